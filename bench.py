@@ -1,0 +1,157 @@
+"""Throughput of the NSPT Langevin step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 2|3|...]
+
+Workload (default): SU(3) Wilson NSPT on a 16^4 periodic lattice, 20 orders
+in beta^-1/2, FP64; RK2 integrator eps = 0.02, zero modes removed and
+Fourier-accelerated stochastic gauge fixing (alpha = 0.05) every step; noise
+generated on the device (Philox4x32-10) inside the timed step.  The field
+starts cold and stays resident in HBM.
+
+Metric: link-order updates/s = 4 V N steps / time.  One JSON line is
+printed with a roofline object for the dominant kernel (FP64 flops per
+launch / its average HIP-event duration over the timed steps) and the CPU
+baseline (the oracle port timed on a bounded sample on this host).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64_TFLOPS = 78.6  # MI355X published FP64 (vector and matrix) peak
+PEAK_HBM_GBS = 8000.0
+
+
+def sp_flops(N, nc=3):
+    """Canonical flops of one truncated series product (survey count)."""
+    if nc == 3:
+        return 216 * N * (N - 1) // 2 + 144 * N
+    return 64 * N * (N - 1) // 2 + 64 * N
+
+
+def ex_flops(N, nc=3):
+    """Canonical flops of one series exp or log."""
+    if nc == 3:
+        return 216 * (N - 1) * N * (N + 1) // 6 + 36 * N * (N - 1)
+    return 64 * (N - 1) * N * (N + 1) // 6 + 16 * N * (N - 1)
+
+
+def kernel_flops(name, N, V, nc=3):
+    """Algorithmic FP64 flops of one launch of a kernel (4V links or V sites)."""
+    SP, EX = sp_flops(N, nc), ex_flops(N, nc)
+    per_link = {"drift": 13 * SP, "update": EX + SP, "update_log": 2 * EX + SP, "zero_mode": EX,
+                "transform": 2 * SP}
+    if name in per_link:
+        return per_link[name] * 4 * V
+    if name == "fa_gexp":
+        return EX * V
+    return 0
+
+
+def step_flops(N, V, integrator, nc=3):
+    SP, EX = sp_flops(N, nc), ex_flops(N, nc)
+    per_link = (30 * SP + 4.25 * EX) if integrator == 1 else (16 * SP + 3.25 * EX)
+    return per_link * 4 * V
+
+
+def cpu_baseline(N, integrator, budget_s=12.0):
+    """Oracle port, one core, 4^4 lattice with the workload's orders and step."""
+    from oracle import oracle as O
+    L, nc = (4, 4, 4, 4), 3
+    V = 256
+    U = O.cold_field(L, nc, N)
+    done, t0 = 0, time.time()
+    while True:
+        eta = O.noise_philox(nc, 1, done, 4 * V)
+        O.step(L, nc, N, U, 0.02, 0.05, integrator=integrator, update=0, gauge_fix=1, zero_modes=1, eta=eta)
+        done += 1
+        if time.time() - t0 > budget_s or done >= 50:
+            break
+    dt = time.time() - t0
+    return {"value": 4 * V * N * done / dt, "unit": "link-order updates/s", "cores": 1, "kind": "port",
+            "sample": "oracle/nspt_oracle.cpp, SU(3) 4^4, N=%d, %s + zero modes + FA, %d steps in %.1f s, 1 core"
+                      % (N, "RK2" if integrator == 1 else "Euler", done, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--L", type=int, default=16)
+    ap.add_argument("--N", type=int, default=20)
+    ap.add_argument("--integrator", choices=["rk2", "euler"], default="rk2")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only; the multi-GPU layout is described in DESIGN.md "
+              "(--gpus %d not supported)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    from prlgt2020_amd import nspt
+    L = (args.L,) * 4
+    N, nc = args.N, 3
+    V = args.L ** 4
+    integ = nspt.RK2 if args.integrator == "rk2" else nspt.EULER
+    ctx = nspt.Context(L, nc, N)
+    p = nspt.Context.params(0.02, 0.05, integ, nspt.GF_FA, 1, nspt.NOISE_DEVICE, seed=1, step=0)
+    ctx.evolve(p, args.warmup, want_obs=False)
+    import ctypes
+    lib = nspt.lib()
+    lib.nspt_profile(ctx.h, 1)
+    t0 = time.perf_counter()
+    ctx.evolve(p, args.steps, want_obs=False)  # returns after the stream is synchronised
+    dt = time.perf_counter() - t0
+    prof = ctx.profile_read()
+    lib.nspt_profile(ctx.h, 0)
+    plaq = ctx.plaquette()
+    value = 4 * V * N * args.steps / dt
+
+    # dominant kernel by total time
+    name = max(prof, key=lambda k: prof[k][0])
+    tot_ms, cnt = prof[name]
+    avg_s = tot_ms / cnt / 1e3
+    fl = kernel_flops(name, N, V, nc)
+    achieved = fl / avg_s / 1e12 if fl else 0.0
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_16x4_N20_rk2.json")
+    if os.path.exists(tpath):
+        traffic = json.load(open(tpath)).get(name)
+    step_tf = step_flops(N, V, integ, nc) * args.steps / dt / 1e12
+
+    out = {
+        "metric": "link-order updates/s per Langevin step (FP64), plain single replica",
+        "value": value,
+        "unit": "link-order updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: cold start, device Philox noise",
+        "config": {"workload": "SU(3) Wilson NSPT %d^4, %d orders, %s eps=0.02, zero modes + FA gauge fixing "
+                               "(alpha=0.05) every step" % (args.L, N, args.integrator.upper()),
+                   "lattice": list(L), "orders": N, "integrator": args.integrator, "replicas": 1},
+        "roofline": {"bound": "mfma", "kernel": name, "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
+                     "avg_launch_ms": avg_s * 1e3, "flops_per_launch": fl,
+                     "step_fp64_tflops": step_tf, "step_frac": step_tf / PEAK_FP64_TFLOPS},
+        "kernels_ms_per_step": {k: v[0] / args.steps for k, v in prof.items() if v[1]},
+        "plaquette_order2": float(plaq[2].real),
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(N, 1 if integ == nspt.RK2 else 0)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,134 @@
+/* nspt.h -- C ABI of the MI355X NSPT Langevin engine (libnspt.so).
+ *
+ * Replaces the C++ function API of the reference's hot path (nspt.h:55-94):
+ * NsptEvolve / gauge_wilson / zero_modes_subtraction /
+ * FAstochastic_gauge_fixing / plaquette_measure operate there on a global
+ * ptGluon_fld through process globals (Quenched.cc:42-67).  Here the same
+ * operations act on an opaque context that owns the field in GPU memory.
+ *
+ * Conventions
+ *  - Status: 0 = ok, negative = error; nspt_last_error() describes the last
+ *    failure of a context.  No C++ exception crosses this boundary.
+ *  - Host field image ("aos"): the byte layout of the reference's
+ *    ptGluon[V] (QCDpt.h:47-48, 249-251; QCDenvNODEpt_test.h:338): for link
+ *    4*site + mu, (1 + nc*nc*alloc_ord) complex doubles (re, im): the order-0
+ *    flag, then order k = 1..alloc_ord as a row-major nc x nc matrix.  Orders
+ *    above ptord are ignored on upload and written as zero on download.
+ *    site = ((x0*L1 + x1)*L2 + x2)*L3 + x3 (lattice.h:37).
+ *  - Noise: 4V matrices of nc*nc complex doubles in link order 4*site + mu,
+ *    the consumption order of gauge_wilson (nspt.cc:365-424); the step uses
+ *    -sqrt(eps)*eta at order 1 (nspt.cc:420-424).
+ *  - Observables of a step ("obs", nspt_obs_len() doubles): plaquette per
+ *    order k = 0..ptord (complex), norm per order 1..ptord (real), trU[mu]
+ *    per order 1..ptord (complex), normalised as the reference writes its
+ *    .txt/.nor/.trU files (nspt.cc:570-597).  They are measured on the field
+ *    entering the step (nspt.cc:392-402).
+ *  - One context per host thread; calls on a context are stream-ordered and
+ *    return after the work is complete unless stated otherwise.
+ */
+#ifndef NSPT_MI355X_H
+#define NSPT_MI355X_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct nspt_ctx nspt_ctx;
+
+/* Lattice and series geometry (Quenched.cc:160-170, choices.h:30, 86-90). */
+typedef struct {
+  int sz[4];     /* taglia L0..L3, periodic boundary conditions */
+  int nc;        /* 3 (SU(3)) or 2 (SU(2)) */
+  int ptord;     /* PTORD: orders 1..ptord are evolved */
+  int alloc_ord; /* allocORD: width of the host field image, >= ptord */
+} nspt_geom;
+
+enum { NSPT_EULER = 0, NSPT_RK2 = 1 };
+enum { NSPT_GF_NONE = 0, NSPT_GF_FA = 1 };
+enum { NSPT_NOISE_HOST = 0, NSPT_NOISE_DEVICE = 1, NSPT_NOISE_ZERO = 2 };
+
+/* One Langevin step (nspt.cc:963-1033, Jacobi update of nspt.cc:428-429). */
+typedef struct {
+  double eps;        /* tau_g of Quench.cfg: Langevin time step */
+  double alpha;      /* gauge-fixing alpha; FA is stable for alpha > 0 */
+  int integrator;    /* NSPT_EULER or NSPT_RK2 */
+  int gauge_fix;     /* NSPT_GF_NONE or NSPT_GF_FA */
+  int zero_modes;    /* 1: zero_modes_subtraction (REMOVE_GLUON_ZEROMOM) */
+  int noise;         /* NSPT_NOISE_HOST (set by nspt_set_noise), _DEVICE, _ZERO */
+  unsigned long long seed; /* device noise: Philox4x32-10 key */
+  unsigned long long step; /* device noise: step counter (advanced by nspt_evolve) */
+} nspt_step_params;
+
+/* Context life cycle.  The field starts cold (flag 1, orders 0), which is
+ * init_status 0 of Quenched.cc:596-604. */
+int nspt_create(const nspt_geom* geom, nspt_ctx** out);
+int nspt_destroy(nspt_ctx* ctx);
+const char* nspt_last_error(const nspt_ctx* ctx);
+int nspt_obs_len(const nspt_ctx* ctx);
+
+/* Field transfer in the ptGluon[V] image (Umu.load / Umu.save payload,
+ * QCDenvNODEpt_test.h:342-374). */
+int nspt_upload(nspt_ctx* ctx, const double* aos);
+int nspt_download(nspt_ctx* ctx, double* aos);
+int nspt_cold_start(nspt_ctx* ctx);
+
+/* Host-generated noise for the next step (NSPT_NOISE_HOST). */
+int nspt_set_noise(nspt_ctx* ctx, const double* eta);
+
+/* NsptEvolve for one step; obs may be null. */
+int nspt_step(nspt_ctx* ctx, const nspt_step_params* p, double* obs);
+
+/* nsteps steps with device noise (p->noise must be NSPT_NOISE_DEVICE or
+ * NSPT_NOISE_ZERO); obs (nsteps * nspt_obs_len doubles) may be null.
+ * p->step is advanced by nsteps. */
+int nspt_evolve(nspt_ctx* ctx, nspt_step_params* p, int nsteps, double* obs);
+
+/* plaquette_measure (nspt.cc:1316-1406): ptord+1 complex. */
+int nspt_plaquette(nspt_ctx* ctx, double* out);
+
+/* Stand-alone FAstochastic_gauge_fixing of the current field
+ * (nspt.cc:866-948). */
+int nspt_fa_gauge_fixing(nspt_ctx* ctx, double alpha);
+
+/* Langevin drift tau_g*reH(U*staples), tau_g = -eps/(2 nc), of the current
+ * field for every link, in the host image layout with flag 0
+ * (nspt.cc:383-417). */
+int nspt_drift(nspt_ctx* ctx, double eps, double* aos_out);
+
+/* Host noise source: ranlxd (ranlxd/ranlxd.cc) seeded as the reference
+ * seeds Rand[0] (Quenched.cc:301-306) and mapped to su(nc) matrices
+ * (MyMath.cc:295-322).  SU(2) draws four uniforms per matrix. */
+typedef struct nspt_rng nspt_rng;
+int nspt_rng_create_ranlxd(int seed, int level, nspt_rng** out);
+int nspt_rng_destroy(nspt_rng* rng);
+int nspt_rng_uniform(nspt_rng* rng, int n, double* out);
+int nspt_rng_noise(nspt_rng* rng, int nc, int nlinks, double* eta);
+
+/* Device noise for inspection: 4V matrices (link order 4*site + mu) of
+ * step `step` with key `seed`. */
+int nspt_device_noise(nspt_ctx* ctx, unsigned long long seed, unsigned long long step, double* eta);
+
+/* Per-kernel timing with HIP events recorded on the context stream around
+ * every launch (no counterpart in the reference, whose MyTime.h timers are
+ * host clocks).  nspt_profile(ctx, 1) resets and enables; the totals (ms)
+ * and launch counts are indexed by the NSPT_PROF_* ids. */
+enum {
+  NSPT_PROF_DRIFT = 0,      /* staples + U*Sigma + reH (+ observables) */
+  NSPT_PROF_UPDATE = 1,     /* exp(W) * U */
+  NSPT_PROF_UPDATE_LOG = 2, /* exp(W) * U, then log for the zero mode */
+  NSPT_PROF_ZERO_MODE = 3,  /* exp(reH(log U - MM)) */
+  NSPT_PROF_FA_DIV = 4,     /* divergence of log U */
+  NSPT_PROF_DFT = 5,        /* one 4-D DFT pass */
+  NSPT_PROF_FA_GEXP = 6,    /* G = exp(reH(W)) per site */
+  NSPT_PROF_TRANSFORM = 7,  /* U <- G U G^dag */
+  NSPT_PROF_NOISE = 8,      /* device noise */
+  NSPT_PROF_REDUCE = 9,     /* deterministic reductions */
+  NSPT_PROF_N = 10
+};
+int nspt_profile(nspt_ctx* ctx, int enable);
+int nspt_profile_read(nspt_ctx* ctx, double* ms_total, long long* count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

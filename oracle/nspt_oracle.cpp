@@ -1,0 +1,965 @@
+// nspt_oracle.cpp -- CPU restatement of the PRlgt2020 NSPT Langevin step.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the oracle that the HIP product path
+// is checked against.  Only tests/, __graft_entry__.smoke() and the
+// cpu_baseline leg of bench.py may load it; the product library never links
+// or calls it.
+//
+// It restates, in plain scalar C++, the arithmetic of the reference:
+//   * series algebra of ptSU3 / ptSU2          QCDpt.h:92-114, 176-220; QCDpt.cc:37-209, 318-537
+//   * 3x3 / 2x2 complex products, dag, Tr      MyMath.h:791-849, 1001, 1064-1080
+//   * periodic lattice indexing + neighbours   lattice.h:37, 139-233, 1194-1222
+//   * staple                                   QCDenvNODEpt_test.h:554-557
+//   * gauge_wilson (Jacobi and on-the-fly)     nspt.cc:258-600
+//   * zero_modes_subtraction                   nspt.cc:605-654
+//   * stochastic_gauge_fixing (in place)       nspt.cc:660-862
+//   * FAstochastic_gauge_fixing                nspt.cc:866-948 (FFTW plan_many semantics of
+//                                              Quenched.cc:827-838 restated as a direct DFT)
+//   * plaquette_measure (OpenMP normalisation) nspt.cc:1316-1406
+//   * noise: SU3rand / SU2rand                 MyMath.cc:256-368
+//   * RAND55 + Box-Muller                      MyRand.h:81-108, MyRand.cc:73-130
+//   * ranlxd (portable path)                   ranlxd/ranlxd.cc:332-601
+//   * RK2 (absent from the reference): second-order Runge-Kutta of Batrouni,
+//     Katz, Kronfeld, Lepage, Svetitsky, Wilson, PRD 32 (1985) 2736, with the
+//     same noise in predictor and corrector; the (1 + C_A eps_raw/6) factor
+//     becomes a two-order shift in the beta^-1/2 expansion.
+//   * Philox4x32-10 device-noise restatement (Salmon et al., SC'11).
+//
+// Pinning of each part (tests/test_oracle_*.py): golden vectors written by
+// oracle/_ref (the reference's own algebra + RNG sources compiled here), the
+// ranlxd known-answer test of ranlxd/test/test.cc:47-102, numpy's FFT for the
+// DFT, and closed forms / invariants (log(exp X) = X, unitarity, gauge
+// covariance, cold-start first step, brute-force plaquette).
+//
+// Series layout: (1 + nc*nc*N) complex, index 0 = flag (order-0 scalar),
+// 1 + (k-1)*nc*nc + r*nc + c = element (r,c) of order k.  This is the memory
+// layout of ptSU3 {Cplx flag; SU3 ptU[allocORD];} with allocORD = N.
+// Field layout: link (site, mu) at series index 4*site + mu (QCDenvNODEpt_test.h:521).
+
+#include <climits>
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+struct cx {
+  double re, im;
+};
+inline cx mk(double r, double i) { return cx{r, i}; }
+inline cx operator+(cx a, cx b) { return cx{a.re + b.re, a.im + b.im}; }
+inline cx operator-(cx a, cx b) { return cx{a.re - b.re, a.im - b.im}; }
+inline cx operator*(cx a, cx b) { return cx{a.re * b.re - a.im * b.im, a.im * b.re + a.re * b.im}; }
+inline cx operator*(double x, cx b) { return cx{x * b.re, x * b.im}; }
+inline cx& operator+=(cx& a, cx b) { a.re += b.re; a.im += b.im; return a; }
+inline cx& operator-=(cx& a, cx b) { a.re -= b.re; a.im -= b.im; return a; }
+inline cx conj(cx a) { return cx{a.re, -a.im}; }
+inline bool is_zero(cx a) { return a.re == 0.0 && a.im == 0.0; }
+
+// ---------------------------------------------------------------- matrices
+// Row-major nc x nc complex matrices (MyMath.h:1697-1706).
+
+// C = A * B; each entry summed over m in ascending order (MyMath.h:791-849).
+inline void mat_mul(int nc, const cx* A, const cx* B, cx* C) {
+  for (int r = 0; r < nc; r++)
+    for (int c = 0; c < nc; c++) {
+      double re = 0.0, im = 0.0;
+      for (int m = 0; m < nc; m++) {
+        const cx a = A[r * nc + m], b = B[m * nc + c];
+        re += a.re * b.re - a.im * b.im;
+        im += a.re * b.im + a.im * b.re;
+      }
+      C[r * nc + c] = cx{re, im};
+    }
+}
+
+// C += A * B
+inline void mat_mac(int nc, const cx* A, const cx* B, cx* C) {
+  cx t[9];
+  mat_mul(nc, A, B, t);
+  for (int e = 0; e < nc * nc; e++) C[e] += t[e];
+}
+
+inline void mat_dag(int nc, const cx* A, cx* B) {
+  for (int r = 0; r < nc; r++)
+    for (int c = 0; c < nc; c++) B[c * nc + r] = conj(A[r * nc + c]);
+}
+
+inline cx mat_tr(int nc, const cx* A) {
+  cx t = A[0];
+  for (int r = 1; r < nc; r++) t = t + A[r * nc + r];
+  return t;
+}
+
+// ---------------------------------------------------------------- series
+struct Ser {
+  int nc, N;
+  std::vector<cx> v;  // 1 + nc*nc*N
+  Ser() : nc(0), N(0) {}
+  Ser(int nc_, int N_, cx flag = cx{0, 0}) : nc(nc_), N(N_), v(1 + nc_ * nc_ * N_, cx{0, 0}) { v[0] = flag; }
+  cx& flag() { return v[0]; }
+  const cx& flag() const { return v[0]; }
+  cx* ord(int k) { return &v[1 + (k - 1) * nc * nc]; }              // k = 1..N
+  const cx* ord(int k) const { return &v[1 + (k - 1) * nc * nc]; }
+  int ne() const { return nc * nc; }
+};
+
+// Truncated Cauchy product L*R (QCDpt.h:92-114): orders i+j <= N from the
+// matrix parts, then the scalar-flag terms R.flag*L_k + L.flag*R_k.
+Ser ser_mul(const Ser& L, const Ser& R) {
+  const int nc = L.nc, N = L.N;
+  Ser C(nc, N);
+  for (int i = 1; i <= N - 1; i++)         // order of R
+    for (int j = 1; j <= N - i; j++)       // order of L
+      mat_mac(nc, L.ord(j), R.ord(i), C.ord(i + j));
+  if (is_zero(L.flag()) && is_zero(R.flag())) {
+    C.flag() = cx{0, 0};
+    return C;
+  }
+  C.flag() = L.flag() * R.flag();
+  for (int k = 1; k <= N; k++) {
+    cx* c = C.ord(k);
+    const cx* l = L.ord(k);
+    const cx* r = R.ord(k);
+    for (int e = 0; e < C.ne(); e++) c[e] += R.flag() * l[e] + L.flag() * r[e];
+  }
+  return C;
+}
+
+Ser ser_add(const Ser& A, const Ser& B) {
+  Ser C = A;
+  for (size_t i = 0; i < C.v.size(); i++) C.v[i] += B.v[i];
+  return C;
+}
+Ser ser_sub(const Ser& A, const Ser& B) {
+  Ser C = A;
+  for (size_t i = 0; i < C.v.size(); i++) C.v[i] -= B.v[i];
+  return C;
+}
+void ser_scale(Ser& A, double x) {
+  for (auto& z : A.v) z = x * z;
+}
+void ser_div(Ser& A, double x) {
+  for (auto& z : A.v) z = cx{z.re / x, z.im / x};
+}
+
+Ser ser_dag(const Ser& A) {
+  Ser B(A.nc, A.N);
+  for (int k = 1; k <= A.N; k++) mat_dag(A.nc, A.ord(k), B.ord(k));
+  B.flag() = conj(A.flag());
+  return B;
+}
+
+// reH (QCDpt.cc:197-209, SU(2): 510-521): per order (M - M^dag)/2 minus Tr/nc.
+void ser_reH(Ser& A) {
+  const int nc = A.nc;
+  const double inv = (nc == 3) ? (1.0 / 3.0) : 0.5;
+  for (int k = 1; k <= A.N; k++) {
+    cx* m = A.ord(k);
+    cx d[9];
+    mat_dag(nc, m, d);
+    for (int e = 0; e < nc * nc; e++) m[e] = 0.5 * (m[e] - d[e]);
+    const cx tr = inv * mat_tr(nc, m);
+    for (int r = 0; r < nc; r++) m[r * nc + r] -= tr;
+  }
+  A.flag() = cx{0, 0};
+}
+
+// Trless (QCDpt.cc:124-136)
+void ser_trless(Ser& A) {
+  const int nc = A.nc;
+  const double inv = (nc == 3) ? (1.0 / 3.0) : 0.5;
+  for (int k = 1; k <= A.N; k++) {
+    cx* m = A.ord(k);
+    const cx z = inv * mat_tr(nc, m);
+    for (int r = 0; r < nc; r++) m[r * nc + r] -= z;
+  }
+  A.flag() = cx{0, 0};
+}
+
+// Tr (QCDpt.cc:115-122): tt[0] = nc*flag, tt[k] = Tr of order k.
+void ser_tr(const Ser& A, cx* tt) {
+  for (int k = 1; k <= A.N; k++) tt[k] = mat_tr(A.nc, A.ord(k));
+  tt[0] = (double)A.nc * A.flag();
+}
+
+// Powers step shared by exp and log: Bnew_{iA+iB} += B_iB * A_iA for iB >= i-1
+// (QCDpt.h:181-187 and 204-211).
+void power_step(const Ser& B, const Ser& A, int i, Ser& Bnew) {
+  const int N = A.N;
+  for (auto& z : Bnew.v) z = cx{0, 0};
+  for (int iA = 1; iA <= N - 1; iA++)
+    for (int iB = i - 1; iB <= N - iA; iB++) mat_mac(A.nc, B.ord(iB), A.ord(iA), Bnew.ord(iA + iB));
+}
+
+// exp (QCDpt.h:200-220): sum_k A^k/k! on the matrix part; result flag = 1.
+Ser ser_exp(const Ser& A) {
+  Ser B = A, Bnew(A.nc, A.N), res = A;
+  for (int i = 2; i <= A.N; i++) {
+    power_step(B, A, i, Bnew);
+    const double den = 1.0 / (double)i;
+    B = Bnew;
+    ser_scale(B, den);
+    for (size_t q = 0; q < res.v.size(); q++) res.v[q] += B.v[q];
+  }
+  res.flag() = cx{1, 0};
+  return res;
+}
+
+// log (QCDpt.h:176-198): sum (-1)^{k+1} X^k/k with X = matrix part of U
+// (flag assumed 1); result flag = 0.
+Ser ser_log(const Ser& U) {
+  Ser B = U, Bnew(U.nc, U.N), res = U;
+  B.flag() = cx{0, 0};
+  double segno = -1.0;
+  for (int i = 2; i <= U.N; i++) {
+    power_step(B, U, i, Bnew);
+    for (int q = 1; q < (int)B.v.size(); q++) B.v[q] = Bnew.v[q];
+    const double aux = segno / (double)i;
+    for (size_t q = 0; q < res.v.size(); q++) res.v[q] += aux * B.v[q];
+    segno = -segno;
+  }
+  res.flag() = cx{0, 0};
+  return res;
+}
+
+// ---------------------------------------------------------------- lattice
+struct Latt {
+  int L[4];
+  int V;
+  std::vector<int> up, dn;  // [site*4 + mu]
+  explicit Latt(const int* sz) {
+    for (int d = 0; d < 4; d++) L[d] = sz[d];
+    V = L[0] * L[1] * L[2] * L[3];
+    up.resize(4 * V);
+    dn.resize(4 * V);
+    for (int s = 0; s < V; s++) {
+      int x[4];
+      coords(s, x);
+      for (int d = 0; d < 4; d++) {
+        int y[4] = {x[0], x[1], x[2], x[3]};
+        y[d] = (x[d] + 1) % L[d];
+        up[s * 4 + d] = index(y);
+        y[d] = (x[d] - 1 + L[d]) % L[d];
+        dn[s * 4 + d] = index(y);
+      }
+    }
+  }
+  // lattice.h:37 GETINDEX, x3 fastest
+  int index(const int* x) const { return ((x[0] * L[1] + x[1]) * L[2] + x[2]) * L[3] + x[3]; }
+  void coords(int s, int* x) const {
+    x[3] = s % L[3];
+    s /= L[3];
+    x[2] = s % L[2];
+    s /= L[2];
+    x[1] = s % L[1];
+    x[0] = s / L[1];
+  }
+};
+
+struct Field {
+  int nc, N, S;
+  const Latt* lat;
+  double* data;  // interleaved complex, [(4*site+mu)*S + j]
+  Ser get(int site, int mu) const {
+    Ser s(nc, N);
+    std::memcpy(s.v.data(), data + 2 * (size_t)(4 * site + mu) * S, sizeof(cx) * S);
+    return s;
+  }
+  void set(int site, int mu, const Ser& s) {
+    std::memcpy(data + 2 * (size_t)(4 * site + mu) * S, s.v.data(), sizeof(cx) * S);
+  }
+};
+
+// staple(n, mu, nu) = U_nu(n+mu) [U_nu(n) U_mu(n+nu)]^dag
+//                   + [U_mu(n-nu) U_nu(n-nu+mu)]^dag U_nu(n-nu)   (QCDenvNODEpt_test.h:554-557)
+Ser staple(const Field& F, int n, int mu, int nu) {
+  const Latt& Z = *F.lat;
+  const int n_pmu = Z.up[n * 4 + mu], n_pnu = Z.up[n * 4 + nu], n_mnu = Z.dn[n * 4 + nu];
+  const int n_mnu_pmu = Z.up[n_mnu * 4 + mu];
+  Ser a = ser_mul(F.get(n_pmu, nu), ser_dag(ser_mul(F.get(n, nu), F.get(n_pnu, mu))));
+  Ser b = ser_mul(ser_dag(ser_mul(F.get(n_mnu, mu), F.get(n_mnu_pmu, nu))), F.get(n_mnu, nu));
+  return ser_add(a, b);
+}
+
+Ser staple_sum(const Field& F, int n, int mu) {
+  Ser W(F.nc, F.N);  // zero(): flag 0
+  for (int nu = 0; nu < 4; nu++)
+    if (nu != mu) {
+      Ser s = staple(F, n, mu, nu);
+      for (size_t q = 0; q < W.v.size(); q++) W.v[q] += s.v[q];
+    }
+  return W;
+}
+
+// Observable accumulators of gauge_wilson (nspt.cc:392-402) and their
+// normalisation (nspt.cc:570-597).
+struct Obs {
+  std::vector<cx> w1;             // N+1
+  std::vector<double> norm;       // N
+  std::vector<cx> trU;            // 4*N
+  void init(int N) {
+    w1.assign(N + 1, cx{0, 0});
+    norm.assign(N, 0.0);
+    trU.assign(4 * N, cx{0, 0});
+  }
+  void accumulate(const Ser& U, const Ser& W, int mu) {
+    const int N = U.N, nc = U.nc;
+    std::vector<cx> w(N + 1);
+    ser_tr(W, w.data());
+    w1[0] += w[0];
+    for (int k = 1; k <= N; k++) {
+      w1[k] += w[k];
+      cx d[9] = {}, p[9] = {};
+      mat_dag(nc, U.ord(k), d);
+      mat_mul(nc, U.ord(k), d, p);
+      norm[k - 1] += mat_tr(nc, p).re;
+      trU[mu * N + (k - 1)] += mat_tr(nc, U.ord(k));
+    }
+  }
+  // out: plaq[2(N+1)], norm[N], trU[2*4N] normalised as written to file
+  void write(int nc, int V, double* out) const {
+    const int N = (int)norm.size();
+    const double pn = (nc == 3) ? 72.0 * V : 48.0 * V;
+    const double nn = (nc == 3) ? 12.0 * V : 8.0 * V;
+    const double tn = (nc == 3) ? 3.0 * V : 2.0 * V;
+    for (int k = 0; k <= N; k++) {
+      out[2 * k] = w1[k].re / pn;
+      out[2 * k + 1] = w1[k].im / pn;
+    }
+    double* o = out + 2 * (N + 1);
+    for (int k = 0; k < N; k++) o[k] = norm[k] / nn;
+    o += N;
+    for (int q = 0; q < 4 * N; q++) {
+      o[2 * q] = trU[q].re / tn;
+      o[2 * q + 1] = trU[q].im / tn;
+    }
+  }
+};
+
+// Drift W = tau_g * reH(U * sum of staples), tau_g = -eps/(2 nc)
+// (nspt.cc:383-417, Quenched.cc:584-589).  Optionally accumulates observables.
+Ser drift(const Field& F, int n, int mu, double tau_g, Obs* obs) {
+  Ser U = F.get(n, mu);
+  Ser W = ser_mul(U, staple_sum(F, n, mu));
+  if (obs) obs->accumulate(U, W, mu);
+  ser_reH(W);
+  ser_scale(W, tau_g);
+  return W;
+}
+
+void sub_noise(Ser& W, double stau, const double* eta) {
+  cx* o1 = W.ord(1);
+  for (int e = 0; e < W.ne(); e++) {
+    o1[e].re -= stau * eta[2 * e];
+    o1[e].im -= stau * eta[2 * e + 1];
+  }
+}
+
+// ---------------------------------------------------------------- DFT
+// Unnormalised multi-dimensional DFT with FFTW plan_many semantics:
+// howmany components, element stride `stride`, distance 1 between
+// components, sign -1 forward / +1 backward, row-major dims (x3 fastest).
+void dft_many(const int* L, int howmany, int stride, cx* data, int sign) {
+  const int V = L[0] * L[1] * L[2] * L[3];
+  const int sstride[4] = {L[1] * L[2] * L[3], L[2] * L[3], L[3], 1};
+  std::vector<cx> line, out;
+  for (int d = 0; d < 4; d++) {
+    const int n = L[d];
+    std::vector<cx> tw(n);
+    for (int j = 0; j < n; j++) {
+      const double ang = 2.0 * M_PI * (double)j / (double)n;
+      tw[j] = cx{std::cos(ang), sign * std::sin(ang)};
+    }
+    line.resize(n);
+    out.resize(n);
+    for (int s = 0; s < V; s++) {
+      // visit each line once: sites with coordinate 0 along d
+      if ((s / sstride[d]) % n != 0) continue;
+      for (int comp = 0; comp < howmany; comp++) {
+        for (int j = 0; j < n; j++) line[j] = data[(size_t)(s + j * sstride[d]) * stride + comp];
+        for (int k = 0; k < n; k++) {
+          cx acc{0, 0};
+          for (int j = 0; j < n; j++) acc += line[j] * tw[(size_t)j * k % n];
+          out[k] = acc;
+        }
+        for (int k = 0; k < n; k++) data[(size_t)(s + k * sstride[d]) * stride + comp] = out[k];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- RNGs
+// ranlxd, portable path (ranlxd/ranlxd.cc:332-531): four interleaved 24-bit
+// subtract-with-borrow lanes, 12 vectors of (low word, high word) x 4 lanes,
+// lags (ir, jr = ir+7 mod 12), luxury p = 202 (level 1) or 397 (level 2).
+struct Ranlxd {
+  int num[96];      // vector v: low words at 8v+0..3, high words at 8v+4..7
+  int carry[4];
+  int pr, prm, ir, jr, is, is_old;
+  int nxt[96];
+  double one_bit;
+
+  Ranlxd(int seed, int level) {
+    one_bit = std::ldexp(1.0, -24);
+    for (int k = 0; k < 96; k++) nxt[k] = (k % 4 == 3) ? (k + 5) % 96 : (k + 1) % 96;
+    pr = (level == 2) ? 397 : 202;
+    int xbit[31];
+    int s = seed;
+    for (int k = 0; k < 31; k++) {
+      xbit[k] = s % 2;
+      s /= 2;
+    }
+    int ibit = 0, jbit = 18;
+    for (int lane = 0; lane < 4; lane++)
+      for (int k = 0; k < 24; k++) {
+        int ix = 0;
+        for (int l = 0; l < 24; l++) {
+          ix = 2 * ix + xbit[ibit];
+          xbit[ibit] = (xbit[ibit] + xbit[jbit]) % 2;
+          ibit = (ibit + 1) % 31;
+          jbit = (jbit + 1) % 31;
+        }
+        if (k % 4 != lane) ix = 16777215 - ix;
+        num[4 * k + lane] = ix;
+      }
+    for (int l = 0; l < 4; l++) carry[l] = 0;
+    ir = 0;
+    jr = 7;
+    is = 91;
+    is_old = 0;
+    prm = pr % 12;
+  }
+
+  void swb(int vi, int vj) {
+    const int base = 0x1000000, mask = 0xffffff;
+    int* xi = &num[8 * vi];
+    const int* xj = &num[8 * vj];
+    for (int l = 0; l < 4; l++) {
+      int d = xj[l] - xi[l] - carry[l];
+      xi[4 + l] += (d < 0);
+      xi[l] = (d + base) & mask;
+    }
+    for (int l = 0; l < 4; l++) {
+      int d = xj[4 + l] - xi[4 + l];
+      carry[l] = (d < 0);
+      xi[4 + l] = (d + base) & mask;
+    }
+  }
+
+  void update() {
+    int vi = ir, vj = jr;
+    for (int k = 0; k < pr; k++) {
+      swb(vi, vj);
+      vi = (vi + 1) % 12;
+      vj = (vj + 1) % 12;
+    }
+    ir = (ir + prm) % 12;
+    jr = (jr + prm) % 12;
+    is = 8 * ir;
+    is_old = is;
+  }
+
+  double next() {
+    is = nxt[is];
+    if (is == is_old) update();
+    return one_bit * ((double)num[is + 4] + one_bit * (double)num[is]);
+  }
+};
+
+// RAND55 additive lagged Fibonacci (MyRand.cc:73-110, MyRand.h:81-91) with
+// the Box-Muller pairing of generate_gauss (MyRand.cc:115-130).  The signed
+// wrap-around of the reference's seeding is reproduced with unsigned 64-bit
+// arithmetic (same bits).
+struct Rand55 {
+  uint64_t ring[55];
+  int nx[55];
+  int n1, n2;
+  bool gen;
+  double r, t;
+  explicit Rand55(long idum) {
+    for (int i = 0; i < 54; i++) nx[i] = i + 1;
+    nx[54] = 0;
+    uint64_t tmp = (uint64_t)161803398L + (uint64_t)idum;
+    ring[54] = tmp;
+    uint64_t aux = 1;
+    int j = 20;
+    for (int i = 0; i < 54; i++, j += 21) {
+      j %= 55;
+      ring[j] = aux;
+      aux += tmp;
+      tmp = ring[j];
+    }
+    n1 = 0;
+    n2 = 31;
+    for (int q = 0; q < 55 * 4; q++) ranf();
+    gen = false;
+    r = t = 0.0;
+  }
+  double ranf() {
+    n1 = nx[n1];
+    n2 = nx[n2];
+    ring[n1] += ring[n2];
+    return (double)ring[n1] * (1.0 / (double)ULONG_MAX);
+  }
+  double gauss() {
+    const double two_pi = 6.28318530717959;  // MY_2pi, MyRand.h:39
+    if (!gen) {
+      t = two_pi * ranf();
+      r = std::sqrt(-std::log(1.0 - ranf()));
+      gen = true;
+      return r * std::cos(t);
+    }
+    gen = false;
+    return r * std::sin(t);
+  }
+};
+
+// su(3) noise from eight uniforms (MyMath.cc:295-322); su(2) from four
+// uniforms of which three are used (the reference reads a fourth element
+// past the end of g[3] at MyMath.cc:352-361; here it is drawn explicitly).
+void noise_from_uniforms(int nc, double* g, double* out /*2*nc*nc*/) {
+  const double twopi = std::atan(1.0) * 8.0;
+  const int ng = (nc == 3) ? 8 : 4;
+  for (int i = 0; i < ng; i += 2) {
+    const double t = twopi * g[i];
+    const double r = std::sqrt(-std::log(1.0 - g[i + 1]));
+    g[i] = r * std::cos(t);
+    g[i + 1] = r * std::sin(t);
+  }
+  if (nc == 3) {
+    g[7] *= std::sqrt(1.0 / 3.0);
+    const double m[18] = {0, g[7] + g[6], g[1], g[0], g[3], g[2], -g[1], g[0], 0, g[7] - g[6],
+                          g[5], g[4], -g[3], g[2], -g[5], g[4], 0, -g[7] * 2};
+    std::memcpy(out, m, sizeof(m));
+  } else {
+    const double m[8] = {0, g[2], g[1], g[0], -g[1], g[0], 0, -g[2]};
+    std::memcpy(out, m, sizeof(m));
+  }
+}
+
+// su(3)/su(2) noise from RAND55 gaussians (MyMath.cc:256-292, 326-346).
+void noise_rand55(int nc, Rand55& R, double* out) {
+  cx P[9];
+  for (auto& z : P) z = cx{0, 0};
+  if (nc == 3) {
+    double g = R.gauss();
+    P[1].im = g; P[3].im = g;
+    g = R.gauss();
+    P[1].re = g; P[3].re = -g;
+    g = R.gauss();
+    P[2].im = g; P[6].im = g;
+    g = R.gauss();
+    P[2].re = g; P[6].re = -g;
+    g = R.gauss();
+    P[5].im = g; P[7].im = g;
+    g = R.gauss();
+    P[5].re = g; P[7].re = -g;
+    g = R.gauss();
+    const double gg = R.gauss() * std::sqrt(1.0 / 3.0);
+    P[0].im = gg + g;
+    P[4].im = gg - g;
+    P[8].im = -2. * gg;
+  } else {
+    double g = R.gauss();
+    P[1].im = g; P[2].im = g;
+    g = R.gauss();
+    P[1].re = g; P[2].re = -g;
+    g = R.gauss();
+    P[0].im = g; P[3].im = -g;
+  }
+  std::memcpy(out, P, sizeof(cx) * nc * nc);
+}
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).
+inline void philox4x32(uint32_t ctr[4], uint32_t key[2]) {
+  uint32_t k0 = key[0], k1 = key[1];
+  for (int round = 0; round < 10; round++) {
+    const uint64_t p0 = (uint64_t)0xD2511F53u * ctr[0];
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * ctr[2];
+    const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    const uint32_t c0 = hi1 ^ ctr[1] ^ k0, c2 = hi0 ^ ctr[3] ^ k1;
+    ctr[0] = c0;
+    ctr[1] = lo1;
+    ctr[2] = c2;
+    ctr[3] = lo0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+}
+
+// Device-noise convention: link l of step `step` uses counters
+// (l, step_lo, step_hi, q) for q = 0,1 and key (seed_lo, seed_hi); each 32-bit
+// word u maps to the uniform (u + 0.5) * 2^-32 in (0,1).
+void noise_philox_link(int nc, uint64_t seed, uint64_t step, uint32_t link, double* out) {
+  double g[8];
+  const int nq = (nc == 3) ? 2 : 1;
+  for (int q = 0; q < nq; q++) {
+    uint32_t ctr[4] = {link, (uint32_t)step, (uint32_t)(step >> 32), (uint32_t)q};
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    philox4x32(ctr, key);
+    for (int w = 0; w < 4; w++) g[4 * q + w] = ((double)ctr[w] + 0.5) * (1.0 / 4294967296.0);
+  }
+  noise_from_uniforms(nc, g, out);
+}
+
+// ---------------------------------------------------------------- step
+struct Params {
+  double eps;        // Langevin step epsilon (tau_g of Quench.cfg)
+  double alpha;      // gauge-fixing alpha (reference sign conventions)
+  int integrator;    // 0 Euler, 1 RK2
+  int update;        // 0 Jacobi, 1 on-the-fly (sequential, in place)
+  int gauge_fix;     // 0 none, 1 Fourier accelerated, 2 default in-place
+  int zero_modes;    // 1: zero_modes_subtraction
+};
+
+struct Geom {
+  int L[4], nc, N;
+};
+
+void zero_modes(Field& F, std::vector<Ser>& MM) {
+  const Latt& Z = *F.lat;
+  for (int mu = 0; mu < 4; mu++) ser_scale(MM[mu], 1.0 / (double)Z.V);  // MM *= rVol
+  for (int s = 0; s < Z.V; s++)
+    for (int mu = 0; mu < 4; mu++) {
+      Ser W = ser_sub(ser_log(F.get(s, mu)), MM[mu]);
+      ser_reH(W);
+      F.set(s, mu, ser_exp(W));
+    }
+}
+
+void fa_gauge_fixing(Field& F, double alpha) {
+  const Latt& Z = *F.lat;
+  const int S = F.S, nc = F.nc, N = F.N;
+  std::vector<cx> Wg((size_t)Z.V * S, cx{0, 0});
+  for (int x = 0; x < Z.V; x++) {
+    Ser W(nc, N);
+    for (int mu = 0; mu < 4; mu++) {
+      Ser d = ser_sub(ser_log(F.get(x, mu)), ser_log(F.get(Z.dn[x * 4 + mu], mu)));
+      for (size_t q = 0; q < W.v.size(); q++) W.v[q] += d.v[q];
+    }
+    std::memcpy(&Wg[(size_t)x * S], W.v.data(), sizeof(cx) * S);
+  }
+  // plan howmany = nc*nc*PTORD + 1, stride = S, dist = 1 (Quenched.cc:827-838)
+  dft_many(Z.L, S, S, Wg.data(), -1);
+  for (int q = 0; q < S; q++) Wg[q] = 0.0 * Wg[q];  // PBC: k = 0 mode removed
+  std::vector<double> p2[4];
+  for (int d = 0; d < 4; d++) {
+    p2[d].resize(Z.L[d]);
+    for (int i = 0; i < Z.L[d]; i++) {
+      const double p = 2 * M_PI * i / Z.L[d];
+      double h = std::sin(p * 0.5);
+      h *= 4 * h;
+      p2[d][i] = h;  // lattice.h:1144-1168
+    }
+  }
+  for (int x = 1; x < Z.V; x++) {
+    int c[4];
+    Z.coords(x, c);
+    const double f = -(alpha / (p2[0][c[0]] + p2[1][c[1]] + p2[2][c[2]] + p2[3][c[3]]));
+    for (int q = 0; q < S; q++) Wg[(size_t)x * S + q] = f * Wg[(size_t)x * S + q];
+  }
+  dft_many(Z.L, S, S, Wg.data(), +1);
+  std::vector<Ser> G(Z.V);
+  for (int x = 0; x < Z.V; x++) {
+    Ser W(nc, N);
+    std::memcpy(W.v.data(), &Wg[(size_t)x * S], sizeof(cx) * S);
+    ser_div(W, (double)Z.V);
+    ser_reH(W);
+    G[x] = ser_exp(W);
+  }
+  for (int x = 0; x < Z.V; x++)
+    for (int mu = 0; mu < 4; mu++)
+      F.set(x, mu, ser_mul(ser_mul(G[x], F.get(x, mu)), ser_dag(G[Z.up[x * 4 + mu]])));
+}
+
+// Default (non-FA) stochastic gauge fixing: sequential and in place
+// (nspt.cc:731-805).
+void sgf_default(Field& F, double alpha) {
+  const Latt& Z = *F.lat;
+  for (int s = 0; s < Z.V; s++) {
+    Ser W(F.nc, F.N);
+    for (int mu = 0; mu < 4; mu++) {
+      Ser d = ser_sub(F.get(s, mu), F.get(Z.dn[s * 4 + mu], mu));
+      for (size_t q = 0; q < W.v.size(); q++) W.v[q] += d.v[q];
+    }
+    W = ser_sub(W, ser_dag(W));
+    ser_trless(W);
+    Ser Wa = W, Wm = W;
+    ser_scale(Wa, alpha);
+    ser_scale(Wm, -alpha);
+    Ser G1 = ser_exp(Wa), G2 = ser_exp(Wm);
+    for (int mu = 0; mu < 4; mu++) {
+      F.set(s, mu, ser_mul(G1, F.get(s, mu)));
+      const int b = Z.dn[s * 4 + mu];
+      F.set(b, mu, ser_mul(F.get(b, mu), G2));
+    }
+  }
+}
+
+// RK2 corrector drift: (D1 + D2)/2 times (1 + (nc eps/6) x^2), x = beta^-1/2.
+Ser rk2_combine(const Ser& D1, const Ser& D2, int nc, double eps) {
+  Ser D = ser_add(D1, D2);
+  ser_scale(D, 0.5);
+  const double c = (double)nc * eps / 6.0;
+  for (int k = D.N; k >= 3; k--) {
+    cx* o = D.ord(k);
+    const cx* p = D.ord(k - 2);
+    for (int e = 0; e < D.ne(); e++) o[e] += c * p[e];
+  }
+  return D;
+}
+
+// One NsptEvolve step (nspt.cc:963-1033).  eta: 4V noise matrices in link
+// order.  obs: plaq[2(N+1)], norm[N], trU[8N] (may be null).
+void step(const Geom& g, double* Udata, const Params& p, const double* eta, double* obs_out) {
+  Latt Z(g.L);
+  const int S = 1 + g.nc * g.nc * g.N, ne2 = 2 * g.nc * g.nc;
+  Field U{g.nc, g.N, S, &Z, Udata};
+  const double stau = std::sqrt(p.eps), tau_g = -p.eps / (2.0 * g.nc);
+  std::vector<Ser> MM(4, Ser(g.nc, g.N));
+  Obs obs;
+  obs.init(g.N);
+
+  if (p.update == 1) {
+    // on the fly: in place, lexicographic site order, mu inner (nspt.cc:365-441)
+    for (int s = 0; s < Z.V; s++)
+      for (int mu = 0; mu < 4; mu++) {
+        Ser W = drift(U, s, mu, tau_g, &obs);
+        sub_noise(W, stau, eta + (size_t)(4 * s + mu) * ne2);
+        Ser Un = ser_mul(ser_exp(W), U.get(s, mu));
+        U.set(s, mu, Un);
+        MM[mu] = ser_add(MM[mu], ser_log(Un));
+      }
+  } else {
+    std::vector<double> Fbuf((size_t)8 * Z.V * S);
+    Field Fn{g.nc, g.N, S, &Z, Fbuf.data()};
+    std::vector<Ser> D1;
+    if (p.integrator == 1) {
+      // predictor U1 = exp(D(U) - sqrt(eps) eta) U
+      D1.resize(4 * Z.V);
+      for (int s = 0; s < Z.V; s++)
+        for (int mu = 0; mu < 4; mu++) {
+          D1[4 * s + mu] = drift(U, s, mu, tau_g, &obs);
+          Ser W = D1[4 * s + mu];
+          sub_noise(W, stau, eta + (size_t)(4 * s + mu) * ne2);
+          Fn.set(s, mu, ser_mul(ser_exp(W), U.get(s, mu)));
+        }
+    }
+    std::vector<double> Gbuf;
+    for (int s = 0; s < Z.V; s++)
+      for (int mu = 0; mu < 4; mu++) {
+        Ser W;
+        if (p.integrator == 1) {
+          // corrector: drift at U1, combined with D(U); applied to U
+          W = rk2_combine(D1[4 * s + mu], drift(Fn, s, mu, tau_g, nullptr), g.nc, p.eps);
+        } else {
+          W = drift(U, s, mu, tau_g, &obs);
+        }
+        sub_noise(W, stau, eta + (size_t)(4 * s + mu) * ne2);
+        Ser Un = ser_mul(ser_exp(W), U.get(s, mu));
+        if (p.integrator == 1) {
+          if (Gbuf.empty()) Gbuf.resize((size_t)8 * Z.V * S);
+          std::memcpy(Gbuf.data() + 2 * (size_t)(4 * s + mu) * S, Un.v.data(), sizeof(cx) * S);
+        } else {
+          Fn.set(s, mu, Un);
+        }
+        MM[mu] = ser_add(MM[mu], ser_log(Un));
+      }
+    // Umu = Fmu (nspt.cc:992)
+    if (p.integrator == 1)
+      std::memcpy(Udata, Gbuf.data(), sizeof(double) * Gbuf.size());
+    else
+      std::memcpy(Udata, Fbuf.data(), sizeof(double) * Fbuf.size());
+  }
+  if (p.zero_modes) zero_modes(U, MM);
+  if (p.gauge_fix == 1) fa_gauge_fixing(U, p.alpha);
+  if (p.gauge_fix == 2) sgf_default(U, p.alpha);
+  if (obs_out) obs.write(g.nc, Z.V, obs_out);
+}
+
+// plaquette_measure, OpenMP form (nspt.cc:1355-1402): sum over mu != nu of
+// Tr(U_mu * staple) / (72 V) (SU(3)) or (48 V) (SU(2)).
+void plaquette_measure(const Geom& g, const double* Udata, double* out) {
+  Latt Z(g.L);
+  const int S = 1 + g.nc * g.nc * g.N;
+  Field U{g.nc, g.N, S, &Z, const_cast<double*>(Udata)};
+  std::vector<cx> w(g.N + 1, cx{0, 0}), t(g.N + 1);
+  for (int s = 0; s < Z.V; s++)
+    for (int mu = 0; mu < 4; mu++) {
+      Ser W = ser_mul(U.get(s, mu), staple_sum(U, s, mu));
+      ser_tr(W, t.data());
+      for (int k = 0; k <= g.N; k++) w[k] += t[k];
+    }
+  const double nrm = (g.nc == 3) ? 72.0 * Z.V : 48.0 * Z.V;
+  for (int k = 0; k <= g.N; k++) {
+    out[2 * k] = w[k].re / nrm;
+    out[2 * k + 1] = w[k].im / nrm;
+  }
+}
+
+Ser from_ptr(int nc, int N, const double* p) {
+  Ser s(nc, N);
+  std::memcpy(s.v.data(), p, sizeof(cx) * s.v.size());
+  return s;
+}
+void to_ptr(const Ser& s, double* p) { std::memcpy(p, s.v.data(), sizeof(cx) * s.v.size()); }
+
+Geom geom_from(const int* gi) {
+  Geom g;
+  for (int d = 0; d < 4; d++) g.L[d] = gi[d];
+  g.nc = gi[4];
+  g.N = gi[5];
+  return g;
+}
+
+}  // namespace
+
+// ================================================================ C ABI
+// geometry arrays are int[6] = {L0, L1, L2, L3, nc, N}; params arrays are
+// double[6] = {eps, alpha, integrator, update, gauge_fix, zero_modes}.
+extern "C" {
+
+void orc_ser_mul(int nc, int N, const double* a, const double* b, double* c) {
+  to_ptr(ser_mul(from_ptr(nc, N, a), from_ptr(nc, N, b)), c);
+}
+void orc_ser_exp(int nc, int N, const double* a, double* c) { to_ptr(ser_exp(from_ptr(nc, N, a)), c); }
+void orc_ser_log(int nc, int N, const double* a, double* c) { to_ptr(ser_log(from_ptr(nc, N, a)), c); }
+void orc_ser_reh(int nc, int N, const double* a, double* c) {
+  Ser s = from_ptr(nc, N, a);
+  ser_reH(s);
+  to_ptr(s, c);
+}
+void orc_ser_trless(int nc, int N, const double* a, double* c) {
+  Ser s = from_ptr(nc, N, a);
+  ser_trless(s);
+  to_ptr(s, c);
+}
+void orc_ser_dag(int nc, int N, const double* a, double* c) { to_ptr(ser_dag(from_ptr(nc, N, a)), c); }
+void orc_ser_tr(int nc, int N, const double* a, double* tt) {
+  ser_tr(from_ptr(nc, N, a), reinterpret_cast<cx*>(tt));
+}
+
+void orc_staple_sum(const int* gi, const double* U, int site, int mu, double* out) {
+  Geom g = geom_from(gi);
+  Latt Z(g.L);
+  Field F{g.nc, g.N, 1 + g.nc * g.nc * g.N, &Z, const_cast<double*>(U)};
+  to_ptr(staple_sum(F, site, mu), out);
+}
+
+// D[4V*S] = tau_g * reH(U * staples) for every link (no noise).
+void orc_drift(const int* gi, const double* U, double eps, double* D) {
+  Geom g = geom_from(gi);
+  Latt Z(g.L);
+  const int S = 1 + g.nc * g.nc * g.N;
+  Field F{g.nc, g.N, S, &Z, const_cast<double*>(U)};
+  for (int s = 0; s < Z.V; s++)
+    for (int mu = 0; mu < 4; mu++)
+      to_ptr(drift(F, s, mu, -eps / (2.0 * g.nc), nullptr), D + 2 * (size_t)(4 * s + mu) * S);
+}
+
+void orc_step(const int* gi, double* U, const double* pp, const double* eta, double* obs) {
+  Params p{pp[0], pp[1], (int)pp[2], (int)pp[3], (int)pp[4], (int)pp[5]};
+  step(geom_from(gi), U, p, eta, obs);
+}
+
+// Stage entry points (Jacobi, Euler): gauge_wilson into F with MM[4*S] and
+// observables; zero modes given MM; gauge fixings in place.
+void orc_gauge_wilson_jacobi(const int* gi, const double* U, double* F, const double* eta, double eps,
+                             double* MMout, double* obs) {
+  Geom g = geom_from(gi);
+  Latt Z(g.L);
+  const int S = 1 + g.nc * g.nc * g.N, ne2 = 2 * g.nc * g.nc;
+  Field Uf{g.nc, g.N, S, &Z, const_cast<double*>(U)};
+  Field Ff{g.nc, g.N, S, &Z, F};
+  std::vector<Ser> MM(4, Ser(g.nc, g.N));
+  Obs o;
+  o.init(g.N);
+  for (int s = 0; s < Z.V; s++)
+    for (int mu = 0; mu < 4; mu++) {
+      Ser W = drift(Uf, s, mu, -eps / (2.0 * g.nc), &o);
+      sub_noise(W, std::sqrt(eps), eta + (size_t)(4 * s + mu) * ne2);
+      Ser Un = ser_mul(ser_exp(W), Uf.get(s, mu));
+      Ff.set(s, mu, Un);
+      MM[mu] = ser_add(MM[mu], ser_log(Un));
+    }
+  if (MMout)
+    for (int mu = 0; mu < 4; mu++) to_ptr(MM[mu], MMout + 2 * (size_t)mu * S);
+  if (obs) o.write(g.nc, Z.V, obs);
+}
+
+void orc_zero_modes(const int* gi, double* U, const double* MMin) {
+  Geom g = geom_from(gi);
+  Latt Z(g.L);
+  const int S = 1 + g.nc * g.nc * g.N;
+  Field F{g.nc, g.N, S, &Z, U};
+  std::vector<Ser> MM(4);
+  for (int mu = 0; mu < 4; mu++) MM[mu] = from_ptr(g.nc, g.N, MMin + 2 * (size_t)mu * S);
+  zero_modes(F, MM);
+}
+
+void orc_fa_gauge_fixing(const int* gi, double* U, double alpha) {
+  Geom g = geom_from(gi);
+  Latt Z(g.L);
+  Field F{g.nc, g.N, 1 + g.nc * g.nc * g.N, &Z, U};
+  fa_gauge_fixing(F, alpha);
+}
+
+void orc_sgf_default(const int* gi, double* U, double alpha) {
+  Geom g = geom_from(gi);
+  Latt Z(g.L);
+  Field F{g.nc, g.N, 1 + g.nc * g.nc * g.N, &Z, U};
+  sgf_default(F, alpha);
+}
+
+void orc_plaquette_measure(const int* gi, const double* U, double* out) {
+  plaquette_measure(geom_from(gi), U, out);
+}
+
+// data: V * stride complex; transforms components 0..howmany-1.
+void orc_dft_many(const int* L, int howmany, int stride, double* data, int sign) {
+  dft_many(L, howmany, stride, reinterpret_cast<cx*>(data), sign);
+}
+
+// ---- RNG
+void* orc_ranlxd_new(int seed, int level) { return new Ranlxd(seed, level); }
+void orc_ranlxd_free(void* r) { delete static_cast<Ranlxd*>(r); }
+void orc_ranlxd_draw(void* r, int n, double* out) {
+  Ranlxd* g = static_cast<Ranlxd*>(r);
+  for (int i = 0; i < n; i++) out[i] = g->next();
+}
+void* orc_rand55_new(long seed) { return new Rand55(seed); }
+void orc_rand55_free(void* r) { delete static_cast<Rand55*>(r); }
+void orc_rand55_ranf(void* r, int n, double* out) {
+  for (int i = 0; i < n; i++) out[i] = static_cast<Rand55*>(r)->ranf();
+}
+void orc_rand55_gauss(void* r, int n, double* out) {
+  for (int i = 0; i < n; i++) out[i] = static_cast<Rand55*>(r)->gauss();
+}
+// nlinks noise matrices (2*nc*nc doubles each) in consumption order.
+void orc_noise_ranlxd(void* r, int nc, int nlinks, double* out) {
+  Ranlxd* g = static_cast<Ranlxd*>(r);
+  const int ng = (nc == 3) ? 8 : 4;
+  for (int l = 0; l < nlinks; l++) {
+    double u[8];
+    for (int i = 0; i < ng; i++) u[i] = g->next();
+    noise_from_uniforms(nc, u, out + (size_t)l * 2 * nc * nc);
+  }
+}
+void orc_noise_rand55(void* r, int nc, int nlinks, double* out) {
+  for (int l = 0; l < nlinks; l++) noise_rand55(nc, *static_cast<Rand55*>(r), out + (size_t)l * 2 * nc * nc);
+}
+void orc_noise_philox(int nc, unsigned long long seed, unsigned long long step, int nlinks, double* out) {
+  for (int l = 0; l < nlinks; l++) noise_philox_link(nc, seed, step, (uint32_t)l, out + (size_t)l * 2 * nc * nc);
+}
+// ranlxd seed of the reference driver: first glibc rand() after srand(seed)
+// (Quenched.cc:301-306).
+int orc_glibc_first_rand(unsigned int seed) {
+  srand(seed);
+  return rand();
+}
+
+}  // extern "C"

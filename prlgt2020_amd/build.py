@@ -1,0 +1,85 @@
+"""Build libnspt.so (HIP, gfx950) in-tree.
+
+    python -m prlgt2020_amd.build [--force] [-j N]
+
+Each (NC, N) kernel instance is one translation unit (nspt_inst.hip compiled
+with -DINST_NC/-DINST_N) so they compile in parallel; the host side
+(nspt_capi.hip, nspt_rng.cpp) is linked with them into
+prlgt2020_amd/libnspt.so.  The instance list mirrors NSPT_INSTANCES of
+csrc/nspt_internal.hpp.
+"""
+import argparse
+import os
+import re
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libnspt.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("NSPT_ARCH", "gfx950")
+
+# packed-FP32 VALU forms are disabled for all device code (FP64 path anyway).
+HIPFLAGS = ["-O3", "--offload-arch=" + ARCH, "-fPIC", "-std=c++17",
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+HEADERS = ["nspt_device.hpp", "nspt_kernels.hpp", "nspt_internal.hpp"]
+
+
+def instances():
+    text = open(os.path.join(CSRC, "nspt_internal.hpp")).read()
+    block = text[text.index("#define NSPT_INSTANCES"):]
+    block = block[:block.index("\n\n")]
+    return [(int(a), int(b)) for a, b in re.findall(r"X\((\d+),\s*(\d+)\)", block)]
+
+
+def _newer(target, sources):
+    if not os.path.exists(target):
+        return True
+    t = os.path.getmtime(target)
+    return any(os.path.getmtime(s) > t for s in sources)
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("build failed: " + " ".join(cmd) + "\n" + r.stdout + r.stderr)
+    return r
+
+
+def build(force=False, jobs=None, verbose=False):
+    os.makedirs(OBJ, exist_ok=True)
+    hdrs = [os.path.join(CSRC, h) for h in HEADERS] + [os.path.join(HERE, "..", "include", "nspt.h")]
+    jobs = jobs or min(8, os.cpu_count() or 4)
+    tasks = []
+    for nc, n in instances():
+        src = os.path.join(CSRC, "nspt_inst.hip")
+        obj = os.path.join(OBJ, "inst_%d_%d.o" % (nc, n))
+        if force or _newer(obj, [src] + hdrs):
+            tasks.append([HIPCC] + HIPFLAGS + ["-DINST_NC=%d" % nc, "-DINST_N=%d" % n, "-c", src, "-o", obj])
+    capi_obj = os.path.join(OBJ, "nspt_capi.o")
+    if force or _newer(capi_obj, [os.path.join(CSRC, "nspt_capi.hip")] + hdrs):
+        tasks.append([HIPCC] + HIPFLAGS + ["-c", os.path.join(CSRC, "nspt_capi.hip"), "-o", capi_obj])
+    rng_obj = os.path.join(OBJ, "nspt_rng.o")
+    if force or _newer(rng_obj, [os.path.join(CSRC, "nspt_rng.cpp")] + hdrs):
+        tasks.append(["g++", "-O2", "-fPIC", "-std=c++17", "-c", os.path.join(CSRC, "nspt_rng.cpp"), "-o", rng_obj])
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        for r in ex.map(_run, tasks):
+            if verbose and r.stderr:
+                print(r.stderr, file=sys.stderr)
+    objs = [os.path.join(OBJ, "inst_%d_%d.o" % inst) for inst in instances()] + [capi_obj, rng_obj]
+    if force or tasks or _newer(LIB, objs):
+        _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", LIB + ".tmp"] + objs)
+        os.replace(LIB + ".tmp", LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--force", action="store_true")
+    ap.add_argument("-j", type=int, default=None)
+    a = ap.parse_args()
+    print(build(force=a.force, jobs=a.j, verbose=True))

@@ -1,0 +1,351 @@
+// nspt_kernels.hpp -- per-link / per-site kernels of the NSPT step, templated
+// on the gauge group (NC = 3 or 2) and the compiled order count N (>= the
+// runtime order; orders above the runtime order read as zero and are never
+// stored, which leaves the lower orders unchanged because a truncated product
+// only feeds orders upward).
+//
+// Link index on the device: idx = mu*V + site (site lexicographic with x3
+// fastest, lattice.h:37).  One wave = one workgroup = LPB links of the same mu.
+#pragma once
+#include "nspt_device.hpp"
+#include "nspt_internal.hpp"
+
+namespace nspt {
+
+__device__ __forceinline__ int site_shift(const Geo& g, int s, int d, int delta) {
+  const int x = (s / g.st[d]) % g.L[d];
+  int xn = x + delta;
+  xn += (xn < 0) ? g.L[d] : 0;
+  xn -= (xn >= g.L[d]) ? g.L[d] : 0;
+  return s + (xn - x) * g.st[d];
+}
+
+template <int NC, int N>
+__device__ __forceinline__ void stage_global(const Lane<NC, N>& ln, const SerField& f, long idx, int nrt, bool ok) {
+  constexpr int NE = NC * NC;
+  if (!ln.lds_ok) return;
+  const unsigned off = (unsigned)(ln.e * f.P + idx);
+  const long plane = (long)NE * f.P;
+#pragma unroll
+  for (int k = 0; k < N; k++) ln.slot[k * NE + ln.e] = (ok && k < nrt) ? ld_u(f.data + k * plane, off) : zero2();
+}
+
+// Sum over the valid link slots of the block of element (k, e) of the staged
+// series; lanes loop over the (k, e) pairs.  Deterministic (ascending slots).
+template <int NC, int N>
+__device__ __forceinline__ double2 slot_sum(const double2* lds, int nvalid, int k, int e) {
+  constexpr int NE = NC * NC;
+  double2 s = zero2();
+  for (int jj = 0; jj < nvalid; jj++) s = c_add(s, lds[jj * N * NE + k * NE + e]);
+  return s;
+}
+
+// ------------------------------------------------------------ drift
+// D = tau_g * reH(U_mu(x) * sum_{nu != mu} staple_{mu nu}(x))
+// (nspt.cc:383-417, staple QCDenvNODEpt_test.h:554-557).
+// mode 0: store D.  mode 1: RK2 corrector, store (D1 + D)/2 * (1 + rk2c x^2).
+// mode 2: observables only.  obs_part (may be null): per block
+// [plaq (N+1) cplx][norm N][trU N cplx] of the block's mu, unnormalised.
+// Scr: per-link scratch series (intermediate right operands).
+template <int NC, int N>
+__global__ __launch_bounds__(64) void k_drift(Geo g, SerField U, SerField D, SerField D1, SerField Scr, SerField Sg,
+                                              double tau_g, double rk2c, int mode, double* obs_part) {
+  constexpr int NE = NC * NC;
+  using LN = Lane<NC, N>;
+  __shared__ double2 lds[LN::LPB * LN::SLOT];
+  __shared__ double2 flagbuf[LN::LPB];
+  LN ln(lds);
+  const int nrt = g.nrt;
+  const int M = min(N, nrt) - 1;
+  const int mu = blockIdx.x / g.nb_mu;
+  const int site0 = (blockIdx.x % g.nb_mu) * LN::LPB;
+  const int nvalid = min(LN::LPB, g.V - site0);
+  const int site = min(site0 + ln.j, g.V - 1);
+  const bool ok = ln.lds_ok && (site0 + ln.j) < g.V;
+  const long V = g.V;
+  const long idx = mu * V + site;
+
+  double2 Out[N];
+  double2 sigf = zero2();
+
+  for (int p = 0; p < 3; p++) {
+    const int nu = (p < mu) ? p : p + 1;
+    const int xpmu = site_shift(g, site, mu, 1);
+    const int xpnu = site_shift(g, site, nu, 1);
+    const int xmnu = site_shift(g, site, nu, -1);
+    const int xmnu_pmu = site_shift(g, xmnu, mu, 1);
+    const long i_nu_x = nu * V + site, i_mu_xpnu = mu * V + xpnu, i_nu_xpmu = nu * V + xpmu;
+    const long i_mu_xmnu = mu * V + xmnu, i_nu_xmnupmu = nu * V + xmnu_pmu, i_nu_xmnu = nu * V + xmnu;
+
+    // T1 = U_nu(x) U_mu(x+nu)
+    stage_global(ln, U, i_nu_x, nrt, ok);
+    __syncthreads();
+    const double2 t1f = mul<NC, N>(ln, false, bop(U, i_mu_xpnu), gflag(U, i_nu_x, ok), gflag(U, i_mu_xpnu, ok), Out,
+                                   M, nrt, ok);
+    gstore<NC, N>(Scr, idx, ln.e, nrt, ok, Out);
+    __syncthreads();
+    // up = U_nu(x+mu) T1^dag
+    stage_global(ln, U, i_nu_xpmu, nrt, ok);
+    __syncthreads();
+    const double2 uf = mul<NC, N>(ln, false, bop(Scr, idx, true), gflag(U, i_nu_xpmu, ok), c_conj(t1f), Out, M, nrt,
+                                  ok);
+    __syncthreads();
+    gaccum<NC, N>(Sg, idx, ln.e, nrt, ok, Out, p == 0);
+    sigf = c_add(sigf, uf);
+    // T2 = U_mu(x-nu) U_nu(x-nu+mu)
+    stage_global(ln, U, i_mu_xmnu, nrt, ok);
+    __syncthreads();
+    const double2 t2f = mul<NC, N>(ln, false, bop(U, i_nu_xmnupmu), gflag(U, i_mu_xmnu, ok),
+                                   gflag(U, i_nu_xmnupmu, ok), Out, M, nrt, ok);
+    __syncthreads();
+    // down = T2^dag U_nu(x-nu)
+    ln.stage(Out);
+    __syncthreads();
+    const double2 df = mul<NC, N>(ln, true, bop(U, i_nu_xmnu), c_conj(t2f), gflag(U, i_nu_xmnu, ok), Out, M, nrt, ok);
+    __syncthreads();
+    gaccum<NC, N>(Sg, idx, ln.e, nrt, ok, Out, false);
+    sigf = c_add(sigf, df);
+  }
+
+  // W = U_mu(x) Sigma
+  stage_global(ln, U, idx, nrt, ok);
+  __syncthreads();
+  const double2 wf = mul<NC, N>(ln, false, bop(Sg, idx), gflag(U, idx, ok), sigf, Out, M, nrt, ok);
+  __syncthreads();
+
+  double* op = obs_part ? obs_part + (long)blockIdx.x * (2 * (nrt + 1) + 3 * nrt) : nullptr;
+  if (op) {
+    // norm_k = sum Re Tr(U_k U_k^dag), trU_k = sum Tr U_k over the staged U
+    const int lane = threadIdx.x;
+    if (lane < nrt) {
+      double nr = 0.0;
+      double2 tr = zero2();
+      for (int jj = 0; jj < nvalid; jj++) {
+        const double2* m = lds + jj * LN::SLOT + lane * NE;
+#pragma unroll
+        for (int q = 0; q < NE; q++) nr += m[q].x * m[q].x + m[q].y * m[q].y;
+#pragma unroll
+        for (int q = 0; q < NC; q++) tr = c_add(tr, m[q * NC + q]);
+      }
+      op[2 * (nrt + 1) + lane] = nr;
+      op[3 * nrt + 2 + 2 * lane] = tr.x;
+      op[3 * nrt + 2 + 2 * lane + 1] = tr.y;
+    }
+    if (ln.lds_ok && ln.e == 0) flagbuf[ln.j] = wf;
+  }
+  __syncthreads();
+  ln.stage(Out);
+  __syncthreads();
+  if (op) {
+    const int lane = threadIdx.x;
+    if (lane <= nrt) {
+      double2 s = zero2();
+      for (int jj = 0; jj < nvalid; jj++) {
+        if (lane == 0) {
+          s = c_add(s, c_scale((double)NC, flagbuf[jj]));
+        } else {
+          const double2* m = lds + jj * LN::SLOT + (lane - 1) * NE;
+#pragma unroll
+          for (int q = 0; q < NC; q++) s = c_add(s, m[q * NC + q]);
+        }
+      }
+      op[2 * lane] = s.x;
+      op[2 * lane + 1] = s.y;
+    }
+  }
+  if (mode == 2) return;
+
+  // D = tau_g reH(W)
+#pragma unroll
+  for (int k = 0; k < N; k++) Out[k] = c_scale(tau_g, reh_elem(ln, k));
+  if (mode == 1) {
+    double2 D1v[N];
+    gload<NC, N>(D1, idx, ln.e, nrt, ok, D1v);
+#pragma unroll
+    for (int k = 0; k < N; k++) Out[k] = c_scale(0.5, c_add(D1v[k], Out[k]));
+#pragma unroll
+    for (int k = N - 1; k >= 2; k--) Out[k] = c_add(Out[k], c_scale(rk2c, Out[k - 2]));
+  }
+  gstore<NC, N>(D, idx, ln.e, nrt, ok, Out);
+}
+
+// ------------------------------------------------------------ update
+// Uout = exp(D - stau*eta at order 1) * Ubase  (nspt.cc:420-430, exp QCDpt.h:200-220);
+// optionally L = log(Uout) (QCDpt.h:176-198) and per-block sums of L for the
+// zero mode (nspt.cc:432-439).  eta: [idx*NE + e] (may be null).
+template <int NC, int N>
+__global__ __launch_bounds__(64) void k_update(Geo g, SerField D, const double2* eta, double stau, SerField Ubase,
+                                               SerField Uout, SerField L, double2* mm_part) {
+  constexpr int NE = NC * NC;
+  using LN = Lane<NC, N>;
+  __shared__ double2 lds[LN::LPB * LN::SLOT];
+  LN ln(lds);
+  const int nrt = g.nrt;
+  const int M = min(N, nrt) - 1;
+  const int mu = blockIdx.x / g.nb_mu;
+  const int site0 = (blockIdx.x % g.nb_mu) * LN::LPB;
+  const int nvalid = min(LN::LPB, g.V - site0);
+  const int site = min(site0 + ln.j, g.V - 1);
+  const bool ok = ln.lds_ok && (site0 + ln.j) < g.V;
+  const long idx = (long)mu * g.V + site;
+
+  double2 S[N];
+  BOp Db = bop(D, idx);
+  Db.eta = eta;
+  Db.stau = stau;
+  horner<NC, N, true>(ln, Db, S, nrt, ok);
+  // Unew = E * Ubase
+  ln.stage(S);
+  __syncthreads();
+  const double2 nf = mul<NC, N>(ln, false, bop(Ubase, idx), one2(), gflag(Ubase, idx, ok), S, M, nrt, ok);
+  gstore<NC, N>(Uout, idx, ln.e, nrt, ok, S);
+  if (ok && ln.e == 0) Uout.flag[idx] = nf;
+  if (!L.data) return;
+  __syncthreads();
+  // L = log(Unew)
+  horner<NC, N, false>(ln, bop(Uout, idx), S, nrt, ok);
+  gstore<NC, N>(L, idx, ln.e, nrt, ok, S);
+  if (ok && ln.e == 0 && L.flag) L.flag[idx] = zero2();
+  if (!mm_part) return;
+  ln.stage(S);
+  __syncthreads();
+  for (int o = threadIdx.x; o < nrt * NE; o += 64)
+    mm_part[(long)blockIdx.x * nrt * NE + o] = slot_sum<NC, N>(lds, nvalid, o / NE, o % NE);
+}
+
+// ------------------------------------------------------------ log field
+// L = log(U) for every link (FA gauge fixing without zero-mode removal).
+template <int NC, int N>
+__global__ __launch_bounds__(64) void k_log(Geo g, SerField U, SerField L) {
+  using LN = Lane<NC, N>;
+  __shared__ double2 lds[LN::LPB * LN::SLOT];
+  LN ln(lds);
+  const int nrt = g.nrt;
+  const int mu = blockIdx.x / g.nb_mu;
+  const int site0 = (blockIdx.x % g.nb_mu) * LN::LPB;
+  const int site = min(site0 + ln.j, g.V - 1);
+  const bool ok = ln.lds_ok && (site0 + ln.j) < g.V;
+  const long idx = (long)mu * g.V + site;
+  double2 S[N];
+  horner<NC, N, false>(ln, bop(U, idx), S, nrt, ok);
+  gstore<NC, N>(L, idx, ln.e, nrt, ok, S);
+  if (ok && ln.e == 0 && L.flag) L.flag[idx] = zero2();
+}
+
+// ------------------------------------------------------------ zero modes
+// X = reH(L - MM_mu), U = exp(X)  (nspt.cc:638-641).  MM: [(mu*nrt + k)*NE + e],
+// already multiplied by 1/V.  X is kept: it is log(U) for the FA step.
+template <int NC, int N>
+__global__ __launch_bounds__(64) void k_zero_mode(Geo g, SerField L, const double2* MM, SerField U, SerField X) {
+  constexpr int NE = NC * NC;
+  using LN = Lane<NC, N>;
+  __shared__ double2 lds[LN::LPB * LN::SLOT];
+  LN ln(lds);
+  const int nrt = g.nrt;
+  const int mu = blockIdx.x / g.nb_mu;
+  const int site0 = (blockIdx.x % g.nb_mu) * LN::LPB;
+  const int site = min(site0 + ln.j, g.V - 1);
+  const bool ok = ln.lds_ok && (site0 + ln.j) < g.V;
+  const long idx = (long)mu * g.V + site;
+  double2 P[N];
+  gload<NC, N>(L, idx, ln.e, nrt, ok, P);
+#pragma unroll
+  for (int k = 0; k < N; k++)
+    if (k < nrt) P[k] = c_sub(P[k], MM[((long)mu * nrt + k) * NE + ln.e]);
+  ln.stage(P);
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < N; k++) P[k] = reh_elem(ln, k);
+  gstore<NC, N>(X, idx, ln.e, nrt, ok, P);
+  __syncthreads();
+  horner<NC, N, true>(ln, bop(X, idx), P, nrt, ok);
+  gstore<NC, N>(U, idx, ln.e, nrt, ok, P);
+  if (ok && ln.e == 0) U.flag[idx] = one2();
+}
+
+// ------------------------------------------------------------ FA: G = exp(reH(W))
+// W per site from the transformed buffer fb[(k*NE + e)*V + site] (already
+// multiplied by -alpha/(p^2 V)), G = exp(reH(W)) (nspt.cc:902-906).
+// Xs: per-site scratch series.
+template <int NC, int N>
+__global__ __launch_bounds__(64) void k_fa_gexp(Geo g, const double2* fb, SerField Xs, SerField G) {
+  constexpr int NE = NC * NC;
+  using LN = Lane<NC, N>;
+  __shared__ double2 lds[LN::LPB * LN::SLOT];
+  LN ln(lds);
+  const int nrt = g.nrt;
+  const int site0 = blockIdx.x * LN::LPB;
+  const int site = min(site0 + ln.j, g.V - 1);
+  const bool ok = ln.lds_ok && (site0 + ln.j) < g.V;
+  double2 P[N];
+#pragma unroll
+  for (int k = 0; k < N; k++) P[k] = (ok && k < nrt) ? fb[((long)k * NE + ln.e) * g.V + site] : zero2();
+  ln.stage(P);
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < N; k++) P[k] = reh_elem(ln, k);
+  gstore<NC, N>(Xs, site, ln.e, nrt, ok, P);
+  __syncthreads();
+  horner<NC, N, true>(ln, bop(Xs, site), P, nrt, ok);
+  gstore<NC, N>(G, site, ln.e, nrt, ok, P);
+  if (ok && ln.e == 0) G.flag[site] = one2();
+}
+
+// ------------------------------------------------------------ gauge transform
+// U_mu(x) <- G(x) U_mu(x) G(x+mu)^dag, in place (nspt.cc:912-916).
+template <int NC, int N>
+__global__ __launch_bounds__(64) void k_gauge_transform(Geo g, SerField G, SerField U) {
+  using LN = Lane<NC, N>;
+  __shared__ double2 lds[LN::LPB * LN::SLOT];
+  LN ln(lds);
+  const int nrt = g.nrt;
+  const int M = min(N, nrt) - 1;
+  const int mu = blockIdx.x / g.nb_mu;
+  const int site0 = (blockIdx.x % g.nb_mu) * LN::LPB;
+  const int site = min(site0 + ln.j, g.V - 1);
+  const bool ok = ln.lds_ok && (site0 + ln.j) < g.V;
+  const long idx = (long)mu * g.V + site;
+  const int xp = site_shift(g, site, mu, 1);
+  double2 Out[N];
+  stage_global(ln, G, site, nrt, ok);
+  __syncthreads();
+  const double2 tf = mul<NC, N>(ln, false, bop(U, idx), gflag(G, site, ok), gflag(U, idx, ok), Out, M, nrt, ok);
+  __syncthreads();
+  ln.stage(Out);
+  __syncthreads();
+  const double2 nf = mul<NC, N>(ln, false, bop(G, xp, true), tf, c_conj(gflag(G, xp, ok)), Out, M, nrt, ok);
+  gstore<NC, N>(U, idx, ln.e, nrt, ok, Out);
+  if (ok && ln.e == 0) U.flag[idx] = nf;
+}
+
+// ------------------------------------------------------------ instance table
+template <int NC, int N>
+KernelSet make_kernel_set() {
+  KernelSet s;
+  s.nc = NC;
+  s.nmax = N;
+  s.lpb = 64 / (NC * NC);
+  s.drift = [](int nblk, hipStream_t st, Geo g, SerField U, SerField D, SerField D1, SerField Scr, SerField Sg,
+               double tau_g, double rk2c, int mode, double* obs) {
+    hipLaunchKernelGGL((k_drift<NC, N>), dim3(nblk), dim3(64), 0, st, g, U, D, D1, Scr, Sg, tau_g, rk2c, mode, obs);
+  };
+  s.update = [](int nblk, hipStream_t st, Geo g, SerField D, const double2* eta, double stau, SerField Ub,
+                SerField Uo, SerField L, double2* mm) {
+    hipLaunchKernelGGL((k_update<NC, N>), dim3(nblk), dim3(64), 0, st, g, D, eta, stau, Ub, Uo, L, mm);
+  };
+  s.logf = [](int nblk, hipStream_t st, Geo g, SerField U, SerField L) {
+    hipLaunchKernelGGL((k_log<NC, N>), dim3(nblk), dim3(64), 0, st, g, U, L);
+  };
+  s.zero_mode = [](int nblk, hipStream_t st, Geo g, SerField L, const double2* MM, SerField U, SerField X) {
+    hipLaunchKernelGGL((k_zero_mode<NC, N>), dim3(nblk), dim3(64), 0, st, g, L, MM, U, X);
+  };
+  s.gexp = [](int nblk, hipStream_t st, Geo g, const double2* fb, SerField Xs, SerField G) {
+    hipLaunchKernelGGL((k_fa_gexp<NC, N>), dim3(nblk), dim3(64), 0, st, g, fb, Xs, G);
+  };
+  s.transform = [](int nblk, hipStream_t st, Geo g, SerField G, SerField U) {
+    hipLaunchKernelGGL((k_gauge_transform<NC, N>), dim3(nblk), dim3(64), 0, st, g, G, U);
+  };
+  return s;
+}
+
+}  // namespace nspt

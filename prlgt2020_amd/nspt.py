@@ -1,0 +1,212 @@
+"""ctypes binding of libnspt.so (include/nspt.h).
+
+The product path: every call goes to the HIP library; there is no CPU
+fallback.  Importing this module without the built library raises.
+
+Host field images are numpy complex128 arrays of shape (4*V, 1 + nc*nc*A) in
+link order 4*site + mu, A = alloc_ord (the ptGluon[V] layout of
+QCDpt.h:47-48 / QCDenvNODEpt_test.h:338).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libnspt.so")
+
+EULER, RK2 = 0, 1
+GF_NONE, GF_FA = 0, 1
+NOISE_HOST, NOISE_DEVICE, NOISE_ZERO = 0, 1, 2
+
+
+class NsptError(RuntimeError):
+    pass
+
+
+class _Geom(ctypes.Structure):
+    _fields_ = [("sz", ctypes.c_int * 4), ("nc", ctypes.c_int), ("ptord", ctypes.c_int),
+                ("alloc_ord", ctypes.c_int)]
+
+
+class StepParams(ctypes.Structure):
+    _fields_ = [("eps", ctypes.c_double), ("alpha", ctypes.c_double), ("integrator", ctypes.c_int),
+                ("gauge_fix", ctypes.c_int), ("zero_modes", ctypes.c_int), ("noise", ctypes.c_int),
+                ("seed", ctypes.c_ulonglong), ("step", ctypes.c_ulonglong)]
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_lib = None
+
+EXPORTS = ["nspt_create", "nspt_destroy", "nspt_last_error", "nspt_obs_len", "nspt_upload", "nspt_download",
+           "nspt_cold_start", "nspt_set_noise", "nspt_step", "nspt_evolve", "nspt_plaquette",
+           "nspt_fa_gauge_fixing", "nspt_drift", "nspt_rng_create_ranlxd", "nspt_rng_destroy",
+           "nspt_rng_uniform", "nspt_rng_noise", "nspt_device_noise", "nspt_profile", "nspt_profile_read"]
+
+PROF_NAMES = ["drift", "update", "update_log", "zero_mode", "fa_div", "dft", "fa_gexp", "transform", "noise",
+              "reduce"]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise NsptError("libnspt.so is not built: run python -m prlgt2020_amd.build")
+        l = ctypes.CDLL(LIB_PATH)
+        l.nspt_create.argtypes = [ctypes.POINTER(_Geom), ctypes.POINTER(ctypes.c_void_p)]
+        l.nspt_destroy.argtypes = [ctypes.c_void_p]
+        l.nspt_last_error.argtypes = [ctypes.c_void_p]
+        l.nspt_last_error.restype = ctypes.c_char_p
+        l.nspt_obs_len.argtypes = [ctypes.c_void_p]
+        for f in ("nspt_upload", "nspt_download", "nspt_set_noise", "nspt_plaquette"):
+            getattr(l, f).argtypes = [ctypes.c_void_p, _dp]
+        l.nspt_cold_start.argtypes = [ctypes.c_void_p]
+        l.nspt_step.argtypes = [ctypes.c_void_p, ctypes.POINTER(StepParams), _dp]
+        l.nspt_evolve.argtypes = [ctypes.c_void_p, ctypes.POINTER(StepParams), ctypes.c_int, _dp]
+        l.nspt_fa_gauge_fixing.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        l.nspt_drift.argtypes = [ctypes.c_void_p, ctypes.c_double, _dp]
+        l.nspt_rng_create_ranlxd.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+        l.nspt_rng_destroy.argtypes = [ctypes.c_void_p]
+        l.nspt_rng_uniform.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+        l.nspt_rng_noise.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]
+        l.nspt_device_noise.argtypes = [ctypes.c_void_p, ctypes.c_ulonglong, ctypes.c_ulonglong, _dp]
+        l.nspt_profile.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        l.nspt_profile_read.argtypes = [ctypes.c_void_p, _dp, ctypes.POINTER(ctypes.c_longlong)]
+        _lib = l
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+class Ranlxd:
+    """Host noise source of the engine (ranlxd, MyMath.cc:295-322 mapping)."""
+
+    def __init__(self, seed, level=1):
+        h = ctypes.c_void_p()
+        rc = lib().nspt_rng_create_ranlxd(seed, level, ctypes.byref(h))
+        if rc != 0:
+            raise NsptError("nspt_rng_create_ranlxd failed (%d)" % rc)
+        self.h = h
+
+    def uniform(self, n):
+        out = np.zeros(n)
+        lib().nspt_rng_uniform(self.h, n, _ptr(out))
+        return out
+
+    def noise(self, nc, nlinks):
+        out = np.zeros((nlinks, nc * nc), np.complex128)
+        lib().nspt_rng_noise(self.h, nc, nlinks, _ptr(out))
+        return out
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.nspt_rng_destroy(self.h)
+            self.h = None
+
+
+class Context:
+    """One lattice field on the GPU (an nspt_ctx)."""
+
+    def __init__(self, L, nc, ptord, alloc_ord=None):
+        self.L = tuple(int(x) for x in L)
+        self.nc, self.N = nc, ptord
+        self.A = ptord if alloc_ord is None else alloc_ord
+        self.V = self.L[0] * self.L[1] * self.L[2] * self.L[3]
+        g = _Geom((ctypes.c_int * 4)(*self.L), nc, ptord, self.A)
+        h = ctypes.c_void_p()
+        rc = lib().nspt_create(ctypes.byref(g), ctypes.byref(h))
+        self.h = h
+        if rc != 0:
+            msg = lib().nspt_last_error(h).decode() if h.value else "nspt_create failed"
+            self.close()
+            raise NsptError("%s (%d)" % (msg, rc))
+        self.obs_len = lib().nspt_obs_len(self.h)
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise NsptError("%s: %s (%d)" % (what, lib().nspt_last_error(self.h).decode(), rc))
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            lib().nspt_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        if _lib is not None:
+            self.close()
+
+    @property
+    def S(self):
+        return 1 + self.nc * self.nc * self.A
+
+    def upload(self, U):
+        U = np.ascontiguousarray(U, dtype=np.complex128)
+        assert U.shape == (4 * self.V, self.S), U.shape
+        self._check(lib().nspt_upload(self.h, _ptr(U)), "upload")
+
+    def download(self):
+        U = np.zeros((4 * self.V, self.S), np.complex128)
+        self._check(lib().nspt_download(self.h, _ptr(U)), "download")
+        return U
+
+    def cold_start(self):
+        self._check(lib().nspt_cold_start(self.h), "cold_start")
+
+    def set_noise(self, eta):
+        eta = np.ascontiguousarray(eta, dtype=np.complex128)
+        assert eta.shape == (4 * self.V, self.nc * self.nc)
+        self._check(lib().nspt_set_noise(self.h, _ptr(eta)), "set_noise")
+
+    @staticmethod
+    def params(eps, alpha=0.0, integrator=EULER, gauge_fix=GF_NONE, zero_modes=1, noise=NOISE_HOST,
+               seed=0, step=0):
+        return StepParams(eps, alpha, integrator, gauge_fix, zero_modes, noise, seed, step)
+
+    def step(self, params, want_obs=True):
+        obs = np.zeros(self.obs_len) if want_obs else None
+        self._check(lib().nspt_step(self.h, ctypes.byref(params), _ptr(obs) if want_obs else None), "step")
+        return obs
+
+    def evolve(self, params, nsteps, want_obs=True):
+        obs = np.zeros(nsteps * self.obs_len) if want_obs else None
+        self._check(lib().nspt_evolve(self.h, ctypes.byref(params), nsteps, _ptr(obs) if want_obs else None),
+                    "evolve")
+        return obs.reshape(nsteps, self.obs_len) if want_obs else None
+
+    def plaquette(self):
+        out = np.zeros(self.N + 1, np.complex128)
+        self._check(lib().nspt_plaquette(self.h, _ptr(out)), "plaquette")
+        return out
+
+    def fa_gauge_fixing(self, alpha):
+        self._check(lib().nspt_fa_gauge_fixing(self.h, alpha), "fa_gauge_fixing")
+
+    def drift(self, eps):
+        D = np.zeros((4 * self.V, self.S), np.complex128)
+        self._check(lib().nspt_drift(self.h, eps, _ptr(D)), "drift")
+        return D
+
+    def profile(self, enable=True):
+        self._check(lib().nspt_profile(self.h, 1 if enable else 0), "profile")
+
+    def profile_read(self):
+        """{kernel: (total_ms, launches)} since the last profile() call."""
+        ms = np.zeros(len(PROF_NAMES))
+        cnt = (ctypes.c_longlong * len(PROF_NAMES))()
+        self._check(lib().nspt_profile_read(self.h, _ptr(ms), cnt), "profile_read")
+        return {n: (ms[i], int(cnt[i])) for i, n in enumerate(PROF_NAMES)}
+
+    def device_noise(self, seed, step):
+        eta = np.zeros((4 * self.V, self.nc * self.nc), np.complex128)
+        self._check(lib().nspt_device_noise(self.h, seed, step, _ptr(eta)), "device_noise")
+        return eta
+
+
+def split_obs(obs, N):
+    """plaq (N+1 complex), norm (N), trU (4 x N complex)."""
+    plaq = obs[0:2 * (N + 1)].view(np.complex128)
+    norm = obs[2 * (N + 1):2 * (N + 1) + N]
+    tru = obs[2 * (N + 1) + N:].view(np.complex128).reshape(4, N)
+    return plaq, norm, tru

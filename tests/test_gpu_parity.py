@@ -1,0 +1,169 @@
+"""Parity of the HIP step (libnspt.so, through the C ABI) with the oracle.
+
+Bar: <= 1e-12 relative per order (max over links of |gpu - oracle| divided
+by the largest oracle coefficient of that order), on the same field and the
+same host-generated ranlxd noise.  The oracle restates nspt.cc:258-1039 in
+Jacobi form (update not on the fly, FA gauge fixing), see oracle/.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def _nspt():
+    from prlgt2020_amd import nspt
+    return nspt
+
+
+def _run_pair(L, nc, N, integrator, gauge_fix, zero_modes, eps=0.05, alpha=0.05, seed=3, noise_seed=1234,
+              relerr=None, steps=1):
+    nspt = _nspt()
+    V = L[0] * L[1] * L[2] * L[3]
+    U = O.random_field(L, nc, N, seed=seed, scale=0.4)
+    ctx = nspt.Context(L, nc, N)
+    ctx.upload(U)
+    rng = nspt.Ranlxd(noise_seed)
+    Uo = U.copy()
+    worst = 0.0
+    for s in range(steps):
+        eta = rng.noise(nc, 4 * V)
+        ctx.set_noise(eta)
+        p = nspt.Context.params(eps, alpha, integrator, gauge_fix, zero_modes, nspt.NOISE_HOST)
+        obs_g = ctx.step(p)
+        obs_o = O.step(L, nc, N, Uo, eps, alpha, integrator=integrator, update=0, gauge_fix=gauge_fix,
+                       zero_modes=zero_modes, eta=eta)
+        Ug = ctx.download()
+        e = relerr(Ug, Uo, nc, N)
+        print("step", s, "per-order rel err", e)
+        assert e.max() <= TOL, e
+        po, pg = O.split_obs(obs_o, N)[0], nspt.split_obs(obs_g, N)[0]
+        assert np.abs(pg - po).max() <= TOL * max(1.0, np.abs(po).max())
+        no, ng = O.split_obs(obs_o, N)[1], nspt.split_obs(obs_g, N)[1]
+        assert np.abs(ng - no).max() <= TOL * max(1.0, np.abs(no).max())
+        to, tg = O.split_obs(obs_o, N)[2], nspt.split_obs(obs_g, N)[2]
+        assert np.abs(tg - to).max() <= TOL * max(1.0, np.abs(to).max())
+        worst = max(worst, e.max())
+    ctx.close()
+    return worst
+
+
+@pytest.mark.parametrize("nc", [3, 2])
+def test_roundtrip(nc):
+    nspt = _nspt()
+    L, N = (2, 2, 2, 4), 5
+    U = O.random_field(L, nc, N, seed=1)
+    Upad = np.zeros((U.shape[0], 1 + nc * nc * 7), np.complex128)
+    Upad[:, :U.shape[1]] = U
+    ctx = nspt.Context(L, nc, N, alloc_ord=7)
+    ctx.upload(Upad)
+    assert np.array_equal(ctx.download(), Upad)
+
+
+@pytest.mark.parametrize("nc,N", [(3, 6), (2, 6), (3, 3)])
+def test_drift_matches_oracle(nc, N, relerr):
+    nspt = _nspt()
+    L = (2, 4, 2, 4)
+    U = O.random_field(L, nc, N, seed=5)
+    ctx = nspt.Context(L, nc, N)
+    ctx.upload(U)
+    Dg = ctx.drift(0.05)
+    Do = O.drift(L, nc, N, U, 0.05)
+    assert relerr(Dg, Do, nc, N)[1:].max() <= TOL
+
+
+@pytest.mark.parametrize("nc", [3, 2])
+def test_plaquette_measure(nc):
+    nspt = _nspt()
+    L, N = (4, 2, 2, 4), 6
+    U = O.random_field(L, nc, N, seed=9)
+    ctx = nspt.Context(L, nc, N)
+    ctx.upload(U)
+    pg, po = ctx.plaquette(), O.plaquette_measure(L, nc, N, U)
+    assert np.abs(pg - po).max() <= TOL * np.abs(po).max()
+
+
+@pytest.mark.parametrize("nc", [3, 2])
+def test_euler_gauge_wilson_only(nc, relerr):
+    _run_pair((2, 2, 4, 4), nc, 6, 0, 0, 0, relerr=relerr)
+
+
+@pytest.mark.parametrize("nc", [3, 2])
+def test_euler_zero_modes(nc, relerr):
+    _run_pair((2, 2, 4, 4), nc, 6, 0, 0, 1, relerr=relerr)
+
+
+@pytest.mark.parametrize("nc", [3, 2])
+def test_euler_zero_modes_fa(nc, relerr):
+    _run_pair((2, 4, 2, 4), nc, 6, 0, 1, 1, relerr=relerr, steps=2)
+
+
+@pytest.mark.parametrize("nc", [3, 2])
+def test_rk2_zero_modes_fa(nc, relerr):
+    _run_pair((2, 2, 4, 4), nc, 6, 1, 1, 1, eps=0.02, relerr=relerr, steps=2)
+
+
+@pytest.mark.parametrize("N", [1, 2, 8, 12, 16, 20])
+def test_orders(N, relerr):
+    _run_pair((2, 2, 2, 4), 3, N, 1, 1, 1, eps=0.02, relerr=relerr)
+
+
+def test_fa_alone(relerr):
+    nspt = _nspt()
+    L, nc, N = (4, 2, 4, 2), 3, 6
+    U = O.random_field(L, nc, N, seed=21)
+    ctx = nspt.Context(L, nc, N)
+    ctx.upload(U)
+    ctx.fa_gauge_fixing(0.05)
+    Uo = U.copy()
+    O.fa_gauge_fixing(L, nc, N, Uo, 0.05)
+    assert relerr(ctx.download(), Uo, nc, N).max() <= TOL
+
+
+@pytest.mark.parametrize("nc", [3, 2])
+def test_device_noise_matches_oracle(nc):
+    nspt = _nspt()
+    L = (2, 2, 4, 4)
+    ctx = nspt.Context(L, nc, 2)
+    eg = ctx.device_noise(77, 5)
+    eo = O.noise_philox(nc, 77, 5, 4 * 64)
+    assert np.abs(eg - eo).max() <= 1e-13
+
+
+def test_cold_start_first_step_closed_form(relerr):
+    """From the cold start all staples are 1 so the drift vanishes and the
+    first Euler step (no zero modes, no gauge fixing) is exp(-sqrt(eps) eta)."""
+    nspt = _nspt()
+    L, nc, N = (2, 2, 2, 4), 3, 6
+    V = 32
+    ctx = nspt.Context(L, nc, N)
+    eta = nspt.Ranlxd(7).noise(nc, 4 * V)
+    ctx.set_noise(eta)
+    ctx.step(nspt.Context.params(0.05, 0.0, nspt.EULER, nspt.GF_NONE, 0, nspt.NOISE_HOST))
+    Ug = ctx.download()
+    ref = np.zeros_like(Ug)
+    for l in range(4 * V):
+        A = np.zeros(1 + 9 * N, np.complex128)
+        A[1:10] = -np.sqrt(0.05) * eta[l]
+        ref[l] = O.exp(nc, N, A)
+    assert relerr(Ug, ref, nc, N).max() <= TOL
+
+
+def test_evolve_device_noise_matches_oracle(relerr):
+    """Several steps with device noise against the oracle fed the same noise."""
+    nspt = _nspt()
+    L, nc, N = (2, 2, 4, 4), 3, 6
+    U = O.random_field(L, nc, N, seed=4, scale=0.3)
+    ctx = nspt.Context(L, nc, N)
+    ctx.upload(U)
+    p = nspt.Context.params(0.02, 0.05, nspt.RK2, nspt.GF_FA, 1, nspt.NOISE_DEVICE, seed=11, step=0)
+    obs = ctx.evolve(p, 3)
+    Uo = U.copy()
+    for s in range(3):
+        eta = O.noise_philox(nc, 11, s, 4 * 64)
+        obs_o = O.step(L, nc, N, Uo, 0.02, 0.05, integrator=1, update=0, gauge_fix=1, zero_modes=1, eta=eta)
+        assert np.abs(obs[s] - obs_o).max() <= 1e-11
+    assert relerr(ctx.download(), Uo, nc, N).max() <= 1e-11
