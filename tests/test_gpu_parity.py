@@ -19,11 +19,24 @@ def _nspt():
     return nspt
 
 
+def evolved_field(L, nc, N, nsteps=40, eps=0.05, seed=5):
+    """A genuine NSPT configuration: nsteps Euler + zero modes + FA steps
+    from the cold start with device noise."""
+    nspt = _nspt()
+    ctx = nspt.Context(L, nc, N)
+    ctx.evolve(nspt.Context.params(eps, 0.05, nspt.EULER, nspt.GF_FA, 1, nspt.NOISE_DEVICE, seed=seed), nsteps,
+               want_obs=False)
+    U = ctx.download()
+    ctx.close()
+    return U
+
+
 def _run_pair(L, nc, N, integrator, gauge_fix, zero_modes, eps=0.05, alpha=0.05, seed=3, noise_seed=1234,
-              relerr=None, steps=1):
+              relerr=None, steps=1, U=None):
     nspt = _nspt()
     V = L[0] * L[1] * L[2] * L[3]
-    U = O.random_field(L, nc, N, seed=seed, scale=0.4)
+    if U is None:
+        U = O.random_field(L, nc, N, seed=seed, scale=0.4)
     ctx = nspt.Context(L, nc, N)
     ctx.upload(U)
     rng = nspt.Ranlxd(noise_seed)
@@ -106,9 +119,19 @@ def test_rk2_zero_modes_fa(nc, relerr):
     _run_pair((2, 2, 4, 4), nc, 6, 1, 1, 1, eps=0.02, relerr=relerr, steps=2)
 
 
-@pytest.mark.parametrize("N", [1, 2, 8, 12, 16, 20])
-def test_orders(N, relerr):
+@pytest.mark.parametrize("N", [1, 2, 8])
+def test_orders_random_field(N, relerr):
     _run_pair((2, 2, 2, 4), 3, N, 1, 1, 1, eps=0.02, relerr=relerr)
+
+
+@pytest.mark.parametrize("nc,N", [(3, 12), (3, 16), (3, 20), (3, 24), (2, 20)])
+def test_orders_evolved_field(nc, N, relerr):
+    """High orders on a Langevin-evolved field (random coefficients of equal
+    size at all orders are not a physical input: their order-20 products
+    cancel to 1e-11 relative in any two FP64 evaluation orders)."""
+    L = (2, 2, 2, 4)
+    U = evolved_field(L, nc, N)
+    _run_pair(L, nc, N, 1, 1, 1, eps=0.02, relerr=relerr, U=U)
 
 
 def test_fa_alone(relerr):
