@@ -31,7 +31,9 @@ HEADERS = ["nspt_device.hpp", "nspt_kernels.hpp", "nspt_internal.hpp"]
 
 def instances():
     text = open(os.path.join(CSRC, "nspt_internal.hpp")).read()
-    block = text[text.index("#define NSPT_INSTANCES"):]
+    # the full list is the last definition (the first is a one-instance
+    # variant for quick experiments)
+    block = text[text.rindex("#define NSPT_INSTANCES"):]
     block = block[:block.index("\n\n")]
     return [(int(a), int(b)) for a, b in re.findall(r"X\((\d+),\s*(\d+)\)", block)]
 

@@ -284,7 +284,7 @@ struct nspt_ctx {
   int nblk_links = 0, nblk_sites = 0, nob = 0, obs_len = 0;
   hipStream_t stream = nullptr;
   std::vector<void*> allocs;
-  SerField U{}, F{}, D{}, D1{}, L{}, X{}, Scr{}, Sg{}, G{}, Xs{};
+  SerField U{}, F{}, D{}, D1{}, L{}, X{}, G{};
   double2* fb = nullptr;
   double2* eta = nullptr;
   double2* eta_tmp = nullptr;
@@ -432,7 +432,7 @@ bool fa_apply(nspt_ctx* c, const SerField& Xsrc, double alpha) {
   }
   {
     Prof pr(c, NSPT_PROF_FA_GEXP);
-    c->ks.gexp(c->nblk_sites, c->stream, c->g, c->fb, c->Xs, c->G);
+    c->ks.gexp(c->nblk_sites, c->stream, c->g, c->fb, c->G);
   }
   {
     Prof pr(c, NSPT_PROF_TRANSFORM);
@@ -475,7 +475,7 @@ bool do_step(nspt_ctx* c, const nspt_step_params* p, double* obs_dev) {
   if (p->integrator == NSPT_RK2) {
     {
       Prof pr(c, NSPT_PROF_DRIFT);
-      c->ks.drift(c->nblk_links, c->stream, c->g, c->U, c->D1, none, c->Scr, c->Sg, tau_g, 0.0, 0, obs_part);
+      c->ks.drift(c->nblk_links, c->stream, c->g, c->U, c->D1, none, tau_g, 0.0, 0, obs_part);
     }
     {
       Prof pr(c, NSPT_PROF_UPDATE);
@@ -483,11 +483,11 @@ bool do_step(nspt_ctx* c, const nspt_step_params* p, double* obs_dev) {
     }
     {
       Prof pr(c, NSPT_PROF_DRIFT);
-      c->ks.drift(c->nblk_links, c->stream, c->g, c->F, c->D, c->D1, c->Scr, c->Sg, tau_g, rk2c, 1, nullptr);
+      c->ks.drift(c->nblk_links, c->stream, c->g, c->F, c->D, c->D1, tau_g, rk2c, 1, nullptr);
     }
   } else {
     Prof pr(c, NSPT_PROF_DRIFT);
-    c->ks.drift(c->nblk_links, c->stream, c->g, c->U, c->D, none, c->Scr, c->Sg, tau_g, 0.0, 0, obs_part);
+    c->ks.drift(c->nblk_links, c->stream, c->g, c->U, c->D, none, tau_g, 0.0, 0, obs_part);
   }
   if (!launch_ok(c, "drift")) return false;
   if (obs_dev && !obs_reduce(c, obs_dev)) return false;
@@ -570,8 +570,7 @@ int nspt_create(const nspt_geom* geom, nspt_ctx** out) {
   const long P = 4L * V;
   bool ok = c->alloc_field(c->U, P, true) && c->alloc_field(c->F, P, true) && c->alloc_field(c->D, P, false) &&
             c->alloc_field(c->D1, P, false) && c->alloc_field(c->L, P, true) && c->alloc_field(c->X, P, false) &&
-            c->alloc_field(c->Scr, P, false) && c->alloc_field(c->Sg, P, false) &&
-            c->alloc_field(c->G, V, true) && c->alloc_field(c->Xs, V, false) &&
+            c->alloc_field(c->G, V, true) &&
             c->alloc(&c->fb, (size_t)c->nrt * c->NE * V) && c->alloc(&c->eta, (size_t)P * c->NE) &&
             c->alloc(&c->eta_tmp, (size_t)P * c->NE) &&
             c->alloc(&c->mm_part, (size_t)c->nblk_links * c->nrt * c->NE) &&
@@ -688,7 +687,7 @@ int nspt_evolve(nspt_ctx* c, nspt_step_params* p, int nsteps, double* obs) {
 int nspt_plaquette(nspt_ctx* c, double* out) {
   if (!c || !out) return -1;
   SerField none{nullptr, nullptr, 0};
-  c->ks.drift(c->nblk_links, c->stream, c->g, c->U, none, none, c->Scr, c->Sg, 0.0, 0.0, 2, c->obs_part);
+  c->ks.drift(c->nblk_links, c->stream, c->g, c->U, none, none, 0.0, 0.0, 2, c->obs_part);
   if (!launch_ok(c, "plaquette") || !obs_reduce(c, c->obs_dev)) return -3;
   if (!c->hip_ok(hipMemcpyAsync(out, c->obs_dev, sizeof(double) * 2 * (c->nrt + 1), hipMemcpyDeviceToHost,
                                 c->stream),
@@ -707,8 +706,7 @@ int nspt_fa_gauge_fixing(nspt_ctx* c, double alpha) {
 int nspt_drift(nspt_ctx* c, double eps, double* aos_out) {
   if (!c || !aos_out) return -1;
   SerField none{nullptr, nullptr, 0};
-  c->ks.drift(c->nblk_links, c->stream, c->g, c->U, c->D, none, c->Scr, c->Sg, -eps / (2.0 * c->nc), 0.0, 0,
-              nullptr);
+  c->ks.drift(c->nblk_links, c->stream, c->g, c->U, c->D, none, -eps / (2.0 * c->nc), 0.0, 0, nullptr);
   if (!launch_ok(c, "drift")) return -3;
   return download_field(c, c->D, aos_out);
 }

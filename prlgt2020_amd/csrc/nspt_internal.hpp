@@ -20,20 +20,24 @@ struct Geo {
 // Launchers of one (NC, N) instance.  nblk is the grid size in waves.
 struct KernelSet {
   int nc = 0, nmax = 0, lpb = 0;
-  void (*drift)(int nblk, hipStream_t, Geo, SerField U, SerField D, SerField D1, SerField Scr, SerField Sg,
-                double tau_g, double rk2c, int mode, double* obs) = nullptr;
+  void (*drift)(int nblk, hipStream_t, Geo, SerField U, SerField D, SerField D1, double tau_g, double rk2c, int mode,
+                double* obs) = nullptr;
   void (*update)(int nblk, hipStream_t, Geo, SerField D, const double2* eta, double stau, SerField Ubase,
                  SerField Uout, SerField L, double2* mm) = nullptr;
   void (*logf)(int nblk, hipStream_t, Geo, SerField U, SerField L) = nullptr;
   void (*zero_mode)(int nblk, hipStream_t, Geo, SerField L, const double2* MM, SerField U, SerField X) = nullptr;
-  void (*gexp)(int nblk, hipStream_t, Geo, const double2* fb, SerField Xs, SerField G) = nullptr;
+  void (*gexp)(int nblk, hipStream_t, Geo, const double2* fb, SerField G) = nullptr;
   void (*transform)(int nblk, hipStream_t, Geo, SerField G, SerField U) = nullptr;
 };
 
 // One function per compiled instance (nspt_inst.hip built once per (NC, N)).
+#ifdef NSPT_SINGLE_INSTANCE
+#define NSPT_INSTANCES(X) X(3, 20)
+#else
 #define NSPT_INSTANCES(X) \
   X(3, 4) X(3, 6) X(3, 8) X(3, 12) X(3, 16) X(3, 20) X(3, 24) \
   X(2, 4) X(2, 6) X(2, 8) X(2, 12) X(2, 16) X(2, 20) X(2, 24)
+#endif
 
 #define NSPT_DECL_SET(nc, n) KernelSet kernel_set_##nc##_##n();
 NSPT_INSTANCES(NSPT_DECL_SET)

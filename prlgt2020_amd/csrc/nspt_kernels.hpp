@@ -1,14 +1,20 @@
 // nspt_kernels.hpp -- per-link / per-site kernels of the NSPT step, templated
 // on the gauge group (NC = 3 or 2) and the compiled order count N (>= the
-// runtime order; orders above the runtime order read as zero and are never
+// runtime order; orders above the runtime order are staged as zero and never
 // stored, which leaves the lower orders unchanged because a truncated product
 // only feeds orders upward).
 //
 // Link index on the device: idx = mu*V + site (site lexicographic with x3
 // fastest, lattice.h:37).  One wave = one workgroup = LPB links of the same mu.
+// Every product stages both operands in LDS first (loads of A and B from HBM
+// issued together), then runs from LDS (see nspt_device.hpp).
 #pragma once
 #include "nspt_device.hpp"
 #include "nspt_internal.hpp"
+
+#ifndef NSPT_WAVES_PER_EU
+#define NSPT_WAVES_PER_EU 1
+#endif
 
 namespace nspt {
 
@@ -20,23 +26,27 @@ __device__ __forceinline__ int site_shift(const Geo& g, int s, int d, int delta)
   return s + (xn - x) * g.st[d];
 }
 
+// Lane-owned element of series idx of field f, all orders, into an LDS slot.
+// Loads are unmasked: lanes of invalid links carry a clamped, valid index.
 template <int NC, int N>
-__device__ __forceinline__ void stage_global(const Lane<NC, N>& ln, const SerField& f, long idx, int nrt, bool ok) {
+__device__ __forceinline__ void stage_global(const Lane<NC, N>& ln, double2* slot, const SerField& f, long idx,
+                                             int nrt) {
   constexpr int NE = NC * NC;
   if (!ln.lds_ok) return;
   const unsigned off = (unsigned)(ln.e * f.P + idx);
   const long plane = (long)NE * f.P;
+  const __amdgpu_buffer_rsrc_t r = rsrc(f.data);
 #pragma unroll
-  for (int k = 0; k < N; k++) ln.slot[k * NE + ln.e] = (ok && k < nrt) ? ld_u(f.data + k * plane, off) : zero2();
+  for (int k = 0; k < N; k++) slot[k * NE + ln.e] = (k < nrt) ? ld_r(r, off, (unsigned)(k * plane * 16)) : zero2();
 }
 
-// Sum over the valid link slots of the block of element (k, e) of the staged
-// series; lanes loop over the (k, e) pairs.  Deterministic (ascending slots).
+// Sum over the valid link slots of the block of element (k, e) of slot A.
+// Deterministic (ascending slots).
 template <int NC, int N>
-__device__ __forceinline__ double2 slot_sum(const double2* lds, int nvalid, int k, int e) {
+__device__ __forceinline__ double2 slot_sum(const double2* slotA0, int nvalid, int k, int e) {
   constexpr int NE = NC * NC;
   double2 s = zero2();
-  for (int jj = 0; jj < nvalid; jj++) s = c_add(s, lds[jj * N * NE + k * NE + e]);
+  for (int jj = 0; jj < nvalid; jj++) s = c_add(s, slotA0[jj * N * NE + k * NE + e]);
   return s;
 }
 
@@ -46,13 +56,13 @@ __device__ __forceinline__ double2 slot_sum(const double2* lds, int nvalid, int 
 // mode 0: store D.  mode 1: RK2 corrector, store (D1 + D)/2 * (1 + rk2c x^2).
 // mode 2: observables only.  obs_part (may be null): per block
 // [plaq (N+1) cplx][norm N][trU N cplx] of the block's mu, unnormalised.
-// Scr: per-link scratch series (intermediate right operands).
 template <int NC, int N>
-__global__ __launch_bounds__(64) void k_drift(Geo g, SerField U, SerField D, SerField D1, SerField Scr, SerField Sg,
-                                              double tau_g, double rk2c, int mode, double* obs_part) {
+__global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_drift(Geo g, SerField U, SerField D, SerField D1,
+                                                                 double tau_g, double rk2c, int mode,
+                                                                 double* obs_part) {
   constexpr int NE = NC * NC;
   using LN = Lane<NC, N>;
-  __shared__ double2 lds[LN::LPB * LN::SLOT];
+  __shared__ double2 lds[LN::LDS];
   __shared__ double2 flagbuf[LN::LPB];
   LN ln(lds);
   const int nrt = g.nrt;
@@ -65,57 +75,78 @@ __global__ __launch_bounds__(64) void k_drift(Geo g, SerField U, SerField D, Ser
   const long V = g.V;
   const long idx = mu * V + site;
 
-  double2 Out[N];
-  double2 sigf = zero2();
+  double2 Out[N], Sig[N];
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    Out[k] = zero2();
+    Sig[k] = zero2();
+  }
+  double2 sigf = zero2(), t1f = zero2(), t2f = zero2(), wf = zero2();
 
-  for (int p = 0; p < 3; p++) {
+  // 12 staple products (4 per plane nu != mu) and W = U_mu(x) Sigma, one
+  // product per iteration so that the product body is emitted once:
+  //   kind 0: T1   = U_nu(x)      U_mu(x+nu)
+  //   kind 1: up   = U_nu(x+mu)   T1^dag          -> Sigma
+  //   kind 2: T2   = U_mu(x-nu)   U_nu(x-nu+mu)
+  //   kind 3: down = T2^dag       U_nu(x-nu)      -> Sigma
+  //   i = 12: W    = U_mu(x)      Sigma
+  for (int i = 0; i < 13; i++) {
+    asm volatile("" ::: "memory");
+    const int p = i >> 2, kind = (i == 12) ? 4 : (i & 3);
     const int nu = (p < mu) ? p : p + 1;
     const int xpmu = site_shift(g, site, mu, 1);
     const int xpnu = site_shift(g, site, nu, 1);
     const int xmnu = site_shift(g, site, nu, -1);
     const int xmnu_pmu = site_shift(g, xmnu, mu, 1);
-    const long i_nu_x = nu * V + site, i_mu_xpnu = mu * V + xpnu, i_nu_xpmu = nu * V + xpmu;
-    const long i_mu_xmnu = mu * V + xmnu, i_nu_xmnupmu = nu * V + xmnu_pmu, i_nu_xmnu = nu * V + xmnu;
-
-    // T1 = U_nu(x) U_mu(x+nu)
-    stage_global(ln, U, i_nu_x, nrt, ok);
+    long ia = idx, ib = idx;
+    double2 a0 = zero2(), b0 = zero2();
+    if (kind == 0) {
+      ia = nu * V + site;
+      ib = mu * V + xpnu;
+    } else if (kind == 1) {
+      ia = nu * V + xpmu;
+      b0 = c_conj(t1f);
+    } else if (kind == 2) {
+      ia = mu * V + xmnu;
+      ib = nu * V + xmnu_pmu;
+    } else if (kind == 3) {
+      ib = nu * V + xmnu;
+      a0 = c_conj(t2f);
+    } else {
+      b0 = sigf;
+    }
+    // operand A: a link, or T2 from registers; operand B: a link, T1 or Sigma
+    if (kind == 3)
+      ln.stage(ln.slotA, Out);
+    else
+      stage_global(ln, ln.slotA, U, ia, nrt);
+    if (kind == 1)
+      ln.stage(ln.slotB, Out);
+    else if (kind == 4)
+      ln.stage(ln.slotB, Sig);
+    else
+      stage_global(ln, ln.slotB, U, ib, nrt);
+    if (kind != 3) a0 = gflag(U, ia, ok);
+    if (kind == 0 || kind == 2 || kind == 3) b0 = gflag(U, ib, ok);
     __syncthreads();
-    const double2 t1f = mul<NC, N>(ln, false, bop(U, i_mu_xpnu), gflag(U, i_nu_x, ok), gflag(U, i_mu_xpnu, ok), Out,
-                                   M, nrt, ok);
-    gstore<NC, N>(Scr, idx, ln.e, nrt, ok, Out);
+    const double2 f = mul<NC, N>(ln, kind == 3, kind == 1, a0, b0, true, Out, M);
+    if (kind == 0) {
+      t1f = f;
+    } else if (kind == 1 || kind == 3) {
+#pragma unroll
+      for (int k = 0; k < N; k++) Sig[k] = c_add(Sig[k], Out[k]);
+      sigf = c_add(sigf, f);
+    } else if (kind == 2) {
+      t2f = f;
+    } else {
+      wf = f;
+    }
     __syncthreads();
-    // up = U_nu(x+mu) T1^dag
-    stage_global(ln, U, i_nu_xpmu, nrt, ok);
-    __syncthreads();
-    const double2 uf = mul<NC, N>(ln, false, bop(Scr, idx, true), gflag(U, i_nu_xpmu, ok), c_conj(t1f), Out, M, nrt,
-                                  ok);
-    __syncthreads();
-    gaccum<NC, N>(Sg, idx, ln.e, nrt, ok, Out, p == 0);
-    sigf = c_add(sigf, uf);
-    // T2 = U_mu(x-nu) U_nu(x-nu+mu)
-    stage_global(ln, U, i_mu_xmnu, nrt, ok);
-    __syncthreads();
-    const double2 t2f = mul<NC, N>(ln, false, bop(U, i_nu_xmnupmu), gflag(U, i_mu_xmnu, ok),
-                                   gflag(U, i_nu_xmnupmu, ok), Out, M, nrt, ok);
-    __syncthreads();
-    // down = T2^dag U_nu(x-nu)
-    ln.stage(Out);
-    __syncthreads();
-    const double2 df = mul<NC, N>(ln, true, bop(U, i_nu_xmnu), c_conj(t2f), gflag(U, i_nu_xmnu, ok), Out, M, nrt, ok);
-    __syncthreads();
-    gaccum<NC, N>(Sg, idx, ln.e, nrt, ok, Out, false);
-    sigf = c_add(sigf, df);
   }
-
-  // W = U_mu(x) Sigma
-  stage_global(ln, U, idx, nrt, ok);
-  __syncthreads();
-  const double2 wf = mul<NC, N>(ln, false, bop(Sg, idx), gflag(U, idx, ok), sigf, Out, M, nrt, ok);
-  __syncthreads();
 
   double* op = obs_part ? obs_part + (long)blockIdx.x * (2 * (nrt + 1) + 3 * nrt) : nullptr;
   if (op) {
-    // norm_k = sum Re Tr(U_k U_k^dag), trU_k = sum Tr U_k over the staged U
+    // norm_k = sum Re Tr(U_k U_k^dag), trU_k = sum Tr U_k over U_mu(x) in slot A
     const int lane = threadIdx.x;
     if (lane < nrt) {
       double nr = 0.0;
@@ -134,7 +165,7 @@ __global__ __launch_bounds__(64) void k_drift(Geo g, SerField U, SerField D, Ser
     if (ln.lds_ok && ln.e == 0) flagbuf[ln.j] = wf;
   }
   __syncthreads();
-  ln.stage(Out);
+  ln.stage(ln.slotA, Out);
   __syncthreads();
   if (op) {
     const int lane = threadIdx.x;
@@ -174,14 +205,15 @@ __global__ __launch_bounds__(64) void k_drift(Geo g, SerField U, SerField D, Ser
 // optionally L = log(Uout) (QCDpt.h:176-198) and per-block sums of L for the
 // zero mode (nspt.cc:432-439).  eta: [idx*NE + e] (may be null).
 template <int NC, int N>
-__global__ __launch_bounds__(64) void k_update(Geo g, SerField D, const double2* eta, double stau, SerField Ubase,
-                                               SerField Uout, SerField L, double2* mm_part) {
+__global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_update(Geo g, SerField D, const double2* eta,
+                                                                  double stau, SerField Ubase, SerField Uout,
+                                                                  SerField L, double2* mm_part) {
   constexpr int NE = NC * NC;
   using LN = Lane<NC, N>;
-  __shared__ double2 lds[LN::LPB * LN::SLOT];
+  __shared__ double2 lds[LN::LDS];
   LN ln(lds);
   const int nrt = g.nrt;
-  const int M = min(N, nrt) - 1;
+  const int n = min(N, nrt);
   const int mu = blockIdx.x / g.nb_mu;
   const int site0 = (blockIdx.x % g.nb_mu) * LN::LPB;
   const int nvalid = min(LN::LPB, g.V - site0);
@@ -190,24 +222,30 @@ __global__ __launch_bounds__(64) void k_update(Geo g, SerField D, const double2*
   const long idx = (long)mu * g.V + site;
 
   double2 S[N];
-  BOp Db = bop(D, idx);
-  Db.eta = eta;
-  Db.stau = stau;
-  horner<NC, N, true>(ln, Db, S, nrt, ok);
-  // Unew = E * Ubase
-  ln.stage(S);
+  // generator W = D - stau*eta (order 1) into slot B
+  gload<NC, N>(D, idx, ln.e, nrt, ok, S);
+  if (eta) S[0] = c_sub(S[0], c_scale(stau, eta[idx * NE + ln.e]));
+  ln.stage(ln.slotB, S);
   __syncthreads();
-  const double2 nf = mul<NC, N>(ln, false, bop(Ubase, idx), one2(), gflag(Ubase, idx, ok), S, M, nrt, ok);
+  horner<NC, N, true>(ln, S, n);
+  // Unew = E * Ubase
+  ln.stage(ln.slotA, S);
+  stage_global(ln, ln.slotB, Ubase, idx, nrt);
+  const double2 ubf = gflag(Ubase, idx, ok);
+  __syncthreads();
+  const double2 nf = mul<NC, N>(ln, false, false, one2(), ubf, true, S, n - 1);
+  __syncthreads();
   gstore<NC, N>(Uout, idx, ln.e, nrt, ok, S);
   if (ok && ln.e == 0) Uout.flag[idx] = nf;
   if (!L.data) return;
-  __syncthreads();
   // L = log(Unew)
-  horner<NC, N, false>(ln, bop(Uout, idx), S, nrt, ok);
+  ln.stage(ln.slotB, S);
+  __syncthreads();
+  horner<NC, N, false>(ln, S, n);
   gstore<NC, N>(L, idx, ln.e, nrt, ok, S);
   if (ok && ln.e == 0 && L.flag) L.flag[idx] = zero2();
   if (!mm_part) return;
-  ln.stage(S);
+  ln.stage(ln.slotA, S);
   __syncthreads();
   for (int o = threadIdx.x; o < nrt * NE; o += 64)
     mm_part[(long)blockIdx.x * nrt * NE + o] = slot_sum<NC, N>(lds, nvalid, o / NE, o % NE);
@@ -216,9 +254,9 @@ __global__ __launch_bounds__(64) void k_update(Geo g, SerField D, const double2*
 // ------------------------------------------------------------ log field
 // L = log(U) for every link (FA gauge fixing without zero-mode removal).
 template <int NC, int N>
-__global__ __launch_bounds__(64) void k_log(Geo g, SerField U, SerField L) {
+__global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_log(Geo g, SerField U, SerField L) {
   using LN = Lane<NC, N>;
-  __shared__ double2 lds[LN::LPB * LN::SLOT];
+  __shared__ double2 lds[LN::LDS];
   LN ln(lds);
   const int nrt = g.nrt;
   const int mu = blockIdx.x / g.nb_mu;
@@ -227,7 +265,9 @@ __global__ __launch_bounds__(64) void k_log(Geo g, SerField U, SerField L) {
   const bool ok = ln.lds_ok && (site0 + ln.j) < g.V;
   const long idx = (long)mu * g.V + site;
   double2 S[N];
-  horner<NC, N, false>(ln, bop(U, idx), S, nrt, ok);
+  stage_global(ln, ln.slotB, U, idx, nrt);
+  __syncthreads();
+  horner<NC, N, false>(ln, S, min(N, nrt));
   gstore<NC, N>(L, idx, ln.e, nrt, ok, S);
   if (ok && ln.e == 0 && L.flag) L.flag[idx] = zero2();
 }
@@ -236,10 +276,11 @@ __global__ __launch_bounds__(64) void k_log(Geo g, SerField U, SerField L) {
 // X = reH(L - MM_mu), U = exp(X)  (nspt.cc:638-641).  MM: [(mu*nrt + k)*NE + e],
 // already multiplied by 1/V.  X is kept: it is log(U) for the FA step.
 template <int NC, int N>
-__global__ __launch_bounds__(64) void k_zero_mode(Geo g, SerField L, const double2* MM, SerField U, SerField X) {
+__global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_zero_mode(Geo g, SerField L, const double2* MM,
+                                                                     SerField U, SerField X) {
   constexpr int NE = NC * NC;
   using LN = Lane<NC, N>;
-  __shared__ double2 lds[LN::LPB * LN::SLOT];
+  __shared__ double2 lds[LN::LDS];
   LN ln(lds);
   const int nrt = g.nrt;
   const int mu = blockIdx.x / g.nb_mu;
@@ -252,13 +293,14 @@ __global__ __launch_bounds__(64) void k_zero_mode(Geo g, SerField L, const doubl
 #pragma unroll
   for (int k = 0; k < N; k++)
     if (k < nrt) P[k] = c_sub(P[k], MM[((long)mu * nrt + k) * NE + ln.e]);
-  ln.stage(P);
+  ln.stage(ln.slotA, P);
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < N; k++) P[k] = reh_elem(ln, k);
   gstore<NC, N>(X, idx, ln.e, nrt, ok, P);
+  ln.stage(ln.slotB, P);
   __syncthreads();
-  horner<NC, N, true>(ln, bop(X, idx), P, nrt, ok);
+  horner<NC, N, true>(ln, P, min(N, nrt));
   gstore<NC, N>(U, idx, ln.e, nrt, ok, P);
   if (ok && ln.e == 0) U.flag[idx] = one2();
 }
@@ -266,12 +308,11 @@ __global__ __launch_bounds__(64) void k_zero_mode(Geo g, SerField L, const doubl
 // ------------------------------------------------------------ FA: G = exp(reH(W))
 // W per site from the transformed buffer fb[(k*NE + e)*V + site] (already
 // multiplied by -alpha/(p^2 V)), G = exp(reH(W)) (nspt.cc:902-906).
-// Xs: per-site scratch series.
 template <int NC, int N>
-__global__ __launch_bounds__(64) void k_fa_gexp(Geo g, const double2* fb, SerField Xs, SerField G) {
+__global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_fa_gexp(Geo g, const double2* fb, SerField G) {
   constexpr int NE = NC * NC;
   using LN = Lane<NC, N>;
-  __shared__ double2 lds[LN::LPB * LN::SLOT];
+  __shared__ double2 lds[LN::LDS];
   LN ln(lds);
   const int nrt = g.nrt;
   const int site0 = blockIdx.x * LN::LPB;
@@ -279,14 +320,14 @@ __global__ __launch_bounds__(64) void k_fa_gexp(Geo g, const double2* fb, SerFie
   const bool ok = ln.lds_ok && (site0 + ln.j) < g.V;
   double2 P[N];
 #pragma unroll
-  for (int k = 0; k < N; k++) P[k] = (ok && k < nrt) ? fb[((long)k * NE + ln.e) * g.V + site] : zero2();
-  ln.stage(P);
+  for (int k = 0; k < N; k++) P[k] = (k < nrt) ? fb[((long)k * NE + ln.e) * g.V + site] : zero2();
+  ln.stage(ln.slotA, P);
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < N; k++) P[k] = reh_elem(ln, k);
-  gstore<NC, N>(Xs, site, ln.e, nrt, ok, P);
+  ln.stage(ln.slotB, P);
   __syncthreads();
-  horner<NC, N, true>(ln, bop(Xs, site), P, nrt, ok);
+  horner<NC, N, true>(ln, P, min(N, nrt));
   gstore<NC, N>(G, site, ln.e, nrt, ok, P);
   if (ok && ln.e == 0) G.flag[site] = one2();
 }
@@ -294,9 +335,9 @@ __global__ __launch_bounds__(64) void k_fa_gexp(Geo g, const double2* fb, SerFie
 // ------------------------------------------------------------ gauge transform
 // U_mu(x) <- G(x) U_mu(x) G(x+mu)^dag, in place (nspt.cc:912-916).
 template <int NC, int N>
-__global__ __launch_bounds__(64) void k_gauge_transform(Geo g, SerField G, SerField U) {
+__global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_gauge_transform(Geo g, SerField G, SerField U) {
   using LN = Lane<NC, N>;
-  __shared__ double2 lds[LN::LPB * LN::SLOT];
+  __shared__ double2 lds[LN::LDS];
   LN ln(lds);
   const int nrt = g.nrt;
   const int M = min(N, nrt) - 1;
@@ -307,15 +348,27 @@ __global__ __launch_bounds__(64) void k_gauge_transform(Geo g, SerField G, SerFi
   const long idx = (long)mu * g.V + site;
   const int xp = site_shift(g, site, mu, 1);
   double2 Out[N];
-  stage_global(ln, G, site, nrt, ok);
-  __syncthreads();
-  const double2 tf = mul<NC, N>(ln, false, bop(U, idx), gflag(G, site, ok), gflag(U, idx, ok), Out, M, nrt, ok);
-  __syncthreads();
-  ln.stage(Out);
-  __syncthreads();
-  const double2 nf = mul<NC, N>(ln, false, bop(G, xp, true), tf, c_conj(gflag(G, xp, ok)), Out, M, nrt, ok);
+#pragma unroll
+  for (int k = 0; k < N; k++) Out[k] = zero2();
+  double2 tf = zero2();
+  for (int i = 0; i < 2; i++) {
+    asm volatile("" ::: "memory");
+    if (i == 0)
+      stage_global(ln, ln.slotA, G, site, nrt);
+    else
+      ln.stage(ln.slotA, Out);
+    if (i == 0)
+      stage_global(ln, ln.slotB, U, idx, nrt);
+    else
+      stage_global(ln, ln.slotB, G, xp, nrt);
+    const double2 a0 = (i == 0) ? gflag(G, site, ok) : tf;
+    const double2 b0 = (i == 0) ? gflag(U, idx, ok) : c_conj(gflag(G, xp, ok));
+    __syncthreads();
+    tf = mul<NC, N>(ln, false, i == 1, a0, b0, true, Out, M);
+    __syncthreads();
+  }
   gstore<NC, N>(U, idx, ln.e, nrt, ok, Out);
-  if (ok && ln.e == 0) U.flag[idx] = nf;
+  if (ok && ln.e == 0) U.flag[idx] = tf;
 }
 
 // ------------------------------------------------------------ instance table
@@ -325,9 +378,9 @@ KernelSet make_kernel_set() {
   s.nc = NC;
   s.nmax = N;
   s.lpb = 64 / (NC * NC);
-  s.drift = [](int nblk, hipStream_t st, Geo g, SerField U, SerField D, SerField D1, SerField Scr, SerField Sg,
-               double tau_g, double rk2c, int mode, double* obs) {
-    hipLaunchKernelGGL((k_drift<NC, N>), dim3(nblk), dim3(64), 0, st, g, U, D, D1, Scr, Sg, tau_g, rk2c, mode, obs);
+  s.drift = [](int nblk, hipStream_t st, Geo g, SerField U, SerField D, SerField D1, double tau_g, double rk2c,
+               int mode, double* obs) {
+    hipLaunchKernelGGL((k_drift<NC, N>), dim3(nblk), dim3(64), 0, st, g, U, D, D1, tau_g, rk2c, mode, obs);
   };
   s.update = [](int nblk, hipStream_t st, Geo g, SerField D, const double2* eta, double stau, SerField Ub,
                 SerField Uo, SerField L, double2* mm) {
@@ -339,8 +392,8 @@ KernelSet make_kernel_set() {
   s.zero_mode = [](int nblk, hipStream_t st, Geo g, SerField L, const double2* MM, SerField U, SerField X) {
     hipLaunchKernelGGL((k_zero_mode<NC, N>), dim3(nblk), dim3(64), 0, st, g, L, MM, U, X);
   };
-  s.gexp = [](int nblk, hipStream_t st, Geo g, const double2* fb, SerField Xs, SerField G) {
-    hipLaunchKernelGGL((k_fa_gexp<NC, N>), dim3(nblk), dim3(64), 0, st, g, fb, Xs, G);
+  s.gexp = [](int nblk, hipStream_t st, Geo g, const double2* fb, SerField G) {
+    hipLaunchKernelGGL((k_fa_gexp<NC, N>), dim3(nblk), dim3(64), 0, st, g, fb, G);
   };
   s.transform = [](int nblk, hipStream_t st, Geo g, SerField G, SerField U) {
     hipLaunchKernelGGL((k_gauge_transform<NC, N>), dim3(nblk), dim3(64), 0, st, g, G, U);

@@ -13,7 +13,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libnspt.so")
+LIB_PATH = os.environ.get("NSPT_LIB") or os.path.join(HERE, "libnspt.so")
 
 EULER, RK2 = 0, 1
 GF_NONE, GF_FA = 0, 1
