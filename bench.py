@@ -88,6 +88,9 @@ def main():
     ap.add_argument("--N", type=int, default=20)
     ap.add_argument("--integrator", choices=["rk2", "euler"], default="rk2")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--copy-calib-mib", type=int, default=0,
+                    help="before the step, copy a buffer of this many MiB device to device (a plain copy kernel "
+                         "of known size that scales the FETCH_SIZE / WRITE_SIZE counters of a --pmc run)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; the multi-GPU layout is described in DESIGN.md "
@@ -95,6 +98,14 @@ def main():
         sys.exit(2)
 
     from prlgt2020_amd import nspt
+    if args.copy_calib_mib:
+        import torch
+        n = args.copy_calib_mib << 20
+        src = torch.ones(n // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     L = (args.L,) * 4
     N, nc = args.N, 3
     V = args.L ** 4

@@ -9,13 +9,13 @@
 //
 // Truncated product C = A*B (the Cauchy product of QCDpt.h:92-114): both
 // operands are staged in LDS (each lane writes its own element of A into
-// slot A and of B into slot B, all orders, loads from HBM issued together).
-// Lane (r, c) then walks the orders of B in windows of QB: it reads column c
-// of B (or of B^dag) for the window into registers and streams rows r of A
-// (or A^dag), accumulating C_{t+q} += A_t[r][:] . B_q[:][c].  Every order
-// index is a compile-time constant; the truncation bound is a wave-uniform
-// runtime value (one uniform branch per row), so one compiled instance
-// serves every runtime order up to N.
+// slot A and of B into slot B, all orders, loads from HBM issued together;
+// daggers are applied while staging).  Lane (r, c) then reads row r of A
+// and column c of B tile by tile (QT orders of each) and accumulates
+// C_{a+b+1} += A_a[r][:] . B_b[:][c] along anti-diagonal bands of tiles,
+// holding only the 2QT-1 outputs a band can reach.  Every order index is a
+// compile-time constant; the truncation bound is a wave-uniform runtime
+// value, so one compiled instance serves every runtime order up to N.
 //
 // exp and log (QCDpt.h:176-220) are evaluated in Horner form with the same
 // product; the Horner polynomial commutes with its generator, so
@@ -57,24 +57,23 @@ __device__ __forceinline__ double2 one2() { return make_double2(1.0, 0.0); }
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// Order window of the right operand: QB orders of column c are read from LDS
-// into registers (plus the next window, prefetched) for each pass over A.
-#ifndef NSPT_QB
-#define NSPT_QB 3
+// Tile edge of the product: pairs of operand orders are taken QT x QT at a time.
+#ifndef NSPT_QT
+#define NSPT_QT 3
 #endif
-template <int N>
-struct QWin {
-  static constexpr int value = N <= 6 ? N : NSPT_QB;
-};
 
 // Lane bookkeeping for one wave = one workgroup.  LDS holds two slots per
-// link (A and B operand), each N orders x NE elements.
+// link (A and B operand), each N orders x NE elements, plus QT-2 orders of
+// padding after the last slot: a tile reads up to QT-2 orders past the end
+// of a slot (into the next slot or the padding), always for products above
+// the truncation bound, whose results are never used.
 template <int NC, int N>
 struct Lane {
   static constexpr int NE = NC * NC;
   static constexpr int LPB = 64 / NE;  // link slots per wave
   static constexpr int SLOT = N * NE;  // LDS complex per slot
-  static constexpr int LDS = 2 * LPB * SLOT;
+  static constexpr int PAD = (NSPT_QT > 2 ? NSPT_QT - 2 : 0) * NE;
+  static constexpr int LDS = 2 * LPB * SLOT + PAD;
   int j, e, r, c;
   bool lds_ok;  // owns an LDS cell (false only for SU(3) lane 63)
   double2* slotA;
@@ -91,32 +90,17 @@ struct Lane {
     slotB = lds + LPB * SLOT + j * SLOT;
   }
 
-  // LDS staging of a lane-owned series (all orders)
-  __device__ __forceinline__ void stage(double2* slot, const double2 (&X)[N]) const {
+  // LDS staging of a lane-owned series (all orders), or of its dagger: the
+  // lane that owns (r, c) writes conj(X) at (c, r).
+  __device__ __forceinline__ void stage(double2* slot, const double2 (&X)[N], bool dag = false) const {
     if (lds_ok) {
+      const int pos = dag ? (c * NC + r) : e;
+      const double sy = dag ? -1.0 : 1.0;
 #pragma unroll
-      for (int k = 0; k < N; k++) slot[k * NE + e] = X[k];
+      for (int k = 0; k < N; k++) slot[k * NE + pos] = make_double2(X[k].x, sy * X[k].y);
     }
   }
-
-  // Element (r,c) of the series staged in slot A or of its dagger.
-  __device__ __forceinline__ double2 elemA(bool dag, int k) const {
-    if (!dag) return slotA[k * NE + e];
-    return c_conj(slotA[k * NE + c * NC + r]);
-  }
 };
-
-template <int NC>
-__device__ __forceinline__ double2 pick(const double2 (&v)[NC], int i) {
-  double x = v[0].x, y = v[0].y;
-#pragma unroll
-  for (int m = 1; m < NC; m++) {
-    const double vx = v[m].x, vy = v[m].y;
-    x = (i == m) ? vx : x;
-    y = (i == m) ? vy : y;
-  }
-  return make_double2(x, y);
-}
 
 // Load of element `off` (32-bit, per lane) of an array whose base is
 // wave-uniform: a raw buffer load with a scalar resource descriptor, so the
@@ -143,101 +127,106 @@ __device__ __forceinline__ void st_u(double2* base, unsigned off, double2 v) {
   *reinterpret_cast<double2*>(reinterpret_cast<char*>(base) + (size_t)(off * 16u)) = v;
 }
 
-// C = A*B with A in slot A (A^dag if dagA, flag a0) and B in slot B (B^dag
-// if dagB, flag b0).  Adds the scalar-flag terms a0*B_k + b0*A_k
-// (QCDpt.h:104-112) and returns a0*b0.  Output orders up to M+1 (0-based
-// index <= M, wave-uniform) are exact; entries above are left undefined and
-// are never read by a caller (truncated products only feed upward).
-// A^dag is handled as conj(sum A[m][r] conj(B[m][c])) so that the streamed
-// rows are used as loaded.  has_b0 = false skips the b0 term (b0 = 0).
+// C = A*B for A staged in slot A (flag a0) and B in slot B (flag b0): the
+// truncated Cauchy product of QCDpt.h:92-114 with the flag terms
+// a0*B_k + b0*A_k; returns the flag a0*b0.  Output order indices up to M
+// (wave-uniform) are exact; entries above are undefined and never read by a
+// caller (truncated products only feed upward).  has_b0 = false skips the
+// b0 term (b0 = 0).  Daggers are taken when staging (Lane::stage).
+//
+// Order index a of A times index b of B lands on index a+b+1.  The pairs
+// (a, b) are visited in QT x QT tiles along bands a0 + b0 = s (s = 0, QT,
+// 2QT, ...): a band reaches outputs s+1 .. s+2QT-1 only, so 2QT-1
+// accumulators suffice, and after band s the outputs s+1 .. s+QT are final.
+// Bands are unrolled (output indices are constants); the tiles of a band
+// form a runtime loop whose operands are double-buffered in registers.
+// Lane (r, c) reads row r of A and column c of B from LDS.
 template <int NC, int N>
-__device__ __forceinline__ double2 mul(const Lane<NC, N>& ln, bool dagA, bool dagB, double2 a0, double2 b0,
-                                       bool has_b0, double2 (&C)[N], int M) {
+__device__ __forceinline__ double2 mul(const Lane<NC, N>& ln, double2 a0, double2 b0, bool has_b0,
+                                       double2 (&C)[N], int M) {
   constexpr int NE = NC * NC;
-  constexpr int QB = QWin<N>::value;
-  constexpr int NW = (N + QB - 1) / QB;
+  constexpr int Q = NSPT_QT;
+  constexpr int NO = 2 * Q - 1;
   M = uniform(M);
-  int oa[NC], ob[NC];
+  const double2* pa = ln.slotA + ln.r * NC;
+  const double2* pb = ln.slotB + ln.c;
+  const double2* own_a = ln.slotA + ln.e;
+  const double2* own_b = ln.slotB + ln.e;
+  auto flags = [&](int k) {
+    double2 v = c_mul(a0, own_b[k * NE]);
+    if (has_b0) v = c_add(v, c_mul(b0, own_a[k * NE]));
+    return v;
+  };
+  double2 acc[NO];
 #pragma unroll
-  for (int m = 0; m < NC; m++) {
-    oa[m] = dagA ? (m * NC + ln.r) : (ln.r * NC + m);
-    ob[m] = dagB ? (ln.c * NC + m) : (m * NC + ln.c);
-  }
-  const double sb = (dagB != dagA) ? -1.0 : 1.0;
-  const double2 a0e = dagA ? c_conj(a0) : a0;
+  for (int o = 0; o < NO; o++) acc[o] = zero2();
+  C[0] = flags(0);
 
-  auto loadwin = [&](int q0, double2 (&W)[QB][NC]) {
+  double2 TA[2][Q][NC], TB[2][Q][NC];
+  auto load = [&](int buf, int a, int b) {
+    const double2* ra = pa + a * NE;
+    const double2* cb = pb + b * NE;
 #pragma unroll
-    for (int qq = 0; qq < QB; qq++) {
-      const int q = q0 + qq;  // order q+1
+    for (int d = 0; d < Q; d++)
 #pragma unroll
       for (int m = 0; m < NC; m++) {
-        double2 v = zero2();
-        if (q < N) {
-          v = ln.slotB[q * NE + ob[m]];
-          v.y *= sb;
-        }
-        W[qq][m] = v;
+        TA[buf][d][m] = ra[d * NE + m];
+        TB[buf][d][m] = cb[d * NE + m * NC];
+      }
+  };
+  // Rows da <= rem of a tile (all of them when rem >= Q - 1): a row whose
+  // pairs all land above the bound is skipped; the pairs of a kept row that
+  // land above it only feed outputs that are never read.  The empty asm
+  // keeps each guarded row a real (uniform) branch instead of a select.
+  auto row = [&](int buf, int da) {
+#pragma unroll
+    for (int m = 0; m < NC; m++)
+#pragma unroll
+      for (int db = 0; db < Q; db++) c_mac(acc[da + db], TA[buf][da][m], TB[buf][db][m]);
+  };
+  auto tile = [&](int buf, int rem) {
+    row(buf, 0);
+#pragma unroll
+    for (int da = 1; da < Q; da++) {
+      if (rem >= Q - 1) {
+        row(buf, da);
+      } else if (da <= rem) {
+        asm volatile("");
+        row(buf, da);
       }
     }
   };
-  auto rowload = [&](int t, double2 (&a)[NC]) {
-    const double2* row = ln.slotA + (t - 1) * NE;
-#pragma unroll
-    for (int m = 0; m < NC; m++) a[m] = row[oa[m]];
+  auto band = [&](int s, int rem) {
+    load(0, 0, s);
+    for (int a = 0;;) {
+      const bool more = a + Q <= s;
+      if (more) load(1, a + Q, s - a - Q);
+      tile(0, rem);
+      if (!more) break;
+      a += Q;
+      const bool more2 = a + Q <= s;
+      if (more2) load(0, a + Q, s - a - Q);
+      tile(1, rem);
+      if (!more2) break;
+      a += Q;
+    }
   };
 
+  // (a guard, not a break: the band loop must unroll fully)
 #pragma unroll
-  for (int k = 0; k < N; k++) C[k] = zero2();
-  // ping-pong buffers (static indices): window w uses Bw[w & 1], row t uses A[t & 1]
-  double2 Bw[2][QB][NC];
-  loadwin(0, Bw[0]);
-
+  for (int s = 0; s + 1 < N; s += Q) {
+    if (s <= M - 1) {
+      const int rem = M - 1 - s;
+      if (rem >= Q - 1)
+        band(s, Q - 1);
+      else
+        band(s, rem);
 #pragma unroll
-  for (int w = 0; w < NW; w++) {
-    const int q0 = w * QB;
-    const int cur = w & 1;
-    if (q0 > M) break;
-    // Rows are re-read from LDS in every window: without this compiler
-    // barrier the identical loads of earlier windows are reused and every
-    // row stays live in registers across the whole product.
-    asm volatile("" ::: "memory");
-    if (w + 1 < NW) {
-      if (q0 + QB <= M) loadwin(q0 + QB, Bw[cur ^ 1]);
+      for (int d = 0; d < Q; d++)
+        if (s + 1 + d < N) C[s + 1 + d] = c_add(acc[d], flags(s + 1 + d));
+#pragma unroll
+      for (int o = 0; o < NO; o++) acc[o] = (o + Q < NO) ? acc[o + Q] : zero2();
     }
-    // a0 * B_k
-#pragma unroll
-    for (int qq = 0; qq < QB; qq++)
-      if (q0 + qq < N) C[q0 + qq] = c_add(C[q0 + qq], c_mul(a0e, pick<NC>(Bw[cur][qq], ln.r)));
-    // Cauchy terms: A order t (LDS index t-1) times B order q0+qq+1 -> index t+q0+qq
-    const int tmax = N - 1 - q0;
-    double2 A[2][NC];
-    if (tmax >= 1) rowload(1, A[1]);
-#pragma unroll
-    for (int t = 1; t < N; t++) {
-      if (t <= tmax) {
-        if (t + q0 > M) break;
-        if (t + 1 <= tmax) rowload(t + 1, A[(t + 1) & 1]);
-        // Every window order is formed for a row that reaches the bound:
-        // products past order M+1 are left undefined and never read, and a
-        // row without inner branches keeps 2*QB independent FMA chains.
-#pragma unroll
-        for (int qq = 0; qq < QB; qq++)
-          if (t + q0 + qq < N) {
-#pragma unroll
-            for (int m = 0; m < NC; m++) c_mac(C[t + q0 + qq], A[t & 1][m], Bw[cur][qq][m]);
-          }
-        __builtin_amdgcn_sched_barrier(0);
-      }
-    }
-  }
-  if (dagA) {
-#pragma unroll
-    for (int k = 0; k < N; k++) C[k].y = -C[k].y;
-  }
-  if (has_b0) {
-#pragma unroll
-    for (int k = 0; k < N; k++) C[k] = c_add(C[k], c_mul(b0, ln.elemA(dagA, k)));
   }
   return c_mul(a0, b0);
 }
@@ -256,7 +245,7 @@ __device__ __forceinline__ void horner(const Lane<NC, N>& ln, double2 (&S)[N], i
   for (int s = 0; s < n; s++) {
     ln.stage(ln.slotA, S);
     __syncthreads();
-    mul<NC, N>(ln, false, false, flag, zero2(), false, S, s);
+    mul<NC, N>(ln, flag, zero2(), false, S, s);
     __syncthreads();
     if (IS_EXP) {
       const double a = 1.0 / (double)(n - s);

@@ -26,18 +26,59 @@ __device__ __forceinline__ int site_shift(const Geo& g, int s, int d, int delta)
   return s + (xn - x) * g.st[d];
 }
 
+// Block order.  The dispatcher hands consecutive workgroups to the 8 XCDs in
+// turn; block_order() renumbers them so that each XCD runs one contiguous
+// range of logical blocks (neighbour links then meet in that XCD's L2), and
+// link kernels take the direction as the fastest logical index so that the
+// four links of a site block, which share most staple links, run together.
+#ifndef NSPT_BLOCK_ORDER
+#define NSPT_BLOCK_ORDER 2
+#endif
+__device__ __forceinline__ int block_order() {
+#if NSPT_BLOCK_ORDER >= 2
+  const int n = gridDim.x, p = blockIdx.x;
+  const int q = n >> 3, r = n & 7, x = p & 7;
+  return x * q + min(x, r) + (p >> 3);
+#else
+  return blockIdx.x;
+#endif
+}
+
+// Link block of this workgroup: direction mu, first site site0, and the
+// index of its per-block partial sums (mu-major, as the reductions expect).
+struct LinkBlock {
+  int mu, site0, part;
+};
+template <int LPB>
+__device__ __forceinline__ LinkBlock link_block(const Geo& g) {
+  const int b = block_order();
+#if NSPT_BLOCK_ORDER >= 1
+  const int mu = b & 3, sb = b >> 2;
+#else
+  const int mu = b / g.nb_mu, sb = b % g.nb_mu;
+#endif
+  return {mu, sb * LPB, mu * g.nb_mu + sb};
+}
+
 // Lane-owned element of series idx of field f, all orders, into an LDS slot.
 // Loads are unmasked: lanes of invalid links carry a clamped, valid index.
+// With dag, the dagger of the series is staged (element (r, c) loaded,
+// conj written at (c, r)).
 template <int NC, int N>
 __device__ __forceinline__ void stage_global(const Lane<NC, N>& ln, double2* slot, const SerField& f, long idx,
-                                             int nrt) {
+                                             int nrt, bool dag = false) {
   constexpr int NE = NC * NC;
   if (!ln.lds_ok) return;
   const unsigned off = (unsigned)(ln.e * f.P + idx);
   const long plane = (long)NE * f.P;
   const __amdgpu_buffer_rsrc_t r = rsrc(f.data);
+  const int pos = dag ? (ln.c * NC + ln.r) : ln.e;
+  const double sy = dag ? -1.0 : 1.0;
 #pragma unroll
-  for (int k = 0; k < N; k++) slot[k * NE + ln.e] = (k < nrt) ? ld_r(r, off, (unsigned)(k * plane * 16)) : zero2();
+  for (int k = 0; k < N; k++) {
+    const double2 v = (k < nrt) ? ld_r(r, off, (unsigned)(k * plane * 16)) : zero2();
+    slot[k * NE + pos] = make_double2(v.x, sy * v.y);
+  }
 }
 
 // Sum over the valid link slots of the block of element (k, e) of slot A.
@@ -67,8 +108,9 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_drift(Geo g, SerField
   LN ln(lds);
   const int nrt = g.nrt;
   const int M = min(N, nrt) - 1;
-  const int mu = blockIdx.x / g.nb_mu;
-  const int site0 = (blockIdx.x % g.nb_mu) * LN::LPB;
+  const LinkBlock lb = link_block<LN::LPB>(g);
+  const int mu = lb.mu;
+  const int site0 = lb.site0;
   const int nvalid = min(LN::LPB, g.V - site0);
   const int site = min(site0 + ln.j, g.V - 1);
   const bool ok = ln.lds_ok && (site0 + ln.j) < g.V;
@@ -87,6 +129,7 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_drift(Geo g, SerField
   // product per iteration so that the product body is emitted once:
   //   kind 0: T1   = U_nu(x)      U_mu(x+nu)
   //   kind 1: up   = U_nu(x+mu)   T1^dag          -> Sigma
+  // (T1^dag and T2^dag are staged as daggers from registers)
   //   kind 2: T2   = U_mu(x-nu)   U_nu(x-nu+mu)
   //   kind 3: down = T2^dag       U_nu(x-nu)      -> Sigma
   //   i = 12: W    = U_mu(x)      Sigma
@@ -117,11 +160,11 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_drift(Geo g, SerField
     }
     // operand A: a link, or T2 from registers; operand B: a link, T1 or Sigma
     if (kind == 3)
-      ln.stage(ln.slotA, Out);
+      ln.stage(ln.slotA, Out, true);
     else
       stage_global(ln, ln.slotA, U, ia, nrt);
     if (kind == 1)
-      ln.stage(ln.slotB, Out);
+      ln.stage(ln.slotB, Out, true);
     else if (kind == 4)
       ln.stage(ln.slotB, Sig);
     else
@@ -129,7 +172,7 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_drift(Geo g, SerField
     if (kind != 3) a0 = gflag(U, ia, ok);
     if (kind == 0 || kind == 2 || kind == 3) b0 = gflag(U, ib, ok);
     __syncthreads();
-    const double2 f = mul<NC, N>(ln, kind == 3, kind == 1, a0, b0, true, Out, M);
+    const double2 f = mul<NC, N>(ln, a0, b0, true, Out, M);
     if (kind == 0) {
       t1f = f;
     } else if (kind == 1 || kind == 3) {
@@ -144,7 +187,7 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_drift(Geo g, SerField
     __syncthreads();
   }
 
-  double* op = obs_part ? obs_part + (long)blockIdx.x * (2 * (nrt + 1) + 3 * nrt) : nullptr;
+  double* op = obs_part ? obs_part + (long)lb.part * (2 * (nrt + 1) + 3 * nrt) : nullptr;
   if (op) {
     // norm_k = sum Re Tr(U_k U_k^dag), trU_k = sum Tr U_k over U_mu(x) in slot A
     const int lane = threadIdx.x;
@@ -214,8 +257,9 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_update(Geo g, SerFiel
   LN ln(lds);
   const int nrt = g.nrt;
   const int n = min(N, nrt);
-  const int mu = blockIdx.x / g.nb_mu;
-  const int site0 = (blockIdx.x % g.nb_mu) * LN::LPB;
+  const LinkBlock lb = link_block<LN::LPB>(g);
+  const int mu = lb.mu;
+  const int site0 = lb.site0;
   const int nvalid = min(LN::LPB, g.V - site0);
   const int site = min(site0 + ln.j, g.V - 1);
   const bool ok = ln.lds_ok && (site0 + ln.j) < g.V;
@@ -233,7 +277,7 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_update(Geo g, SerFiel
   stage_global(ln, ln.slotB, Ubase, idx, nrt);
   const double2 ubf = gflag(Ubase, idx, ok);
   __syncthreads();
-  const double2 nf = mul<NC, N>(ln, false, false, one2(), ubf, true, S, n - 1);
+  const double2 nf = mul<NC, N>(ln, one2(), ubf, true, S, n - 1);
   __syncthreads();
   gstore<NC, N>(Uout, idx, ln.e, nrt, ok, S);
   if (ok && ln.e == 0) Uout.flag[idx] = nf;
@@ -248,7 +292,7 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_update(Geo g, SerFiel
   ln.stage(ln.slotA, S);
   __syncthreads();
   for (int o = threadIdx.x; o < nrt * NE; o += 64)
-    mm_part[(long)blockIdx.x * nrt * NE + o] = slot_sum<NC, N>(lds, nvalid, o / NE, o % NE);
+    mm_part[(long)lb.part * nrt * NE + o] = slot_sum<NC, N>(lds, nvalid, o / NE, o % NE);
 }
 
 // ------------------------------------------------------------ log field
@@ -259,8 +303,9 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_log(Geo g, SerField U
   __shared__ double2 lds[LN::LDS];
   LN ln(lds);
   const int nrt = g.nrt;
-  const int mu = blockIdx.x / g.nb_mu;
-  const int site0 = (blockIdx.x % g.nb_mu) * LN::LPB;
+  const LinkBlock lb = link_block<LN::LPB>(g);
+  const int mu = lb.mu;
+  const int site0 = lb.site0;
   const int site = min(site0 + ln.j, g.V - 1);
   const bool ok = ln.lds_ok && (site0 + ln.j) < g.V;
   const long idx = (long)mu * g.V + site;
@@ -283,8 +328,9 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_zero_mode(Geo g, SerF
   __shared__ double2 lds[LN::LDS];
   LN ln(lds);
   const int nrt = g.nrt;
-  const int mu = blockIdx.x / g.nb_mu;
-  const int site0 = (blockIdx.x % g.nb_mu) * LN::LPB;
+  const LinkBlock lb = link_block<LN::LPB>(g);
+  const int mu = lb.mu;
+  const int site0 = lb.site0;
   const int site = min(site0 + ln.j, g.V - 1);
   const bool ok = ln.lds_ok && (site0 + ln.j) < g.V;
   const long idx = (long)mu * g.V + site;
@@ -315,7 +361,7 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_fa_gexp(Geo g, const 
   __shared__ double2 lds[LN::LDS];
   LN ln(lds);
   const int nrt = g.nrt;
-  const int site0 = blockIdx.x * LN::LPB;
+  const int site0 = block_order() * LN::LPB;
   const int site = min(site0 + ln.j, g.V - 1);
   const bool ok = ln.lds_ok && (site0 + ln.j) < g.V;
   double2 P[N];
@@ -341,8 +387,9 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_gauge_transform(Geo g
   LN ln(lds);
   const int nrt = g.nrt;
   const int M = min(N, nrt) - 1;
-  const int mu = blockIdx.x / g.nb_mu;
-  const int site0 = (blockIdx.x % g.nb_mu) * LN::LPB;
+  const LinkBlock lb = link_block<LN::LPB>(g);
+  const int mu = lb.mu;
+  const int site0 = lb.site0;
   const int site = min(site0 + ln.j, g.V - 1);
   const bool ok = ln.lds_ok && (site0 + ln.j) < g.V;
   const long idx = (long)mu * g.V + site;
@@ -360,11 +407,11 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_gauge_transform(Geo g
     if (i == 0)
       stage_global(ln, ln.slotB, U, idx, nrt);
     else
-      stage_global(ln, ln.slotB, G, xp, nrt);
+      stage_global(ln, ln.slotB, G, xp, nrt, true);
     const double2 a0 = (i == 0) ? gflag(G, site, ok) : tf;
     const double2 b0 = (i == 0) ? gflag(U, idx, ok) : c_conj(gflag(G, xp, ok));
     __syncthreads();
-    tf = mul<NC, N>(ln, false, i == 1, a0, b0, true, Out, M);
+    tf = mul<NC, N>(ln, a0, b0, true, Out, M);
     __syncthreads();
   }
   gstore<NC, N>(U, idx, ln.e, nrt, ok, Out);
