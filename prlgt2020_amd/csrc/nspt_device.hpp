@@ -57,9 +57,14 @@ __device__ __forceinline__ double2 one2() { return make_double2(1.0, 0.0); }
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// Tile edge of the product: pairs of operand orders are taken QT x QT at a time.
+// Tile edge of the product: pairs of operand orders are taken QT x QT at a
+// time.  NSPT_TILE_DB = 1 double-buffers the tile operands in registers
+// (the next tile is read from LDS while the current one is multiplied).
 #ifndef NSPT_QT
-#define NSPT_QT 3
+#define NSPT_QT 4
+#endif
+#ifndef NSPT_TILE_DB
+#define NSPT_TILE_DB 0
 #endif
 
 // Lane bookkeeping for one wave = one workgroup.  LDS holds two slots per
@@ -132,7 +137,8 @@ __device__ __forceinline__ void st_u(double2* base, unsigned off, double2 v) {
 // a0*B_k + b0*A_k; returns the flag a0*b0.  Output order indices up to M
 // (wave-uniform) are exact; entries above are undefined and never read by a
 // caller (truncated products only feed upward).  has_b0 = false skips the
-// b0 term (b0 = 0).  Daggers are taken when staging (Lane::stage).
+// b0 term (b0 = 0).  Daggers are taken when staging (Lane::stage).  With
+// ACC the product is added to C (orders up to M) instead of replacing it.
 //
 // Order index a of A times index b of B lands on index a+b+1.  The pairs
 // (a, b) are visited in QT x QT tiles along bands a0 + b0 = s (s = 0, QT,
@@ -141,7 +147,7 @@ __device__ __forceinline__ void st_u(double2* base, unsigned off, double2 v) {
 // Bands are unrolled (output indices are constants); the tiles of a band
 // form a runtime loop whose operands are double-buffered in registers.
 // Lane (r, c) reads row r of A and column c of B from LDS.
-template <int NC, int N>
+template <int NC, int N, bool ACC = false>
 __device__ __forceinline__ double2 mul(const Lane<NC, N>& ln, double2 a0, double2 b0, bool has_b0,
                                        double2 (&C)[N], int M) {
   constexpr int NE = NC * NC;
@@ -160,7 +166,7 @@ __device__ __forceinline__ double2 mul(const Lane<NC, N>& ln, double2 a0, double
   double2 acc[NO];
 #pragma unroll
   for (int o = 0; o < NO; o++) acc[o] = zero2();
-  C[0] = flags(0);
+  C[0] = ACC ? c_add(C[0], flags(0)) : flags(0);
 
   double2 TA[2][Q][NC], TB[2][Q][NC];
   auto load = [&](int buf, int a, int b) {
@@ -197,6 +203,13 @@ __device__ __forceinline__ double2 mul(const Lane<NC, N>& ln, double2 a0, double
     }
   };
   auto band = [&](int s, int rem) {
+#if NSPT_TILE_DB == 0
+    for (int a = 0; a <= s; a += Q) {
+      load(0, a, s - a);
+      tile(0, rem);
+    }
+    return;
+#endif
     load(0, 0, s);
     for (int a = 0;;) {
       const bool more = a + Q <= s;
@@ -223,7 +236,10 @@ __device__ __forceinline__ double2 mul(const Lane<NC, N>& ln, double2 a0, double
         band(s, rem);
 #pragma unroll
       for (int d = 0; d < Q; d++)
-        if (s + 1 + d < N) C[s + 1 + d] = c_add(acc[d], flags(s + 1 + d));
+        if (s + 1 + d < N) {
+          const double2 v = c_add(acc[d], flags(s + 1 + d));
+          C[s + 1 + d] = ACC ? c_add(C[s + 1 + d], v) : v;
+        }
 #pragma unroll
       for (int o = 0; o < NO; o++) acc[o] = (o + Q < NO) ? acc[o + Q] : zero2();
     }

@@ -129,49 +129,65 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_drift(Geo g, SerField
   // product per iteration so that the product body is emitted once:
   //   kind 0: T1   = U_nu(x)      U_mu(x+nu)
   //   kind 1: up   = U_nu(x+mu)   T1^dag          -> Sigma
-  // (T1^dag and T2^dag are staged as daggers from registers)
   //   kind 2: T2   = U_mu(x-nu)   U_nu(x-nu+mu)
   //   kind 3: down = T2^dag       U_nu(x-nu)      -> Sigma
   //   i = 12: W    = U_mu(x)      Sigma
-  for (int i = 0; i < 13; i++) {
-    asm volatile("" ::: "memory");
-    const int p = i >> 2, kind = (i == 12) ? 4 : (i & 3);
+  // (T1^dag and T2^dag are staged as daggers from registers).  The links of
+  // product i+1 are loaded into PA/PB while product i runs from LDS.
+  const int xpmu = site_shift(g, site, mu, 1);
+  auto kind_of = [](int i) { return (i == 12) ? 4 : (i & 3); };
+  auto links_of = [&](int i, long& ia, long& ib) {
+    const int p = i >> 2, kind = kind_of(i);
     const int nu = (p < mu) ? p : p + 1;
-    const int xpmu = site_shift(g, site, mu, 1);
     const int xpnu = site_shift(g, site, nu, 1);
     const int xmnu = site_shift(g, site, nu, -1);
-    const int xmnu_pmu = site_shift(g, xmnu, mu, 1);
-    long ia = idx, ib = idx;
-    double2 a0 = zero2(), b0 = zero2();
+    ia = idx;
+    ib = idx;
     if (kind == 0) {
       ia = nu * V + site;
       ib = mu * V + xpnu;
     } else if (kind == 1) {
       ia = nu * V + xpmu;
-      b0 = c_conj(t1f);
     } else if (kind == 2) {
       ia = mu * V + xmnu;
-      ib = nu * V + xmnu_pmu;
+      ib = nu * V + site_shift(g, xmnu, mu, 1);
     } else if (kind == 3) {
       ib = nu * V + xmnu;
-      a0 = c_conj(t2f);
-    } else {
-      b0 = sigf;
     }
-    // operand A: a link, or T2 from registers; operand B: a link, T1 or Sigma
+  };
+  double2 PA[N], PB[N], paf = zero2(), pbf = zero2();
+  auto fetch = [&](int i) {
+    long ia, ib;
+    links_of(i, ia, ib);
+    const int kind = kind_of(i);
+    if (kind != 3) {
+      gload<NC, N>(U, ia, ln.e, nrt, ok, PA);
+      paf = gflag(U, ia, ok);
+    }
+    if (kind == 0 || kind == 2 || kind == 3) {
+      gload<NC, N>(U, ib, ln.e, nrt, ok, PB);
+      pbf = gflag(U, ib, ok);
+    }
+  };
+  fetch(0);
+  for (int i = 0; i < 13; i++) {
+    asm volatile("" ::: "memory");
+    const int kind = kind_of(i);
+    // operand A: a link, or T2^dag; operand B: a link, T1^dag or Sigma
     if (kind == 3)
       ln.stage(ln.slotA, Out, true);
     else
-      stage_global(ln, ln.slotA, U, ia, nrt);
+      ln.stage(ln.slotA, PA);
     if (kind == 1)
       ln.stage(ln.slotB, Out, true);
     else if (kind == 4)
       ln.stage(ln.slotB, Sig);
     else
-      stage_global(ln, ln.slotB, U, ib, nrt);
-    if (kind != 3) a0 = gflag(U, ia, ok);
-    if (kind == 0 || kind == 2 || kind == 3) b0 = gflag(U, ib, ok);
+      ln.stage(ln.slotB, PB);
+    const double2 a0 = (kind == 3) ? c_conj(t2f) : paf;
+    const double2 b0 = (kind == 1) ? c_conj(t1f) : (kind == 4) ? sigf : pbf;
     __syncthreads();
+    if (i + 1 < 13) fetch(i + 1);
     const double2 f = mul<NC, N>(ln, a0, b0, true, Out, M);
     if (kind == 0) {
       t1f = f;
