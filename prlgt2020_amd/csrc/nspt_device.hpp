@@ -13,8 +13,10 @@
 // daggers are applied while staging).  Lane (r, c) then reads row r of A
 // and column c of B tile by tile (QT orders of each) and accumulates
 // C_{a+b+1} += A_a[r][:] . B_b[:][c] along anti-diagonal bands of tiles,
-// holding only the 2QT-1 outputs a band can reach.  Every order index is a
-// compile-time constant; the truncation bound is a wave-uniform runtime
+// highest band first, holding only the 2QT-1 outputs a band can reach.
+// Finished orders go to registers or straight back into an operand slot
+// (in place), so chained products need no restaging.  Every order index is
+// a compile-time constant; the truncation bound is a wave-uniform runtime
 // value, so one compiled instance serves every runtime order up to N.
 //
 // exp and log (QCDpt.h:176-220) are evaluated in Horner form with the same
@@ -23,7 +25,7 @@
 //   log:  S <- c_i + S*X,  c_i = (-1)^{i+1}/i, then log = S*X
 // each step truncated at the order it can still reach.  The flop count is
 // the reference's (N-1)N(N+1)/6 matrix products per exp/log.  The generator
-// X stays in slot B for all steps.
+// X stays in slot B and S in slot A for all steps.
 //
 // HBM layout of a series field with P series (structure of arrays):
 //   data[(k*NE + e)*P + idx]  double2, k = 0..N-1 is order k+1
@@ -58,13 +60,9 @@ __device__ __forceinline__ double2 one2() { return make_double2(1.0, 0.0); }
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 // Tile edge of the product: pairs of operand orders are taken QT x QT at a
-// time.  NSPT_TILE_DB = 1 double-buffers the tile operands in registers
-// (the next tile is read from LDS while the current one is multiplied).
+// time.
 #ifndef NSPT_QT
 #define NSPT_QT 4
-#endif
-#ifndef NSPT_TILE_DB
-#define NSPT_TILE_DB 0
 #endif
 
 // Lane bookkeeping for one wave = one workgroup.  LDS holds two slots per
@@ -132,24 +130,33 @@ __device__ __forceinline__ void st_u(double2* base, unsigned off, double2 v) {
   *reinterpret_cast<double2*>(reinterpret_cast<char*>(base) + (size_t)(off * 16u)) = v;
 }
 
-// C = A*B for A staged in slot A (flag a0) and B in slot B (flag b0): the
-// truncated Cauchy product of QCDpt.h:92-114 with the flag terms
-// a0*B_k + b0*A_k; returns the flag a0*b0.  Output order indices up to M
-// (wave-uniform) are exact; entries above are undefined and never read by a
-// caller (truncated products only feed upward).  has_b0 = false skips the
-// b0 term (b0 = 0).  Daggers are taken when staging (Lane::stage).  With
-// ACC the product is added to C (orders up to M) instead of replacing it.
+// Truncated product A*B for A staged in slot A (flag a0) and B in slot B
+// (flag b0): the Cauchy product of QCDpt.h:92-114 with the flag terms
+// a0*B_k + b0*A_k.  Returns the flag a0*b0.  has_b0 = false skips the b0
+// term (b0 = 0).  Daggers are taken when staging (Lane::stage).
+//
+// Destination (wave-uniform choice):
+//   to_lds: order k is written, times `scale` and daggered if `dag`, into
+//           LDS slot `dst` -- which may be slot A or slot B itself: the
+//           product is formed in place (see below);
+//   else:   C[k] += order k (registers).
+// Output order indices up to M (wave-uniform) are exact; entries above are
+// undefined and never read by a caller (truncated products only feed
+// upward).
 //
 // Order index a of A times index b of B lands on index a+b+1.  The pairs
-// (a, b) are visited in QT x QT tiles along bands a0 + b0 = s (s = 0, QT,
-// 2QT, ...): a band reaches outputs s+1 .. s+2QT-1 only, so 2QT-1
-// accumulators suffice, and after band s the outputs s+1 .. s+QT are final.
-// Bands are unrolled (output indices are constants); the tiles of a band
-// form a runtime loop whose operands are double-buffered in registers.
-// Lane (r, c) reads row r of A and column c of B from LDS.
-template <int NC, int N, bool ACC = false>
+// (a, b) are visited in QT x QT tiles along bands a0 + b0 = s, from the
+// highest band down to s = 0.  Band s reaches outputs s+1 .. s+2QT-1 and
+// reads operand orders below s+QT only, so 2QT-1 accumulators suffice;
+// after band s the outputs s+QT .. s+2QT-1 are final and every operand
+// order >= s is dead, which is what lets a finished order overwrite the
+// operand slots.  Bands are unrolled (output indices are constants); the
+// tiles of a band form a runtime loop.  Lane (r, c) reads row r of A and
+// column c of B from LDS.
+template <int NC, int N>
 __device__ __forceinline__ double2 mul(const Lane<NC, N>& ln, double2 a0, double2 b0, bool has_b0,
-                                       double2 (&C)[N], int M) {
+                                       double2 (&C)[N], int M, bool to_lds, double2* dst, bool dag,
+                                       double scale) {
   constexpr int NE = NC * NC;
   constexpr int Q = NSPT_QT;
   constexpr int NO = 2 * Q - 1;
@@ -158,76 +165,68 @@ __device__ __forceinline__ double2 mul(const Lane<NC, N>& ln, double2 a0, double
   const double2* pb = ln.slotB + ln.c;
   const double2* own_a = ln.slotA + ln.e;
   const double2* own_b = ln.slotB + ln.e;
-  auto flags = [&](int k) {
-    double2 v = c_mul(a0, own_b[k * NE]);
+  const int pos = dag ? (ln.c * NC + ln.r) : ln.e;
+  const double sx = scale, sy = dag ? -scale : scale;
+  // Order k is final with value v (flag terms still to add).  The flag
+  // terms read order k of both operands before it is overwritten.
+  auto retire = [&](int k, double2 v) {
+    v = c_add(v, c_mul(a0, own_b[k * NE]));
     if (has_b0) v = c_add(v, c_mul(b0, own_a[k * NE]));
-    return v;
+    if (to_lds) {
+      if (ln.lds_ok) dst[k * NE + pos] = make_double2(sx * v.x, sy * v.y);
+    } else {
+      C[k] = c_add(C[k], v);
+    }
   };
   double2 acc[NO];
 #pragma unroll
   for (int o = 0; o < NO; o++) acc[o] = zero2();
-  C[0] = ACC ? c_add(C[0], flags(0)) : flags(0);
 
-  double2 TA[2][Q][NC], TB[2][Q][NC];
-  auto load = [&](int buf, int a, int b) {
+  double2 TA[Q][NC], TB[Q][NC];
+  auto load = [&](int a, int b) {
     const double2* ra = pa + a * NE;
     const double2* cb = pb + b * NE;
 #pragma unroll
     for (int d = 0; d < Q; d++)
 #pragma unroll
       for (int m = 0; m < NC; m++) {
-        TA[buf][d][m] = ra[d * NE + m];
-        TB[buf][d][m] = cb[d * NE + m * NC];
+        TA[d][m] = ra[d * NE + m];
+        TB[d][m] = cb[d * NE + m * NC];
       }
   };
-  // Rows da <= rem of a tile (all of them when rem >= Q - 1): a row whose
-  // pairs all land above the bound is skipped; the pairs of a kept row that
-  // land above it only feed outputs that are never read.  The empty asm
-  // keeps each guarded row a real (uniform) branch instead of a select.
-  auto row = [&](int buf, int da) {
+  auto row = [&](int da) {
 #pragma unroll
     for (int m = 0; m < NC; m++)
 #pragma unroll
-      for (int db = 0; db < Q; db++) c_mac(acc[da + db], TA[buf][da][m], TB[buf][db][m]);
+      for (int db = 0; db < Q; db++) c_mac(acc[da + db], TA[da][m], TB[db][m]);
   };
-  auto tile = [&](int buf, int rem) {
-    row(buf, 0);
+  // Rows da <= rem of a tile (all of them when rem >= QT - 1): a row whose
+  // pairs all land above the bound is skipped; the pairs of a kept row that
+  // land above it only feed outputs that are never read.  The empty asm
+  // keeps each guarded row a real (uniform) branch instead of a select.
+  auto tile = [&](int rem) {
+    row(0);
 #pragma unroll
     for (int da = 1; da < Q; da++) {
       if (rem >= Q - 1) {
-        row(buf, da);
+        row(da);
       } else if (da <= rem) {
         asm volatile("");
-        row(buf, da);
+        row(da);
       }
     }
   };
   auto band = [&](int s, int rem) {
-#if NSPT_TILE_DB == 0
     for (int a = 0; a <= s; a += Q) {
-      load(0, a, s - a);
-      tile(0, rem);
-    }
-    return;
-#endif
-    load(0, 0, s);
-    for (int a = 0;;) {
-      const bool more = a + Q <= s;
-      if (more) load(1, a + Q, s - a - Q);
-      tile(0, rem);
-      if (!more) break;
-      a += Q;
-      const bool more2 = a + Q <= s;
-      if (more2) load(0, a + Q, s - a - Q);
-      tile(1, rem);
-      if (!more2) break;
-      a += Q;
+      load(a, s - a);
+      tile(rem);
     }
   };
 
+  constexpr int STOP = N >= 2 ? ((N - 2) / Q) * Q : 0;
   // (a guard, not a break: the band loop must unroll fully)
 #pragma unroll
-  for (int s = 0; s + 1 < N; s += Q) {
+  for (int s = STOP; s >= 0; s -= Q) {
     if (s <= M - 1) {
       const int rem = M - 1 - s;
       if (rem >= Q - 1)
@@ -236,49 +235,60 @@ __device__ __forceinline__ double2 mul(const Lane<NC, N>& ln, double2 a0, double
         band(s, rem);
 #pragma unroll
       for (int d = 0; d < Q; d++)
-        if (s + 1 + d < N) {
-          const double2 v = c_add(acc[d], flags(s + 1 + d));
-          C[s + 1 + d] = ACC ? c_add(C[s + 1 + d], v) : v;
-        }
+        if (s + Q + d < N) retire(s + Q + d, acc[Q - 1 + d]);
+      // carry outputs s+1 .. s+QT-1 to the next band down
 #pragma unroll
-      for (int o = 0; o < NO; o++) acc[o] = (o + Q < NO) ? acc[o + Q] : zero2();
+      for (int o = NO - 1; o >= 0; o--) acc[o] = (o >= Q) ? acc[o - Q] : zero2();
     }
   }
+  // outputs 1 .. QT-1 (carried past band 0, or flag terms only) and 0
+#pragma unroll
+  for (int d = 1; d < Q; d++)
+    if (d < N) retire(d, acc[Q - 1 + d]);
+  retire(0, zero2());
   return c_mul(a0, b0);
+}
+
+// Product into LDS (in place allowed) / added to registers.
+template <int NC, int N>
+__device__ __forceinline__ double2 mul_lds(const Lane<NC, N>& ln, double2 a0, double2 b0, bool has_b0, int M,
+                                           double2* dst, bool dag, double scale = 1.0) {
+  double2 unused[N];
+  return mul<NC, N>(ln, a0, b0, has_b0, unused, M, true, dst, dag, scale);
 }
 
 // Horner evaluation of exp(X) (IS_EXP) or log(1 + X) for the generator X
 // staged in slot B (its flag ignored, as exp and log of QCDpt.h:176-220
-// ignore it).  Result orders in S; the result flag is 1 (exp) or 0 (log).
-// n = number of orders formed (<= N).  The caller has synchronised after
-// staging X; returns synchronised.
+// ignore it).  The result orders are left in slot A; the result flag is 1
+// (exp) or 0 (log).  n = number of orders formed (<= N).  Each step forms
+// S*X in place in slot A.
 template <int NC, int N, bool IS_EXP>
-__device__ __forceinline__ void horner(const Lane<NC, N>& ln, double2 (&S)[N], int n) {
+__device__ __forceinline__ void horner(const Lane<NC, N>& ln, int n) {
+  constexpr int NE = NC * NC;
   n = uniform(n);
+  if (ln.lds_ok) {
 #pragma unroll
-  for (int k = 0; k < N; k++) S[k] = zero2();
+    for (int k = 0; k < N; k++) ln.slotA[k * NE + ln.e] = zero2();
+  }
   double2 flag = IS_EXP ? one2() : make_double2((n % 2 == 1 ? 1.0 : -1.0) / (double)n, 0.0);
   for (int s = 0; s < n; s++) {
-    ln.stage(ln.slotA, S);
     __syncthreads();
-    mul<NC, N>(ln, flag, zero2(), false, S, s);
-    __syncthreads();
-    if (IS_EXP) {
-      const double a = 1.0 / (double)(n - s);
-#pragma unroll
-      for (int k = 0; k < N; k++) S[k] = c_scale(a, S[k]);
-    } else {
+    const double a = IS_EXP ? 1.0 / (double)(n - s) : 1.0;
+    mul_lds<NC, N>(ln, flag, zero2(), false, s, ln.slotA, false, a);
+    if (!IS_EXP) {
       const int i = n - 1 - s;  // next coefficient c_i
       flag = (i >= 1) ? make_double2((i % 2 == 1 ? 1.0 : -1.0) / (double)i, 0.0) : zero2();
     }
   }
+  __syncthreads();
 }
 
-// reH of the series staged in slot A (QCDpt.cc:197-209): (M - M^dag)/2 minus Tr/NC.
+// reH of order k of the series in an LDS slot (QCDpt.cc:197-209):
+// (M - M^dag)/2 minus Tr/NC, element (r, c) of this lane.
 template <int NC, int N>
-__device__ __forceinline__ double2 reh_elem(const Lane<NC, N>& ln, int k) {
+__device__ __forceinline__ double2 reh_elem(const Lane<NC, N>& ln, const double2* slot, int k) {
   constexpr int NE = NC * NC;
-  const double2* m = ln.slotA + k * NE;
+  const double2* m = slot + k * NE;
   double2 d[NC];
 #pragma unroll
   for (int q = 0; q < NC; q++) d[q] = c_scale(0.5, c_sub(m[q * NC + q], c_conj(m[q * NC + q])));
@@ -289,6 +299,14 @@ __device__ __forceinline__ double2 reh_elem(const Lane<NC, N>& ln, int k) {
   double2 v = c_scale(0.5, c_sub(m[ln.e], c_conj(m[ln.c * NC + ln.r])));
   if (ln.r == ln.c) v = c_sub(v, tr);
   return v;
+}
+
+// This lane's element of every order of a series in an LDS slot.
+template <int NC, int N>
+__device__ __forceinline__ void unstage(const Lane<NC, N>& ln, const double2* slot, double2 (&X)[N]) {
+  constexpr int NE = NC * NC;
+#pragma unroll
+  for (int k = 0; k < N; k++) X[k] = slot[k * NE + ln.e];
 }
 
 // Global loads/stores of a lane-owned series.  Orders >= nrt read as zero and

@@ -117,23 +117,21 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_drift(Geo g, SerField
   const long V = g.V;
   const long idx = mu * V + site;
 
-  double2 Out[N], Sig[N];
+  double2 Sig[N];
 #pragma unroll
-  for (int k = 0; k < N; k++) {
-    Out[k] = zero2();
-    Sig[k] = zero2();
-  }
+  for (int k = 0; k < N; k++) Sig[k] = zero2();
   double2 sigf = zero2(), t1f = zero2(), t2f = zero2(), wf = zero2();
 
   // 12 staple products (4 per plane nu != mu) and W = U_mu(x) Sigma, one
   // product per iteration so that the product body is emitted once:
-  //   kind 0: T1   = U_nu(x)      U_mu(x+nu)
-  //   kind 1: up   = U_nu(x+mu)   T1^dag          -> Sigma
-  //   kind 2: T2   = U_mu(x-nu)   U_nu(x-nu+mu)
-  //   kind 3: down = T2^dag       U_nu(x-nu)      -> Sigma
-  //   i = 12: W    = U_mu(x)      Sigma
-  // (T1^dag and T2^dag are staged as daggers from registers).  The links of
-  // product i+1 are loaded into PA/PB while product i runs from LDS.
+  //   kind 0: T1   = U_nu(x)      U_mu(x+nu)   -> T1^dag into slot B
+  //   kind 1: up   = U_nu(x+mu)   T1^dag       -> Sigma +=
+  //   kind 2: T2   = U_mu(x-nu)   U_nu(x-nu+mu) -> T2^dag into slot A
+  //   kind 3: down = T2^dag       U_nu(x-nu)   -> Sigma +=
+  //   i = 12: W    = U_mu(x)      Sigma        -> slot B
+  // T1^dag and T2^dag are formed in place in the slot that the next product
+  // reads.  The links of product i+1 are loaded into PA/PB while product i
+  // runs from LDS.
   const int xpmu = site_shift(g, site, mu, 1);
   auto kind_of = [](int i) { return (i == 12) ? 4 : (i & 3); };
   auto links_of = [&](int i, long& ia, long& ib) {
@@ -173,40 +171,32 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_drift(Geo g, SerField
   for (int i = 0; i < 13; i++) {
     asm volatile("" ::: "memory");
     const int kind = kind_of(i);
-    // operand A: a link, or T2^dag; operand B: a link, T1^dag or Sigma
-    if (kind == 3)
-      ln.stage(ln.slotA, Out, true);
-    else
-      ln.stage(ln.slotA, PA);
-    if (kind == 1)
-      ln.stage(ln.slotB, Out, true);
-    else if (kind == 4)
-      ln.stage(ln.slotB, Sig);
-    else
-      ln.stage(ln.slotB, PB);
+    if (kind != 3) ln.stage(ln.slotA, PA);
+    if (kind == 0 || kind == 2 || kind == 3) ln.stage(ln.slotB, PB);
+    if (kind == 4) ln.stage(ln.slotB, Sig);
     const double2 a0 = (kind == 3) ? c_conj(t2f) : paf;
     const double2 b0 = (kind == 1) ? c_conj(t1f) : (kind == 4) ? sigf : pbf;
     __syncthreads();
     if (i + 1 < 13) fetch(i + 1);
-    const double2 f = mul<NC, N>(ln, a0, b0, true, Out, M);
-    if (kind == 0) {
+    const bool to_lds = (kind == 0 || kind == 2 || kind == 4);
+    double2* dst = (kind == 2) ? ln.slotA : ln.slotB;
+    const double2 f = mul<NC, N>(ln, a0, b0, true, Sig, M, to_lds, dst, kind != 4, 1.0);
+    if (kind == 0)
       t1f = f;
-    } else if (kind == 1 || kind == 3) {
-#pragma unroll
-      for (int k = 0; k < N; k++) Sig[k] = c_add(Sig[k], Out[k]);
-      sigf = c_add(sigf, f);
-    } else if (kind == 2) {
+    else if (kind == 2)
       t2f = f;
-    } else {
+    else if (kind == 4)
       wf = f;
-    }
+    else
+      sigf = c_add(sigf, f);
     __syncthreads();
   }
+  // slot A: U_mu(x); slot B: W
 
   double* op = obs_part ? obs_part + (long)lb.part * (2 * (nrt + 1) + 3 * nrt) : nullptr;
   if (op) {
-    // norm_k = sum Re Tr(U_k U_k^dag), trU_k = sum Tr U_k over U_mu(x) in slot A
     const int lane = threadIdx.x;
+    // norm_k = sum Re Tr(U_k U_k^dag), trU_k = sum Tr U_k over U_mu(x)
     if (lane < nrt) {
       double nr = 0.0;
       double2 tr = zero2();
@@ -222,19 +212,15 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_drift(Geo g, SerField
       op[3 * nrt + 2 + 2 * lane + 1] = tr.y;
     }
     if (ln.lds_ok && ln.e == 0) flagbuf[ln.j] = wf;
-  }
-  __syncthreads();
-  ln.stage(ln.slotA, Out);
-  __syncthreads();
-  if (op) {
-    const int lane = threadIdx.x;
+    __syncthreads();
+    // plaquette sums Tr W_k (k = 0 from the flags)
     if (lane <= nrt) {
       double2 s = zero2();
       for (int jj = 0; jj < nvalid; jj++) {
         if (lane == 0) {
           s = c_add(s, c_scale((double)NC, flagbuf[jj]));
         } else {
-          const double2* m = lds + jj * LN::SLOT + (lane - 1) * NE;
+          const double2* m = lds + LN::LPB * LN::SLOT + jj * LN::SLOT + (lane - 1) * NE;
 #pragma unroll
           for (int q = 0; q < NC; q++) s = c_add(s, m[q * NC + q]);
         }
@@ -246,8 +232,9 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_drift(Geo g, SerField
   if (mode == 2) return;
 
   // D = tau_g reH(W)
+  double2 Out[N];
 #pragma unroll
-  for (int k = 0; k < N; k++) Out[k] = c_scale(tau_g, reh_elem(ln, k));
+  for (int k = 0; k < N; k++) Out[k] = c_scale(tau_g, reh_elem(ln, ln.slotB, k));
   if (mode == 1) {
     double2 D1v[N];
     gload<NC, N>(D1, idx, ln.e, nrt, ok, D1v);
@@ -286,27 +273,25 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_update(Geo g, SerFiel
   gload<NC, N>(D, idx, ln.e, nrt, ok, S);
   if (eta) S[0] = c_sub(S[0], c_scale(stau, eta[idx * NE + ln.e]));
   ln.stage(ln.slotB, S);
-  __syncthreads();
-  horner<NC, N, true>(ln, S, n);
-  // Unew = E * Ubase
-  ln.stage(ln.slotA, S);
-  stage_global(ln, ln.slotB, Ubase, idx, nrt);
+  // Ubase is loaded while the exponential is formed
+  gload<NC, N>(Ubase, idx, ln.e, nrt, ok, S);
   const double2 ubf = gflag(Ubase, idx, ok);
+  horner<NC, N, true>(ln, n);
+  // Unew = E * Ubase, formed in place in slot B
+  ln.stage(ln.slotB, S);
   __syncthreads();
-  const double2 nf = mul<NC, N>(ln, one2(), ubf, true, S, n - 1);
+  const double2 nf = mul_lds<NC, N>(ln, one2(), ubf, true, n - 1, ln.slotB, false);
   __syncthreads();
+  unstage(ln, ln.slotB, S);
   gstore<NC, N>(Uout, idx, ln.e, nrt, ok, S);
   if (ok && ln.e == 0) Uout.flag[idx] = nf;
   if (!L.data) return;
-  // L = log(Unew)
-  ln.stage(ln.slotB, S);
-  __syncthreads();
-  horner<NC, N, false>(ln, S, n);
+  // L = log(Unew): generator Unew - 1 is already in slot B
+  horner<NC, N, false>(ln, n);
+  unstage(ln, ln.slotA, S);
   gstore<NC, N>(L, idx, ln.e, nrt, ok, S);
   if (ok && ln.e == 0 && L.flag) L.flag[idx] = zero2();
   if (!mm_part) return;
-  ln.stage(ln.slotA, S);
-  __syncthreads();
   for (int o = threadIdx.x; o < nrt * NE; o += 64)
     mm_part[(long)lb.part * nrt * NE + o] = slot_sum<NC, N>(lds, nvalid, o / NE, o % NE);
 }
@@ -327,8 +312,8 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_log(Geo g, SerField U
   const long idx = (long)mu * g.V + site;
   double2 S[N];
   stage_global(ln, ln.slotB, U, idx, nrt);
-  __syncthreads();
-  horner<NC, N, false>(ln, S, min(N, nrt));
+  horner<NC, N, false>(ln, min(N, nrt));
+  unstage(ln, ln.slotA, S);
   gstore<NC, N>(L, idx, ln.e, nrt, ok, S);
   if (ok && ln.e == 0 && L.flag) L.flag[idx] = zero2();
 }
@@ -358,11 +343,11 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_zero_mode(Geo g, SerF
   ln.stage(ln.slotA, P);
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < N; k++) P[k] = reh_elem(ln, k);
+  for (int k = 0; k < N; k++) P[k] = reh_elem(ln, ln.slotA, k);
   gstore<NC, N>(X, idx, ln.e, nrt, ok, P);
   ln.stage(ln.slotB, P);
-  __syncthreads();
-  horner<NC, N, true>(ln, P, min(N, nrt));
+  horner<NC, N, true>(ln, min(N, nrt));
+  unstage(ln, ln.slotA, P);
   gstore<NC, N>(U, idx, ln.e, nrt, ok, P);
   if (ok && ln.e == 0) U.flag[idx] = one2();
 }
@@ -386,10 +371,10 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_fa_gexp(Geo g, const 
   ln.stage(ln.slotA, P);
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < N; k++) P[k] = reh_elem(ln, k);
+  for (int k = 0; k < N; k++) P[k] = reh_elem(ln, ln.slotA, k);
   ln.stage(ln.slotB, P);
-  __syncthreads();
-  horner<NC, N, true>(ln, P, min(N, nrt));
+  horner<NC, N, true>(ln, min(N, nrt));
+  unstage(ln, ln.slotA, P);
   gstore<NC, N>(G, site, ln.e, nrt, ok, P);
   if (ok && ln.e == 0) G.flag[site] = one2();
 }
@@ -410,28 +395,23 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_gauge_transform(Geo g
   const bool ok = ln.lds_ok && (site0 + ln.j) < g.V;
   const long idx = (long)mu * g.V + site;
   const int xp = site_shift(g, site, mu, 1);
-  double2 Out[N];
-#pragma unroll
-  for (int k = 0; k < N; k++) Out[k] = zero2();
-  double2 tf = zero2();
-  for (int i = 0; i < 2; i++) {
-    asm volatile("" ::: "memory");
-    if (i == 0)
-      stage_global(ln, ln.slotA, G, site, nrt);
-    else
-      ln.stage(ln.slotA, Out);
-    if (i == 0)
-      stage_global(ln, ln.slotB, U, idx, nrt);
-    else
-      stage_global(ln, ln.slotB, G, xp, nrt, true);
-    const double2 a0 = (i == 0) ? gflag(G, site, ok) : tf;
-    const double2 b0 = (i == 0) ? gflag(U, idx, ok) : c_conj(gflag(G, xp, ok));
-    __syncthreads();
-    tf = mul<NC, N>(ln, a0, b0, true, Out, M);
-    __syncthreads();
-  }
-  gstore<NC, N>(U, idx, ln.e, nrt, ok, Out);
-  if (ok && ln.e == 0) U.flag[idx] = tf;
+  double2 P[N];
+  stage_global(ln, ln.slotA, G, site, nrt);
+  stage_global(ln, ln.slotB, U, idx, nrt);
+  const double2 gf = gflag(G, site, ok), uf = gflag(U, idx, ok);
+  // G(x+mu) is loaded while T = G(x) U_mu(x) is formed in place in slot A
+  gload<NC, N>(G, xp, ln.e, nrt, ok, P);
+  const double2 gpf = c_conj(gflag(G, xp, ok));
+  __syncthreads();
+  const double2 tf = mul_lds<NC, N>(ln, gf, uf, true, M, ln.slotA, false);
+  __syncthreads();
+  ln.stage(ln.slotB, P, true);
+  __syncthreads();
+  const double2 nf = mul_lds<NC, N>(ln, tf, gpf, true, M, ln.slotB, false);
+  __syncthreads();
+  unstage(ln, ln.slotB, P);
+  gstore<NC, N>(U, idx, ln.e, nrt, ok, P);
+  if (ok && ln.e == 0) U.flag[idx] = nf;
 }
 
 // ------------------------------------------------------------ instance table
