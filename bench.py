@@ -131,9 +131,14 @@ def main():
     fl = kernel_flops(name, N, V, nc)
     achieved = fl / avg_s / 1e12 if fl else 0.0
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_16x4_N20_rk2.json")
-    if os.path.exists(tpath):
-        traffic = json.load(open(tpath)).get(name)
+    tpath = os.path.join(ROOT, "profiles", "traffic_su3_16x4_N20_rk2.md")
+    if os.path.exists(tpath) and (args.L, N, args.integrator) == (16, 20, "rk2"):
+        # HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE counter runs
+        # recorded there (calibrated against a copy of known size)
+        for line in open(tpath):
+            f = line.split()
+            if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == name:
+                traffic = float(f[2])
     step_tf = step_flops(N, V, integ, nc) * args.steps / dt / 1e12
 
     out = {
