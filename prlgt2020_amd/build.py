@@ -27,6 +27,11 @@ HIPFLAGS = ["-O3", "--offload-arch=" + ARCH, "-fPIC", "-std=c++17",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 HEADERS = ["nspt_device.hpp", "nspt_kernels.hpp", "nspt_internal.hpp"]
+HOST = os.path.join(HERE, "host")
+BIN = os.path.join(HERE, "bin")
+# Quenched drivers built by default: (NC, allocORD); any other pair with
+# python -m prlgt2020_amd.build --driver NC ALLOCORD
+DRIVERS = [(3, 6), (3, 8), (3, 20), (2, 6)]
 
 
 def instances():
@@ -73,15 +78,39 @@ def build(force=False, jobs=None, verbose=False):
             if verbose and r.stderr:
                 print(r.stderr, file=sys.stderr)
     objs = [os.path.join(OBJ, "inst_%d_%d.o" % inst) for inst in instances()] + [capi_obj, rng_obj]
+    relinked = False
     if force or tasks or _newer(LIB, objs):
         _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", LIB + ".tmp"] + objs)
         os.replace(LIB + ".tmp", LIB)
+        relinked = True
+    for nc, ao in DRIVERS:
+        build_driver(nc, ao, force=force or relinked)
     return LIB
+
+
+def driver_path(nc, alloc_ord):
+    return os.path.join(BIN, "quenched_su%d_a%d" % (nc, alloc_ord))
+
+
+def build_driver(nc, alloc_ord, force=False):
+    """The Quenched driver (host/quenched.cc + host/nspt_host.cc) for one
+    gauge group and allocORD, linked against libnspt.so."""
+    os.makedirs(BIN, exist_ok=True)
+    out = driver_path(nc, alloc_ord)
+    srcs = [os.path.join(HOST, "quenched.cc"), os.path.join(HOST, "nspt_host.cc")]
+    deps = srcs + [os.path.join(HOST, "nspt_host.h"), os.path.join(HERE, "..", "include", "nspt.h")]
+    if force or _newer(out, deps):
+        _run(["g++", "-O2", "-std=c++17", "-Wall", "-DNC=%d" % nc, "-DallocORD=%d" % alloc_ord] + srcs +
+             ["-L" + HERE, "-lnspt", "-Wl,-rpath,$ORIGIN/..", "-o", out])
+    return out
 
 
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--force", action="store_true")
     ap.add_argument("-j", type=int, default=None)
+    ap.add_argument("--driver", nargs=2, type=int, metavar=("NC", "ALLOCORD"), help="build one more driver")
     a = ap.parse_args()
     print(build(force=a.force, jobs=a.j, verbose=True))
+    if a.driver:
+        print(build_driver(a.driver[0], a.driver[1], force=True))

@@ -1,0 +1,190 @@
+// nspt_host.cc -- the host drop-in's types and entry points over libnspt.so.
+// Every Langevin step runs on the GPU; nothing here computes a series.
+#include "nspt_host.h"
+
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "../../include/nspt.h"
+
+int PTORD = allocORD;
+nspt_params_t nspt_pars;
+act_params_t act_pars;
+mi355x_opts_t mi355x_opts = {0, 1, 1, 0, 1};
+Cplx* w = nullptr;
+std::ofstream plaqfile, normfile, trufile;
+
+double Cplx::mod() const { return std::sqrt(re * re + im * im); }
+
+void ptSU3::zero() {
+  flag = Cplx();
+  for (int k = 0; k < allocORD; k++)
+    for (int e = 0; e < NC * NC; e++) ptU[k].whr[e] = Cplx();
+}
+void ptSU3::id() {
+  zero();
+  flag = Cplx(1.0, 0.0);
+}
+
+latt::latt(const int* sz) {
+  Size = 1;
+  for (int d = 0; d < dim; d++) {
+    Sz[d] = sz[d];
+    Size *= sz[d];
+  }
+  TSize = Size;
+}
+int latt::get(const int* n) const { return ((n[0] * Sz[1] + n[1]) * Sz[2] + n[2]) * Sz[3] + n[3]; }
+int latt::get(int n, int step, int dir) const {
+  int x[dim];
+  for (int d = dim - 1; d >= 0; d--) {
+    x[d] = n % Sz[d];
+    n /= Sz[d];
+  }
+  x[dir] = ((x[dir] + step) % Sz[dir] + Sz[dir]) % Sz[dir];
+  return get(x);
+}
+
+ptGluon_fld::ptGluon_fld(latt* z) : Z(z), W(new ptGluon[z->TSize]()) {}
+
+int ptGluon_fld::save(const char* filename) {
+  std::ofstream os(filename, std::ofstream::binary);
+  os.write(reinterpret_cast<const char*>(W), sizeof(ptGluon) * Z->Size);
+  if (os.rdstate() != 0) std::cerr << "Error writing " << filename << "\n";
+  return 0;
+}
+int ptGluon_fld::save(const char* filename, const Cplx* placchetta) {
+  std::ofstream os(filename, std::ofstream::binary);
+  os.write(reinterpret_cast<const char*>(W), sizeof(ptGluon) * Z->Size);
+  os.write(reinterpret_cast<const char*>(placchetta), sizeof(Cplx) * (allocORD + 1));
+  if (os.rdstate() != 0) std::cerr << "Error writing " << filename << "\n";
+  return 0;
+}
+int ptGluon_fld::load(const char* filename) {
+  std::ifstream is(filename, std::ifstream::binary);
+  if (!is) return -1;
+  is.read(reinterpret_cast<char*>(W), sizeof(ptGluon) * Z->Size);
+  return is ? 0 : -1;
+}
+int ptGluon_fld::load(const char* filename, Cplx* placchetta) {
+  std::ifstream is(filename, std::ifstream::binary);
+  if (!is) return -1;
+  is.read(reinterpret_cast<char*>(W), sizeof(ptGluon) * Z->Size);
+  is.read(reinterpret_cast<char*>(placchetta), sizeof(Cplx) * (allocORD + 1));
+  return is ? 0 : -1;
+}
+
+namespace {
+
+struct Engine {
+  nspt_ctx* ctx = nullptr;
+  nspt_rng* rng = nullptr;
+  ptGluon_fld* field = nullptr;
+  std::vector<double> eta, obs;
+  unsigned long long step = 0;
+};
+Engine E;
+
+[[noreturn]] void fail(const char* what) {
+  std::fprintf(stderr, "MI355X engine: %s: %s\n", what, E.ctx ? nspt_last_error(E.ctx) : "no context");
+  std::exit(1);
+}
+
+void attach(ptGluon_fld& U) {
+  if (E.ctx && E.field == &U) return;
+  mi355x_release();
+  nspt_geom g = {{U.Z->Sz[0], U.Z->Sz[1], U.Z->Sz[2], U.Z->Sz[3]}, NC, PTORD, allocORD};
+  if (nspt_create(&g, &E.ctx) != 0) fail("nspt_create");
+  E.field = &U;
+  if (nspt_upload(E.ctx, reinterpret_cast<const double*>(U.W)) != 0) fail("nspt_upload");
+}
+
+void write_step(const double* o) {
+  const int N = PTORD;
+  const double* plaq = o;                  // N+1 complex
+  const double* nrm = o + 2 * (N + 1);     // N
+  const double* tru = nrm + N;             // [mu][order] complex
+  for (int i = 0; i <= N; i++) plaqfile << plaq[2 * i] << "\t" << plaq[2 * i + 1] << std::endl;
+  for (int i = 0; i < N; i++) {
+    normfile << nrm[i] << std::endl;
+    for (int mu = 0; mu < dim; mu++)
+      trufile << tru[2 * (mu * N + i)] << "\t" << tru[2 * (mu * N + i) + 1] << "\t";
+    trufile << std::endl;
+  }
+}
+
+}  // namespace
+
+void mi355x_seed_noise(unsigned long seed) {
+  if (E.rng) nspt_rng_destroy(E.rng);
+  E.rng = nullptr;
+  std::srand((unsigned)seed);
+  if (nspt_rng_create_ranlxd(std::rand(), 1, &E.rng) != 0) fail("nspt_rng_create_ranlxd");
+  mi355x_opts.noise_seed = seed;
+}
+
+void NsptEvolve(ptGluon_fld& Umu) {
+  attach(Umu);
+  const int V = Umu.Z->Size;
+  const int ol = nspt_obs_len(E.ctx);
+  const double eps = act_pars.stau * act_pars.stau;
+  nspt_step_params p = {eps, act_pars.alpha, mi355x_opts.integrator ? NSPT_RK2 : NSPT_EULER,
+                        mi355x_opts.fa ? NSPT_GF_FA : NSPT_GF_NONE, mi355x_opts.zero_modes,
+                        mi355x_opts.device_noise ? NSPT_NOISE_DEVICE : NSPT_NOISE_HOST,
+                        mi355x_opts.noise_seed, E.step};
+  const int beat = nspt_pars.Beat;
+  E.obs.resize((size_t)beat * ol);
+  if (mi355x_opts.device_noise) {
+    if (nspt_evolve(E.ctx, &p, beat, E.obs.data()) != 0) fail("nspt_evolve");
+    E.step = p.step;
+  } else {
+    if (!E.rng) mi355x_seed_noise(mi355x_opts.noise_seed);
+    E.eta.resize((size_t)4 * V * NC * NC * 2);
+    for (int t = 0; t < beat; t++) {
+      nspt_rng_noise(E.rng, NC, 4 * V, E.eta.data());
+      if (nspt_set_noise(E.ctx, E.eta.data()) != 0) fail("nspt_set_noise");
+      if (nspt_step(E.ctx, &p, E.obs.data() + (size_t)t * ol) != 0) fail("nspt_step");
+    }
+  }
+  for (int t = 0; t < beat; t++) write_step(E.obs.data() + (size_t)t * ol);
+}
+
+void mi355x_sync_to_host(ptGluon_fld& Umu) {
+  if (!E.ctx || E.field != &Umu) return;
+  if (nspt_download(E.ctx, reinterpret_cast<double*>(Umu.W)) != 0) fail("nspt_download");
+}
+
+void mi355x_sync_to_device(ptGluon_fld& Umu) {
+  if (!E.ctx || E.field != &Umu) {
+    attach(Umu);
+    return;
+  }
+  if (nspt_upload(E.ctx, reinterpret_cast<const double*>(Umu.W)) != 0) fail("nspt_upload");
+}
+
+void plaquette_measure(ptGluon_fld& Umu, act_params_t& /*act*/) {
+  attach(Umu);
+  std::vector<double> out(2 * (PTORD + 1));
+  if (nspt_plaquette(E.ctx, out.data()) != 0) fail("nspt_plaquette");
+  for (int i = 0; i <= allocORD; i++) w[i] = (i <= PTORD) ? Cplx(out[2 * i], out[2 * i + 1]) : Cplx();
+  mi355x_sync_to_host(Umu);
+}
+
+int plaquette_check(const Cplx* w1, const Cplx* w2) {
+  for (int i = 0; i <= PTORD; i++)
+    if ((w1[i] - w2[i]).mod() > 1e-7) return 1;
+  return 0;
+}
+
+void mi355x_release() {
+  if (E.ctx) nspt_destroy(E.ctx);
+  E.ctx = nullptr;
+  E.field = nullptr;
+}
+
+ptGluon_fld::~ptGluon_fld() {
+  if (E.field == this) mi355x_release();
+  delete[] W;
+}

@@ -1,0 +1,143 @@
+// nspt_host.h -- host C++ drop-in for the reference's NSPT API, running the
+// Langevin step on the MI355X through libnspt.so (include/nspt.h).
+//
+// Kept from the reference (same names, same memory layout, same meaning):
+//   Cplx, SU3 matrix storage      MyMath.h:65-159, 505-520 (whr[NC*NC] row-major)
+//   ptSU3 {flag; ptU[allocORD]}   QCDpt.h:42-52 (order-0 flag, orders 1..allocORD)
+//   ptGluon {ptSU3 U[dim]}        QCDpt.h:249-251
+//   latt (sizes, site index)      lattice.h:37, 81-233
+//   ptGluon_fld (W, handle, get, save/load)   QCDenvNODEpt_test.h:331-426
+//   nspt_params_t, act_params_t   input.h:35-70
+//   NsptEvolve, plaquette_measure, plaquette_check   nspt.h:55-94, nspt.cc:1048-1058
+// The series types are data containers; the arithmetic of the step runs on
+// the GPU.  The field lives in GPU memory between NsptEvolve calls: the host
+// image W is refreshed by mi355x_sync_to_host (done by plaquette_measure and
+// before every save in the driver).
+//
+// Compile-time switches as in the reference's choices.h: allocORD (default 6,
+// choices.h:30) and NC (3; 2 for the SU(2) build, _runSU2).
+#pragma once
+#include <cstdio>
+#include <fstream>
+#include <iostream>
+
+#ifndef allocORD
+#define allocORD 6
+#endif
+#ifndef NC
+#define NC 3
+#endif
+
+constexpr int dim = 4;
+extern int PTORD;
+
+struct Cplx {
+  double re, im;
+  Cplx(double r = 0.0, double i = 0.0) : re(r), im(i) {}
+  Cplx operator-(const Cplx& b) const { return Cplx(re - b.re, im - b.im); }
+  double mod() const;
+};
+
+struct SU3 {
+  Cplx whr[NC * NC];
+};
+
+struct ptSU3 {
+  Cplx flag;
+  SU3 ptU[allocORD];
+  void zero();
+  void id();
+};
+
+struct ptGluon {
+  ptSU3 U[dim];
+};
+
+static_assert(sizeof(ptSU3) == sizeof(double) * 2 * (1 + NC * NC * allocORD), "ptSU3 layout");
+static_assert(sizeof(ptGluon) == dim * sizeof(ptSU3), "ptGluon layout");
+
+// Periodic 4-D lattice, site = ((x0*L1 + x1)*L2 + x2)*L3 + x3 (lattice.h:37).
+class latt {
+ public:
+  int Sz[dim];
+  int Size;
+  int TSize;
+  explicit latt(const int* sz);
+  int get(const int* n) const;
+  // neighbour of site n at distance step (+1/-1) along dir
+  int get(int n, int step, int dir) const;
+};
+
+class ptGluon_fld {
+ public:
+  latt* Z;
+  ptGluon* W;
+  explicit ptGluon_fld(latt* z);
+  ~ptGluon_fld();
+  ptGluon_fld(const ptGluon_fld&) = delete;
+  ptGluon_fld& operator=(const ptGluon_fld&) = delete;
+  ptSU3* handle() { return W->U; }
+  ptSU3& get(int n, int mu) { return W[n].U[mu]; }
+  ptGluon& get(int* n) { return W[Z->get(n)]; }
+  // Raw image of W (QCDenvNODEpt_test.h:342-348) / with the plaquette
+  // appended (allocORD+1 Cplx, QCDenvNODEpt_test.h:360-368).
+  int save(const char* filename);
+  int save(const char* filename, const Cplx* placchetta);
+  int load(const char* filename);
+  int load(const char* filename, Cplx* placchetta);
+};
+
+typedef struct {
+  int Sweep;
+  int Beat;
+  int Kill;
+  int Save;
+  int Init;
+  char* plaqn;
+  char* confn;
+  char* damon;
+  char* normn;
+  char* trun;
+  char* logn;
+} nspt_params_t;
+
+typedef struct {
+  int iVol;
+  int* sz;
+  double rVol;
+  double tau_g;  // eps as read; -eps/(2 NC) after QuenchedAllocate (Quenched.cc:583-588)
+  double stau;   // sqrt(eps)
+  double alpha;
+  double c0, c1;
+} act_params_t;
+
+// Engine choices the reference makes with compile-time switches.
+typedef struct {
+  int integrator;    // 0 Euler (the reference), 1 RK2
+  int fa;            // 1: Fourier-accelerated gauge fixing every step
+  int zero_modes;    // 1: REMOVE_GLUON_ZEROMOM
+  int device_noise;  // 0: ranlxd stream of the reference (Rand[0]); 1: device Philox
+  unsigned long long noise_seed;
+} mi355x_opts_t;
+
+extern nspt_params_t nspt_pars;
+extern act_params_t act_pars;
+extern mi355x_opts_t mi355x_opts;
+// Plaquette of plaquette_measure (PTORD+1, normalised), as the reference's global w.
+extern Cplx* w;
+// Output streams written once per Langevin step (nspt.cc:541-597).
+extern std::ofstream plaqfile, normfile, trufile;
+
+// Seeds the host noise stream as the reference seeds Rand[0]: srand(seed);
+// ranlxd(rand()) (Quenched.cc:301-306).
+void mi355x_seed_noise(unsigned long seed);
+// nspt_pars.Beat Langevin steps (nspt.cc:952-1039); observables of every
+// step go to plaqfile / normfile / trufile.
+void NsptEvolve(ptGluon_fld& Umu);
+// Plaquette per order into w (nspt.cc:1316-1406); refreshes the host image.
+void plaquette_measure(ptGluon_fld& Umu, act_params_t& act);
+// 0 if equal to 1e-7 per order (nspt.cc:1048-1058).
+int plaquette_check(const Cplx* w1, const Cplx* w2);
+void mi355x_sync_to_host(ptGluon_fld& Umu);
+void mi355x_sync_to_device(ptGluon_fld& Umu);
+void mi355x_release();

@@ -1,0 +1,273 @@
+// quenched.cc -- the Quenched driver (Quenched.cc:126-871) over the MI355X
+// engine: the same Quench.cfg and damocle.dag inputs, command-line options,
+// output file names and formats, checkpoint files and sweep/beat structure.
+//
+//   quenched [options]     (reads ./Quench.cfg, ./damocle.dag by default)
+//
+// Options of the reference (Quenched.cc:193-256): -l L (all four sizes),
+// -s SWEEP, -b BEAT, -r seed, -g tau_g, -a v (alpha = -v), -t init_status,
+// -p plaquette file stem, -c configuration stem, -d damocle file, -n PTORD,
+// -h.  Engine options (compile-time switches in the reference): -I 1 (RK2
+// integrator), -F 0 (no gauge fixing), -Z 0 (keep zero modes), -D 1
+// (device noise instead of the reference's ranlxd stream).
+//
+// The gauge fixing is the Fourier-accelerated one (nspt.cc:866-948); it is
+// stable for alpha > 0 only, so the shipped Quench.cfg (alpha -0.05, written
+// for the sequential gauge fixing) needs "-a -0.05" (alpha = +0.05).
+#include <cmath>
+#include <ctime>
+#include <cstdlib>
+#include <cstring>
+#include <getopt.h>
+#include <string>
+#include <vector>
+
+#include "nspt_host.h"
+
+namespace {
+
+std::ofstream logfile;
+int nInt = 0;
+int config_counter = 0;
+Cplx* w1 = nullptr;
+
+// "KEY value" with the key checked (Quenched.cc:106-127)
+int get_val(FILE* fp, const char* str, const char* fmt, void* val) {
+  char c[128];
+  if (1 != std::fscanf(fp, "%127s", c)) {
+    std::fprintf(stderr, "Error reading input file at %s\n", str);
+    std::exit(1);
+  }
+  if (std::strcmp(str, c) != 0) {
+    std::fprintf(stderr, "Error reading input file expected %s found %s\n", str, c);
+    std::exit(1);
+  }
+  if (1 != std::fscanf(fp, fmt, val)) {
+    std::fprintf(stderr, "Error reading input file at %s\nCannot read value format %s\n", str, fmt);
+    std::exit(1);
+  }
+  return 0;
+}
+
+const char* type_g() { return NC == 3 ? "SU3_wil" : "SU2_wil"; }
+
+void stamp(const char* what) {
+  std::time_t t;
+  std::time(&t);
+  logfile << what << std::ctime(&t) << std::endl;
+}
+
+int initialize(int argc, char** argv) {
+  unsigned long seed0 = 0;
+  act_pars.sz = new int[dim];
+  nspt_pars.plaqn = new char[256];
+  nspt_pars.confn = new char[256];
+  nspt_pars.damon = new char[256];
+  nspt_pars.normn = new char[256];
+  nspt_pars.logn = new char[256];
+  nspt_pars.trun = new char[256];
+
+  FILE* fp = std::fopen("Quench.cfg", "r");
+  if (!fp) {
+    std::cout << "Error: cannot read the configuration file Quench.cfg." << std::endl;
+    return 1;
+  }
+  if (std::fscanf(fp, "taglia %d %d %d %d\n", &act_pars.sz[0], &act_pars.sz[1], &act_pars.sz[2], &act_pars.sz[3]) != 4) {
+    std::fprintf(stderr, "Error reading input file at taglia\n");
+    std::exit(1);
+  }
+  get_val(fp, "SWEEP", "%d", &nspt_pars.Sweep);
+  get_val(fp, "BEAT", "%d", &nspt_pars.Beat);
+  get_val(fp, "tau_g", "%lf", &act_pars.tau_g);
+  get_val(fp, "alpha", "%lf", &act_pars.alpha);
+  get_val(fp, "init_status", "%d", &nspt_pars.Init);
+  get_val(fp, "plaq_out", "%255s", nspt_pars.plaqn);
+  get_val(fp, "last_conf", "%255s", nspt_pars.confn);
+  get_val(fp, "PTORD", "%d", &PTORD);
+  get_val(fp, "seed", "%lu", &seed0);  // one thread: the first seed line (NTHR = 1)
+  std::fclose(fp);
+  std::strcpy(nspt_pars.damon, "damocle.dag");
+
+  int opt;
+  while ((opt = getopt(argc, argv, "l:s:b:d:r:g:a:t:p:c:n:I:F:Z:D:h")) != -1) {
+    switch (opt) {
+      case 'l': act_pars.sz[0] = act_pars.sz[1] = act_pars.sz[2] = act_pars.sz[3] = std::atoi(optarg); break;
+      case 's': nspt_pars.Sweep = std::atoi(optarg); break;
+      case 'b': nspt_pars.Beat = std::atoi(optarg); break;
+      case 'r': seed0 = std::strtoul(optarg, nullptr, 10); break;
+      case 'n': PTORD = std::atoi(optarg); break;
+      case 'g': act_pars.tau_g = std::atof(optarg); break;
+      case 'a': act_pars.alpha = -std::atof(optarg); break;
+      case 't': nspt_pars.Init = std::atoi(optarg); break;
+      case 'p': std::strncpy(nspt_pars.plaqn, optarg, 255); break;
+      case 'c': std::strncpy(nspt_pars.confn, optarg, 255); break;
+      case 'd': std::strncpy(nspt_pars.damon, optarg, 255); break;
+      case 'I': mi355x_opts.integrator = std::atoi(optarg); break;
+      case 'F': mi355x_opts.fa = std::atoi(optarg); break;
+      case 'Z': mi355x_opts.zero_modes = std::atoi(optarg); break;
+      case 'D': mi355x_opts.device_noise = std::atoi(optarg); break;
+      case 'h':
+        std::printf("\n-l\t\tlattice size (L0 = L1 = L2 = L3)\n-s\t\tnumber of iterations (SWEEP)\n"
+                    "-b\t\titerations between damocle reads (BEAT)\n-r\t\trandom seed\n-g\t\ttau gluons\n"
+                    "-a\t\talpha (alpha = -value)\n-t\t\tinitial state: 0 cold, otherwise from configuration\n"
+                    "-p\t\tplaquette file name\n-c\t\tsaved configuration name\n-d\t\tdamocle file name\n"
+                    "-n\t\tperturbative order\n-I\t\tintegrator: 0 Euler, 1 RK2\n-F\t\tFA gauge fixing: 1 on, 0 off\n"
+                    "-Z\t\tzero-mode subtraction: 1 on, 0 off\n-D\t\tnoise: 0 host ranlxd, 1 device Philox\n\n");
+        std::exit(0);
+    }
+  }
+  if (PTORD < 1 || PTORD > allocORD) {
+    std::fprintf(stderr, "PTORD = %d outside 1..allocORD = %d\n", PTORD, allocORD);
+    return 1;
+  }
+  {
+    FILE* df = std::fopen(nspt_pars.damon, "r");
+    if (!df) {
+      std::cout << "Error: damocle file " << nspt_pars.damon << " not found." << std::endl;
+      std::exit(-1);
+    }
+    std::fclose(df);
+  }
+  mi355x_opts.noise_seed = seed0;
+  if (!mi355x_opts.device_noise) mi355x_seed_noise(seed0);
+  if (mi355x_opts.fa && act_pars.alpha <= 0.0)
+    std::fprintf(stderr, "warning: Fourier-accelerated gauge fixing needs alpha > 0 (alpha = %g)\n", act_pars.alpha);
+
+  // file names, non-MPI form (Quenched.cc:388-423)
+  const double a = std::fabs(act_pars.alpha), tg = act_pars.tau_g;
+  const int* z = act_pars.sz;
+  std::string plaq_stem = nspt_pars.plaqn, conf_stem = nspt_pars.confn;
+  std::snprintf(nspt_pars.plaqn, 256, "%s_%s_nf0_%dx%dx%dx%d_a%2.2f_tg%3.3f_o%d.txt", plaq_stem.c_str(), type_g(),
+                z[0], z[1], z[2], z[3], a, tg, PTORD);
+  std::snprintf(nspt_pars.confn, 256, "%s%s_nf0_%dx%dx%dx%d_a%2.2f_tg%3.3f_o%d", conf_stem.c_str(), type_g(), z[0],
+                z[1], z[2], z[3], a, tg, allocORD);
+  std::snprintf(nspt_pars.logn, 256, "%s_nf0_%dx%dx%dx%d_a%2.2f_tg%3.3f_o%d.log", type_g(), z[0], z[1], z[2], z[3],
+                a, tg, PTORD);
+  std::snprintf(nspt_pars.normn, 256, "%s_nf0_%dx%dx%dx%d_a%2.2f_tg%3.3f_o%d.nor", type_g(), z[0], z[1], z[2],
+                z[3], a, tg, PTORD);
+  std::snprintf(nspt_pars.trun, 256, "%s_nf0_%dx%dx%dx%d_a%2.2f_tg%3.3f_o%d.trU", type_g(), z[0], z[1], z[2], z[3],
+                a, tg, PTORD);
+
+  plaqfile.open(nspt_pars.plaqn, std::ios_base::app);
+  plaqfile.precision(30);
+  plaqfile << std::scientific;
+  normfile.open(nspt_pars.normn);
+  logfile.open(nspt_pars.logn);
+  trufile.open(nspt_pars.trun);
+  if (!plaqfile.is_open()) {
+    std::cout << "Error: cannot open " << nspt_pars.plaqn << std::endl;
+    return 1;
+  }
+
+  logfile << "Gauge group\t\t" << (NC == 3 ? "SU3" : "SU2") << std::endl
+          << "\tgauge field update not \"on the fly\"" << std::endl
+          << "Gauge field boundary conditions\t\tPERIODIC" << std::endl
+          << "Taglia\t\t" << z[0] << " " << z[1] << " " << z[2] << " " << z[3] << std::endl
+          << "Sweep\t\t" << nspt_pars.Sweep << std::endl
+          << "Beat\t\t" << nspt_pars.Beat << std::endl
+          << "Tau_g\t\t" << act_pars.tau_g << std::endl
+          << "Alpha\t\t" << act_pars.alpha << std::endl
+          << "Init status\t" << nspt_pars.Init << std::endl
+          << "Ordine\t\t" << PTORD << std::endl
+          << std::endl
+          << "Placchetta\t" << nspt_pars.plaqn << std::endl
+          << "Configurazione\t" << nspt_pars.confn << std::endl
+          << "Damocle\t\t" << nspt_pars.damon << std::endl
+          << "Norma\t\t" << nspt_pars.normn << std::endl
+          << "Tr(U)\t\t" << nspt_pars.trun << std::endl
+          << "Integrator\t" << (mi355x_opts.integrator ? "RK2" : "Euler") << std::endl
+          << "Noise\t\t" << (mi355x_opts.device_noise ? "device Philox4x32-10" : "ranlxd (Rand[0])") << std::endl
+          << std::endl;
+  return 0;
+}
+
+void transform_tau() {
+  act_pars.stau = std::sqrt(act_pars.tau_g);
+  act_pars.tau_g /= -(2.0 * NC);  // _SQRT_BETA_EXPANSION_ (Quenched.cc:583-588)
+}
+
+int allocate(ptGluon_fld& Umu) {
+  act_pars.iVol = act_pars.sz[0] * act_pars.sz[1] * act_pars.sz[2] * act_pars.sz[3];
+  act_pars.rVol = 1.0 / act_pars.iVol;
+  transform_tau();
+  if (!nspt_pars.Init) {
+    for (int i = 0; i < Umu.Z->TSize; i++)
+      for (int mu = 0; mu < dim; mu++) Umu.W[i].U[mu].id();
+  } else {
+    if (Umu.load(nspt_pars.confn, w1)) {
+      std::cout << "Error reading the configuration " << nspt_pars.confn << "." << std::endl;
+      return 1;
+    }
+    plaquette_measure(Umu, act_pars);
+    if (plaquette_check(w, w1)) {
+      std::cout << "The plaquette does not match the one stored with the configuration." << std::endl;
+      for (int i = 0; i <= PTORD; i++) std::printf("%e %e\t%e %e\n", w[i].re, w[i].im, w1[i].re, w1[i].im);
+    }
+    std::cout << "\nPlaquette:" << std::endl;
+    for (int i = 0; i <= PTORD; i++) std::printf("%e %e\n", w[i].re, w[i].im);
+  }
+  stamp("Inizio run\t\t");
+  return 0;
+}
+
+// damocle.dag re-read after every BEAT steps (Quenched.cc:665-725)
+int update_parameters(ptGluon_fld& Umu) {
+  nInt += nspt_pars.Beat;
+  logfile << "______________________________________" << std::endl << std::endl;
+  stamp("Damocle Update\t\t");
+  logfile << "# Iterazioni\t\t" << nInt << std::endl;
+  FILE* df = std::fopen(nspt_pars.damon, "r");
+  if (!df) {
+    std::cout << "Error: damocle file " << nspt_pars.damon << " not found." << std::endl;
+    std::exit(-1);
+  }
+  get_val(df, "SWEEP", "%d", &nspt_pars.Sweep);
+  get_val(df, "BEAT", "%d", &nspt_pars.Beat);
+  get_val(df, "tau_g", "%lf", &act_pars.tau_g);
+  get_val(df, "alpha", "%lf", &act_pars.alpha);
+  get_val(df, "save_config", "%d", &nspt_pars.Save);
+  get_val(df, "kill_run", "%d", &nspt_pars.Kill);
+  std::fclose(df);
+  logfile << "Tau_g\t\t\t" << act_pars.tau_g << std::endl << std::endl;
+  logfile.flush();
+  transform_tau();
+  if (nspt_pars.Save == 1) {
+    plaquette_measure(Umu, act_pars);
+    char name[300];
+    std::snprintf(name, sizeof(name), "%s_%d.dat", nspt_pars.confn, ++config_counter);
+    Umu.save(name, w);
+  }
+  if (nspt_pars.Kill == 1) nspt_pars.Sweep = 0;
+  return 0;
+}
+
+int finalize(ptGluon_fld& Umu, int t) {
+  logfile << "__________________________________" << std::endl;
+  stamp("Termine simulazione\t\t");
+  logfile << "# Iterazioni\t\t" << t << std::endl;
+  plaquette_measure(Umu, act_pars);
+  std::printf("Plaquette:\n");
+  for (int i = 0; i <= PTORD; i++) std::printf("%e\t%e\n", w[i].re, w[i].im);
+  Umu.save(nspt_pars.confn, w);
+  return 0;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  w = new Cplx[allocORD + 1];
+  w1 = new Cplx[allocORD + 1];
+  if (initialize(argc, argv)) return 1;
+  latt LL(act_pars.sz);
+  ptGluon_fld Umu(&LL);
+  if (allocate(Umu)) return 1;
+  int t1 = 0;
+  while (t1 < nspt_pars.Sweep) {
+    t1 += nspt_pars.Beat;
+    NsptEvolve(Umu);
+    if (update_parameters(Umu)) return 0;
+  }
+  finalize(Umu, t1);
+  mi355x_release();
+  return 0;
+}

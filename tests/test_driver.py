@@ -1,0 +1,125 @@
+"""The Quenched driver (prlgt2020_amd/host): inputs, output names and formats,
+checkpoints, and one run compared with the oracle under the same ranlxd noise.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "prlgt2020_amd", "bin")
+
+
+def driver(nc, alloc_ord):
+    p = os.path.join(BIN, "quenched_su%d_a%d" % (nc, alloc_ord))
+    if not os.path.exists(p):
+        pytest.skip("driver not built")
+    return p
+
+
+def write_inputs(d, L=4, sweep=20, beat=10, tau=0.05, alpha=0.05, ptord=6, seed=1, save=1, init=0):
+    with open(os.path.join(d, "Quench.cfg"), "w") as f:
+        f.write("taglia\t\t%d %d %d %d\nSWEEP\t\t%d\nBEAT \t\t%d\ntau_g\t\t%g\nalpha\t\t%g\ninit_status\t%d\n"
+                "plaq_out\t./plaq\nlast_conf\t./conf_\nPTORD\t\t%d\nseed %d\nseed 77\n"
+                % (L, L, L, L, sweep, beat, tau, alpha, init, ptord, seed))
+    with open(os.path.join(d, "damocle.dag"), "w") as f:
+        f.write("SWEEP\t\t%d\nBEAT\t\t%d\ntau_g\t\t%g\nalpha\t\t%g\nsave_config\t%d\nkill_run\t0\n"
+                % (sweep, beat, tau, alpha, save))
+
+
+def names(L, alpha, tau, ptord, alloc_ord, nc=3):
+    stem = "SU%d_wil_nf0_%dx%dx%dx%d_a%2.2f_tg%3.3f" % (nc, L, L, L, L, abs(alpha), tau)
+    return {"plaq": "plaq_%s_o%d.txt" % (stem, ptord), "nor": "%s_o%d.nor" % (stem, ptord),
+            "trU": "%s_o%d.trU" % (stem, ptord), "log": "%s_o%d.log" % (stem, ptord),
+            "conf": "conf_%s_o%d" % (stem, alloc_ord)}
+
+
+def has_gpu():
+    return os.path.exists("/dev/kfd")
+
+
+def test_driver_help(tmp_path):
+    write_inputs(tmp_path)
+    out = subprocess.run([driver(3, 6), "-h"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0
+    for opt in ("-l", "-s", "-b", "-r", "-g", "-a", "-t", "-p", "-c", "-d", "-n", "-I", "-D"):
+        assert opt + "\t" in out.stdout
+
+
+def test_driver_rejects_bad_input(tmp_path):
+    write_inputs(tmp_path)
+    with open(tmp_path / "Quench.cfg") as f:
+        text = f.read()
+    with open(tmp_path / "Quench.cfg", "w") as f:
+        f.write(text.replace("BEAT", "BEATS"))
+    out = subprocess.run([driver(3, 6)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert out.returncode != 0 and "expected BEAT found BEATS" in out.stderr
+
+
+@pytest.mark.skipif(has_gpu(), reason="checks the failure without a GPU")
+def test_driver_fails_loudly_without_gpu_and_names_outputs_like_the_reference(tmp_path):
+    write_inputs(tmp_path, L=4, tau=0.05, alpha=0.05)
+    out = subprocess.run([driver(3, 6)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert out.returncode != 0
+    assert "MI355X engine: nspt_create" in out.stderr
+    n = names(4, 0.05, 0.05, 6, 6)
+    for k in ("plaq", "nor", "trU", "log"):
+        assert (tmp_path / n[k]).exists(), n[k]
+
+
+def _read_cols(path, ncols):
+    rows = [l.split() for l in open(path) if l.strip()]
+    return np.array([[float(x) for x in r[:ncols]] for r in rows])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nc", [3, 2])
+def test_driver_run_matches_oracle(tmp_path, nc):
+    """20 Euler steps (Jacobi, zero modes, FA alpha 0.05) of a cold 4^4 lattice
+    at N = 6 with the reference's noise stream (srand(1); ranlxd(rand()))."""
+    from oracle import oracle as O
+    L, N, eps, alpha, steps, beat = 4, 6, 0.05, 0.05, 20, 10
+    write_inputs(tmp_path, L=L, sweep=steps, beat=beat, tau=eps, alpha=alpha, ptord=N, seed=1)
+    print("driver run: SU(%d) 4^4 N=6 Euler FA, %d steps" % (nc, steps), flush=True)
+    out = subprocess.run([driver(nc, 6)], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    n = names(L, alpha, eps, N, 6, nc)
+
+    # oracle with the same noise
+    Lt = (L, L, L, L)
+    V = L ** 4
+    U = O.cold_field(Lt, nc, N)
+    rng = O.Ranlxd(O.glibc_first_rand(1))
+    plaq, nor, tru = [], [], []
+    for _ in range(steps):
+        eta = rng.noise(nc, 4 * V)
+        obs = O.step(Lt, nc, N, U, eps, alpha, integrator=0, update=0, gauge_fix=1, zero_modes=1, eta=eta)
+        p, nr, t = O.split_obs(obs, N)
+        plaq.append(np.stack([p.real, p.imag], 1))
+        nor.append(nr)
+        tru.append(np.concatenate([np.stack([t[mu].real, t[mu].imag], 1) for mu in range(4)], 1))
+    plaq = np.concatenate(plaq)
+    nor = np.concatenate(nor)
+    tru = np.concatenate(tru)
+
+    got_p = _read_cols(tmp_path / n["plaq"], 2)
+    got_n = _read_cols(tmp_path / n["nor"], 1)[:, 0]
+    got_t = _read_cols(tmp_path / n["trU"], 8)
+    assert got_p.shape == plaq.shape and got_n.shape == nor.shape and got_t.shape == tru.shape
+    scale = np.abs(plaq).max()
+    assert np.abs(got_p - plaq).max() < 1e-12 * scale
+    # .nor and .trU are written with the stream's default 6 significant digits, as the reference does
+    # (relative rounding up to 5e-6)
+    assert np.allclose(got_n, nor, rtol=1e-5, atol=1e-14)
+    assert np.allclose(got_t, tru, rtol=1e-5, atol=1e-9)
+
+    # checkpoints: after each beat (save_config 1) and at the end, ptGluon[V] + plaquette
+    S = 1 + nc * nc * 6
+    size = 4 * V * S * 16 + 7 * 16
+    for f in (n["conf"] + "_1.dat", n["conf"] + "_2.dat", n["conf"]):
+        assert os.path.getsize(tmp_path / f) == size, f
+    raw = np.fromfile(tmp_path / n["conf"], dtype=np.complex128)
+    field = raw[:4 * V * S].reshape(4 * V, S)
+    assert np.abs(field - U).max() < 1e-12 * np.abs(U).max()
+    assert np.abs(raw[4 * V * S:] - O.plaquette_measure(Lt, nc, N, U)).max() < 1e-12

@@ -1,0 +1,42 @@
+"""Long runs agree statistically with what the theory fixes.
+
+With zero modes removed and eps -> 0, equipartition over the 3(V-1)(NC^2-1)
+transverse order-1 modes gives the order-2 plaquette coefficient
+<P_2> = -(NC^2-1)(1-1/V)/4 (SU(3), 4^4: -1.99219); odd orders average to
+zero.  RK2's eps^2 bias at eps = 0.02 is far below the statistical error
+allowed here.  The error is estimated by blocking (blocks longer than the
+autocorrelation time).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def blocked(x, nblocks=40):
+    b = np.array_split(x, nblocks)
+    m = np.array([v.mean() for v in b])
+    return x.mean(), m.std(ddof=1) / np.sqrt(nblocks)
+
+
+@pytest.mark.parametrize("nc", [3, 2])
+def test_order2_plaquette_equipartition(nc):
+    from prlgt2020_amd import nspt
+    L, N, eps = (4, 4, 4, 4), 4, 0.02
+    V = 256
+    want = -(nc * nc - 1) * (1 - 1 / V) / 4
+    print("equipartition run: SU(%d) 4^4 N=4 RK2 eps=%g, device noise" % (nc, eps), flush=True)
+    ctx = nspt.Context(L, nc, N)
+    p = nspt.Context.params(eps, 0.05, nspt.RK2, nspt.GF_FA, 1, nspt.NOISE_DEVICE, seed=11, step=0)
+    ctx.evolve(p, 1000, want_obs=False)
+    obs = ctx.evolve(p, 16000)
+    plaq = obs[:, :2 * (N + 1)].view(np.complex128)
+    m2, e2 = blocked(plaq[:, 2].real)
+    m1, e1 = blocked(plaq[:, 1].real)
+    m3, e3 = blocked(plaq[:, 3].real)
+    print("order 2: %.5f +- %.5f (expected %.5f); order 1: %.2e +- %.1e; order 3: %.2e +- %.1e"
+          % (m2, e2, want, m1, e1, m3, e3))
+    assert abs(m2 - want) < max(5 * e2, 0.005 * abs(want))
+    assert abs(m1) < 5 * e1 + 1e-12
+    assert abs(m3) < 5 * e3 + 1e-12
+    assert np.allclose(plaq[:, 0], 1.0)
