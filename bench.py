@@ -79,6 +79,40 @@ def cpu_baseline(N, integrator, budget_s=12.0):
                       % (N, "RK2" if integrator == 1 else "Euler", done, dt)}
 
 
+def batched_sweep(nsteps, warmup):
+    """Epsilon-replica sweep: SU(3) 4 x 12^4 + 4 x 16^4, N = 16, RK2 + zero
+    modes + FA, eps in {0.01, 0.015, 0.02, 0.03} per volume, distinct seeds.
+    Plain: each replica stepped alone (nspt_evolve, its own launches).
+    Batched: one graph-replayed step of all replicas, tiles autotuned."""
+    from prlgt2020_amd import nspt
+    N = 16
+    Ls = [(12,) * 4] * 4 + [(16,) * 4] * 4
+    eps = [0.01, 0.015, 0.02, 0.03] * 2
+    ps = [nspt.Context.params(e, 0.05, nspt.RK2, nspt.GF_FA, 1, nspt.NOISE_DEVICE, seed=100 + r, step=0)
+          for r, e in enumerate(eps)]
+    work = sum(4 * L[0] ** 4 * N for L in Ls)  # link-orders per step of all replicas
+    b = nspt.Batch(Ls, 3, N, ps)
+    b.evolve(warmup, want_obs=False)
+    qt, tune_ms = b.tune(steps=3)
+    # plain: the same replicas, one after the other, each with its own launches
+    lib = nspt.lib()
+    t0 = time.perf_counter()
+    for r, rep in enumerate(b.replicas):
+        p = nspt.Context.params(eps[r], 0.05, nspt.RK2, nspt.GF_FA, 1, nspt.NOISE_DEVICE, seed=100 + r,
+                                step=10 ** 6)
+        rep.evolve(p, nsteps, want_obs=True)
+    t_plain = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    b.evolve(nsteps, want_obs=True)
+    t_batch = time.perf_counter() - t0
+    b.close()
+    return {"workload": "SU(3) 4x12^4 + 4x16^4, 16 orders, RK2 + zero modes + FA, eps {0.01,0.015,0.02,0.03} "
+                        "per volume, distinct seeds, device noise",
+            "steps": nsteps, "plain_value": work * nsteps / t_plain, "batched_value": work * nsteps / t_batch,
+            "unit": "link-order updates/s", "speedup": t_plain / t_batch, "tile_edge": qt,
+            "tune_ms_per_step": [round(x, 3) for x in tune_ms if x > 0]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,6 +122,7 @@ def main():
     ap.add_argument("--N", type=int, default=20)
     ap.add_argument("--integrator", choices=["rk2", "euler"], default="rk2")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched epsilon-replica sweep")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="before the step, copy a buffer of this many MiB device to device (a plain copy kernel "
                          "of known size that scales the FETCH_SIZE / WRITE_SIZE counters of a --pmc run)")
@@ -164,6 +199,9 @@ def main():
         "kernels_ms_per_step": {k: v[0] / args.steps for k, v in prof.items() if v[1]},
         "plaquette_order2": float(plaq[2].real),
     }
+    ctx.close()
+    if not args.no_batched:
+        out["batched"] = batched_sweep(max(4, args.steps), 2)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(N, 1 if integ == nspt.RK2 else 0)
     print(json.dumps(out))

@@ -128,6 +128,30 @@ enum {
 int nspt_profile(nspt_ctx* ctx, int enable);
 int nspt_profile_read(nspt_ctx* ctx, double* ms_total, long long* count);
 
+/* Independent replicas stepped together: the epsilon-sweep setting of
+ * NSPT, where many runs of different volume and time step feed the
+ * eps -> 0 extrapolation (no counterpart in the reference, which runs one
+ * replica per process, Quenched.cc:262).  Replicas share nc and ptord; each
+ * has its own geometry and step parameters and must draw device noise
+ * (NSPT_NOISE_DEVICE) with its own seed.  One step of every replica is
+ * captured once in a HIP graph, one branch per replica running
+ * concurrently, and replayed; results equal nspt_evolve on each replica
+ * alone. */
+typedef struct nspt_batch nspt_batch;
+int nspt_batch_create(const nspt_geom* geoms, const nspt_step_params* params, int nrep, nspt_batch** out);
+int nspt_batch_destroy(nspt_batch* b);
+const char* nspt_batch_last_error(const nspt_batch* b);
+/* nsteps steps of every replica.  obs (may be null): replica-major, replica
+ * r holding nsteps * nspt_obs_len(replica r) doubles. */
+int nspt_batch_evolve(nspt_batch* b, int nsteps, double* obs);
+/* Context of replica r (download, plaquette, ...), owned by the batch. */
+int nspt_batch_replica(nspt_batch* b, int r, nspt_ctx** ctx);
+/* Autotuner: for every compiled order-tile edge of the replicas' kernel
+ * instance, time `steps` replays of the batch step; keep the fastest.
+ * Fields and counters are restored afterwards.  ms_per_step (may be null,
+ * up to 4 entries) receives the measured times in candidate order. */
+int nspt_batch_tune(nspt_batch* b, int steps, int* qt_chosen, double* ms_per_step);
+
 #ifdef __cplusplus
 }
 #endif

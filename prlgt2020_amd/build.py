@@ -40,7 +40,7 @@ def instances():
     # variant for quick experiments)
     block = text[text.rindex("#define NSPT_INSTANCES"):]
     block = block[:block.index("\n\n")]
-    return [(int(a), int(b)) for a, b in re.findall(r"X\((\d+),\s*(\d+)\)", block)]
+    return [(int(a), int(b), int(q)) for a, b, q in re.findall(r"X\((\d+),\s*(\d+),\s*(\d+)\)", block)]
 
 
 def _newer(target, sources):
@@ -62,11 +62,12 @@ def build(force=False, jobs=None, verbose=False):
     hdrs = [os.path.join(CSRC, h) for h in HEADERS] + [os.path.join(HERE, "..", "include", "nspt.h")]
     jobs = jobs or min(8, os.cpu_count() or 4)
     tasks = []
-    for nc, n in instances():
+    for nc, n, q in instances():
         src = os.path.join(CSRC, "nspt_inst.hip")
-        obj = os.path.join(OBJ, "inst_%d_%d.o" % (nc, n))
+        obj = os.path.join(OBJ, "inst_%d_%d_q%d.o" % (nc, n, q))
         if force or _newer(obj, [src] + hdrs):
-            tasks.append([HIPCC] + HIPFLAGS + ["-DINST_NC=%d" % nc, "-DINST_N=%d" % n, "-c", src, "-o", obj])
+            tasks.append([HIPCC] + HIPFLAGS + ["-DINST_NC=%d" % nc, "-DINST_N=%d" % n, "-DINST_QT=%d" % q, "-c", src,
+                          "-o", obj])
     capi_obj = os.path.join(OBJ, "nspt_capi.o")
     if force or _newer(capi_obj, [os.path.join(CSRC, "nspt_capi.hip")] + hdrs):
         tasks.append([HIPCC] + HIPFLAGS + ["-c", os.path.join(CSRC, "nspt_capi.hip"), "-o", capi_obj])
@@ -77,7 +78,7 @@ def build(force=False, jobs=None, verbose=False):
         for r in ex.map(_run, tasks):
             if verbose and r.stderr:
                 print(r.stderr, file=sys.stderr)
-    objs = [os.path.join(OBJ, "inst_%d_%d.o" % inst) for inst in instances()] + [capi_obj, rng_obj]
+    objs = [os.path.join(OBJ, "inst_%d_%d_q%d.o" % inst) for inst in instances()] + [capi_obj, rng_obj]
     relinked = False
     if force or tasks or _newer(LIB, objs):
         _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", LIB + ".tmp"] + objs)

@@ -4,6 +4,7 @@
 // include/nspt.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -95,7 +96,11 @@ __device__ __forceinline__ void philox(uint32_t c[4], uint32_t k0, uint32_t k1) 
 // (4*site+mu, step_lo, step_hi, q), q = 0 (and 1 for SU(3)), key = seed;
 // uniforms (u + 0.5) 2^-32; Box-Muller with variance 1/2 and the su(N)
 // mapping of MyMath.cc:295-322.
-__global__ void k_noise_philox(int nc, int V, unsigned long long seed, unsigned long long step, double2* eta) {
+// step_dev (may be null): a device counter added to step, so that a step
+// replayed from a captured graph draws fresh noise.
+__global__ void k_noise_philox(int nc, int V, unsigned long long seed, unsigned long long step0,
+                               const unsigned long long* step_dev, double2* eta) {
+  const unsigned long long step = step0 + (step_dev ? *step_dev : 0ull);
   for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < 4L * V; idx += (long)gridDim.x * blockDim.x) {
   const long mu = idx / V, site = idx % V;
   const uint32_t link = (uint32_t)(4 * site + mu);
@@ -151,7 +156,19 @@ __global__ void k_reduce_rows(const double* in, long ld, long row0, long nrows, 
 
 // Per-step observables from the per-mu block sums (layout per row:
 // plaq (N+1) cplx, norm N, trU N cplx), normalised as nspt.cc:570-597.
-__global__ void k_obs_finalize(const double* obs_mu, int nrt, int nc, int V, double* out) {
+// Advances the noise step and observable row counters of a captured step.
+__global__ void k_count(unsigned long long* ctr) {
+  if (threadIdx.x == 0) {
+    ctr[0] += 1;
+    ctr[1] += 1;
+  }
+}
+
+// slot_dev (may be null): a device counter selecting the output row, for
+// steps replayed from a captured graph.
+__global__ void k_obs_finalize(const double* obs_mu, int nrt, int nc, int V, double* out,
+                               const unsigned long long* slot_dev, int obs_len) {
+  if (slot_dev) out += (long)(*slot_dev) * obs_len;
   const int nob = 2 * (nrt + 1) + 3 * nrt;
   const double pn = (nc == 3 ? 72.0 : 48.0) * V, nn = (nc == 3 ? 12.0 : 8.0) * V, tn = (nc == 3 ? 3.0 : 2.0) * V;
   for (int t = threadIdx.x; t < 2 * (nrt + 1) + nrt; t += blockDim.x) {
@@ -255,14 +272,32 @@ __global__ void k_dft(Geo g, double2* fb, int d, int sign, int scale, double alp
   line[site] = acc;
 }
 
-KernelSet find_set(int nc, int n) {
+// Smallest compiled N >= n; among those the tile edge qt if compiled, else
+// the default edge 4.
+KernelSet find_set(int nc, int n, int qt = 4) {
   KernelSet best;
   auto consider = [&](KernelSet s) {
-    if (s.nc == nc && s.nmax >= n && (best.nmax == 0 || s.nmax < best.nmax)) best = s;
+    if (s.nc != nc || s.nmax < n) return;
+    if (best.nmax == 0 || s.nmax < best.nmax) {
+      best = s;
+    } else if (s.nmax == best.nmax && s.qt == qt && best.qt != qt) {
+      best = s;
+    }
   };
-#define NSPT_CONSIDER(a, b) consider(nspt::kernel_set_##a##_##b());
+#define NSPT_CONSIDER(a, b, q) consider(nspt::kernel_set_##a##_##b##_q##q());
   NSPT_INSTANCES(NSPT_CONSIDER)
 #undef NSPT_CONSIDER
+  if (best.nmax && best.qt != qt) {
+    KernelSet dflt = best;
+    best = KernelSet();
+    auto prefer4 = [&](KernelSet s) {
+      if (s.nc == nc && s.nmax == dflt.nmax && s.qt == 4) best = s;
+    };
+#define NSPT_CONSIDER4(a, b, q) prefer4(nspt::kernel_set_##a##_##b##_q##q());
+    NSPT_INSTANCES(NSPT_CONSIDER4)
+#undef NSPT_CONSIDER4
+    if (!best.nmax) best = dflt;
+  }
   return best;
 }
 
@@ -297,6 +332,10 @@ struct nspt_ctx {
   int obs_cap = 0;
   double2* aos_buf = nullptr;
   bool noise_ready = false;
+  // Captured-graph mode: noise step and observable row come from ctr_dev
+  // ({step offset, row}), advanced by k_count at the end of each step.
+  bool graph = false;
+  unsigned long long* ctr_dev = nullptr;
   std::string err;
   // per-kernel HIP-event timing (nspt_profile)
   bool prof = false;
@@ -405,7 +444,7 @@ bool obs_reduce(nspt_ctx* c, double* out_dev) {
       return false;
   Prof pr(c, NSPT_PROF_REDUCE);
   hipLaunchKernelGGL(k_obs_finalize, dim3(1), dim3(256), 0, c->stream, (const double*)c->obs_mu, c->nrt, c->nc, c->V,
-                     out_dev);
+                     out_dev, c->graph ? (const unsigned long long*)(c->ctr_dev + 1) : nullptr, c->obs_len);
   return launch_ok(c, "obs_finalize");
 }
 
@@ -449,7 +488,7 @@ bool make_noise(nspt_ctx* c, const nspt_step_params* p, const double2** eta) {
   if (p->noise == NSPT_NOISE_DEVICE) {
     Prof pr(c, NSPT_PROF_NOISE);
     hipLaunchKernelGGL(k_noise_philox, dim3(grid_for(4L * c->V, 256)), dim3(256), 0, c->stream, c->nc, c->V,
-                       p->seed, p->step, c->eta);
+                       p->seed, p->step, c->graph ? (const unsigned long long*)c->ctr_dev : nullptr, c->eta);
     *eta = c->eta;
     return launch_ok(c, "noise");
   }
@@ -577,7 +616,8 @@ int nspt_create(const nspt_geom* geom, nspt_ctx** out) {
             c->alloc(&c->MM, (size_t)4 * c->nrt * c->NE) && c->alloc(&c->obs_part, (size_t)c->nblk_links * c->nob) &&
             c->alloc(&c->obs_mu, (size_t)4 * c->nob) &&
             c->alloc(&c->red_tmp, (size_t)(c->nblk_links / 256 + 2) * 2 * c->nrt * c->NE + 4096) &&
-            c->alloc(&c->aos_buf, (size_t)P * (1 + (size_t)c->NE * c->A)) && ensure_obs(c, 1);
+            c->alloc(&c->aos_buf, (size_t)P * (1 + (size_t)c->NE * c->A)) && c->alloc(&c->ctr_dev, 2) &&
+            ensure_obs(c, 1);
   if (!ok) return -3;
   return nspt_cold_start(c);
 }
@@ -715,7 +755,7 @@ int nspt_device_noise(nspt_ctx* c, unsigned long long seed, unsigned long long s
   if (!c || !eta) return -1;
   const size_t n = 4UL * c->V * c->NE;
   hipLaunchKernelGGL(k_noise_philox, dim3(grid_for(4L * c->V, 256)), dim3(256), 0, c->stream, c->nc, c->V, seed,
-                     step, c->eta_tmp);
+                     step, (const unsigned long long*)nullptr, c->eta_tmp);
   hipLaunchKernelGGL(k_eta_permute, dim3(grid_for((long)n, 256)), dim3(256), 0, c->stream,
                      (const double2*)c->eta_tmp, c->eta, c->NE, c->V, 0);
   if (!launch_ok(c, "device_noise")) return -3;
@@ -743,6 +783,277 @@ int nspt_profile_read(nspt_ctx* c, double* ms_total, long long* count) {
     ms_total[i] = c->prof_ms[i];
     count[i] = c->prof_cnt[i];
   }
+  return 0;
+}
+
+}  // extern "C"
+
+// ================================================================== batches
+// Independent replicas stepped together.  A step of every replica is
+// captured once into a HIP graph -- one branch per replica, on the
+// replica's own stream, so the branches run concurrently -- and replayed.
+// Two graphs are kept because a step swaps the roles of the U and F
+// buffers: graph 0 is captured for the current buffer roles, graph 1 for the
+// swapped ones; after each replay the host-side roles are swapped to match.
+struct nspt_batch {
+  std::vector<nspt_ctx*> reps;
+  std::vector<nspt_step_params> params;
+  hipStream_t master = nullptr;
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> join;
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  int parity = 0;
+  bool with_obs = false;
+  int cap = 0;  // observable rows per replica held on the device
+  std::string err;
+  int fail(const std::string& m) {
+    err = m;
+    return -1;
+  }
+};
+
+namespace {
+
+void batch_drop_graphs(nspt_batch* b) {
+  for (auto& e : b->exec) {
+    if (e) (void)hipGraphExecDestroy(e);
+    e = nullptr;
+  }
+}
+
+// Capture the two step graphs (with or without observables).
+bool batch_build(nspt_batch* b, bool with_obs) {
+  batch_drop_graphs(b);
+  for (auto* c : b->reps) {
+    c->graph = true;
+    c->prof = false;
+  }
+  bool ok = true;
+  for (int g = 0; g < 2 && ok; g++) {
+    if (hipStreamBeginCapture(b->master, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      b->err = "hipStreamBeginCapture failed";
+      ok = false;
+      break;
+    }
+    (void)hipEventRecord(b->fork, b->master);
+    for (size_t r = 0; r < b->reps.size() && ok; r++) {
+      nspt_ctx* c = b->reps[r];
+      (void)hipStreamWaitEvent(c->stream, b->fork, 0);
+      if (!do_step(c, &b->params[r], with_obs ? c->obs_dev : nullptr)) {
+        b->err = "replica " + std::to_string(r) + ": " + c->err;
+        ok = false;
+      }
+      hipLaunchKernelGGL(k_count, dim3(1), dim3(64), 0, c->stream, c->ctr_dev);
+      (void)hipEventRecord(b->join[r], c->stream);
+      (void)hipStreamWaitEvent(b->master, b->join[r], 0);
+    }
+    hipGraph_t graph = nullptr;
+    if (hipStreamEndCapture(b->master, &graph) != hipSuccess || !graph) {
+      if (ok) b->err = "hipStreamEndCapture failed";
+      ok = false;
+    }
+    if (ok && hipGraphInstantiate(&b->exec[g], graph, nullptr, nullptr, 0) != hipSuccess) {
+      b->err = "hipGraphInstantiate failed";
+      ok = false;
+    }
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+  // each capture performed one U/F swap per replica: two captures restore
+  // the roles that graph 0 expects
+  if (!ok) {
+    batch_drop_graphs(b);
+  }
+  for (auto* c : b->reps) c->graph = false;
+  b->parity = 0;
+  b->with_obs = with_obs;
+  return ok;
+}
+
+bool batch_sync(nspt_batch* b) {
+  if (hipStreamSynchronize(b->master) != hipSuccess) {
+    b->err = "hipStreamSynchronize failed";
+    return false;
+  }
+  return true;
+}
+
+// n replays; the host-side U/F roles follow the device state
+bool batch_replay(nspt_batch* b, int n) {
+  for (int s = 0; s < n; s++) {
+    if (hipGraphLaunch(b->exec[b->parity], b->master) != hipSuccess) {
+      b->err = "hipGraphLaunch failed";
+      return false;
+    }
+    for (auto* c : b->reps) std::swap(c->U, c->F);
+    b->parity ^= 1;
+  }
+  return true;
+}
+
+bool batch_set_counters(nspt_batch* b, unsigned long long step, unsigned long long row) {
+  for (auto* c : b->reps) {
+    const unsigned long long v[2] = {step, row};
+    if (!c->hip_ok(hipMemcpyAsync(c->ctr_dev, v, sizeof(v), hipMemcpyHostToDevice, b->master), "counters")) {
+      b->err = c->err;
+      return false;
+    }
+  }
+  return batch_sync(b);
+}
+
+}  // namespace
+
+extern "C" {
+
+int nspt_batch_create(const nspt_geom* geoms, const nspt_step_params* params, int nrep, nspt_batch** out) {
+  if (!geoms || !params || !out || nrep < 1) return -1;
+  nspt_batch* b = new nspt_batch();
+  *out = b;
+  for (int r = 0; r < nrep; r++) {
+    if (geoms[r].nc != geoms[0].nc || geoms[r].ptord != geoms[0].ptord)
+      return b->fail("nspt_batch_create: replicas must share nc and ptord");
+    if (params[r].noise != NSPT_NOISE_DEVICE)
+      return b->fail("nspt_batch_create: replicas draw device noise (NSPT_NOISE_DEVICE)");
+    if (params[r].integrator != NSPT_EULER && params[r].integrator != NSPT_RK2)
+      return b->fail("nspt_batch_create: unknown integrator");
+    nspt_ctx* c = nullptr;
+    const int rc = nspt_create(&geoms[r], &c);
+    if (rc != 0) {
+      b->err = std::string("nspt_batch_create: replica ") + std::to_string(r) + ": " + (c ? c->err : "create failed");
+      if (c) nspt_destroy(c);
+      return -2;
+    }
+    b->reps.push_back(c);
+    b->params.push_back(params[r]);
+  }
+  if (hipStreamCreate(&b->master) != hipSuccess || hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess)
+    return b->fail("nspt_batch_create: stream/event creation failed");
+  b->join.resize(nrep);
+  for (auto& e : b->join)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return b->fail("event creation failed");
+  return batch_set_counters(b, 0, 0) ? 0 : -3;
+}
+
+int nspt_batch_destroy(nspt_batch* b) {
+  if (!b) return -1;
+  batch_drop_graphs(b);
+  for (auto* c : b->reps) nspt_destroy(c);
+  for (auto e : b->join) (void)hipEventDestroy(e);
+  if (b->fork) (void)hipEventDestroy(b->fork);
+  if (b->master) (void)hipStreamDestroy(b->master);
+  delete b;
+  return 0;
+}
+
+const char* nspt_batch_last_error(const nspt_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+int nspt_batch_replica(nspt_batch* b, int r, nspt_ctx** ctx) {
+  if (!b || !ctx || r < 0 || r >= (int)b->reps.size()) return -1;
+  *ctx = b->reps[r];
+  return 0;
+}
+
+int nspt_batch_evolve(nspt_batch* b, int nsteps, double* obs) {
+  if (!b || nsteps < 0) return -1;
+  const bool want = obs != nullptr;
+  const int chunk = 256;
+  if (want && b->cap < chunk) {
+    for (auto* c : b->reps)
+      if (!ensure_obs(c, chunk)) return b->fail(c->err);
+    b->cap = chunk;
+    batch_drop_graphs(b);
+  }
+  if (!b->exec[0] || b->with_obs != want) {
+    if (!batch_build(b, want)) return -3;
+  }
+  // replica r's rows: obs + r * nsteps * obs_len(r)
+  std::vector<size_t> base(b->reps.size(), 0);
+  for (size_t r = 1; r < b->reps.size(); r++) base[r] = base[r - 1] + (size_t)nsteps * b->reps[r - 1]->obs_len;
+  int done = 0;
+  while (done < nsteps) {
+    const int n = want ? std::min(chunk, nsteps - done) : nsteps;
+    if (want) {
+      // the row counter restarts for each chunk; the noise step counter runs on
+      for (auto* c : b->reps) {
+        const unsigned long long zero = 0;
+        if (hipMemcpyAsync(c->ctr_dev + 1, &zero, sizeof(zero), hipMemcpyHostToDevice, b->master) != hipSuccess)
+          return b->fail("counter reset failed");
+      }
+    }
+    if (!batch_replay(b, n)) return -3;
+    if (want) {
+      for (size_t r = 0; r < b->reps.size(); r++) {
+        nspt_ctx* c = b->reps[r];
+        if (hipMemcpyAsync(obs + base[r] + (size_t)done * c->obs_len, c->obs_dev, sizeof(double) * n * c->obs_len,
+                           hipMemcpyDeviceToHost, b->master) != hipSuccess)
+          return b->fail("observable copy failed");
+      }
+    }
+    if (!batch_sync(b)) return -3;
+    done += n;
+  }
+  return 0;
+}
+
+int nspt_batch_tune(nspt_batch* b, int steps, int* qt_chosen, double* ms_per_step) {
+  if (!b || steps < 1) return -1;
+  nspt_ctx* c0 = b->reps[0];
+  // candidate tile edges compiled for the replicas' instance
+  std::vector<int> cand;
+  for (int q : {4, 3, 5, 2}) {
+    const KernelSet ks = find_set(c0->nc, c0->nrt, q);
+    if (ks.qt == q && ks.nmax == c0->ks.nmax) cand.push_back(q);
+  }
+  // snapshot fields and counters: tuning steps are not part of the run
+  std::vector<double2*> snap(b->reps.size(), nullptr);
+  std::vector<unsigned long long> ctr(2 * b->reps.size());
+  for (size_t r = 0; r < b->reps.size(); r++) {
+    nspt_ctx* c = b->reps[r];
+    const size_t n = (size_t)c->nrt * c->NE * 4 * c->V + 4 * (size_t)c->V;
+    if (hipMalloc(&snap[r], n * sizeof(double2)) != hipSuccess) return b->fail("tune: snapshot allocation failed");
+    (void)hipMemcpyAsync(snap[r], c->U.data, (n - 4 * (size_t)c->V) * sizeof(double2), hipMemcpyDeviceToDevice,
+                         b->master);
+    (void)hipMemcpyAsync(snap[r] + (n - 4 * (size_t)c->V), c->U.flag, 4 * (size_t)c->V * sizeof(double2),
+                         hipMemcpyDeviceToDevice, b->master);
+    (void)hipMemcpyAsync(&ctr[2 * r], c->ctr_dev, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, b->master);
+  }
+  if (!batch_sync(b)) return -3;
+  double best = 1e30;
+  int best_q = c0->ks.qt;
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  for (size_t i = 0; i < cand.size(); i++) {
+    for (auto* c : b->reps) c->ks = find_set(c->nc, c->nrt, cand[i]);
+    if (!batch_build(b, false) || !batch_replay(b, 1)) return -3;
+    (void)hipEventRecord(e0, b->master);
+    if (!batch_replay(b, steps)) return -3;
+    (void)hipEventRecord(e1, b->master);
+    if (!batch_sync(b)) return -3;
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    const double per = ms / steps;
+    if (ms_per_step && i < 4) ms_per_step[i] = per;
+    if (per < best) {
+      best = per;
+      best_q = cand[i];
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  for (auto* c : b->reps) c->ks = find_set(c->nc, c->nrt, best_q);
+  batch_drop_graphs(b);
+  for (size_t r = 0; r < b->reps.size(); r++) {
+    nspt_ctx* c = b->reps[r];
+    const size_t n = (size_t)c->nrt * c->NE * 4 * c->V;
+    (void)hipMemcpyAsync(c->U.data, snap[r], n * sizeof(double2), hipMemcpyDeviceToDevice, b->master);
+    (void)hipMemcpyAsync(c->U.flag, snap[r] + n, 4 * (size_t)c->V * sizeof(double2), hipMemcpyDeviceToDevice,
+                         b->master);
+    (void)hipMemcpyAsync(c->ctr_dev, &ctr[2 * r], 2 * sizeof(unsigned long long), hipMemcpyHostToDevice, b->master);
+  }
+  if (!batch_sync(b)) return -3;
+  for (auto* p : snap) (void)hipFree(p);
+  if (qt_chosen) *qt_chosen = best_q;
   return 0;
 }
 

@@ -1,10 +1,17 @@
-// nspt_inst.hip -- one (NC, N) instance of the link/site kernels.  Compiled
-// once per instance with -DINST_NC=<nc> -DINST_N=<n>.
+// nspt_inst.hip -- one (NC, N, tile edge) instance of the link/site
+// kernels.  Compiled once per instance with -DINST_NC=<nc> -DINST_N=<n>
+// -DINST_QT=<edge>.
+#ifndef INST_QT
+#define INST_QT 4
+#endif
+#ifndef NSPT_QT
+#define NSPT_QT INST_QT
+#endif
 #include "nspt_kernels.hpp"
 
-#define NSPT_CAT2(a, b, c) kernel_set_##a##_##b##c
-#define NSPT_CAT(a, b) NSPT_CAT2(a, b, )
+#define NSPT_CAT2(a, b, q) kernel_set_##a##_##b##_q##q
+#define NSPT_CAT(a, b, q) NSPT_CAT2(a, b, q)
 
 namespace nspt {
-KernelSet NSPT_CAT(INST_NC, INST_N)() { return make_kernel_set<INST_NC, INST_N>(); }
+KernelSet NSPT_CAT(INST_NC, INST_N, INST_QT)() { return make_kernel_set<INST_NC, INST_N>(); }
 }  // namespace nspt

@@ -19,7 +19,7 @@ struct Geo {
 
 // Launchers of one (NC, N) instance.  nblk is the grid size in waves.
 struct KernelSet {
-  int nc = 0, nmax = 0, lpb = 0;
+  int nc = 0, nmax = 0, lpb = 0, qt = 0;  // qt: order-tile edge of the products
   void (*drift)(int nblk, hipStream_t, Geo, SerField U, SerField D, SerField D1, double tau_g, double rk2c, int mode,
                 double* obs) = nullptr;
   void (*update)(int nblk, hipStream_t, Geo, SerField D, const double2* eta, double stau, SerField Ubase,
@@ -30,16 +30,18 @@ struct KernelSet {
   void (*transform)(int nblk, hipStream_t, Geo, SerField G, SerField U) = nullptr;
 };
 
-// One function per compiled instance (nspt_inst.hip built once per (NC, N)).
+// One function per compiled instance (nspt_inst.hip built once per
+// (NC, N, tile edge)).  The default tile edge is 4; the SU(3) instances used
+// by the production configurations also come with edge 3 for the autotuner.
 #ifdef NSPT_SINGLE_INSTANCE
-#define NSPT_INSTANCES(X) X(3, 20)
+#define NSPT_INSTANCES(X) X(3, 20, 4)
 #else
 #define NSPT_INSTANCES(X) \
-  X(3, 4) X(3, 6) X(3, 8) X(3, 12) X(3, 16) X(3, 20) X(3, 24) \
-  X(2, 4) X(2, 6) X(2, 8) X(2, 12) X(2, 16) X(2, 20) X(2, 24)
+  X(3, 4, 4) X(3, 6, 4) X(3, 8, 4) X(3, 12, 4) X(3, 16, 4) X(3, 16, 3) X(3, 20, 4) X(3, 20, 3) X(3, 24, 4) \
+  X(2, 4, 4) X(2, 6, 4) X(2, 8, 4) X(2, 12, 4) X(2, 16, 4) X(2, 20, 4) X(2, 24, 4)
 #endif
 
-#define NSPT_DECL_SET(nc, n) KernelSet kernel_set_##nc##_##n();
+#define NSPT_DECL_SET(nc, n, q) KernelSet kernel_set_##nc##_##n##_q##q();
 NSPT_INSTANCES(NSPT_DECL_SET)
 #undef NSPT_DECL_SET
 

@@ -421,6 +421,7 @@ KernelSet make_kernel_set() {
   s.nc = NC;
   s.nmax = N;
   s.lpb = 64 / (NC * NC);
+  s.qt = NSPT_QT;
   s.drift = [](int nblk, hipStream_t st, Geo g, SerField U, SerField D, SerField D1, double tau_g, double rk2c,
                int mode, double* obs) {
     hipLaunchKernelGGL((k_drift<NC, N>), dim3(nblk), dim3(64), 0, st, g, U, D, D1, tau_g, rk2c, mode, obs);

@@ -41,7 +41,9 @@ _lib = None
 EXPORTS = ["nspt_create", "nspt_destroy", "nspt_last_error", "nspt_obs_len", "nspt_upload", "nspt_download",
            "nspt_cold_start", "nspt_set_noise", "nspt_step", "nspt_evolve", "nspt_plaquette",
            "nspt_fa_gauge_fixing", "nspt_drift", "nspt_rng_create_ranlxd", "nspt_rng_destroy",
-           "nspt_rng_uniform", "nspt_rng_noise", "nspt_device_noise", "nspt_profile", "nspt_profile_read"]
+           "nspt_rng_uniform", "nspt_rng_noise", "nspt_device_noise", "nspt_profile", "nspt_profile_read",
+           "nspt_batch_create", "nspt_batch_destroy", "nspt_batch_last_error", "nspt_batch_evolve",
+           "nspt_batch_replica", "nspt_batch_tune"]
 
 PROF_NAMES = ["drift", "update", "update_log", "zero_mode", "fa_div", "dft", "fa_gexp", "transform", "noise",
               "reduce"]
@@ -72,6 +74,14 @@ def lib():
         l.nspt_device_noise.argtypes = [ctypes.c_void_p, ctypes.c_ulonglong, ctypes.c_ulonglong, _dp]
         l.nspt_profile.argtypes = [ctypes.c_void_p, ctypes.c_int]
         l.nspt_profile_read.argtypes = [ctypes.c_void_p, _dp, ctypes.POINTER(ctypes.c_longlong)]
+        l.nspt_batch_create.argtypes = [ctypes.POINTER(_Geom), ctypes.POINTER(StepParams), ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_void_p)]
+        l.nspt_batch_destroy.argtypes = [ctypes.c_void_p]
+        l.nspt_batch_last_error.argtypes = [ctypes.c_void_p]
+        l.nspt_batch_last_error.restype = ctypes.c_char_p
+        l.nspt_batch_evolve.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+        l.nspt_batch_replica.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+        l.nspt_batch_tune.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int), _dp]
         _lib = l
     return _lib
 
@@ -202,6 +212,76 @@ class Context:
         eta = np.zeros((4 * self.V, self.nc * self.nc), np.complex128)
         self._check(lib().nspt_device_noise(self.h, seed, step, _ptr(eta)), "device_noise")
         return eta
+
+
+class _ReplicaView(Context):
+    """A replica of a Batch: a Context that the batch owns."""
+
+    def __init__(self, h, L, nc, ptord):
+        self.L = tuple(int(x) for x in L)
+        self.nc, self.N, self.A = nc, ptord, ptord
+        self.V = self.L[0] * self.L[1] * self.L[2] * self.L[3]
+        self.h = h
+        self.obs_len = lib().nspt_obs_len(self.h)
+
+    def close(self):
+        self.h = None
+
+
+class Batch:
+    """Independent replicas stepped together (nspt_batch_*): lattices L[r],
+    step parameters params[r] (device noise, own seeds), common nc, ptord."""
+
+    def __init__(self, Ls, nc, ptord, params):
+        n = len(Ls)
+        geoms = (_Geom * n)(*[_Geom((ctypes.c_int * 4)(*L), nc, ptord, ptord) for L in Ls])
+        ps = (StepParams * n)(*params)
+        h = ctypes.c_void_p()
+        rc = lib().nspt_batch_create(geoms, ps, n, ctypes.byref(h))
+        self.h = h
+        if rc != 0:
+            msg = lib().nspt_batch_last_error(h).decode() if h.value else "nspt_batch_create failed"
+            self.close()
+            raise NsptError("%s (%d)" % (msg, rc))
+        self.replicas = []
+        for r, L in enumerate(Ls):
+            c = ctypes.c_void_p()
+            lib().nspt_batch_replica(self.h, r, ctypes.byref(c))
+            self.replicas.append(_ReplicaView(c, L, nc, ptord))
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise NsptError("%s: %s (%d)" % (what, lib().nspt_batch_last_error(self.h).decode(), rc))
+
+    def evolve(self, nsteps, want_obs=True):
+        """Returns a list of (nsteps, obs_len) arrays, one per replica."""
+        if not want_obs:
+            self._check(lib().nspt_batch_evolve(self.h, nsteps, None), "batch_evolve")
+            return None
+        lens = [r.obs_len for r in self.replicas]
+        obs = np.zeros(nsteps * sum(lens))
+        self._check(lib().nspt_batch_evolve(self.h, nsteps, _ptr(obs)), "batch_evolve")
+        out, o = [], 0
+        for n in lens:
+            out.append(obs[o:o + nsteps * n].reshape(nsteps, n))
+            o += nsteps * n
+        return out
+
+    def tune(self, steps=3):
+        """Returns (chosen tile edge, ms per step of each candidate)."""
+        q = ctypes.c_int()
+        ms = np.zeros(4)
+        self._check(lib().nspt_batch_tune(self.h, steps, ctypes.byref(q), _ptr(ms)), "batch_tune")
+        return q.value, ms
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            lib().nspt_batch_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        if _lib is not None:
+            self.close()
 
 
 def split_obs(obs, N):

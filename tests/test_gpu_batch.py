@@ -1,0 +1,78 @@
+"""Batched replicas (nspt_batch_*): a graph-replayed step of several
+replicas of different volume, time step, integrator and seed gives the same
+fields and observables as each replica evolved alone with nspt_evolve."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _replicas(nspt):
+    Ls = [(4, 4, 4, 4), (4, 4, 6, 4), (6, 4, 4, 4)]
+    ps = [nspt.Context.params(0.02, 0.05, nspt.RK2, nspt.GF_FA, 1, nspt.NOISE_DEVICE, seed=11, step=0),
+          nspt.Context.params(0.01, 0.05, nspt.EULER, nspt.GF_FA, 1, nspt.NOISE_DEVICE, seed=12, step=0),
+          nspt.Context.params(0.03, 0.05, nspt.RK2, nspt.GF_NONE, 0, nspt.NOISE_DEVICE, seed=13, step=5)]
+    return Ls, ps
+
+
+@pytest.mark.parametrize("nc", [3, 2])
+def test_batch_equals_single_replicas(nc):
+    from prlgt2020_amd import nspt
+    N = 6
+    Ls, ps = _replicas(nspt)
+    print("batch of %d SU(%d) replicas, N=%d" % (len(Ls), nc, N), flush=True)
+    b = nspt.Batch(Ls, nc, N, ps)
+    obs_b = b.evolve(4)
+    obs_b2 = b.evolve(3)
+    for r, (L, p) in enumerate(zip(Ls, ps)):
+        c = nspt.Context(L, nc, N)
+        q = nspt.Context.params(p.eps, p.alpha, p.integrator, p.gauge_fix, p.zero_modes, p.noise, p.seed, p.step)
+        o1 = c.evolve(q, 4)
+        o2 = c.evolve(q, 3)
+        assert np.array_equal(b.replicas[r].download(), c.download()), r
+        assert np.array_equal(obs_b[r], o1) and np.array_equal(obs_b2[r], o2), r
+        c.close()
+    b.close()
+
+
+def test_batch_without_observables_and_long_run():
+    from prlgt2020_amd import nspt
+    N = 4
+    Ls, ps = _replicas(nspt)
+    b = nspt.Batch(Ls, 3, N, ps)
+    b.evolve(300, want_obs=False)
+    obs = b.evolve(300)  # more steps than one observable chunk (256)
+    for r, (L, p) in enumerate(zip(Ls, ps)):
+        c = nspt.Context(L, 3, N)
+        q = nspt.Context.params(p.eps, p.alpha, p.integrator, p.gauge_fix, p.zero_modes, p.noise, p.seed, p.step)
+        c.evolve(q, 300, want_obs=False)
+        o = c.evolve(q, 300)
+        assert np.array_equal(b.replicas[r].download(), c.download()), r
+        assert np.array_equal(obs[r], o), r
+        c.close()
+    b.close()
+
+
+def test_tuner_restores_the_run():
+    from prlgt2020_amd import nspt
+    N = 16  # compiled with tile edges 4 and 3
+    Ls, ps = _replicas(nspt)
+    b = nspt.Batch(Ls, 3, N, ps)
+    b.evolve(2, want_obs=False)
+    before = [r.download() for r in b.replicas]
+    q, ms = b.tune(steps=2)
+    assert q in (3, 4) and ms[0] > 0 and ms[1] > 0
+    after = [r.download() for r in b.replicas]
+    for x, y in zip(before, after):
+        assert np.array_equal(x, y)
+    # the run continues with the chosen tiles from the restored state and counters
+    obs = b.evolve(2)
+    ref = nspt.Context(Ls[0], 3, N)
+    p = ps[0]
+    q0 = nspt.Context.params(p.eps, p.alpha, p.integrator, p.gauge_fix, p.zero_modes, p.noise, p.seed, p.step)
+    ref.evolve(q0, 2, want_obs=False)
+    o = ref.evolve(q0, 2)
+    # a different tile edge only changes rounding
+    assert np.abs(obs[0] - o).max() < 1e-12
+    b.close()
+    ref.close()
