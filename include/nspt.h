@@ -152,6 +152,18 @@ int nspt_batch_replica(nspt_batch* b, int r, nspt_ctx** ctx);
  * up to 4 entries) receives the measured times in candidate order. */
 int nspt_batch_tune(nspt_batch* b, int steps, int* qt_chosen, double* ms_per_step);
 
+/* The Langevin drift as a reverse-mode gradient of the order-truncated
+ * Wilson action (no counterpart in the reference, whose drift is the staple
+ * sum of nspt.cc:383-417).  Generalised field: order 0 is a full NC x NC
+ * matrix (e.g. Haar-random), so each link carries (ptord+1) matrices, order
+ * 0 first, 4V links in link order 4*site + mu (no flag).  grad (may be
+ * null, same layout): G = df/dRe U + i df/dIm U of
+ * f = sum_{x, mu<nu} Re Tr P_{mu nu}(x)_{ptord}; one reverse pass seeded at
+ * the top order yields every order of the staple sum, Sigma_m =
+ * G_{ptord-m}^dag.  drift (may be null, same layout): -eps/(2 NC) reH(U Sigma)
+ * per order 0..ptord.  Compiled for ptord 4, 6, 12; alloc_ord is ignored. */
+int nspt_adjoint_drift(const nspt_geom* geom, const double* U, double eps, double* grad, double* drift);
+
 #ifdef __cplusplus
 }
 #endif

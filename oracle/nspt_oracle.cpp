@@ -616,6 +616,121 @@ struct Params {
   int zero_modes;    // 1: zero_modes_subtraction
 };
 
+// ---------------------------------------------------------------- adjoint
+// Generalised series with a matrix at order 0 (random configurations with a
+// Haar-random order 0): orders k = 0..N, each NC x NC, [(k*NE + e)].  The
+// Wilson action truncated at order K is f = sum_{x, mu<nu} Re Tr P_{mu nu}(x)_K
+// with P = U_mu(x) U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag (the plaquettes of
+// nspt.cc:1327-1346).  Its gradient is taken in reverse mode, each
+// plaquette's products recomputed in the backward pass.  Complex gradient
+// convention: G = df/dRe U + i df/dIm U, so df = Re Tr(G^dag dU).
+struct GSer {
+  int nc, N;
+  std::vector<cx> v;  // (N+1) * nc*nc
+  GSer(int nc_, int N_) : nc(nc_), N(N_), v((size_t)(N_ + 1) * nc_ * nc_, cx{0, 0}) {}
+  int ne() const { return nc * nc; }
+  cx* ord(int k) { return v.data() + (size_t)k * ne(); }
+  const cx* ord(int k) const { return v.data() + (size_t)k * ne(); }
+};
+
+// full Cauchy product C_k = sum_{i+j=k} A_i B_j, k = 0..N
+GSer gmul(const GSer& A, const GSer& B) {
+  GSer C(A.nc, A.N);
+  for (int i = 0; i <= A.N; i++)
+    for (int j = 0; i + j <= A.N; j++) mat_mac(A.nc, A.ord(i), B.ord(j), C.ord(i + j));
+  return C;
+}
+GSer gdag(const GSer& A) {
+  GSer D(A.nc, A.N);
+  for (int k = 0; k <= A.N; k++) mat_dag(A.nc, A.ord(k), D.ord(k));
+  return D;
+}
+// adjoints of C = A B: Abar_i = sum_k Cbar_k B_{k-i}^dag, Bbar_j = sum_k A_{k-j}^dag Cbar_k
+GSer gadj_a(const GSer& Cbar, const GSer& B) {
+  const GSer Bd = gdag(B);
+  GSer Ab(B.nc, B.N);
+  for (int i = 0; i <= B.N; i++)
+    for (int k = i; k <= B.N; k++) mat_mac(B.nc, Cbar.ord(k), Bd.ord(k - i), Ab.ord(i));
+  return Ab;
+}
+GSer gadj_b(const GSer& A, const GSer& Cbar) {
+  const GSer Ad = gdag(A);
+  GSer Bb(A.nc, A.N);
+  for (int j = 0; j <= A.N; j++)
+    for (int k = j; k <= A.N; k++) mat_mac(A.nc, Ad.ord(k - j), Cbar.ord(k), Bb.ord(j));
+  return Bb;
+}
+void gacc(GSer& a, const GSer& b) {
+  for (size_t i = 0; i < a.v.size(); i++) a.v[i] += b.v[i];
+}
+
+struct GField {
+  int nc, N;
+  const Latt* lat;
+  double* data;  // [(4*site + mu) * (N+1)*NE] complex
+  size_t S() const { return (size_t)(N + 1) * nc * nc; }
+  GSer get(int site, int mu) const {
+    GSer g(nc, N);
+    std::memcpy(g.v.data(), data + 2 * (4 * (size_t)site + mu) * S(), sizeof(cx) * S());
+    return g;
+  }
+};
+
+double re_tr(int nc, const cx* m) {
+  double t = 0;
+  for (int q = 0; q < nc; q++) t += m[q * nc + q].re;
+  return t;
+}
+
+// Re Tr P_{mu nu}(x)_K
+double gplaq(const GField& U, int x, int mu, int nu, int K) {
+  const Latt& Z = *U.lat;
+  const GSer P = gmul(gmul(gmul(U.get(x, mu), U.get(Z.up[4 * x + mu], nu)), gdag(U.get(Z.up[4 * x + nu], mu))),
+                      gdag(U.get(x, nu)));
+  return re_tr(U.nc, P.ord(K));
+}
+
+// reverse-mode gradient of f_K into G (same layout as U)
+void gaction_grad(const GField& U, int K, double* Gout) {
+  const Latt& Z = *U.lat;
+  const int nc = U.nc, N = U.N;
+  std::vector<GSer> G(4 * (size_t)Z.V, GSer(nc, N));
+  GSer Pbar(nc, N);
+  for (int q = 0; q < nc; q++) Pbar.ord(K)[q * nc + q] = cx{1, 0};
+  for (int x = 0; x < Z.V; x++)
+    for (int mu = 0; mu < 4; mu++)
+      for (int nu = mu + 1; nu < 4; nu++) {
+        const int l1 = 4 * x + mu, l2 = 4 * Z.up[4 * x + mu] + nu, l3 = 4 * Z.up[4 * x + nu] + mu, l4 = 4 * x + nu;
+        const GSer U1 = U.get(x, mu), U2 = U.get(l2 / 4, nu), U3d = gdag(U.get(l3 / 4, mu)),
+                   U4d = gdag(U.get(x, nu));
+        // forward: T1 = U1 U2, T2 = T1 U3^dag, P = T2 U4^dag
+        const GSer T1 = gmul(U1, U2);
+        const GSer T2 = gmul(T1, U3d);
+        // backward
+        const GSer T2b = gadj_a(Pbar, U4d);
+        gacc(G[l4], gdag(gadj_b(T2, Pbar)));
+        const GSer T1b = gadj_a(T2b, U3d);
+        gacc(G[l3], gdag(gadj_b(T1, T2b)));
+        gacc(G[l1], gadj_a(T1b, U2));
+        gacc(G[l2], gadj_b(U1, T1b));
+      }
+  for (size_t l = 0; l < G.size(); l++) std::memcpy(Gout + 2 * l * U.S(), G[l].v.data(), sizeof(cx) * U.S());
+}
+
+// sum of the staples of link (x, mu) (QCDenvNODEpt_test.h:554-557), matrix order 0
+GSer gstaple_sum(const GField& U, int x, int mu) {
+  const Latt& Z = *U.lat;
+  GSer S(U.nc, U.N);
+  for (int nu = 0; nu < 4; nu++) {
+    if (nu == mu) continue;
+    const int xpmu = Z.up[4 * x + mu], xpnu = Z.up[4 * x + nu], xmnu = Z.dn[4 * x + nu];
+    const int xmnu_pmu = Z.up[4 * xmnu + mu];
+    gacc(S, gmul(gmul(U.get(xpmu, nu), gdag(U.get(xpnu, mu))), gdag(U.get(x, nu))));
+    gacc(S, gmul(gmul(gdag(U.get(xmnu_pmu, nu)), gdag(U.get(xmnu, mu))), U.get(xmnu, nu)));
+  }
+  return S;
+}
+
 struct Geom {
   int L[4], nc, N;
 };
@@ -960,6 +1075,49 @@ void orc_noise_philox(int nc, unsigned long long seed, unsigned long long step, 
 int orc_glibc_first_rand(unsigned int seed) {
   srand(seed);
   return rand();
+}
+
+// ---- adjoint (generalised series, matrix order 0)
+void orc_gser_mul(int nc, int N, const double* a, const double* b, double* out) {
+  GSer A(nc, N), B(nc, N);
+  std::memcpy(A.v.data(), a, sizeof(cx) * A.v.size());
+  std::memcpy(B.v.data(), b, sizeof(cx) * B.v.size());
+  const GSer C = gmul(A, B);
+  std::memcpy(out, C.v.data(), sizeof(cx) * C.v.size());
+}
+// f_K = sum_{x, mu<nu} Re Tr P_K (site < 0), or the sum over the six
+// plaquettes that contain link (site, mu) (the part of f_K that depends on it)
+double orc_gaction(const int* gi, const double* U, int K, int site, int mu) {
+  const Geom g = geom_from(gi);
+  Latt Z(g.L);
+  GField F{g.nc, g.N, &Z, const_cast<double*>(U)};
+  double f = 0;
+  if (site < 0) {
+    for (int x = 0; x < Z.V; x++)
+      for (int m = 0; m < 4; m++)
+        for (int n = m + 1; n < 4; n++) f += gplaq(F, x, m, n, K);
+    return f;
+  }
+  for (int nu = 0; nu < 4; nu++) {
+    if (nu == mu) continue;
+    const int a = std::min(mu, nu), b = std::max(mu, nu);
+    f += gplaq(F, site, a, b, K);
+    f += gplaq(F, Z.dn[4 * site + nu], a, b, K);
+  }
+  return f;
+}
+void orc_gaction_grad(const int* gi, const double* U, int K, double* G) {
+  const Geom g = geom_from(gi);
+  Latt Z(g.L);
+  GField F{g.nc, g.N, &Z, const_cast<double*>(U)};
+  gaction_grad(F, K, G);
+}
+void orc_gstaple_sum(const int* gi, const double* U, int site, int mu, double* out) {
+  const Geom g = geom_from(gi);
+  Latt Z(g.L);
+  GField F{g.nc, g.N, &Z, const_cast<double*>(U)};
+  const GSer S = gstaple_sum(F, site, mu);
+  std::memcpy(out, S.v.data(), sizeof(cx) * S.v.size());
 }
 
 }  // extern "C"

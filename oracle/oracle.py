@@ -36,6 +36,7 @@ def _ptr(a):
     return a.ctypes.data_as(_dp)
 
 
+
 _lib = None
 
 
@@ -62,6 +63,10 @@ def lib():
         _lib.orc_gauge_wilson_jacobi.argtypes = [_ip, _dp, _dp, _dp, ctypes.c_double, _dp, _dp]
         _lib.orc_fa_gauge_fixing.argtypes = [_ip, _dp, ctypes.c_double]
         _lib.orc_sgf_default.argtypes = [_ip, _dp, ctypes.c_double]
+        _lib.orc_gaction.restype = ctypes.c_double
+        _lib.orc_gaction.argtypes = [_ip, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        _lib.orc_gaction_grad.argtypes = [_ip, _dp, ctypes.c_int, _dp]
+        _lib.orc_gstaple_sum.argtypes = [_ip, _dp, ctypes.c_int, ctypes.c_int, _dp]
     return _lib
 
 
@@ -269,6 +274,61 @@ def random_field(L, nc, N, seed=0, scale=0.3):
     U = np.empty_like(A)
     for i in range(4 * V):
         U[i] = exp(nc, N, A[i])
+    return U
+
+
+# ---------------------------------------------------------------- adjoint
+# Generalised series: orders 0..N, all NC x NC matrices, shape ((N+1)*NC^2,);
+# fields (4V, (N+1)*NC^2) in link order 4*site + mu.
+def gser_len(nc, N):
+    return (N + 1) * nc * nc
+
+
+def gmul(nc, N, a, b):
+    a = np.ascontiguousarray(a, np.complex128)
+    b = np.ascontiguousarray(b, np.complex128)
+    out = np.zeros(gser_len(nc, N), np.complex128)
+    lib().orc_gser_mul(nc, N, _ptr(a), _ptr(b), _ptr(out))
+    return out
+
+
+def gaction(L, nc, N, U, K, site=-1, mu=0):
+    """sum_{x, mu<nu} Re Tr P_K, or (site >= 0) the six plaquettes of a link."""
+    U = np.ascontiguousarray(U, np.complex128)
+    return lib().orc_gaction(_geom(L, nc, N), _ptr(U), K, site, mu)
+
+
+def gaction_grad(L, nc, N, U, K):
+    U = np.ascontiguousarray(U, np.complex128)
+    G = np.zeros_like(U)
+    lib().orc_gaction_grad(_geom(L, nc, N), _ptr(U), K, _ptr(G))
+    return G
+
+
+def gstaple_sum(L, nc, N, U, site, mu):
+    U = np.ascontiguousarray(U, np.complex128)
+    out = np.zeros(gser_len(nc, N), np.complex128)
+    lib().orc_gstaple_sum(_geom(L, nc, N), _ptr(U), site, mu, _ptr(out))
+    return out
+
+
+def haar_sun(rng, nc, n):
+    """n Haar-random SU(nc) matrices (QR of complex Gaussians, phases fixed)."""
+    z = (rng.normal(size=(n, nc, nc)) + 1j * rng.normal(size=(n, nc, nc))) / np.sqrt(2)
+    q, r = np.linalg.qr(z)
+    d = np.diagonal(r, axis1=1, axis2=2)
+    q = q * (d / np.abs(d))[:, None, :]
+    det = np.linalg.det(q)
+    return q / (det ** (1.0 / nc))[:, None, None]
+
+
+def random_gfield(L, nc, N, seed=7, scale=0.1):
+    """Order 0 Haar-random SU(nc), orders 1..N Gaussian su(nc) of the given scale."""
+    V = L[0] * L[1] * L[2] * L[3]
+    rng = np.random.default_rng(seed)
+    U = np.zeros((4 * V, gser_len(nc, N)), np.complex128)
+    U[:, :nc * nc] = haar_sun(rng, nc, 4 * V).reshape(4 * V, nc * nc)
+    U[:, nc * nc:] = random_algebra(rng, nc, N, scale, 4 * V)[:, 1:]
     return U
 
 

@@ -71,6 +71,9 @@ def build(force=False, jobs=None, verbose=False):
     capi_obj = os.path.join(OBJ, "nspt_capi.o")
     if force or _newer(capi_obj, [os.path.join(CSRC, "nspt_capi.hip")] + hdrs):
         tasks.append([HIPCC] + HIPFLAGS + ["-c", os.path.join(CSRC, "nspt_capi.hip"), "-o", capi_obj])
+    adj_obj = os.path.join(OBJ, "nspt_adjoint.o")
+    if force or _newer(adj_obj, [os.path.join(CSRC, "nspt_adjoint.hip")] + hdrs):
+        tasks.append([HIPCC] + HIPFLAGS + ["-c", os.path.join(CSRC, "nspt_adjoint.hip"), "-o", adj_obj])
     rng_obj = os.path.join(OBJ, "nspt_rng.o")
     if force or _newer(rng_obj, [os.path.join(CSRC, "nspt_rng.cpp")] + hdrs):
         tasks.append(["g++", "-O2", "-fPIC", "-std=c++17", "-c", os.path.join(CSRC, "nspt_rng.cpp"), "-o", rng_obj])
@@ -78,7 +81,7 @@ def build(force=False, jobs=None, verbose=False):
         for r in ex.map(_run, tasks):
             if verbose and r.stderr:
                 print(r.stderr, file=sys.stderr)
-    objs = [os.path.join(OBJ, "inst_%d_%d_q%d.o" % inst) for inst in instances()] + [capi_obj, rng_obj]
+    objs = [os.path.join(OBJ, "inst_%d_%d_q%d.o" % inst) for inst in instances()] + [capi_obj, adj_obj, rng_obj]
     relinked = False
     if force or tasks or _newer(LIB, objs):
         _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", LIB + ".tmp"] + objs)

@@ -1,0 +1,322 @@
+// nspt_adjoint.hip -- the Langevin drift as a reverse-mode gradient of the
+// order-truncated Wilson action (no counterpart in the reference, whose
+// drift is the staple sum of nspt.cc:383-417).
+//
+// Field: generalised series, order 0 a full matrix (random configurations
+// with Haar-random order 0), orders k = 0..NG-1 stored SoA as
+// data[(k*NE + e)*P + idx], idx = mu*V + site.  Action truncated at order K:
+//   f = sum_{x, mu<nu} Re Tr P_{mu nu}(x)_K,
+//   P = U_mu(x) U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag.
+// Gradient convention G = df/dRe U + i df/dIm U (df = Re Tr(G^dag dU)).
+//
+// k_plaq_adjoint: one workgroup per LPB plaquettes of one plane.  Forward:
+// T1 = U1 U2, T2 = T1 U3^dag.  Backward from Pbar = delta_{kK} 1:
+//   T2bar = adjA(Pbar, U4^dag)   U4bar = adjB(T2, Pbar)^dag
+//   T1bar = adjA(T2bar, U3^dag)  U3bar = adjB(T1, T2bar)^dag
+//   U1bar = adjA(T1bar, U2)      U2bar = adjB(U1, T1bar)
+// with adjA(Cbar, B)_i = sum_k Cbar_k B_{k-i}^dag and
+//      adjB(A, Cbar)_j = sum_k A_{k-j}^dag Cbar_k.
+// Both adjoints are ordinary truncated products once the cotangent is
+// staged with its orders reversed (order index n of the product is order
+// NG-1-n of the adjoint), so they run on the LDS product of
+// nspt_device.hpp (full Cauchy form).  Intermediate series are recomputed
+// per plaquette and kept in registers; nothing is stored between kernels
+// except each link's share of the gradient: a link belongs to six
+// plaquettes, each writing one of its six slots (no races), which
+// k_adjoint_drift sums in a fixed order.  The slot buffer is a field of
+// 6 x 4V series (plane stride 6 x 4V), series index slot*4V + link.
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/nspt.h"
+#include "nspt_device.hpp"
+#include "nspt_internal.hpp"
+
+namespace nspt {
+namespace {
+
+__device__ __forceinline__ int shift_site(const Geo& g, int s, int d, int delta) {
+  const int x = (s / g.st[d]) % g.L[d];
+  int xn = x + delta;
+  xn += (xn < 0) ? g.L[d] : 0;
+  xn -= (xn >= g.L[d]) ? g.L[d] : 0;
+  return s + (xn - x) * g.st[d];
+}
+
+// index of direction b among the three directions other than a
+__device__ __forceinline__ int other_index(int a, int b) { return b < a ? b : b - 1; }
+
+template <int NC, int NG>
+__device__ __forceinline__ void load_ser(const Lane<NC, NG>& ln, const double2* f, long P, long idx,
+                                         double2 (&X)[NG]) {
+  constexpr int NE = NC * NC;
+#pragma unroll
+  for (int k = 0; k < NG; k++) X[k] = f[((long)k * NE + ln.e) * P + idx];
+}
+template <int NC, int NG>
+__device__ __forceinline__ void store_ser(const Lane<NC, NG>& ln, double2* f, long P, long idx, bool ok,
+                                          const double2 (&X)[NG], bool dag) {
+  constexpr int NE = NC * NC;
+  // the dagger of a lane-owned series needs the transposed element: go
+  // through LDS (the caller has a free slot in ln.slotA)
+  if (!dag) {
+    if (ok)
+#pragma unroll
+      for (int k = 0; k < NG; k++) f[((long)k * NE + ln.e) * P + idx] = X[k];
+    return;
+  }
+  __syncthreads();
+  ln.stage(ln.slotA, X, true);
+  __syncthreads();
+  if (ok)
+#pragma unroll
+    for (int k = 0; k < NG; k++) f[((long)k * NE + ln.e) * P + idx] = ln.slotA[k * NE + ln.e];
+}
+
+// Full Cauchy product of slot A and slot B into registers: C_n, n = 0..NG-1.
+template <int NC, int NG>
+__device__ __forceinline__ void conv(const Lane<NC, NG>& ln, double2 (&C)[NG]) {
+#pragma unroll
+  for (int k = 0; k < NG; k++) C[k] = zero2();
+  __syncthreads();
+  mul<NC, NG, 0>(ln, zero2(), zero2(), false, C, NG - 1, false, nullptr, false, 1.0);
+  __syncthreads();
+}
+
+template <int NC, int NG>
+__global__ __launch_bounds__(64, 1) void k_plaq_adjoint(Geo g, const double2* U, double2* G6, int K) {
+  constexpr int NE = NC * NC;
+  using LN = Lane<NC, NG>;
+  __shared__ double2 lds[LN::LDS + NE];
+  LN ln(lds);
+  const int b = blockIdx.x;
+  const int pl = b % 6, sb = b / 6;
+  const int pm[6] = {0, 0, 0, 1, 1, 2}, pn[6] = {1, 2, 3, 2, 3, 3};
+  const int mu = pm[pl], nu = pn[pl];
+  const int x = min(sb * LN::LPB + ln.j, g.V - 1);
+  const bool ok = ln.lds_ok && (sb * LN::LPB + ln.j) < g.V;
+  const long V = g.V, P = 4 * V;
+  const int xpmu = shift_site(g, x, mu, 1), xpnu = shift_site(g, x, nu, 1);
+  const long l1 = mu * V + x, l2 = nu * V + xpmu, l3 = mu * V + xpnu, l4 = nu * V + x;
+
+  double2 X[NG], T1[NG], T2[NG], Tb[NG], R[NG];
+  // forward
+  load_ser(ln, U, P, l1, X);
+  ln.stage(ln.slotA, X);
+  load_ser(ln, U, P, l2, X);
+  ln.stage(ln.slotB, X);
+  conv(ln, T1);
+  ln.stage(ln.slotA, T1);
+  load_ser(ln, U, P, l3, X);
+  ln.stage(ln.slotB, X, true);
+  conv(ln, T2);
+  // Pbar = delta_{kK} 1
+#pragma unroll
+  for (int k = 0; k < NG; k++) Tb[k] = (k == K && ln.r == ln.c) ? one2() : zero2();
+  // T2bar = adjA(Pbar, U4^dag): reversed Pbar times (U4^dag)^dag = U4
+  ln.stage(ln.slotA, Tb, false, true);
+  load_ser(ln, U, P, l4, X);
+  ln.stage(ln.slotB, X);
+  conv(ln, R);
+  // U4bar = adjB(T2, Pbar)^dag: T2^dag times reversed Pbar, reversed, daggered
+  ln.stage(ln.slotA, T2, true);
+  ln.stage(ln.slotB, Tb, false, true);
+#pragma unroll
+  for (int k = 0; k < NG; k++) Tb[k] = R[NG - 1 - k];  // Tb = T2bar
+  conv(ln, R);
+  {
+    double2 Y[NG];
+#pragma unroll
+    for (int k = 0; k < NG; k++) Y[k] = R[NG - 1 - k];
+    store_ser(ln, G6, 6 * P, (long)(2 * other_index(nu, mu) + 0) * P + l4, ok, Y, true);
+  }
+  // T1bar = adjA(T2bar, U3^dag): reversed T2bar times U3
+  __syncthreads();
+  ln.stage(ln.slotA, Tb, false, true);
+  load_ser(ln, U, P, l3, X);
+  ln.stage(ln.slotB, X);
+  conv(ln, R);
+  // U3bar = adjB(T1, T2bar)^dag: T1^dag times reversed T2bar
+  ln.stage(ln.slotA, T1, true);
+  ln.stage(ln.slotB, Tb, false, true);
+#pragma unroll
+  for (int k = 0; k < NG; k++) Tb[k] = R[NG - 1 - k];  // Tb = T1bar
+  conv(ln, R);
+  {
+    double2 Y[NG];
+#pragma unroll
+    for (int k = 0; k < NG; k++) Y[k] = R[NG - 1 - k];
+    store_ser(ln, G6, 6 * P, (long)(2 * other_index(mu, nu) + 1) * P + l3, ok, Y, true);
+  }
+  // U1bar = adjA(T1bar, U2): reversed T1bar times U2^dag
+  __syncthreads();
+  ln.stage(ln.slotA, Tb, false, true);
+  load_ser(ln, U, P, l2, X);
+  ln.stage(ln.slotB, X, true);
+  conv(ln, R);
+  {
+    double2 Y[NG];
+#pragma unroll
+    for (int k = 0; k < NG; k++) Y[k] = R[NG - 1 - k];
+    store_ser(ln, G6, 6 * P, (long)(2 * other_index(mu, nu) + 0) * P + l1, ok, Y, false);
+  }
+  // U2bar = adjB(U1, T1bar): U1^dag times reversed T1bar
+  load_ser(ln, U, P, l1, X);
+  __syncthreads();
+  ln.stage(ln.slotA, X, true);
+  ln.stage(ln.slotB, Tb, false, true);
+  conv(ln, R);
+  {
+    double2 Y[NG];
+#pragma unroll
+    for (int k = 0; k < NG; k++) Y[k] = R[NG - 1 - k];
+    store_ser(ln, G6, 6 * P, (long)(2 * other_index(nu, mu) + 1) * P + l2, ok, Y, false);
+  }
+}
+
+// Per link: G = sum of the six slots; Sigma_m = G_{K-m}^dag (K = NG-1);
+// D = tau reH(U Sigma) per order 0..NG-1.  Gout / Dout may be null.
+template <int NC, int NG>
+__global__ __launch_bounds__(64, 1) void k_adjoint_drift(Geo g, const double2* U, const double2* G6,
+                                                         double2* Gout, double2* Dout, double tau) {
+  constexpr int NE = NC * NC;
+  using LN = Lane<NC, NG>;
+  __shared__ double2 lds[LN::LDS + NE];
+  LN ln(lds);
+  const int b = blockIdx.x;
+  const int mu = b & 3, sb = b >> 2;
+  const int x = min(sb * LN::LPB + ln.j, g.V - 1);
+  const bool ok = ln.lds_ok && (sb * LN::LPB + ln.j) < g.V;
+  const long V = g.V, P = 4 * V, idx = mu * V + x;
+  double2 Gs[NG], X[NG];
+#pragma unroll
+  for (int k = 0; k < NG; k++) Gs[k] = zero2();
+  for (int slot = 0; slot < 6; slot++) {
+    load_ser(ln, G6, 6 * P, (long)slot * P + idx, X);
+#pragma unroll
+    for (int k = 0; k < NG; k++) Gs[k] = c_add(Gs[k], X[k]);
+  }
+  if (Gout && ok)
+#pragma unroll
+    for (int k = 0; k < NG; k++) Gout[((long)k * NE + ln.e) * P + idx] = Gs[k];
+  if (!Dout) return;
+  load_ser(ln, U, P, idx, X);
+  ln.stage(ln.slotA, X);
+  ln.stage(ln.slotB, Gs, true, true);  // Sigma
+  double2 W[NG];
+  conv(ln, W);
+  ln.stage(ln.slotA, W);
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NG; k++) X[k] = c_scale(tau, reh_elem(ln, ln.slotA, k));
+  if (ok)
+#pragma unroll
+    for (int k = 0; k < NG; k++) Dout[((long)k * NE + ln.e) * P + idx] = X[k];
+}
+
+// host image (4V x NG*NE complex, link order 4*site+mu) <-> SoA
+__global__ void k_g_aos_to_soa(const double2* aos, int S, int V, double2* soa) {
+  const long P = 4L * V, n = P * S;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+    const long j = t / P, idx = t % P;  // j = k*NE + e
+    const long site = idx % V, mu = idx / V;
+    soa[t] = aos[(4 * site + mu) * S + j];
+  }
+}
+__global__ void k_g_soa_to_aos(const double2* soa, int S, int V, double2* aos) {
+  const long P = 4L * V, n = P * S;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+    const long link = t / S, j = t % S;
+    const long site = link / 4, mu = link % 4;
+    aos[t] = soa[j * P + mu * V + site];
+  }
+}
+
+struct AdjointSet {
+  int nc, ng;
+  void (*plaq)(int nblk, Geo g, const double2* U, double2* G6, int K);
+  void (*drift)(int nblk, Geo g, const double2* U, const double2* G6, double2* Gout, double2* Dout, double tau);
+};
+
+template <int NC, int NG>
+AdjointSet adjoint_set() {
+  AdjointSet s;
+  s.nc = NC;
+  s.ng = NG;
+  s.plaq = [](int nblk, Geo g, const double2* U, double2* G6, int K) {
+    hipLaunchKernelGGL((k_plaq_adjoint<NC, NG>), dim3(nblk), dim3(64), 0, 0, g, U, G6, K);
+  };
+  s.drift = [](int nblk, Geo g, const double2* U, const double2* G6, double2* Gout, double2* Dout, double tau) {
+    hipLaunchKernelGGL((k_adjoint_drift<NC, NG>), dim3(nblk), dim3(64), 0, 0, g, U, G6, Gout, Dout, tau);
+  };
+  return s;
+}
+
+// compiled orders (NG = ptord + 1)
+AdjointSet find_adjoint(int nc, int ng) {
+  const AdjointSet all[] = {adjoint_set<3, 5>(), adjoint_set<3, 7>(), adjoint_set<3, 13>(),
+                            adjoint_set<2, 5>(), adjoint_set<2, 7>(), adjoint_set<2, 13>()};
+  for (const auto& s : all)
+    if (s.nc == nc && s.ng == ng) return s;
+  return AdjointSet{0, 0, nullptr, nullptr};
+}
+
+int grid_n(long n) {
+  long gsz = (n + 255) / 256;
+  return (int)(gsz > 65536 ? 65536 : (gsz < 1 ? 1 : gsz));
+}
+
+}  // namespace
+}  // namespace nspt
+
+extern "C" int nspt_adjoint_drift(const nspt_geom* geom, const double* U, double eps, double* grad, double* drift) {
+  using namespace nspt;
+  if (!geom || !U) return -1;
+  const int nc = geom->nc, N = geom->ptord, NG = N + 1, NE = nc * nc;
+  const AdjointSet set = find_adjoint(nc, NG);
+  if (!set.plaq) return -2;  // orders 4, 6, 12 are compiled
+  long V = 1;
+  for (int d = 0; d < 4; d++) {
+    if (geom->sz[d] < 2 || geom->sz[d] > 32) return -1;
+    V *= geom->sz[d];
+  }
+  Geo g{};
+  for (int d = 0; d < 4; d++) g.L[d] = geom->sz[d];
+  g.st[3] = 1;
+  g.st[2] = g.L[3];
+  g.st[1] = g.L[2] * g.L[3];
+  g.st[0] = g.L[1] * g.L[2] * g.L[3];
+  g.V = (int)V;
+  g.nrt = NG;
+  const int lpb = 64 / NE;
+  const long nb = (V + lpb - 1) / lpb, P = 4 * V, S = (long)NG * NE;
+  double2 *u = nullptr, *aos = nullptr, *g6 = nullptr, *go = nullptr, *dout = nullptr;
+  int rc = 0;
+  auto ok = [&](hipError_t e) {
+    if (e != hipSuccess && rc == 0) rc = -3;
+    return rc == 0;
+  };
+  if (ok(hipMalloc(&u, sizeof(double2) * S * P)) && ok(hipMalloc(&aos, sizeof(double2) * S * P)) &&
+      ok(hipMalloc(&g6, sizeof(double2) * 6 * S * P)) && ok(hipMalloc(&go, sizeof(double2) * S * P)) &&
+      ok(hipMalloc(&dout, sizeof(double2) * S * P)) &&
+      ok(hipMemcpy(aos, U, sizeof(double2) * S * P, hipMemcpyHostToDevice))) {
+    hipLaunchKernelGGL(k_g_aos_to_soa, dim3(grid_n(S * P)), dim3(256), 0, 0, (const double2*)aos, (int)S, (int)V, u);
+    set.plaq((int)(6 * nb), g, u, g6, N);
+    set.drift((int)(4 * nb), g, u, g6, go, dout, -eps / (2.0 * nc));
+    ok(hipGetLastError());
+    if (rc == 0 && grad) {
+      hipLaunchKernelGGL(k_g_soa_to_aos, dim3(grid_n(S * P)), dim3(256), 0, 0, (const double2*)go, (int)S, (int)V, aos);
+      ok(hipMemcpy(grad, aos, sizeof(double2) * S * P, hipMemcpyDeviceToHost));
+    }
+    if (rc == 0 && drift) {
+      hipLaunchKernelGGL(k_g_soa_to_aos, dim3(grid_n(S * P)), dim3(256), 0, 0, (const double2*)dout, (int)S, (int)V,
+                         aos);
+      ok(hipMemcpy(drift, aos, sizeof(double2) * S * P, hipMemcpyDeviceToHost));
+    }
+    ok(hipDeviceSynchronize());
+  }
+  for (double2* p : {u, aos, g6, go, dout})
+    if (p) (void)hipFree(p);
+  return rc;
+}

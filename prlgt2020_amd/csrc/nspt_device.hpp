@@ -94,13 +94,18 @@ struct Lane {
   }
 
   // LDS staging of a lane-owned series (all orders), or of its dagger: the
-  // lane that owns (r, c) writes conj(X) at (c, r).
-  __device__ __forceinline__ void stage(double2* slot, const double2 (&X)[N], bool dag = false) const {
+  // lane that owns (r, c) writes conj(X) at (c, r).  With rev, order k of
+  // the slot receives order N-1-k of X.
+  __device__ __forceinline__ void stage(double2* slot, const double2 (&X)[N], bool dag = false,
+                                        bool rev = false) const {
     if (lds_ok) {
       const int pos = dag ? (c * NC + r) : e;
       const double sy = dag ? -1.0 : 1.0;
 #pragma unroll
-      for (int k = 0; k < N; k++) slot[k * NE + pos] = make_double2(X[k].x, sy * X[k].y);
+      for (int k = 0; k < N; k++) {
+        const double2 v = rev ? X[N - 1 - k] : X[k];
+        slot[k * NE + pos] = make_double2(v.x, sy * v.y);
+      }
     }
   }
 };
@@ -153,7 +158,12 @@ __device__ __forceinline__ void st_u(double2* base, unsigned off, double2 v) {
 // operand slots.  Bands are unrolled (output indices are constants); the
 // tiles of a band form a runtime loop.  Lane (r, c) reads row r of A and
 // column c of B from LDS.
-template <int NC, int N>
+//
+// OFF = 0 forms the full Cauchy product C_{a+b} of series whose order 0 is a
+// matrix stored at index 0 (no flags; a0, b0 unused): the same bands with
+// outputs shifted down by one.  It reads up to QT-1 (not QT-2) orders past a
+// slot, so its LDS needs NE more complex of padding.
+template <int NC, int N, int OFF = 1>
 __device__ __forceinline__ double2 mul(const Lane<NC, N>& ln, double2 a0, double2 b0, bool has_b0,
                                        double2 (&C)[N], int M, bool to_lds, double2* dst, bool dag,
                                        double scale) {
@@ -170,8 +180,10 @@ __device__ __forceinline__ double2 mul(const Lane<NC, N>& ln, double2 a0, double
   // Order k is final with value v (flag terms still to add).  The flag
   // terms read order k of both operands before it is overwritten.
   auto retire = [&](int k, double2 v) {
-    v = c_add(v, c_mul(a0, own_b[k * NE]));
-    if (has_b0) v = c_add(v, c_mul(b0, own_a[k * NE]));
+    if (OFF == 1) {
+      v = c_add(v, c_mul(a0, own_b[k * NE]));
+      if (has_b0) v = c_add(v, c_mul(b0, own_a[k * NE]));
+    }
     if (to_lds) {
       if (ln.lds_ok) dst[k * NE + pos] = make_double2(sx * v.x, sy * v.y);
     } else {
@@ -223,30 +235,33 @@ __device__ __forceinline__ double2 mul(const Lane<NC, N>& ln, double2 a0, double
     }
   };
 
-  constexpr int STOP = N >= 2 ? ((N - 2) / Q) * Q : 0;
+  // band s reaches outputs s+OFF .. s+OFF+2QT-2
+  constexpr int TOP = N - 1 - OFF;
+  constexpr int STOP = TOP >= 0 ? (TOP / Q) * Q : 0;
   // (a guard, not a break: the band loop must unroll fully)
 #pragma unroll
   for (int s = STOP; s >= 0; s -= Q) {
-    if (s <= M - 1) {
-      const int rem = M - 1 - s;
+    if (s + OFF <= M) {
+      const int rem = M - OFF - s;
       if (rem >= Q - 1)
         band(s, Q - 1);
       else
         band(s, rem);
 #pragma unroll
       for (int d = 0; d < Q; d++)
-        if (s + Q + d < N) retire(s + Q + d, acc[Q - 1 + d]);
-      // carry outputs s+1 .. s+QT-1 to the next band down
+        if (s + OFF + Q - 1 + d < N) retire(s + OFF + Q - 1 + d, acc[Q - 1 + d]);
+      // carry outputs s+OFF .. s+OFF+QT-2 to the next band down
 #pragma unroll
       for (int o = NO - 1; o >= 0; o--) acc[o] = (o >= Q) ? acc[o - Q] : zero2();
     }
   }
-  // outputs 1 .. QT-1 (carried past band 0, or flag terms only) and 0
+  // outputs OFF .. OFF+QT-2 (carried past band 0, or flag terms only), and
+  // the flag-only index 0 of the flagged form
 #pragma unroll
-  for (int d = 1; d < Q; d++)
-    if (d < N) retire(d, acc[Q - 1 + d]);
-  retire(0, zero2());
-  return c_mul(a0, b0);
+  for (int d = 0; d + 1 < Q; d++)
+    if (OFF + d < N) retire(OFF + d, acc[Q + d]);
+  if (OFF == 1) retire(0, zero2());
+  return OFF == 1 ? c_mul(a0, b0) : zero2();
 }
 
 // Product into LDS (in place allowed) / added to registers.

@@ -43,7 +43,7 @@ EXPORTS = ["nspt_create", "nspt_destroy", "nspt_last_error", "nspt_obs_len", "ns
            "nspt_fa_gauge_fixing", "nspt_drift", "nspt_rng_create_ranlxd", "nspt_rng_destroy",
            "nspt_rng_uniform", "nspt_rng_noise", "nspt_device_noise", "nspt_profile", "nspt_profile_read",
            "nspt_batch_create", "nspt_batch_destroy", "nspt_batch_last_error", "nspt_batch_evolve",
-           "nspt_batch_replica", "nspt_batch_tune"]
+           "nspt_batch_replica", "nspt_batch_tune", "nspt_adjoint_drift"]
 
 PROF_NAMES = ["drift", "update", "update_log", "zero_mode", "fa_div", "dft", "fa_gexp", "transform", "noise",
               "reduce"]
@@ -82,6 +82,7 @@ def lib():
         l.nspt_batch_evolve.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
         l.nspt_batch_replica.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
         l.nspt_batch_tune.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int), _dp]
+        l.nspt_adjoint_drift.argtypes = [ctypes.POINTER(_Geom), _dp, ctypes.c_double, _dp, _dp]
         _lib = l
     return _lib
 
@@ -282,6 +283,19 @@ class Batch:
     def __del__(self):
         if _lib is not None:
             self.close()
+
+
+def adjoint_drift(L, nc, N, U, eps):
+    """Reverse-mode gradient of sum Re Tr P_N and the drift -eps/(2nc) reH(U Sigma)
+    of a generalised field U (4V, (N+1)*nc*nc), order 0 a matrix.  Returns (G, D)."""
+    U = np.ascontiguousarray(U, np.complex128)
+    g = _Geom((ctypes.c_int * 4)(*L), nc, N, N)
+    G = np.zeros_like(U)
+    D = np.zeros_like(U)
+    rc = lib().nspt_adjoint_drift(ctypes.byref(g), _ptr(U), eps, _ptr(G), _ptr(D))
+    if rc != 0:
+        raise NsptError("nspt_adjoint_drift failed (%d)" % rc)
+    return G, D
 
 
 def split_obs(obs, N):
