@@ -120,6 +120,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--L", type=int, default=16)
     ap.add_argument("--N", type=int, default=20)
+    ap.add_argument("--nc", type=int, choices=[3, 2], default=3)
+    ap.add_argument("--eps", type=float, default=0.02)
     ap.add_argument("--integrator", choices=["rk2", "euler"], default="rk2")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched epsilon-replica sweep")
@@ -142,11 +144,11 @@ def main():
         torch.cuda.synchronize()
         del src, dst
     L = (args.L,) * 4
-    N, nc = args.N, 3
+    N, nc = args.N, args.nc
     V = args.L ** 4
     integ = nspt.RK2 if args.integrator == "rk2" else nspt.EULER
     ctx = nspt.Context(L, nc, N)
-    p = nspt.Context.params(0.02, 0.05, integ, nspt.GF_FA, 1, nspt.NOISE_DEVICE, seed=1, step=0)
+    p = nspt.Context.params(args.eps, 0.05, integ, nspt.GF_FA, 1, nspt.NOISE_DEVICE, seed=1, step=0)
     ctx.evolve(p, args.warmup, want_obs=False)
     import ctypes
     lib = nspt.lib()
@@ -189,8 +191,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic: cold start, device Philox noise",
-        "config": {"workload": "SU(3) Wilson NSPT %d^4, %d orders, %s eps=0.02, zero modes + FA gauge fixing "
-                               "(alpha=0.05) every step" % (args.L, N, args.integrator.upper()),
+        "config": {"workload": "SU(%d) Wilson NSPT %d^4, %d orders, %s eps=%g, zero modes + FA gauge fixing "
+                               "(alpha=0.05) every step" % (nc, args.L, N, args.integrator.upper(), args.eps),
                    "lattice": list(L), "orders": N, "integrator": args.integrator, "replicas": 1},
         "roofline": {"bound": "mfma", "kernel": name, "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,

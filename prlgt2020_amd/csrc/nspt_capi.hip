@@ -591,7 +591,7 @@ int nspt_create(const nspt_geom* geom, nspt_ctx** out) {
   }
   c->V = (int)V;
   c->ks = find_set(c->nc, c->nrt);
-  if (!c->ks.drift) return bad("nspt_create: no compiled kernel instance for this order (max 24)");
+  if (!c->ks.drift) return bad("nspt_create: no compiled kernel instance for this order (SU(3) up to 32, SU(2) up to 24)");
   for (int d = 0; d < 4; d++) c->g.L[d] = geom->sz[d];
   c->g.st[3] = 1;
   c->g.st[2] = c->g.L[3];

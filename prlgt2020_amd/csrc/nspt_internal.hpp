@@ -37,7 +37,7 @@ struct KernelSet {
 #define NSPT_INSTANCES(X) X(3, 20, 4)
 #else
 #define NSPT_INSTANCES(X) \
-  X(3, 4, 4) X(3, 6, 4) X(3, 8, 4) X(3, 12, 4) X(3, 16, 4) X(3, 16, 3) X(3, 20, 4) X(3, 20, 3) X(3, 24, 4) \
+  X(3, 4, 4) X(3, 6, 4) X(3, 8, 4) X(3, 12, 4) X(3, 16, 4) X(3, 16, 3) X(3, 20, 4) X(3, 20, 3) X(3, 24, 4) X(3, 32, 4) \
   X(2, 4, 4) X(2, 6, 4) X(2, 8, 4) X(2, 12, 4) X(2, 16, 4) X(2, 20, 4) X(2, 24, 4)
 #endif
 

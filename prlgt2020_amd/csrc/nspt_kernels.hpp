@@ -153,17 +153,22 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_drift(Geo g, SerField
       ib = nu * V + xmnu;
     }
   };
-  double2 PA[N], PB[N], paf = zero2(), pbf = zero2();
+  // Up to N = 24 the next product's links are prefetched into registers;
+  // above that the registers are needed for Sigma and the product, and the
+  // links are staged straight from HBM.
+  constexpr bool PF = N <= 24;
+  constexpr int NP = PF ? N : 1;
+  double2 PA[NP], PB[NP], paf = zero2(), pbf = zero2();
   auto fetch = [&](int i) {
     long ia, ib;
     links_of(i, ia, ib);
     const int kind = kind_of(i);
     if (kind != 3) {
-      gload<NC, N>(U, ia, ln.e, nrt, ok, PA);
+      if constexpr (PF) gload<NC, N>(U, ia, ln.e, nrt, ok, PA);
       paf = gflag(U, ia, ok);
     }
     if (kind == 0 || kind == 2 || kind == 3) {
-      gload<NC, N>(U, ib, ln.e, nrt, ok, PB);
+      if constexpr (PF) gload<NC, N>(U, ib, ln.e, nrt, ok, PB);
       pbf = gflag(U, ib, ok);
     }
   };
@@ -171,8 +176,15 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_drift(Geo g, SerField
   for (int i = 0; i < 13; i++) {
     asm volatile("" ::: "memory");
     const int kind = kind_of(i);
-    if (kind != 3) ln.stage(ln.slotA, PA);
-    if (kind == 0 || kind == 2 || kind == 3) ln.stage(ln.slotB, PB);
+    if constexpr (PF) {
+      if (kind != 3) ln.stage(ln.slotA, PA);
+      if (kind == 0 || kind == 2 || kind == 3) ln.stage(ln.slotB, PB);
+    } else {
+      long ia, ib;
+      links_of(i, ia, ib);
+      if (kind != 3) stage_global(ln, ln.slotA, U, ia, nrt);
+      if (kind == 0 || kind == 2 || kind == 3) stage_global(ln, ln.slotB, U, ib, nrt);
+    }
     if (kind == 4) ln.stage(ln.slotB, Sig);
     const double2 a0 = (kind == 3) ? c_conj(t2f) : paf;
     const double2 b0 = (kind == 1) ? c_conj(t1f) : (kind == 4) ? sigf : pbf;

@@ -147,7 +147,7 @@ def _conditioning_envelope(L, nc, N, U, eta, trials=3):
     return U1, env
 
 
-@pytest.mark.parametrize("nc,N", [(3, 20), (3, 24), (2, 20)])
+@pytest.mark.parametrize("nc,N", [(3, 20), (3, 24), (3, 32), (2, 20), (2, 24)])
 def test_high_orders_within_fp64_conditioning(nc, N, relerr):
     """Orders above ~16: one step is ill-conditioned in FP64 (a 1e-16
     relative perturbation of the input moves the oracle's own order-20 output
