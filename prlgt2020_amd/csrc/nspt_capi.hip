@@ -202,11 +202,14 @@ __global__ void k_fa_div(Geo g, int ncomp, const double2* X, double2* fb) {
 }
 
 // One pass of the unnormalised 4-D DFT along direction d (sign -1 forward,
-// +1 backward, FFTW convention of Quenched.cc:827-838), as a direct DFT of
-// each line from an LDS tile of L_d x T elements (T consecutive sites of
-// the fastest other direction).  With scale != 0 the output is multiplied
-// by -alpha/(V * p^2(k)), p^2 = sum_mu 4 sin^2(pi k_mu / L_mu), and the k = 0
-// mode is set to zero (nspt.cc:881-894, 902).
+// +1 backward, FFTW convention of Quenched.cc:827-838) on an LDS tile of
+// L_d x T elements (T consecutive sites of the fastest other direction):
+// a mixed-radix Stockham FFT of each line, one output per thread per stage,
+// radices the prime factors of L_d (smallest first), ping-ponging between
+// two tile buffers so that the output comes out in natural order.  With
+// scale != 0 the output is multiplied by -alpha/(V * p^2(k)),
+// p^2 = sum_mu 4 sin^2(pi k_mu / L_mu), and the k = 0 mode is set to zero
+// (nspt.cc:881-894, 902).
 __global__ void k_dft(Geo g, double2* fb, int d, int sign, int scale, double alpha) {
   extern __shared__ double2 sm[];
   const int n = g.L[d];
@@ -224,8 +227,12 @@ __global__ void k_dft(Geo g, double2* fb, int d, int sign, int scale, double alp
   const int x0 = (int)(tt / g.L[f[1]]);
   const long base = (long)x0 * g.st[f[0]] + (long)x1 * g.st[f[1]];
   const int tid = threadIdx.x;
-  double2* tile = sm;
-  double2* tw = sm + n * T;
+  double2* X = sm;
+  double2* Y = sm + n * T;
+  double2* tw = sm + 2 * n * T;  // tw[k] = exp(sign 2 pi i k / n)
+  // per stage and output index: first input, input stride, twiddle step of
+  // the r-point DFT and the output twiddle, built once per block
+  int* tab = reinterpret_cast<int*>(tw + n);  // [stage][n][4]
   int a = 0, b = 0;
   const bool act = tid < n * T;
   if (act) {
@@ -239,26 +246,61 @@ __global__ void k_dft(Geo g, double2* fb, int d, int sign, int scale, double alp
   }
   const long site = base + (long)a * g.st[d] + (long)b * g.st[bdim];
   double2* line = fb + comp * V;
-  if (act) tile[a * T + b] = line[site];
+  if (act) X[a * T + b] = line[site];
   if (tid < n) {
     double s, c;
-    sincospi(2.0 * tid / (double)n, &s, &c);
+    sincospi(2.0 * (double)tid / (double)n, &s, &c);
     tw[tid] = make_double2(c, sign * s);
+    // stages: length len = r * m, stride st; output o = q + st*(r*p + t) is
+    // the r-point DFT of x[q + st*(p + u*m)], u = 0..r-1, times w_len^{p t}
+    int len = n, st = 1, stage = 0;
+    while (len > 1) {
+      int r = 2;
+      while (len % r) r++;
+      const int m = len / r;
+      const int qq = tid % st, rest = tid / st, t = rest % r, p = rest / r;
+      int* e = tab + (stage * n + tid) * 4;
+      e[0] = qq + st * p;
+      e[1] = st * m;
+      e[2] = (t * (n / r)) % n;
+      e[3] = (p * t * (n / len)) % n;
+      len = m;
+      st *= r;
+      stage++;
+    }
+  }
+  int len = n, stage = 0;
+  while (len > 1) {
+    int r = 2;
+    while (len % r) r++;
+    __syncthreads();
+    if (act) {
+      const int* e = tab + (stage * n + a) * 4;
+      int in = e[0], wi = 0;
+      const int istep = e[1], wstep = e[2];
+      double2 acc = make_double2(0.0, 0.0);
+      for (int u = 0; u < r; u++) {
+        const double2 v = X[in * T + b];
+        const double2 w = tw[wi];
+        acc.x = fma(v.x, w.x, acc.x);
+        acc.x = fma(-v.y, w.y, acc.x);
+        acc.y = fma(v.x, w.y, acc.y);
+        acc.y = fma(v.y, w.x, acc.y);
+        in += istep;
+        wi += wstep;
+        wi -= (wi >= n) ? n : 0;
+      }
+      Y[a * T + b] = nspt::c_mul(acc, tw[e[3]]);
+    }
+    double2* sw = X;
+    X = Y;
+    Y = sw;
+    len /= r;
+    stage++;
   }
   __syncthreads();
   if (!act) return;
-  double2 acc = make_double2(0.0, 0.0);
-  int ph = 0;
-  for (int j = 0; j < n; j++) {
-    const double2 v = tile[j * T + b];
-    const double2 w = tw[ph];
-    acc.x = fma(v.x, w.x, acc.x);
-    acc.x = fma(-v.y, w.y, acc.x);
-    acc.y = fma(v.x, w.y, acc.y);
-    acc.y = fma(v.y, w.x, acc.y);
-    ph += a;
-    ph -= (ph >= n) ? n : 0;
-  }
+  double2 acc = X[a * T + b];
   if (scale) {
     double p2 = 0.0;
     for (int mu = 0; mu < 4; mu++) {
@@ -463,7 +505,13 @@ bool fa_apply(nspt_ctx* c, const SerField& Xsrc, double alpha) {
     const int n = c->g.L[d], T = c->g.L[d == 3 ? 2 : 3];
     const long tiles = (long)c->V / ((long)n * T) * ncomp;
     const int block = ((n * T + 63) / 64) * 64;
-    const size_t shm = sizeof(double2) * (n * T + n);
+    int nstage = 0;
+    for (int len = n; len > 1; nstage++) {
+      int r = 2;
+      while (len % r) r++;
+      len /= r;
+    }
+    const size_t shm = sizeof(double2) * (2 * n * T + n) + sizeof(int) * 4 * (size_t)nstage * n;
     Prof pr(c, NSPT_PROF_DFT);
     hipLaunchKernelGGL(k_dft, dim3((unsigned)tiles), dim3(block), shm, c->stream, c->g, c->fb, d, sign,
                        pass == 3 ? 1 : 0, alpha);

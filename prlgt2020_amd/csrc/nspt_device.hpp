@@ -198,13 +198,16 @@ __device__ __forceinline__ double2 mul(const Lane<NC, N>& ln, double2 a0, double
   auto load = [&](int a, int b) {
     const double2* ra = pa + a * NE;
     const double2* cb = pb + b * NE;
+    // columns of B first, then the rows of A in the order they are used,
+    // so that the first row's products wait for fewer reads
 #pragma unroll
     for (int d = 0; d < Q; d++)
 #pragma unroll
-      for (int m = 0; m < NC; m++) {
-        TA[d][m] = ra[d * NE + m];
-        TB[d][m] = cb[d * NE + m * NC];
-      }
+      for (int m = 0; m < NC; m++) TB[d][m] = cb[d * NE + m * NC];
+#pragma unroll
+    for (int d = 0; d < Q; d++)
+#pragma unroll
+      for (int m = 0; m < NC; m++) TA[d][m] = ra[d * NE + m];
   };
   auto row = [&](int da) {
 #pragma unroll

@@ -167,9 +167,11 @@ def test_high_orders_within_fp64_conditioning(nc, N, relerr):
     assert (e <= bar).all(), (e, bar)
 
 
-def test_fa_alone(relerr):
+@pytest.mark.parametrize("L", [(4, 2, 4, 2), (6, 3, 2, 5), (3, 4, 5, 2), (2, 6, 2, 6)])
+def test_fa_alone(relerr, L):
+    """FA gauge fixing (the 4-D FFT passes of every radix: extents 2..6)."""
     nspt = _nspt()
-    L, nc, N = (4, 2, 4, 2), 3, 6
+    nc, N = 3, 6
     U = O.random_field(L, nc, N, seed=21)
     ctx = nspt.Context(L, nc, N)
     ctx.upload(U)
