@@ -182,136 +182,378 @@ __global__ void k_obs_finalize(const double* obs_mu, int nrt, int nc, int V, dou
   }
 }
 
-// W(x) = sum_mu X_mu(x) - X_mu(x-mu) per component (nspt.cc:871-877);
-// fb[comp*V + site].
-__global__ void k_fa_div(Geo g, int ncomp, const double2* X, double2* fb) {
-  const long V = g.V;
-  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < ncomp * V; t += (long)gridDim.x * blockDim.x) {
-    const long comp = t / V;
-    const int site = (int)(t % V);
-    const double2* xc = X + comp * 4 * V;
-    double2 w = make_double2(0.0, 0.0);
-    for (int mu = 0; mu < 4; mu++) {
-      const int x = (site / g.st[mu]) % g.L[mu];
-      const int xm = (x == 0) ? site + (g.L[mu] - 1) * g.st[mu] : site - g.st[mu];
-      const double2 a = xc[mu * V + site], b = xc[mu * V + xm];
-      w = nspt::c_add(w, nspt::c_sub(a, b));
-    }
-    fb[t] = w;
+// ------------------------------------------------------------ FA transform
+// The unnormalised 4-D DFT of the divergence of X (FFTW's sign convention,
+// Quenched.cc:827-838), the momentum-space factor and the inverse DFT
+// (nspt.cc:871-902), in three passes over fb[comp*V + site]:
+//   pass 0: W = sum_mu X_mu(x) - X_mu(x-mu), then forward FFT along 2 and 3;
+//   pass 1: forward FFT along 0 and 1, times -alpha/(V p^2(k)) with
+//           p^2 = sum_mu 4 sin^2(pi k_mu / L_mu) and the k = 0 mode zeroed,
+//           then backward FFT along 1 and 0;
+//   pass 2: backward FFT along 2 and 3.
+// A block (one wave) holds one tile of L_da x L_db x C sites in LDS, C
+// consecutive values of a third direction: direction 3 in pass 1 (sites
+// (x0, x1) x C consecutive x3), direction 1 in passes 0 and 2 (C consecutive
+// (x2, x3) planes), so every load and store is a run of consecutive sites;
+// C makes about 64 lines per axis, one per lane.  Each line is transformed by
+// one lane in registers: a mixed-radix Stockham FFT unrolled at compile time
+// for the extents in NSPT_FFT_SIZES, a direct DFT read from LDS for any
+// other extent.
+constexpr int FFT_THREADS = 64;
+constexpr int FFT_MAXE = 2048;  // tile sites: L_da * L_db * C
+constexpr int FFT_MAXN = 32;    // lattice extents are at most 32
+
+#define NSPT_FFT_SIZES(X) X(2) X(3) X(4) X(5) X(6) X(8) X(10) X(12) X(16) X(20) X(24) X(32)
+
+__host__ __device__ constexpr int smallest_factor(int n) {
+  int r = 2;
+  while (n % r) r++;
+  return r;
+}
+
+__host__ __device__ constexpr int fft_nstages(int n) {
+  int s = 0;
+  for (int len = n; len > 1; s++) len /= smallest_factor(len);
+  return s;
+}
+
+__host__ inline bool fft_unrolled(int n, int maxn) {
+  bool hit = false;
+#define NSPT_FFT_HIT(N) hit = hit || (n == N && N <= maxn);
+  NSPT_FFT_SIZES(NSPT_FFT_HIT)
+#undef NSPT_FFT_HIT
+  return hit;
+}
+
+// acc + v w^k with w = exp(-2 pi i / n) (FWD) or its conjugate; w holds the
+// forward twiddles.  k is known at compile time once unrolled, so the
+// trivial twiddles 1, -1 and -+i cost no multiplication.
+template <int n, bool FWD>
+__device__ __forceinline__ double2 tw_mac(double2 acc, double2 v, const double2* w, int k) {
+  if (k == 0) return nspt::c_add(acc, v);
+  if (2 * k == n) return nspt::c_sub(acc, v);
+  if (4 * k == n || 4 * k == 3 * n) {
+    // w^k = -i (forward, k = n/4) or +i; the sign flips for 3n/4 and backward
+    const bool minus_i = (4 * k == n) == FWD;
+    return minus_i ? make_double2(acc.x + v.y, acc.y - v.x) : make_double2(acc.x - v.y, acc.y + v.x);
+  }
+  const double tx = w[k].x, ty = FWD ? w[k].y : -w[k].y;
+  return make_double2(fma(v.x, tx, fma(-v.y, ty, acc.x)), fma(v.x, ty, fma(v.y, tx, acc.y)));
+}
+
+// One Stockham stage (length len = r*m, stride st) from x into y: output
+// q + st*(r*p + t) is the r-point DFT of x[q + st*(p + u*m)], u = 0..r-1,
+// times w_len^{p t}; then the remaining stages from y into x.
+template <int n, int len, int st, bool FWD>
+__device__ __forceinline__ void fft_stages(double2 (&x)[n], double2 (&y)[n], const double2* w) {
+  if constexpr (len > 1) {
+    constexpr int r = smallest_factor(len), m = len / r;
+#pragma unroll
+    for (int q = 0; q < st; q++)
+#pragma unroll
+      for (int p = 0; p < m; p++)
+#pragma unroll
+        for (int t = 0; t < r; t++) {
+          double2 acc = x[q + st * p];
+#pragma unroll
+          for (int u = 1; u < r; u++) acc = tw_mac<n, FWD>(acc, x[q + st * (p + u * m)], w, (u * t * (n / r)) % n);
+          y[q + st * (r * p + t)] = tw_mac<n, FWD>(make_double2(0.0, 0.0), acc, w, (p * t * (n / len)) % n);
+        }
+    fft_stages<n, m, st * r, FWD>(y, x, w);
   }
 }
 
-// One pass of the unnormalised 4-D DFT along direction d (sign -1 forward,
-// +1 backward, FFTW convention of Quenched.cc:827-838) on an LDS tile of
-// L_d x T elements (T consecutive sites of the fastest other direction):
-// a mixed-radix Stockham FFT of each line, one output per thread per stage,
-// radices the prime factors of L_d (smallest first), ping-ponging between
-// two tile buffers so that the output comes out in natural order.  With
-// scale != 0 the output is multiplied by -alpha/(V * p^2(k)),
-// p^2 = sum_mu 4 sin^2(pi k_mu / L_mu), and the k = 0 mode is set to zero
-// (nspt.cc:881-894, 902).
-__global__ void k_dft(Geo g, double2* fb, int d, int sign, int scale, double alpha) {
-  extern __shared__ double2 sm[];
-  const int n = g.L[d];
-  const int bdim = (d == 3) ? 2 : 3;
-  const int T = g.L[bdim];
-  const long V = g.V;
-  const long tiles = V / ((long)n * T);
-  const long comp = blockIdx.x / tiles;
-  long tt = blockIdx.x % tiles;
-  // the two remaining directions, slower one first
-  int f[2], q = 0;
-  for (int dd = 0; dd < 4; dd++)
-    if (dd != d && dd != bdim) f[q++] = dd;
-  const int x1 = (int)(tt % g.L[f[1]]);
-  const int x0 = (int)(tt / g.L[f[1]]);
-  const long base = (long)x0 * g.st[f[0]] + (long)x1 * g.st[f[1]];
-  const int tid = threadIdx.x;
-  double2* X = sm;
-  double2* Y = sm + n * T;
-  double2* tw = sm + 2 * n * T;  // tw[k] = exp(sign 2 pi i k / n)
-  // per stage and output index: first input, input stride, twiddle step of
-  // the r-point DFT and the output twiddle, built once per block
-  int* tab = reinterpret_cast<int*>(tw + n);  // [stage][n][4]
-  int a = 0, b = 0;
-  const bool act = tid < n * T;
-  if (act) {
-    if (d < 3) {
-      a = tid / T;
-      b = tid % T;
-    } else {
-      a = tid % n;
-      b = tid / n;
-    }
+// w: the n forward twiddles in LDS (every lane reads the same entry)
+template <int n, bool FWD>
+__device__ __forceinline__ void fft_line(double2 (&x)[n], const double2* w) {
+  double2 y[n];
+  fft_stages<n, n, 1, FWD>(x, y, w);
+  if constexpr (fft_nstages(n) % 2 == 1) {
+#pragma unroll
+    for (int i = 0; i < n; i++) x[i] = y[i];
   }
-  const long site = base + (long)a * g.st[d] + (long)b * g.st[bdim];
-  double2* line = fb + comp * V;
-  if (act) X[a * T + b] = line[site];
-  if (tid < n) {
-    double s, c;
-    sincospi(2.0 * (double)tid / (double)n, &s, &c);
-    tw[tid] = make_double2(c, sign * s);
-    // stages: length len = r * m, stride st; output o = q + st*(r*p + t) is
-    // the r-point DFT of x[q + st*(p + u*m)], u = 0..r-1, times w_len^{p t}
-    int len = n, st = 1, stage = 0;
-    while (len > 1) {
-      int r = 2;
-      while (len % r) r++;
-      const int m = len / r;
-      const int qq = tid % st, rest = tid / st, t = rest % r, p = rest / r;
-      int* e = tab + (stage * n + tid) * 4;
-      e[0] = qq + st * p;
-      e[1] = st * m;
-      e[2] = (t * (n / r)) % n;
-      e[3] = (p * t * (n / len)) % n;
-      len = m;
-      st *= r;
-      stage++;
-    }
+}
+
+// Per-block tables in LDS: forward twiddles of both axes, 4 sin^2(pi k/L) of
+// every direction, the tile and (direct DFT only) a second tile buffer.
+struct FftShared {
+  double2* twa;  // exp(-2 pi i k / L_da)
+  double2* twb;
+  double* s2[4];
+  double2* tile;
+  double2* tmp;
+};
+
+// Line transform along one tile axis: nl = nh * C lines, line t = (hi, ic)
+// (ic = t % C when c_inner, else hi = t % nh) starting at LDS offset
+// hi * pitch_hi + ic * pitch_c, element stride ls.  mode 0 forward, 1
+// backward, 2 forward, the momentum factor, backward (axis b of pass 1,
+// where hi = k0 and the element index is k1).
+struct FftAxis {
+  int n, nh, C, ls, pitch_hi, pitch_c;
+  bool c_inner;
+};
+
+__device__ __forceinline__ void line_of(const FftAxis& ax, int t, int& hi, int& ic) {
+  if (ax.c_inner) {
+    hi = t / ax.C;
+    ic = t % ax.C;
+  } else {
+    hi = t % ax.nh;
+    ic = t / ax.nh;
   }
-  int len = n, stage = 0;
-  while (len > 1) {
-    int r = 2;
-    while (len % r) r++;
-    __syncthreads();
-    if (act) {
-      const int* e = tab + (stage * n + a) * 4;
-      int in = e[0], wi = 0;
-      const int istep = e[1], wstep = e[2];
+}
+
+struct FftScale {
+  double alpha, invV;
+  int k2, k3base;  // momentum along 2 and the first momentum along 3 of the tile (pass 1)
+};
+
+__device__ __forceinline__ double fa_factor(const FftShared& sh, const FftScale& sc, int k0, int k1, int k3off) {
+  const int k3 = sc.k3base + k3off;
+  const double p2 = sh.s2[0][k0] + sh.s2[1][k1] + sh.s2[2][sc.k2] + sh.s2[3][k3];
+  return (k0 == 0 && k1 == 0 && sc.k2 == 0 && k3 == 0) ? 0.0 : -sc.alpha * sc.invV / p2;
+}
+
+template <int n>
+__device__ void axis_fft_fixed(const FftShared& sh, const double2* tw, const FftAxis& ax, int mode,
+                               const FftScale& sc) {
+  for (int t = threadIdx.x; t < ax.nh * ax.C; t += blockDim.x) {
+    int hi, ic;
+    line_of(ax, t, hi, ic);
+    double2* base = sh.tile + hi * ax.pitch_hi + ic * ax.pitch_c;
+    double2 x[n];
+#pragma unroll
+    for (int j = 0; j < n; j++) x[j] = base[j * ax.ls];
+    if (mode == 1)
+      fft_line<n, false>(x, tw);
+    else
+      fft_line<n, true>(x, tw);
+    if (mode == 2) {
+#pragma unroll
+      for (int j = 0; j < n; j++) x[j] = nspt::c_scale(fa_factor(sh, sc, hi, j, ic), x[j]);
+      fft_line<n, false>(x, tw);
+    }
+#pragma unroll
+    for (int j = 0; j < n; j++) base[j * ax.ls] = x[j];
+  }
+}
+
+// Direct DFT of every line for extents without an unrolled FFT: each output
+// read from the tile, written to the second buffer, copied back.
+__device__ void axis_dft_direct(const FftShared& sh, const double2* tw, const FftAxis& ax, int mode,
+                                const FftScale& sc) {
+  const int n = ax.n;
+  const int passes = mode == 2 ? 2 : 1;
+  for (int ps = 0; ps < passes; ps++) {
+    const double cj = (mode == 1 || ps == 1) ? -1.0 : 1.0;
+    for (int o = threadIdx.x; o < ax.nh * ax.C * n; o += blockDim.x) {
+      int hi, ic;
+      line_of(ax, o / n, hi, ic);
+      const int k = o % n;
+      const int off = hi * ax.pitch_hi + ic * ax.pitch_c;
       double2 acc = make_double2(0.0, 0.0);
-      for (int u = 0; u < r; u++) {
-        const double2 v = X[in * T + b];
+      int wi = 0;
+      for (int j = 0; j < n; j++) {
+        const double2 v = sh.tile[off + j * ax.ls];
         const double2 w = tw[wi];
-        acc.x = fma(v.x, w.x, acc.x);
-        acc.x = fma(-v.y, w.y, acc.x);
-        acc.y = fma(v.x, w.y, acc.y);
-        acc.y = fma(v.y, w.x, acc.y);
-        in += istep;
-        wi += wstep;
+        const double wy = cj * w.y;
+        acc.x = fma(v.x, w.x, fma(-v.y, wy, acc.x));
+        acc.y = fma(v.x, wy, fma(v.y, w.x, acc.y));
+        wi += k;
         wi -= (wi >= n) ? n : 0;
       }
-      Y[a * T + b] = nspt::c_mul(acc, tw[e[3]]);
+      if (mode == 2 && ps == 0) acc = nspt::c_scale(fa_factor(sh, sc, hi, k, ic), acc);
+      sh.tmp[off + k * ax.ls] = acc;
     }
-    double2* sw = X;
-    X = Y;
-    Y = sw;
-    len /= r;
-    stage++;
+    __syncthreads();
+    for (int o = threadIdx.x; o < ax.nh * ax.C * n; o += blockDim.x) {
+      int hi, ic;
+      line_of(ax, o / n, hi, ic);
+      const int off = hi * ax.pitch_hi + ic * ax.pitch_c + (o % n) * ax.ls;
+      sh.tile[off] = sh.tmp[off];
+    }
+    __syncthreads();
+  }
+}
+
+// MAXN bounds the unrolled sizes of one kernel variant, and so its registers.
+template <int MAXN>
+__device__ void axis_transform(const FftShared& sh, const double2* tw, const FftAxis& ax, int mode,
+                               const FftScale& sc) {
+  switch (ax.n) {
+#define NSPT_FFT_CASE(N)                                  \
+  case N:                                                 \
+    if constexpr (N <= MAXN) {                            \
+      axis_fft_fixed<N>(sh, tw, ax, mode, sc);            \
+      break;                                              \
+    }                                                     \
+    [[fallthrough]];
+    NSPT_FFT_SIZES(NSPT_FFT_CASE)
+#undef NSPT_FFT_CASE
+    default:
+      axis_dft_direct(sh, tw, ax, mode, sc);
   }
   __syncthreads();
-  if (!act) return;
-  double2 acc = X[a * T + b];
-  if (scale) {
-    double p2 = 0.0;
-    for (int mu = 0; mu < 4; mu++) {
-      const int k = (int)((site / g.st[mu]) % g.L[mu]);
-      const double h = sinpi((double)k / (double)g.L[mu]);
-      p2 += 4.0 * h * h;
-    }
-    const double fac = (site == 0) ? 0.0 : -alpha / (p2 * (double)V);
-    acc = nspt::c_scale(fac, acc);
+}
+
+// Tile geometry of one pass.  Pass 1: LDS offset ia*ra + ib*C + ic with
+// ra = L_1*C + 1; passes 0, 2: ic*cs + ia*ra + ib with ra = L_3 + 1,
+// cs = L_2*ra.  The extra site per row spreads the rows over the LDS banks.
+struct FftTile {
+  int da, db, na, nb, C, E, tiles, ra, cs;
+};
+
+// C: the largest divisor of the batch direction's extent that keeps the
+// tile within FFT_MAXE sites and the lines per axis within one per lane.
+__host__ __device__ inline FftTile fft_tile(const Geo& g, int pass) {
+  FftTile t;
+  t.da = pass == 1 ? 0 : 2;
+  t.db = pass == 1 ? 1 : 3;
+  t.na = g.L[t.da];
+  t.nb = g.L[t.db];
+  const int Lc = g.L[pass == 1 ? 3 : 1];
+  const int mn = t.na < t.nb ? t.na : t.nb;
+  t.C = 1;
+  for (int c = 2; c <= Lc; c++)
+    if (Lc % c == 0 && t.na * t.nb * c <= FFT_MAXE && mn * c <= FFT_THREADS) t.C = c;
+  t.E = t.na * t.nb * t.C;
+  t.tiles = g.V / t.E;
+  if (pass == 1) {
+    t.ra = t.nb * t.C + 1;
+    t.cs = 1;
+  } else {
+    t.ra = t.nb + 1;
+    t.cs = t.na * t.ra;
   }
-  line[site] = acc;
+  return t;
+}
+
+__host__ __device__ inline int fft_tile_words(const FftTile& t) {  // double2 per tile buffer
+  return t.C * t.na * t.ra;
+}
+
+__host__ inline size_t fft_shm(const FftTile& t, const Geo& g, bool direct) {
+  const int nsin = (g.L[0] + g.L[1] + g.L[2] + g.L[3] + 1) / 2;  // in double2 units
+  return sizeof(double2) * ((size_t)t.na + t.nb + nsin + (direct ? 2 : 1) * (size_t)fft_tile_words(t));
+}
+
+template <int MAXN>
+__global__ __launch_bounds__(FFT_THREADS) void k_fa_fft(Geo g, int pass, const double2* Xsrc, double2* fb,
+                                                       double alpha) {
+  extern __shared__ double2 sm[];
+  const FftTile t = fft_tile(g, pass);
+  const long V = g.V;
+  const long comp = blockIdx.x / t.tiles;
+  const int tt = (int)(blockIdx.x % t.tiles);
+  const int L0 = g.L[0], L1 = g.L[1], L2 = g.L[2], L3 = g.L[3];
+  // tile origin: (x2, first x3) in pass 1, (x0, first x1) in passes 0 and 2
+  const int nch = (pass == 1 ? L3 : L1) / t.C;
+  const int o0 = tt / nch;
+  const int o1 = (tt % nch) * t.C;
+  const long base = pass == 1 ? (long)o0 * g.st[2] + o1 : (long)o0 * g.st[0] + (long)o1 * g.st[1];
+  FftShared sh;
+  sh.twa = sm;
+  sh.twb = sm + t.na;
+  double* s2 = reinterpret_cast<double*>(sh.twb + t.nb);
+  sh.s2[0] = s2;
+  sh.s2[1] = s2 + L0;
+  sh.s2[2] = s2 + L0 + L1;
+  sh.s2[3] = s2 + L0 + L1 + L2;
+  sh.tile = sh.twb + t.nb + (L0 + L1 + L2 + L3 + 1) / 2;
+  sh.tmp = sh.tile + fft_tile_words(t);
+  for (int k = threadIdx.x; k < t.na + t.nb; k += blockDim.x) {
+    const bool on_a = k < t.na;
+    const int n = on_a ? t.na : t.nb;
+    const int i = on_a ? k : k - t.na;
+    double sn, cs;
+    sincospi(2.0 * (double)i / (double)n, &sn, &cs);
+    sh.twa[k] = make_double2(cs, -sn);  // twb follows twa
+  }
+  if (pass == 1) {
+    // s2 holds 4 sin^2(pi i / L_mu) for mu = 0..3 back to back
+    for (int k = threadIdx.x; k < L0 + L1 + L2 + L3; k += blockDim.x) {
+      int i = k, n = L0;
+      if (i >= n) {
+        i -= n;
+        n = L1;
+        if (i >= n) {
+          i -= n;
+          n = L2;
+          if (i >= n) {
+            i -= n;
+            n = L3;
+          }
+        }
+      }
+      const double h = sinpi((double)i / (double)n);
+      s2[k] = 4.0 * h * h;
+    }
+  }
+
+  double2* line = fb + comp * V;
+  const double2* xc = Xsrc + comp * 4 * V;
+  for (int l = threadIdx.x; l < t.E; l += FFT_THREADS) {
+    int ia, ib, ic, site, off;
+    if (pass == 1) {
+      ic = l % t.C;
+      ib = (l / t.C) % t.nb;
+      ia = l / (t.C * t.nb);
+      site = (int)(base + (long)ia * g.st[0] + (long)ib * g.st[1] + ic);
+      off = ia * t.ra + ib * t.C + ic;
+    } else {
+      ib = l % t.nb;
+      ia = (l / t.nb) % t.na;
+      ic = l / (t.na * t.nb);
+      site = (int)(base + (long)ic * g.st[1] + (long)ia * g.st[2] + ib);
+      off = ic * t.cs + ia * t.ra + ib;
+    }
+    double2 w;
+    if (pass == 0) {
+      // divergence at x = (o0, o1 + ic, ia, ib)
+      const int x[4] = {o0, o1 + ic, ia, ib};
+      w = make_double2(0.0, 0.0);
+#pragma unroll
+      for (int mu = 0; mu < 4; mu++) {
+        const int xm = (x[mu] == 0) ? site + (g.L[mu] - 1) * g.st[mu] : site - g.st[mu];
+        w = nspt::c_add(w, nspt::c_sub(xc[mu * V + site], xc[mu * V + xm]));
+      }
+    } else {
+      w = line[site];
+    }
+    sh.tile[off] = w;
+  }
+  __syncthreads();
+  FftAxis axa, axb;  // lines (ib, ic) along a, (ia, ic) along b
+  if (pass == 1) {
+    axa = FftAxis{t.na, t.nb, t.C, t.ra, t.C, 1, true};
+    axb = FftAxis{t.nb, t.na, t.C, t.C, t.ra, 1, true};
+  } else {
+    axa = FftAxis{t.na, t.nb, t.C, t.ra, 1, t.cs, false};
+    axb = FftAxis{t.nb, t.na, t.C, 1, t.ra, t.cs, false};
+  }
+  const FftScale sc{alpha, 1.0 / (double)V, o0, o1};
+  if (pass == 2) {
+    axis_transform<MAXN>(sh, sh.twa, axa, 1, sc);
+    axis_transform<MAXN>(sh, sh.twb, axb, 1, sc);
+  } else {
+    axis_transform<MAXN>(sh, sh.twa, axa, 0, sc);
+    axis_transform<MAXN>(sh, sh.twb, axb, pass == 1 ? 2 : 0, sc);
+    if (pass == 1) axis_transform<MAXN>(sh, sh.twa, axa, 1, sc);
+  }
+  for (int l = threadIdx.x; l < t.E; l += FFT_THREADS) {
+    int site, off;
+    if (pass == 1) {
+      const int ic = l % t.C, ib = (l / t.C) % t.nb, ia = l / (t.C * t.nb);
+      site = (int)(base + (long)ia * g.st[0] + (long)ib * g.st[1] + ic);
+      off = ia * t.ra + ib * t.C + ic;
+    } else {
+      const int ib = l % t.nb, ia = (l / t.nb) % t.na, ic = l / (t.na * t.nb);
+      site = (int)(base + (long)ic * g.st[1] + (long)ia * g.st[2] + ib);
+      off = ic * t.cs + ia * t.ra + ib;
+    }
+    line[site] = sh.tile[off];
+  }
 }
 
 // Smallest compiled N >= n; among those the tile edge qt if compiled, else
@@ -492,30 +734,20 @@ bool obs_reduce(nspt_ctx* c, double* out_dev) {
 
 bool fa_apply(nspt_ctx* c, const SerField& Xsrc, double alpha) {
   const long ncomp = (long)c->nrt * c->NE;
-  {
-    Prof pr(c, NSPT_PROF_FA_DIV);
-    hipLaunchKernelGGL(k_fa_div, dim3(grid_for(ncomp * c->V, 256)), dim3(256), 0, c->stream, c->g, (int)ncomp,
-                     (const double2*)Xsrc.data, c->fb);
-  }
-  if (!launch_ok(c, "fa_div")) return false;
-  const int order_fwd[4] = {3, 2, 1, 0};
-  for (int pass = 0; pass < 8; pass++) {
-    const int d = pass < 4 ? order_fwd[pass] : order_fwd[7 - pass];
-    const int sign = pass < 4 ? -1 : +1;
-    const int n = c->g.L[d], T = c->g.L[d == 3 ? 2 : 3];
-    const long tiles = (long)c->V / ((long)n * T) * ncomp;
-    const int block = ((n * T + 63) / 64) * 64;
-    int nstage = 0;
-    for (int len = n; len > 1; nstage++) {
-      int r = 2;
-      while (len % r) r++;
-      len /= r;
-    }
-    const size_t shm = sizeof(double2) * (2 * n * T + n) + sizeof(int) * 4 * (size_t)nstage * n;
-    Prof pr(c, NSPT_PROF_DFT);
-    hipLaunchKernelGGL(k_dft, dim3((unsigned)tiles), dim3(block), shm, c->stream, c->g, c->fb, d, sign,
-                       pass == 3 ? 1 : 0, alpha);
-    if (!launch_ok(c, "dft")) return false;
+  for (int pass = 0; pass < 3; pass++) {
+    const FftTile t = fft_tile(c->g, pass);
+    const int maxn = std::max(t.na, t.nb) <= 16 ? 16 : FFT_MAXN;
+    const bool direct = !fft_unrolled(t.na, maxn) || !fft_unrolled(t.nb, maxn);
+    const dim3 grid((unsigned)((long)t.tiles * ncomp));
+    const size_t shm = fft_shm(t, c->g, direct);
+    Prof pr(c, pass == 0 ? NSPT_PROF_FA_DIV : NSPT_PROF_DFT);
+    if (maxn == 16)
+      hipLaunchKernelGGL(k_fa_fft<16>, grid, dim3(FFT_THREADS), shm, c->stream, c->g, pass,
+                         (const double2*)Xsrc.data, c->fb, alpha);
+    else
+      hipLaunchKernelGGL(k_fa_fft<FFT_MAXN>, grid, dim3(FFT_THREADS), shm, c->stream, c->g, pass,
+                         (const double2*)Xsrc.data, c->fb, alpha);
+    if (!launch_ok(c, pass == 0 ? "fa_div" : "fa_fft")) return false;
   }
   {
     Prof pr(c, NSPT_PROF_FA_GEXP);
