@@ -141,17 +141,20 @@ __global__ void k_noise_philox(int nc, int V, unsigned long long seed, unsigned 
   }
 }
 
-// Deterministic column sums: out[chunk][c] = sum of rows
-// [row0 + chunk*rows_per, ...) of in (row stride ld), optional scale.
-__global__ void k_reduce_rows(const double* in, long ld, long row0, long nrows, int ncols, long rows_per,
-                              double* out, double scale) {
+// Deterministic column sums, one level of a tree over the four directions
+// at once (blockIdx.z = mu): rows [mu*in_mu + chunk*per, ...) of the rows
+// `rows` of direction mu (row stride ld) are summed into row
+// mu*out_mu + chunk of out (row stride out_ld), times scale.
+__global__ void k_reduce_rows(const double* in, long ld, long in_mu, long rows, int ncols, long per, double* out,
+                              long out_ld, long out_mu, double scale) {
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= ncols) return;
-  const long r0 = row0 + blockIdx.y * rows_per;
-  const long r1 = min(row0 + nrows, r0 + rows_per);
+  const long mu = blockIdx.z;
+  const long r0 = mu * in_mu + blockIdx.y * per;
+  const long r1 = min(mu * in_mu + rows, r0 + per);
   double s = 0.0;
   for (long r = r0; r < r1; r++) s += in[r * ld + col];
-  out[(long)blockIdx.y * ncols + col] = s * scale;
+  out[(mu * out_mu + blockIdx.y) * out_ld + col] = s * scale;
 }
 
 // Per-step observables from the per-mu block sums (layout per row:
@@ -708,31 +711,52 @@ void prof_collect(nspt_ctx* c) {
   c->marks.clear();
 }
 
-// deterministic reduction of rows [row0, row0+nrows) of a (rows x ncols)
-// matrix into out[ncols] (times scale)
-bool reduce(nspt_ctx* c, const double* in, long ld, long row0, long nrows, int ncols, double* out, double scale) {
+// Deterministic reduction, for each direction mu, of the rows
+// [mu*nb_mu, (mu+1)*nb_mu) of a (rows x ncols) matrix (row stride ld) into
+// out[mu*ncols + col] (times scale): a tree of launches summing REDUCE_PER
+// rows per thread, the four directions in each launch.
+constexpr long REDUCE_PER = 32;
+
+size_t reduce_tmp_doubles(long nb_mu, int ncols) {
+  return 2 * (size_t)4 * ((nb_mu + REDUCE_PER - 1) / REDUCE_PER) * ncols;
+}
+
+bool reduce4(nspt_ctx* c, const double* in, long ld, int ncols, double* out, double scale) {
   Prof pr(c, NSPT_PROF_REDUCE);
-  const long per = 256;
-  const long nch = (nrows + per - 1) / per;
-  dim3 g1((ncols + 63) / 64, (unsigned)nch);
-  hipLaunchKernelGGL(k_reduce_rows, g1, dim3(64), 0, c->stream, in, ld, row0, nrows, ncols, per, c->red_tmp, 1.0);
-  dim3 g2((ncols + 63) / 64, 1);
-  hipLaunchKernelGGL(k_reduce_rows, g2, dim3(64), 0, c->stream, (const double*)c->red_tmp, (long)ncols, 0L, nch,
-                     ncols, nch, out, scale);
+  const long nb_mu = c->g.nb_mu;
+  const size_t half = reduce_tmp_doubles(nb_mu, ncols) / 2;
+  const double* src = in;
+  long src_ld = ld, rows = nb_mu;
+  for (int level = 0;; level++) {
+    const long nch = (rows + REDUCE_PER - 1) / REDUCE_PER;
+    const dim3 grid((ncols + 63) / 64, (unsigned)nch, 4);
+    if (nch == 1) {
+      hipLaunchKernelGGL(k_reduce_rows, grid, dim3(64), 0, c->stream, src, src_ld, rows, rows, ncols, REDUCE_PER, out,
+                         (long)ncols, 1L, scale);
+      break;
+    }
+    double* dst = c->red_tmp + (level % 2) * half;
+    hipLaunchKernelGGL(k_reduce_rows, grid, dim3(64), 0, c->stream, src, src_ld, rows, rows, ncols, REDUCE_PER, dst,
+                       (long)ncols, nch, 1.0);
+    src = dst;
+    src_ld = ncols;
+    rows = nch;
+  }
   return launch_ok(c, "reduce");
 }
 
 bool obs_reduce(nspt_ctx* c, double* out_dev) {
-  for (int mu = 0; mu < 4; mu++)
-    if (!reduce(c, c->obs_part, c->nob, (long)mu * c->g.nb_mu, c->g.nb_mu, c->nob, c->obs_mu + mu * c->nob, 1.0))
-      return false;
+  if (!reduce4(c, c->obs_part, c->nob, c->nob, c->obs_mu, 1.0)) return false;
   Prof pr(c, NSPT_PROF_REDUCE);
   hipLaunchKernelGGL(k_obs_finalize, dim3(1), dim3(256), 0, c->stream, (const double*)c->obs_mu, c->nrt, c->nc, c->V,
                      out_dev, c->graph ? (const unsigned long long*)(c->ctr_dev + 1) : nullptr, c->obs_len);
   return launch_ok(c, "obs_finalize");
 }
 
-bool fa_apply(nspt_ctx* c, const SerField& Xsrc, double alpha) {
+// FA gauge fixing of U with Xsrc = log U up to a constant per direction.
+// u_from_log: U is not in c->U but is exp(Xsrc - MM) (MM may be null),
+// formed by the transform.
+bool fa_apply(nspt_ctx* c, const SerField& Xsrc, double alpha, const double2* MM, bool u_from_log) {
   const long ncomp = (long)c->nrt * c->NE;
   for (int pass = 0; pass < 3; pass++) {
     const FftTile t = fft_tile(c->g, pass);
@@ -755,7 +779,8 @@ bool fa_apply(nspt_ctx* c, const SerField& Xsrc, double alpha) {
   }
   {
     Prof pr(c, NSPT_PROF_TRANSFORM);
-    c->ks.transform(c->nblk_links, c->stream, c->g, c->G, c->U);
+    c->ks.transform(c->nblk_links, c->stream, c->g, c->G, c->U, u_from_log ? Xsrc : SerField{nullptr, nullptr, 0},
+                    MM);
   }
   return launch_ok(c, "fa gexp/transform");
 }
@@ -820,18 +845,21 @@ bool do_step(nspt_ctx* c, const nspt_step_params* p, double* obs_dev) {
   const SerField* xsrc = &c->L;
   if (p->zero_modes) {
     const int ncol = 2 * c->nrt * c->NE;
-    for (int mu = 0; mu < 4; mu++)
-      if (!reduce(c, reinterpret_cast<const double*>(c->mm_part), ncol, (long)mu * c->g.nb_mu, c->g.nb_mu, ncol,
-                  reinterpret_cast<double*>(c->MM) + (long)mu * ncol, 1.0 / (double)c->V))
-        return false;
-    {
+    if (!reduce4(c, reinterpret_cast<const double*>(c->mm_part), ncol, ncol, reinterpret_cast<double*>(c->MM),
+                 1.0 / (double)c->V))
+      return false;
+    if (p->gauge_fix != NSPT_GF_FA) {
       Prof pr(c, NSPT_PROF_ZERO_MODE);
       c->ks.zero_mode(c->nblk_links, c->stream, c->g, c->L, c->MM, c->U, c->X);
+      return launch_ok(c, "zero_mode");
     }
-    if (!launch_ok(c, "zero_mode")) return false;
-    xsrc = &c->X;
+    // FA follows: the divergence of L - MM is that of L, and the gauge
+    // transform forms U = exp(L - MM) itself
   }
-  if (p->gauge_fix == NSPT_GF_FA && !fa_apply(c, *xsrc, p->alpha)) return false;
+  if (p->gauge_fix == NSPT_GF_FA &&
+      !fa_apply(c, *xsrc, p->alpha, p->zero_modes ? reinterpret_cast<const double2*>(c->MM) : nullptr,
+                p->zero_modes != 0))
+    return false;
   return true;
 }
 
@@ -895,7 +923,7 @@ int nspt_create(const nspt_geom* geom, nspt_ctx** out) {
             c->alloc(&c->mm_part, (size_t)c->nblk_links * c->nrt * c->NE) &&
             c->alloc(&c->MM, (size_t)4 * c->nrt * c->NE) && c->alloc(&c->obs_part, (size_t)c->nblk_links * c->nob) &&
             c->alloc(&c->obs_mu, (size_t)4 * c->nob) &&
-            c->alloc(&c->red_tmp, (size_t)(c->nblk_links / 256 + 2) * 2 * c->nrt * c->NE + 4096) &&
+            c->alloc(&c->red_tmp, reduce_tmp_doubles(c->g.nb_mu, std::max(c->nob, 2 * c->nrt * c->NE))) &&
             c->alloc(&c->aos_buf, (size_t)P * (1 + (size_t)c->NE * c->A)) && c->alloc(&c->ctr_dev, 2) &&
             ensure_obs(c, 1);
   if (!ok) return -3;
@@ -1019,7 +1047,7 @@ int nspt_plaquette(nspt_ctx* c, double* out) {
 int nspt_fa_gauge_fixing(nspt_ctx* c, double alpha) {
   if (!c) return -1;
   c->ks.logf(c->nblk_links, c->stream, c->g, c->U, c->L);
-  if (!launch_ok(c, "log") || !fa_apply(c, c->L, alpha)) return -3;
+  if (!launch_ok(c, "log") || !fa_apply(c, c->L, alpha, nullptr, false)) return -3;
   return c->hip_ok(hipStreamSynchronize(c->stream), "sync") ? 0 : -3;
 }
 

@@ -70,11 +70,15 @@ __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirs
 // padding after the last slot: a tile reads up to QT-2 orders past the end
 // of a slot (into the next slot or the padding), always for products above
 // the truncation bound, whose results are never used.
+#ifndef NSPT_SLOT_PAD
+#define NSPT_SLOT_PAD 0
+#endif
+
 template <int NC, int N>
 struct Lane {
   static constexpr int NE = NC * NC;
-  static constexpr int LPB = 64 / NE;  // link slots per wave
-  static constexpr int SLOT = N * NE;  // LDS complex per slot
+  static constexpr int LPB = 64 / NE;                  // link slots per wave
+  static constexpr int SLOT = N * NE + NSPT_SLOT_PAD;  // LDS complex per slot (pitch)
   static constexpr int PAD = (NSPT_QT > 2 ? NSPT_QT - 2 : 0) * NE;
   static constexpr int LDS = 2 * LPB * SLOT + PAD;
   int j, e, r, c;

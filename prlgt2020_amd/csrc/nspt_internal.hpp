@@ -27,7 +27,8 @@ struct KernelSet {
   void (*logf)(int nblk, hipStream_t, Geo, SerField U, SerField L) = nullptr;
   void (*zero_mode)(int nblk, hipStream_t, Geo, SerField L, const double2* MM, SerField U, SerField X) = nullptr;
   void (*gexp)(int nblk, hipStream_t, Geo, const double2* fb, SerField G) = nullptr;
-  void (*transform)(int nblk, hipStream_t, Geo, SerField G, SerField U) = nullptr;
+  void (*transform)(int nblk, hipStream_t, Geo, SerField G, SerField U, SerField Xexp,
+                    const double2* MM) = nullptr;
 };
 
 // One function per compiled instance (nspt_inst.hip built once per

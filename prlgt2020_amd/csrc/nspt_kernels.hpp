@@ -87,7 +87,7 @@ template <int NC, int N>
 __device__ __forceinline__ double2 slot_sum(const double2* slotA0, int nvalid, int k, int e) {
   constexpr int NE = NC * NC;
   double2 s = zero2();
-  for (int jj = 0; jj < nvalid; jj++) s = c_add(s, slotA0[jj * N * NE + k * NE + e]);
+  for (int jj = 0; jj < nvalid; jj++) s = c_add(s, slotA0[jj * Lane<NC, N>::SLOT + k * NE + e]);
   return s;
 }
 
@@ -260,8 +260,10 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_drift(Geo g, SerField
 
 // ------------------------------------------------------------ update
 // Uout = exp(D - stau*eta at order 1) * Ubase  (nspt.cc:420-430, exp QCDpt.h:200-220);
-// optionally L = log(Uout) (QCDpt.h:176-198) and per-block sums of L for the
-// zero mode (nspt.cc:432-439).  eta: [idx*NE + e] (may be null).
+// optionally L = reH(log(Uout)) (QCDpt.h:176-198, 197-209) and per-block sums
+// of L for the zero mode (nspt.cc:432-439).  reH is linear, so the zero-mode
+// generator reH(log U - mean) of nspt.cc:638 is L - mean(L).
+// eta: [idx*NE + e] (may be null).
 template <int NC, int N>
 __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_update(Geo g, SerField D, const double2* eta,
                                                                   double stau, SerField Ubase, SerField Uout,
@@ -298,14 +300,17 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_update(Geo g, SerFiel
   gstore<NC, N>(Uout, idx, ln.e, nrt, ok, S);
   if (ok && ln.e == 0) Uout.flag[idx] = nf;
   if (!L.data) return;
-  // L = log(Unew): generator Unew - 1 is already in slot B
+  // L = reH(log(Unew)): generator Unew - 1 is already in slot B
   horner<NC, N, false>(ln, n);
-  unstage(ln, ln.slotA, S);
+#pragma unroll
+  for (int k = 0; k < N; k++) S[k] = reh_elem(ln, ln.slotA, k);
   gstore<NC, N>(L, idx, ln.e, nrt, ok, S);
   if (ok && ln.e == 0 && L.flag) L.flag[idx] = zero2();
   if (!mm_part) return;
+  ln.stage(ln.slotB, S);
+  __syncthreads();
   for (int o = threadIdx.x; o < nrt * NE; o += 64)
-    mm_part[(long)lb.part * nrt * NE + o] = slot_sum<NC, N>(lds, nvalid, o / NE, o % NE);
+    mm_part[(long)lb.part * nrt * NE + o] = slot_sum<NC, N>(lds + LN::LPB * LN::SLOT, nvalid, o / NE, o % NE);
 }
 
 // ------------------------------------------------------------ log field
@@ -332,7 +337,7 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_log(Geo g, SerField U
 
 // ------------------------------------------------------------ zero modes
 // X = reH(L - MM_mu), U = exp(X)  (nspt.cc:638-641).  MM: [(mu*nrt + k)*NE + e],
-// already multiplied by 1/V.  X is kept: it is log(U) for the FA step.
+// already multiplied by 1/V.  X is kept: it is log(U) for an FA step.
 template <int NC, int N>
 __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_zero_mode(Geo g, SerField L, const double2* MM,
                                                                      SerField U, SerField X) {
@@ -392,9 +397,13 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_fa_gexp(Geo g, const 
 }
 
 // ------------------------------------------------------------ gauge transform
-// U_mu(x) <- G(x) U_mu(x) G(x+mu)^dag, in place (nspt.cc:912-916).
+// U_mu(x) <- G(x) U_mu(x) G(x+mu)^dag, in place (nspt.cc:912-916).  With
+// Xexp.data set, U_mu(x) is not read but formed as exp(Xexp - MM_mu) (MM may
+// be null): the field after zero-mode removal (nspt.cc:638-641), from the
+// projected log L of the update and the mean MM (layout as in k_zero_mode).
 template <int NC, int N>
-__global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_gauge_transform(Geo g, SerField G, SerField U) {
+__global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_gauge_transform(Geo g, SerField G, SerField U,
+                                                                           SerField Xexp, const double2* MM) {
   using LN = Lane<NC, N>;
   __shared__ double2 lds[LN::LDS];
   LN ln(lds);
@@ -408,9 +417,27 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_gauge_transform(Geo g
   const long idx = (long)mu * g.V + site;
   const int xp = site_shift(g, site, mu, 1);
   double2 P[N];
-  stage_global(ln, ln.slotA, G, site, nrt);
-  stage_global(ln, ln.slotB, U, idx, nrt);
-  const double2 gf = gflag(G, site, ok), uf = gflag(U, idx, ok);
+  double2 uf;
+  if (Xexp.data) {
+    gload<NC, N>(Xexp, idx, ln.e, nrt, ok, P);
+    if (MM) {
+#pragma unroll
+      for (int k = 0; k < N; k++)
+        if (k < nrt) P[k] = c_sub(P[k], MM[((long)mu * nrt + k) * (NC * NC) + ln.e]);
+    }
+    ln.stage(ln.slotB, P);
+    horner<NC, N, true>(ln, M + 1);
+    unstage(ln, ln.slotA, P);
+    __syncthreads();
+    stage_global(ln, ln.slotA, G, site, nrt);
+    ln.stage(ln.slotB, P);
+    uf = one2();
+  } else {
+    stage_global(ln, ln.slotA, G, site, nrt);
+    stage_global(ln, ln.slotB, U, idx, nrt);
+    uf = gflag(U, idx, ok);
+  }
+  const double2 gf = gflag(G, site, ok);
   // G(x+mu) is loaded while T = G(x) U_mu(x) is formed in place in slot A
   gload<NC, N>(G, xp, ln.e, nrt, ok, P);
   const double2 gpf = c_conj(gflag(G, xp, ok));
@@ -451,8 +478,8 @@ KernelSet make_kernel_set() {
   s.gexp = [](int nblk, hipStream_t st, Geo g, const double2* fb, SerField G) {
     hipLaunchKernelGGL((k_fa_gexp<NC, N>), dim3(nblk), dim3(64), 0, st, g, fb, G);
   };
-  s.transform = [](int nblk, hipStream_t st, Geo g, SerField G, SerField U) {
-    hipLaunchKernelGGL((k_gauge_transform<NC, N>), dim3(nblk), dim3(64), 0, st, g, G, U);
+  s.transform = [](int nblk, hipStream_t st, Geo g, SerField G, SerField U, SerField Xexp, const double2* MM) {
+    hipLaunchKernelGGL((k_gauge_transform<NC, N>), dim3(nblk), dim3(64), 0, st, g, G, U, Xexp, MM);
   };
   return s;
 }
