@@ -167,7 +167,10 @@ __device__ __forceinline__ void st_u(double2* base, unsigned off, double2 v) {
 // matrix stored at index 0 (no flags; a0, b0 unused): the same bands with
 // outputs shifted down by one.  It reads up to QT-1 (not QT-2) orders past a
 // slot, so its LDS needs NE more complex of padding.
-template <int NC, int N, int OFF = 1>
+//
+// ADD (with to_lds, dag false): order k is written as C[k] + scale*(A*B)_k,
+// C holding this lane's element of an addend series.
+template <int NC, int N, int OFF = 1, bool ADD = false>
 __device__ __forceinline__ double2 mul(const Lane<NC, N>& ln, double2 a0, double2 b0, bool has_b0,
                                        double2 (&C)[N], int M, bool to_lds, double2* dst, bool dag,
                                        double scale) {
@@ -189,7 +192,12 @@ __device__ __forceinline__ double2 mul(const Lane<NC, N>& ln, double2 a0, double
       if (has_b0) v = c_add(v, c_mul(b0, own_a[k * NE]));
     }
     if (to_lds) {
-      if (ln.lds_ok) dst[k * NE + pos] = make_double2(sx * v.x, sy * v.y);
+      if (ln.lds_ok) {
+        if (ADD)
+          dst[k * NE + pos] = make_double2(fma(sx, v.x, C[k].x), fma(sy, v.y, C[k].y));
+        else
+          dst[k * NE + pos] = make_double2(sx * v.x, sy * v.y);
+      }
     } else {
       C[k] = c_add(C[k], v);
     }
@@ -277,6 +285,22 @@ __device__ __forceinline__ double2 mul_lds(const Lane<NC, N>& ln, double2 a0, do
                                            double2* dst, bool dag, double scale = 1.0) {
   double2 unused[N];
   return mul<NC, N>(ln, a0, b0, has_b0, unused, M, true, dst, dag, scale);
+}
+
+// exp(X)*B by Horner from the right, T <- B + (X*T)/k for k = n..1, which
+// costs what exp(X) alone costs (each step is truncated at the order it can
+// still reach) and saves the product with B.  X (flag 0) must be staged in
+// slot A and B in slot B; bv holds this lane's element of every order of B,
+// bflag its flag.  T is formed in place in slot B, which holds exp(X)*B
+// (flag bflag) on return.
+template <int NC, int N>
+__device__ __forceinline__ void horner_exp_times(const Lane<NC, N>& ln, int n, double2 bflag, double2 (&bv)[N]) {
+  n = uniform(n);
+  for (int s = 0; s < n; s++) {
+    __syncthreads();
+    mul<NC, N, 1, true>(ln, zero2(), bflag, true, bv, s, true, ln.slotB, false, 1.0 / (double)(n - s));
+  }
+  __syncthreads();
 }
 
 // Horner evaluation of exp(X) (IS_EXP) or log(1 + X) for the generator X

@@ -282,23 +282,19 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_update(Geo g, SerFiel
   const bool ok = ln.lds_ok && (site0 + ln.j) < g.V;
   const long idx = (long)mu * g.V + site;
 
-  double2 S[N];
-  // generator W = D - stau*eta (order 1) into slot B
-  gload<NC, N>(D, idx, ln.e, nrt, ok, S);
-  if (eta) S[0] = c_sub(S[0], c_scale(stau, eta[idx * NE + ln.e]));
-  ln.stage(ln.slotB, S);
-  // Ubase is loaded while the exponential is formed
+  // generator W = D - stau*eta (order 1) into slot A, Ubase into slot B
+  double2 S[N], W[N];
+  gload<NC, N>(D, idx, ln.e, nrt, ok, W);
   gload<NC, N>(Ubase, idx, ln.e, nrt, ok, S);
   const double2 ubf = gflag(Ubase, idx, ok);
-  horner<NC, N, true>(ln, n);
-  // Unew = E * Ubase, formed in place in slot B
+  if (eta) W[0] = c_sub(W[0], c_scale(stau, eta[idx * NE + ln.e]));
+  ln.stage(ln.slotA, W);
   ln.stage(ln.slotB, S);
-  __syncthreads();
-  const double2 nf = mul_lds<NC, N>(ln, one2(), ubf, true, n - 1, ln.slotB, false);
-  __syncthreads();
+  // Unew = exp(W) * Ubase by Horner from the right, in place in slot B
+  horner_exp_times<NC, N>(ln, n, ubf, S);
   unstage(ln, ln.slotB, S);
   gstore<NC, N>(Uout, idx, ln.e, nrt, ok, S);
-  if (ok && ln.e == 0) Uout.flag[idx] = nf;
+  if (ok && ln.e == 0) Uout.flag[idx] = ubf;
   if (!L.data) return;
   // L = reH(log(Unew)): generator Unew - 1 is already in slot B
   horner<NC, N, false>(ln, n);
@@ -417,26 +413,36 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_gauge_transform(Geo g
   const long idx = (long)mu * g.V + site;
   const int xp = site_shift(g, site, mu, 1);
   double2 P[N];
-  double2 uf;
   if (Xexp.data) {
+    // T = exp(X - MM) G(x+mu)^dag by Horner from the right (the exponential
+    // is never formed alone), then U = G(x) T
+    double2 Q[N];
     gload<NC, N>(Xexp, idx, ln.e, nrt, ok, P);
+    gload<NC, N>(G, xp, ln.e, nrt, ok, Q);
+    const double2 gpf = c_conj(gflag(G, xp, ok));
     if (MM) {
 #pragma unroll
       for (int k = 0; k < N; k++)
         if (k < nrt) P[k] = c_sub(P[k], MM[((long)mu * nrt + k) * (NC * NC) + ln.e]);
     }
-    ln.stage(ln.slotB, P);
-    horner<NC, N, true>(ln, M + 1);
-    unstage(ln, ln.slotA, P);
+    ln.stage(ln.slotA, P);
+    ln.stage(ln.slotB, Q, true);
     __syncthreads();
+    unstage(ln, ln.slotB, Q);  // this lane's element of G(x+mu)^dag
+    horner_exp_times<NC, N>(ln, M + 1, gpf, Q);
     stage_global(ln, ln.slotA, G, site, nrt);
-    ln.stage(ln.slotB, P);
-    uf = one2();
-  } else {
-    stage_global(ln, ln.slotA, G, site, nrt);
-    stage_global(ln, ln.slotB, U, idx, nrt);
-    uf = gflag(U, idx, ok);
+    const double2 gf = gflag(G, site, ok);
+    __syncthreads();
+    const double2 nf = mul_lds<NC, N>(ln, gf, gpf, true, M, ln.slotB, false);
+    __syncthreads();
+    unstage(ln, ln.slotB, P);
+    gstore<NC, N>(U, idx, ln.e, nrt, ok, P);
+    if (ok && ln.e == 0) U.flag[idx] = nf;
+    return;
   }
+  stage_global(ln, ln.slotA, G, site, nrt);
+  stage_global(ln, ln.slotB, U, idx, nrt);
+  const double2 uf = gflag(U, idx, ok);
   const double2 gf = gflag(G, site, ok);
   // G(x+mu) is loaded while T = G(x) U_mu(x) is formed in place in slot A
   gload<NC, N>(G, xp, ln.e, nrt, ok, P);
