@@ -64,6 +64,14 @@ __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirs
 #ifndef NSPT_QT
 #define NSPT_QT 4
 #endif
+// The drift's products are full-length (truncated at the top order only), so
+// a larger tile pays there; the Horner steps of exp/log are mostly short
+// products, where the partial bands of a larger tile waste more than its
+// extra reuse saves.
+#ifndef NSPT_QT_DRIFT
+#define NSPT_QT_DRIFT (NSPT_QT == 4 ? 5 : NSPT_QT)
+#endif
+#define NSPT_QT_MAX (NSPT_QT_DRIFT > NSPT_QT ? NSPT_QT_DRIFT : NSPT_QT)
 
 // Lane bookkeeping for one wave = one workgroup.  LDS holds two slots per
 // link (A and B operand), each N orders x NE elements, plus QT-2 orders of
@@ -79,7 +87,7 @@ struct Lane {
   static constexpr int NE = NC * NC;
   static constexpr int LPB = 64 / NE;                  // link slots per wave
   static constexpr int SLOT = N * NE + NSPT_SLOT_PAD;  // LDS complex per slot (pitch)
-  static constexpr int PAD = (NSPT_QT > 2 ? NSPT_QT - 2 : 0) * NE;
+  static constexpr int PAD = (NSPT_QT_MAX > 2 ? NSPT_QT_MAX - 2 : 0) * NE;
   static constexpr int LDS = 2 * LPB * SLOT + PAD;
   int j, e, r, c;
   bool lds_ok;  // owns an LDS cell (false only for SU(3) lane 63)
@@ -170,12 +178,12 @@ __device__ __forceinline__ void st_u(double2* base, unsigned off, double2 v) {
 //
 // ADD (with to_lds, dag false): order k is written as C[k] + scale*(A*B)_k,
 // C holding this lane's element of an addend series.
-template <int NC, int N, int OFF = 1, bool ADD = false>
+template <int NC, int N, int OFF = 1, bool ADD = false, int Q = NSPT_QT>
 __device__ __forceinline__ double2 mul(const Lane<NC, N>& ln, double2 a0, double2 b0, bool has_b0,
                                        double2 (&C)[N], int M, bool to_lds, double2* dst, bool dag,
                                        double scale) {
   constexpr int NE = NC * NC;
-  constexpr int Q = NSPT_QT;
+  static_assert(Q <= NSPT_QT_MAX, "LDS padding covers tiles up to NSPT_QT_MAX");
   constexpr int NO = 2 * Q - 1;
   M = uniform(M);
   const double2* pa = ln.slotA + ln.r * NC;

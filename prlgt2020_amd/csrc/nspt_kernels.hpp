@@ -192,7 +192,7 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_drift(Geo g, SerField
     if (i + 1 < 13) fetch(i + 1);
     const bool to_lds = (kind == 0 || kind == 2 || kind == 4);
     double2* dst = (kind == 2) ? ln.slotA : ln.slotB;
-    const double2 f = mul<NC, N>(ln, a0, b0, true, Sig, M, to_lds, dst, kind != 4, 1.0);
+    const double2 f = mul<NC, N, 1, false, NSPT_QT_DRIFT>(ln, a0, b0, true, Sig, M, to_lds, dst, kind != 4, 1.0);
     if (kind == 0)
       t1f = f;
     else if (kind == 2)
