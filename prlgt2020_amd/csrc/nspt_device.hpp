@@ -85,7 +85,10 @@ __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirs
 template <int NC, int N>
 struct Lane {
   static constexpr int NE = NC * NC;
-  static constexpr int LPB = 64 / NE;                  // link slots per wave
+  // link slots per wave: 64 / NE, except SU(3) beyond 24 orders, where 7
+  // links' slots (65 KB at N = 32) would leave room for two waves per CU;
+  // 4 links (36 lanes, 37 KB) let four waves share it
+  static constexpr int LPB = (NC == 3 && N > 24) ? 4 : 64 / NE;
   static constexpr int SLOT = N * NE + NSPT_SLOT_PAD;  // LDS complex per slot (pitch)
   static constexpr int PAD = (NSPT_QT_MAX > 2 ? NSPT_QT_MAX - 2 : 0) * NE;
   static constexpr int LDS = 2 * LPB * SLOT + PAD;

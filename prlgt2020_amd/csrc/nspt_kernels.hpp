@@ -465,7 +465,7 @@ KernelSet make_kernel_set() {
   KernelSet s;
   s.nc = NC;
   s.nmax = N;
-  s.lpb = 64 / (NC * NC);
+  s.lpb = Lane<NC, N>::LPB;
   s.qt = NSPT_QT;
   s.drift = [](int nblk, hipStream_t st, Geo g, SerField U, SerField D, SerField D1, double tau_g, double rk2c,
                int mode, double* obs) {
