@@ -72,23 +72,41 @@ __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirs
 #define NSPT_QT_DRIFT (NSPT_QT == 4 ? 5 : NSPT_QT)
 #endif
 #define NSPT_QT_MAX (NSPT_QT_DRIFT > NSPT_QT ? NSPT_QT_DRIFT : NSPT_QT)
+#ifndef NSPT_SLOT_PAD
+#define NSPT_SLOT_PAD 0
+#endif
 
 // Lane bookkeeping for one wave = one workgroup.  LDS holds two slots per
 // link (A and B operand), each N orders x NE elements, plus QT-2 orders of
 // padding after the last slot: a tile reads up to QT-2 orders past the end
 // of a slot (into the next slot or the padding), always for products above
 // the truncation bound, whose results are never used.
-#ifndef NSPT_SLOT_PAD
-#define NSPT_SLOT_PAD 0
-#endif
+// Link slots per wave.  The kernels run one wave per SIMD (register bound),
+// so a CU holds at most four; when the slots of 64/NE links leave LDS for
+// fewer waves, fewer links per wave can pay: the choice maximises waves per
+// CU (LDS in 1 KiB granules, 160 KiB per CU, plus a flag per link) times
+// working lanes.  SU(3): 7 links up to N = 24, 4 at N = 32; SU(2): 16, but
+// 15 at N = 20 and 13 at N = 24.
+template <int NC, int N>
+constexpr int links_per_wave() {
+  constexpr int NE = NC * NC;
+  int best = 1, best_score = 0;
+  for (int l = 64 / NE; l >= 1; l--) {
+    const int bytes = (2 * l * (N * NE + NSPT_SLOT_PAD) + (NSPT_QT_MAX - 2) * NE + l) * 16;
+    const int waves = 163840 / ((bytes + 1023) / 1024 * 1024);
+    const int score = (waves < 4 ? waves : 4) * l * NE;
+    if (score > best_score) {
+      best_score = score;
+      best = l;
+    }
+  }
+  return best;
+}
 
 template <int NC, int N>
 struct Lane {
   static constexpr int NE = NC * NC;
-  // link slots per wave: 64 / NE, except SU(3) beyond 24 orders, where 7
-  // links' slots (65 KB at N = 32) would leave room for two waves per CU;
-  // 4 links (36 lanes, 37 KB) let four waves share it
-  static constexpr int LPB = (NC == 3 && N > 24) ? 4 : 64 / NE;
+  static constexpr int LPB = links_per_wave<NC, N>();
   static constexpr int SLOT = N * NE + NSPT_SLOT_PAD;  // LDS complex per slot (pitch)
   static constexpr int PAD = (NSPT_QT_MAX > 2 ? NSPT_QT_MAX - 2 : 0) * NE;
   static constexpr int LDS = 2 * LPB * SLOT + PAD;
