@@ -86,6 +86,17 @@ int nspt_evolve(nspt_ctx* ctx, nspt_step_params* p, int nsteps, double* obs);
 /* plaquette_measure (nspt.cc:1316-1406): ptord+1 complex. */
 int nspt_plaquette(nspt_ctx* ctx, double* out);
 
+/* ComputeLoops (nspt.cc:1156-1312): Tr of the sums over sites and planes
+ * mu < nu of the Wilson loops w21 (2x1 + 1x2), w22 (2x2), w42 (4x2 + 2x4)
+ * and w44 (4x4), as the reference's l21, l22, l42, l44 before its output
+ * normalisation: 4 x (ptord+1) complex, unnormalised. */
+int nspt_loops(nspt_ctx* ctx, double* out);
+
+/* U <- exp(reH(log U)) for every link: the re-unitarisation that
+ * ptGluon_fld::loadlowerorder applies after reading a configuration of
+ * fewer orders (QCDenvNODEpt_test.h:415-418). */
+int nspt_unitarize(nspt_ctx* ctx);
+
 /* Stand-alone FAstochastic_gauge_fixing of the current field
  * (nspt.cc:866-948). */
 int nspt_fa_gauge_fixing(nspt_ctx* ctx, double alpha);

@@ -16,6 +16,7 @@
 //   * FAstochastic_gauge_fixing                nspt.cc:866-948 (FFTW plan_many semantics of
 //                                              Quenched.cc:827-838 restated as a direct DFT)
 //   * plaquette_measure (OpenMP normalisation) nspt.cc:1316-1406
+//   * ComputeLoops (1x2, 2x2, 2x4, 4x4 loops)  nspt.cc:1156-1312
 //   * noise: SU3rand / SU2rand                 MyMath.cc:256-368
 //   * RAND55 + Box-Muller                      MyRand.h:81-108, MyRand.cc:73-130
 //   * ranlxd (portable path)                   ranlxd/ranlxd.cc:332-601
@@ -916,6 +917,61 @@ void plaquette_measure(const Geom& g, const double* Udata, double* out) {
   }
 }
 
+// ComputeLoops (nspt.cc:1156-1312): per plane mu < nu and site x, the sums
+//   w21 += 2x1 + 1x2 rectangles, w22 += 2x2, w42 += 4x2 + 2x4, w44 += 4x4,
+// with the reference's neighbour chain (a .. t) and product order; out gets
+// Tr of each sum (QCDpt.cc:115-122), 4 x (N+1) complex, unnormalised.
+void compute_loops(const Geom& gm, const double* Udata, double* out) {
+  Latt Z(gm.L);
+  const int nc = gm.nc, N = gm.N;
+  const int S = 1 + nc * nc * N;
+  Field F{nc, N, S, &Z, const_cast<double*>(Udata)};
+  auto U = [&](int site, int mu) { return F.get(site, mu); };
+  auto up = [&](int site, int mu) { return Z.up[site * 4 + mu]; };
+  Ser w21(nc, N), w22(nc, N), w42(nc, N), w44(nc, N);
+  for (int mu = 0; mu < 3; mu++)
+    for (int nu = mu + 1; nu < 4; nu++)
+      for (int x = 0; x < Z.V; x++) {
+        const int a = up(x, mu), b = up(a, mu), c = up(b, nu), e = up(x, nu), d = up(e, mu);
+        const int f = up(c, nu), g = up(d, nu), h = up(e, nu);
+        const Ser bl = ser_mul(U(x, mu), U(a, mu));
+        const Ser ml = ser_mul(U(h, mu), U(g, mu));
+        const Ser cd = ser_mul(U(b, nu), U(c, nu));
+        const Ser ld = ser_mul(U(x, nu), U(e, nu));
+        w22 = ser_add(w22, ser_mul(ser_mul(bl, cd), ser_dag(ser_mul(ld, ml))));
+        const Ser r21 = ser_mul(ser_mul(bl, U(b, nu)), ser_dag(ser_mul(ser_mul(U(x, nu), U(e, mu)), U(d, mu))));
+        const Ser r12 =
+            ser_mul(ser_mul(ser_mul(U(x, mu), U(a, nu)), U(d, nu)), ser_dag(ser_mul(ld, U(h, mu))));
+        w21 = ser_add(w21, ser_add(r21, r12));
+        const int i = up(h, nu), j = up(f, nu), k = up(j, nu), n = up(i, nu), l = up(n, mu);
+        const int m = up(b, mu), o = up(m, mu), p = up(o, nu), q = up(p, nu), r = up(f, mu);
+        const int s = up(q, nu), t = up(k, mu);
+        const Ser br = ser_mul(U(b, mu), U(m, mu));
+        const Ser mr = ser_mul(U(f, mu), U(r, mu));
+        const Ser tr = ser_mul(U(k, mu), U(t, mu));
+        const Ser tl = ser_mul(U(n, mu), U(l, mu));
+        const Ser lu = ser_mul(U(h, nu), U(i, nu));
+        const Ser cu = ser_mul(U(f, nu), U(j, nu));
+        const Ser ru = ser_mul(U(q, nu), U(s, nu));
+        const Ser rd = ser_mul(U(o, nu), U(p, nu));
+        const Ser in42 = ser_add(ser_mul(ser_mul(br, rd), ser_dag(ser_mul(ml, mr))),
+                                 ser_mul(ser_mul(cd, cu), ser_dag(ser_mul(lu, tl))));
+        w42 = ser_add(w42, ser_mul(ser_mul(bl, in42), ser_dag(ld)));
+        const Ser top = ser_mul(ser_mul(ser_mul(bl, br), rd), ru);
+        const Ser bot = ser_mul(ser_mul(ser_mul(ld, lu), tl), tr);
+        w44 = ser_add(w44, ser_mul(top, ser_dag(bot)));
+      }
+  const Ser* ws[4] = {&w21, &w22, &w42, &w44};
+  std::vector<cx> tt(N + 1);
+  for (int w = 0; w < 4; w++) {
+    ser_tr(*ws[w], tt.data());
+    for (int kk = 0; kk <= N; kk++) {
+      out[2 * (w * (N + 1) + kk)] = tt[kk].re;
+      out[2 * (w * (N + 1) + kk) + 1] = tt[kk].im;
+    }
+  }
+}
+
 Ser from_ptr(int nc, int N, const double* p) {
   Ser s(nc, N);
   std::memcpy(s.v.data(), p, sizeof(cx) * s.v.size());
@@ -1030,6 +1086,7 @@ void orc_sgf_default(const int* gi, double* U, double alpha) {
   sgf_default(F, alpha);
 }
 
+void orc_compute_loops(const int* gi, const double* U, double* out) { compute_loops(geom_from(gi), U, out); }
 void orc_plaquette_measure(const int* gi, const double* U, double* out) {
   plaquette_measure(geom_from(gi), U, out);
 }

@@ -178,6 +178,14 @@ def plaquette_measure(L, nc, N, U):
     return out
 
 
+def compute_loops(L, nc, N, U):
+    """Tr of the ComputeLoops sums w21, w22, w42, w44 (nspt.cc:1156-1312):
+    array (4, N+1) complex, unnormalised."""
+    out = np.zeros(4 * (N + 1), np.complex128)
+    lib().orc_compute_loops(_geom(L, nc, N), _ptr(U), _ptr(out))
+    return out.reshape(4, N + 1)
+
+
 def staple_sum(L, nc, N, U, site, mu):
     out = np.zeros(ser_len(nc, N), np.complex128)
     lib().orc_staple_sum(_geom(L, nc, N), _ptr(U), site, mu, _ptr(out))

@@ -607,6 +607,10 @@ struct nspt_ctx {
   hipStream_t stream = nullptr;
   std::vector<void*> allocs;
   SerField U{}, F{}, D{}, D1{}, L{}, X{}, G{};
+  // Wilson-loop measurement, allocated on first use: 2-link and 4-link
+  // fields, per-block rectangle sums, reduction scratch, the six sums
+  SerField L2{}, L4{};
+  double *loop_part = nullptr, *loop_tmp = nullptr, *loop_res = nullptr;
   double2* fb = nullptr;
   double2* eta = nullptr;
   double2* eta_tmp = nullptr;
@@ -711,31 +715,32 @@ void prof_collect(nspt_ctx* c) {
   c->marks.clear();
 }
 
-// Deterministic reduction, for each direction mu, of the rows
-// [mu*nb_mu, (mu+1)*nb_mu) of a (rows x ncols) matrix (row stride ld) into
-// out[mu*ncols + col] (times scale): a tree of launches summing REDUCE_PER
-// rows per thread, the four directions in each launch.
+// Deterministic reduction, for each of `groups` groups g, of the rows
+// [g*rows, (g+1)*rows) of a (groups*rows x ncols) matrix (row stride ld)
+// into out[g*ncols + col] (times scale): a tree of launches summing
+// REDUCE_PER rows per thread, all groups in each launch.  tmp holds
+// reduce_tmp_doubles(rows, groups, ncols) doubles.
 constexpr long REDUCE_PER = 32;
 
-size_t reduce_tmp_doubles(long nb_mu, int ncols) {
-  return 2 * (size_t)4 * ((nb_mu + REDUCE_PER - 1) / REDUCE_PER) * ncols;
+size_t reduce_tmp_doubles(long rows, int groups, int ncols) {
+  return 2 * (size_t)groups * ((rows + REDUCE_PER - 1) / REDUCE_PER) * ncols;
 }
 
-bool reduce4(nspt_ctx* c, const double* in, long ld, int ncols, double* out, double scale) {
+bool reduce_groups(nspt_ctx* c, const double* in, long ld, long rows, int groups, int ncols, double* out,
+                   double scale, double* tmp) {
   Prof pr(c, NSPT_PROF_REDUCE);
-  const long nb_mu = c->g.nb_mu;
-  const size_t half = reduce_tmp_doubles(nb_mu, ncols) / 2;
+  const size_t half = reduce_tmp_doubles(rows, groups, ncols) / 2;
   const double* src = in;
-  long src_ld = ld, rows = nb_mu;
+  long src_ld = ld;
   for (int level = 0;; level++) {
     const long nch = (rows + REDUCE_PER - 1) / REDUCE_PER;
-    const dim3 grid((ncols + 63) / 64, (unsigned)nch, 4);
+    const dim3 grid((ncols + 63) / 64, (unsigned)nch, (unsigned)groups);
     if (nch == 1) {
       hipLaunchKernelGGL(k_reduce_rows, grid, dim3(64), 0, c->stream, src, src_ld, rows, rows, ncols, REDUCE_PER, out,
                          (long)ncols, 1L, scale);
       break;
     }
-    double* dst = c->red_tmp + (level % 2) * half;
+    double* dst = tmp + (level % 2) * half;
     hipLaunchKernelGGL(k_reduce_rows, grid, dim3(64), 0, c->stream, src, src_ld, rows, rows, ncols, REDUCE_PER, dst,
                        (long)ncols, nch, 1.0);
     src = dst;
@@ -743,6 +748,11 @@ bool reduce4(nspt_ctx* c, const double* in, long ld, int ncols, double* out, dou
     rows = nch;
   }
   return launch_ok(c, "reduce");
+}
+
+// the four link directions of the per-block partial sums
+bool reduce4(nspt_ctx* c, const double* in, long ld, int ncols, double* out, double scale) {
+  return reduce_groups(c, in, ld, c->g.nb_mu, 4, ncols, out, scale, c->red_tmp);
 }
 
 bool obs_reduce(nspt_ctx* c, double* out_dev) {
@@ -923,7 +933,7 @@ int nspt_create(const nspt_geom* geom, nspt_ctx** out) {
             c->alloc(&c->mm_part, (size_t)c->nblk_links * c->nrt * c->NE) &&
             c->alloc(&c->MM, (size_t)4 * c->nrt * c->NE) && c->alloc(&c->obs_part, (size_t)c->nblk_links * c->nob) &&
             c->alloc(&c->obs_mu, (size_t)4 * c->nob) &&
-            c->alloc(&c->red_tmp, reduce_tmp_doubles(c->g.nb_mu, std::max(c->nob, 2 * c->nrt * c->NE))) &&
+            c->alloc(&c->red_tmp, reduce_tmp_doubles(c->g.nb_mu, 4, std::max(c->nob, 2 * c->nrt * c->NE))) &&
             c->alloc(&c->aos_buf, (size_t)P * (1 + (size_t)c->NE * c->A)) && c->alloc(&c->ctr_dev, 2) &&
             ensure_obs(c, 1);
   if (!ok) return -3;
@@ -1041,6 +1051,50 @@ int nspt_plaquette(nspt_ctx* c, double* out) {
                                 c->stream),
                  "plaq"))
     return -3;
+  return c->hip_ok(hipStreamSynchronize(c->stream), "sync") ? 0 : -3;
+}
+
+int nspt_loops(nspt_ctx* c, double* out) {
+  if (!c || !out) return -1;
+  const int nrt = c->nrt, ncols = 2 * (nrt + 1);
+  const long rows = 6L * c->g.nb_mu;  // per rectangle type: 6 planes x site blocks
+  if (!c->loop_part) {
+    const long P = 4L * c->V;
+    if (!(c->alloc_field(c->L2, P, true) && c->alloc_field(c->L4, P, true) &&
+          c->alloc(&c->loop_part, (size_t)6 * rows * ncols) &&
+          c->alloc(&c->loop_tmp, reduce_tmp_doubles(rows, 6, ncols)) && c->alloc(&c->loop_res, (size_t)6 * ncols))) {
+      c->loop_part = nullptr;
+      return -2;
+    }
+  }
+  c->ks.link_pair(c->nblk_links, c->stream, c->g, c->U, c->L2, 1);
+  c->ks.link_pair(c->nblk_links, c->stream, c->g, c->L2, c->L4, 2);
+  if (!launch_ok(c, "long links")) return -3;
+  c->ks.loops((int)(6 * rows), c->stream, c->g, c->U, c->L2, c->L4, c->loop_part);
+  if (!launch_ok(c, "loops") || !reduce_groups(c, c->loop_part, ncols, rows, 6, ncols, c->loop_res, 1.0, c->loop_tmp))
+    return -3;
+  std::vector<double> r((size_t)6 * ncols);
+  if (!c->hip_ok(hipMemcpyAsync(r.data(), c->loop_res, sizeof(double) * r.size(), hipMemcpyDeviceToHost, c->stream),
+                 "loops") ||
+      !c->hip_ok(hipStreamSynchronize(c->stream), "sync"))
+    return -3;
+  // w21 = 2x1 + 1x2, w22 = 2x2, w42 = 4x2 + 2x4, w44 = 4x4
+  const int from[4][2] = {{0, 1}, {2, -1}, {3, 4}, {5, -1}};
+  for (int w = 0; w < 4; w++)
+    for (int q = 0; q < ncols; q++)
+      out[w * ncols + q] = r[(size_t)from[w][0] * ncols + q] + (from[w][1] >= 0 ? r[(size_t)from[w][1] * ncols + q] : 0.0);
+  return 0;
+}
+
+int nspt_unitarize(nspt_ctx* c) {
+  if (!c) return -1;
+  // U <- exp(reH(log U)): the log kernel, then the zero-mode kernel with a zero mean
+  c->ks.logf(c->nblk_links, c->stream, c->g, c->U, c->L);
+  if (!launch_ok(c, "log") ||
+      !c->hip_ok(hipMemsetAsync(c->MM, 0, sizeof(double2) * 4 * c->nrt * c->NE, c->stream), "memset"))
+    return -3;
+  c->ks.zero_mode(c->nblk_links, c->stream, c->g, c->L, c->MM, c->U, c->X);
+  if (!launch_ok(c, "unitarize")) return -3;
   return c->hip_ok(hipStreamSynchronize(c->stream), "sync") ? 0 : -3;
 }
 

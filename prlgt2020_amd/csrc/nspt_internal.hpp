@@ -29,6 +29,8 @@ struct KernelSet {
   void (*gexp)(int nblk, hipStream_t, Geo, const double2* fb, SerField G) = nullptr;
   void (*transform)(int nblk, hipStream_t, Geo, SerField G, SerField U, SerField Xexp,
                     const double2* MM) = nullptr;
+  void (*link_pair)(int nblk, hipStream_t, Geo, SerField A, SerField out, int k) = nullptr;
+  void (*loops)(int nblk, hipStream_t, Geo, SerField U, SerField L2, SerField L4, double* part) = nullptr;
 };
 
 // One function per compiled instance (nspt_inst.hip built once per

@@ -26,6 +26,12 @@ __device__ __forceinline__ int site_shift(const Geo& g, int s, int d, int delta)
   return s + (xn - x) * g.st[d];
 }
 
+// Shift by delta >= 0 sites along d, any number of times around the lattice.
+__device__ __forceinline__ int site_shift_far(const Geo& g, int s, int d, int delta) {
+  const int x = (s / g.st[d]) % g.L[d];
+  return s + ((x + delta) % g.L[d] - x) * g.st[d];
+}
+
 // Block order.  The dispatcher hands consecutive workgroups to the 8 XCDs in
 // turn; block_order() renumbers them so that each XCD runs one contiguous
 // range of logical blocks (neighbour links then meet in that XCD's L2), and
@@ -459,6 +465,127 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_gauge_transform(Geo g
   if (ok && ln.e == 0) U.flag[idx] = nf;
 }
 
+// ------------------------------------------------------------ Wilson loops
+// Long links out_mu(x) = A_mu(x) A'_mu(x + k mu) with A' = A: the 2-link
+// field (k = 1, A = U) and the 4-link field (k = 2, A = the 2-link field)
+// of the loop measurement.
+template <int NC, int N>
+__global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_link_pair(Geo g, SerField A, SerField out, int k) {
+  using LN = Lane<NC, N>;
+  __shared__ double2 lds[LN::LDS];
+  LN ln(lds);
+  const int nrt = g.nrt;
+  const LinkBlock lb = link_block<LN::LPB>(g);
+  const int mu = lb.mu;
+  const int site = min(lb.site0 + ln.j, g.V - 1);
+  const bool ok = ln.lds_ok && (lb.site0 + ln.j) < g.V;
+  const long idx = (long)mu * g.V + site;
+  const long idx2 = (long)mu * g.V + site_shift_far(g, site, mu, k);
+  stage_global(ln, ln.slotA, A, idx, nrt);
+  stage_global(ln, ln.slotB, A, idx2, nrt);
+  const double2 fa = gflag(A, idx, ok), fb = gflag(A, idx2, ok);
+  __syncthreads();
+  const double2 f = mul_lds<NC, N>(ln, fa, fb, true, min(N, nrt) - 1, ln.slotB, false);
+  __syncthreads();
+  double2 P[N];
+  unstage(ln, ln.slotB, P);
+  gstore<NC, N>(out, idx, ln.e, nrt, ok, P);
+  if (ok && ln.e == 0) out.flag[idx] = f;
+}
+
+// One rectangle type of one plane per block (ComputeLoops, nspt.cc:1156-1312):
+// the lm x ln loop at x in plane (mu, nu) is Tr(T1 T2^dag) with
+//   T1 = F(lm)_mu(x) F(ln)_nu(x + lm mu),  T2 = F(ln)_nu(x) F(lm)_mu(x + ln nu),
+// F(1) = U, F(2) = the 2-link, F(4) = the 4-link field.  Rectangles
+// r = 0..5: 2x1, 1x2, 2x2, 4x2, 2x4, 4x4; planes p = 0..5: (0,1), (0,2),
+// (0,3), (1,2), (1,3), (2,3).  Block b -> site block b % nb, plane
+// (b / nb) % 6, rectangle b / (6 nb); its sum over the block's sites of Tr,
+// orders 0..nrt (complex), goes to row b of part (2 (nrt+1) doubles).
+template <int NC, int N>
+__global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_loops(Geo g, SerField U, SerField L2, SerField L4,
+                                                                 double* part) {
+  constexpr int NE = NC * NC;
+  using LN = Lane<NC, N>;
+  __shared__ double2 lds[LN::LDS];
+  static_assert((N + 1) * 64 <= LN::LDS, "lane sums fit in the slots");
+  LN ln(lds);
+  const int nrt = g.nrt;
+  const int M = min(N, nrt) - 1;
+  const int nb = g.nb_mu;
+  const int b = block_order();
+  const int sb = b % nb, plane = (b / nb) % 6, rect = b / (6 * nb);
+  const int mu = plane < 3 ? 0 : (plane < 5 ? 1 : 2);
+  const int nu = plane < 3 ? plane + 1 : (plane < 5 ? plane - 1 : 3);
+  const int lm = (rect == 1) ? 1 : (rect == 3 || rect == 5) ? 4 : 2;
+  const int lnu = (rect == 0) ? 1 : (rect == 4 || rect == 5) ? 4 : 2;
+  // (pointers picked one by one: a runtime pick of a whole struct goes through scratch)
+  auto F = [&](int len) {
+    SerField f;
+    f.data = len == 1 ? U.data : (len == 2 ? L2.data : L4.data);
+    f.flag = len == 1 ? U.flag : (len == 2 ? L2.flag : L4.flag);
+    f.P = U.P;
+    return f;
+  };
+  const int site0 = sb * LN::LPB;
+  const int site = min(site0 + ln.j, g.V - 1);
+  const bool ok = ln.lds_ok && (site0 + ln.j) < g.V;
+  const long V = g.V;
+  const long i1 = mu * V + site, i2 = nu * V + site_shift_far(g, site, mu, lm);
+  const long i3 = nu * V + site, i4 = mu * V + site_shift_far(g, site, nu, lnu);
+  double2 P[N], Q[N];
+  // T1
+  stage_global(ln, ln.slotA, F(lm), i1, nrt);
+  stage_global(ln, ln.slotB, F(lnu), i2, nrt);
+  const double2 fa = gflag(F(lm), i1, ok), fb = gflag(F(lnu), i2, ok);
+  __syncthreads();
+  const double2 f1 = mul_lds<NC, N>(ln, fa, fb, true, M, ln.slotA, false);
+  __syncthreads();
+  unstage(ln, ln.slotA, P);
+  __syncthreads();
+  // T2
+  stage_global(ln, ln.slotA, F(lnu), i3, nrt);
+  stage_global(ln, ln.slotB, F(lm), i4, nrt);
+  const double2 fc = gflag(F(lnu), i3, ok), fd = gflag(F(lm), i4, ok);
+  __syncthreads();
+  const double2 f2 = mul_lds<NC, N>(ln, fc, fd, true, M, ln.slotA, false);
+  __syncthreads();
+  unstage(ln, ln.slotA, Q);
+  // this lane's share of Tr(T1 T2^dag), order k+1 in acc[k]: the pairs
+  // P_i conj(Q_j) with i + j + 1 = k over element (r, c), and on the
+  // diagonal the flag terms f1 conj(Q_k) + conj(f2) P_k
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    P[k] = (k <= M) ? P[k] : zero2();
+    Q[k] = (k <= M) ? Q[k] : zero2();
+  }
+  double2 acc[N];
+#pragma unroll
+  for (int k = 0; k < N; k++) acc[k] = zero2();
+#pragma unroll
+  for (int i = 0; i < N; i++)
+#pragma unroll
+    for (int j = 0; i + j + 1 < N; j++) c_mac(acc[i + j + 1], P[i], c_conj(Q[j]));
+  if (ln.r == ln.c) {
+#pragma unroll
+    for (int k = 0; k < N; k++) acc[k] = c_add(acc[k], c_add(c_mul(f1, c_conj(Q[k])), c_mul(c_conj(f2), P[k])));
+  }
+  const double2 t0 = (ok && ln.e == 0) ? c_scale((double)NC, c_mul(f1, c_conj(f2))) : zero2();
+  __syncthreads();
+  const int lane = threadIdx.x;
+#pragma unroll
+  for (int k = 0; k < N; k++) lds[k * 64 + lane] = ok ? acc[k] : zero2();
+  lds[N * 64 + lane] = t0;
+  __syncthreads();
+  if (lane <= nrt) {
+    const double2* col = lds + (lane == 0 ? N : lane - 1) * 64;
+    double2 sum = zero2();
+    for (int l = 0; l < 64; l++) sum = c_add(sum, col[l]);
+    double* row = part + (long)b * 2 * (nrt + 1);
+    row[2 * lane] = sum.x;
+    row[2 * lane + 1] = sum.y;
+  }
+}
+
 // ------------------------------------------------------------ instance table
 template <int NC, int N>
 KernelSet make_kernel_set() {
@@ -486,6 +613,12 @@ KernelSet make_kernel_set() {
   };
   s.transform = [](int nblk, hipStream_t st, Geo g, SerField G, SerField U, SerField Xexp, const double2* MM) {
     hipLaunchKernelGGL((k_gauge_transform<NC, N>), dim3(nblk), dim3(64), 0, st, g, G, U, Xexp, MM);
+  };
+  s.link_pair = [](int nblk, hipStream_t st, Geo g, SerField A, SerField out, int k) {
+    hipLaunchKernelGGL((k_link_pair<NC, N>), dim3(nblk), dim3(64), 0, st, g, A, out, k);
+  };
+  s.loops = [](int nblk, hipStream_t st, Geo g, SerField U, SerField L2, SerField L4, double* part) {
+    hipLaunchKernelGGL((k_loops<NC, N>), dim3(nblk), dim3(64), 0, st, g, U, L2, L4, part);
   };
   return s;
 }

@@ -14,7 +14,7 @@ nspt_params_t nspt_pars;
 act_params_t act_pars;
 mi355x_opts_t mi355x_opts = {0, 1, 1, 0, 1};
 Cplx* w = nullptr;
-std::ofstream plaqfile, normfile, trufile;
+std::ofstream plaqfile, normfile, trufile, loop_nxm;
 
 double Cplx::mod() const { return std::sqrt(re * re + im * im); }
 
@@ -170,6 +170,59 @@ void plaquette_measure(ptGluon_fld& Umu, act_params_t& /*act*/) {
   if (nspt_plaquette(E.ctx, out.data()) != 0) fail("nspt_plaquette");
   for (int i = 0; i <= allocORD; i++) w[i] = (i <= PTORD) ? Cplx(out[2 * i], out[2 * i + 1]) : Cplx();
   mi355x_sync_to_host(Umu);
+}
+
+int ptGluon_fld::loadlowerorder(const char* filename, int loworder, Cplx* plaquette) {
+  if (loworder > allocORD) {
+    std::cout << "ERROR: trying to load more orders than the ones allocated." << std::endl;
+    return -1;
+  }
+  std::ifstream handle(filename, std::ios::binary | std::ios::ate);
+  if (!handle.is_open()) {
+    std::cout << "ERROR: file " << filename << " not found." << std::endl;
+    return -1;
+  }
+  const std::streamoff want =
+      (std::streamoff)sizeof(double) * (2 * (NC * NC * loworder + 1) * dim * (std::streamoff)Z->Size + 2 * (loworder + 1));
+  if (handle.tellg() != want) {
+    std::cout << "ERROR: not possible to read the configuration, unexpected file dimension." << std::endl;
+    return -1;
+  }
+  handle.seekg(0);
+  for (int x = 0; x < Z->Size; x++)
+    for (int mu = 0; mu < dim; mu++) {
+      ptSU3& u = W[x].U[mu];
+      handle.read(reinterpret_cast<char*>(&u.flag), sizeof(Cplx));
+      for (int k = 0; k < loworder; k++) handle.read(reinterpret_cast<char*>(u.ptU[k].whr), sizeof(Cplx) * NC * NC);
+      // every order above loworder is zero (the reference's loop starts one
+      // order higher and relies on fresh, zero memory for order loworder+1)
+      for (int k = loworder; k < allocORD; k++)
+        for (int a = 0; a < NC * NC; a++) u.ptU[k].whr[a] = Cplx();
+    }
+  // the plaquette stored with it: loworder+1 Cplx
+  handle.read(reinterpret_cast<char*>(plaquette), sizeof(Cplx) * (loworder + 1));
+  if (!handle) {
+    std::cout << "ERROR: reading " << filename << " failed." << std::endl;
+    return -1;
+  }
+  // the field has to be unitary
+  mi355x_sync_to_device(*this);
+  if (nspt_unitarize(E.ctx) != 0) fail("nspt_unitarize");
+  mi355x_sync_to_host(*this);
+  return 0;
+}
+
+void ComputeLoops(ptGluon_fld& Umu) {
+  attach(Umu);
+  const int n1 = PTORD + 1;
+  std::vector<double> l((size_t)4 * 2 * n1);  // l21, l22, l42, l44: Tr per order
+  if (nspt_loops(E.ctx, l.data()) != 0) fail("nspt_loops");
+  // normalisation of nspt.cc:1287-1311 (3*2*3 for either group, as there)
+  const double loopNorm = 1. / (3 * 2 * 3 * (Umu.Z->Size));
+  const double half[4] = {.5, 1., .5, 1.};
+  for (int w = 0; w < 4; w++)
+    for (int i1 = 0; i1 <= PTORD; i1 += 2) loop_nxm << l[2 * ((size_t)w * n1 + i1)] * half[w] * loopNorm << "\t";
+  loop_nxm << std::endl;
 }
 
 int plaquette_check(const Cplx* w1, const Cplx* w2) {

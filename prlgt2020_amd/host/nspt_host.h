@@ -9,6 +9,7 @@
 //   ptGluon_fld (W, handle, get, save/load)   QCDenvNODEpt_test.h:331-426
 //   nspt_params_t, act_params_t   input.h:35-70
 //   NsptEvolve, plaquette_measure, plaquette_check   nspt.h:55-94, nspt.cc:1048-1058
+//   ComputeLoops                  nspt.h:94, nspt.cc:1156-1312
 // The series types are data containers; the arithmetic of the step runs on
 // the GPU.  The field lives in GPU memory between NsptEvolve calls: the host
 // image W is refreshed by mi355x_sync_to_host (done by plaquette_measure and
@@ -85,6 +86,11 @@ class ptGluon_fld {
   int save(const char* filename, const Cplx* placchetta);
   int load(const char* filename);
   int load(const char* filename, Cplx* placchetta);
+  // A configuration saved with loworder <= allocORD orders (its plaquette,
+  // loworder+1 Cplx, appended): orders above loworder set to zero, every
+  // link re-unitarised as exp(reH(log U)) on the engine
+  // (QCDenvNODEpt_test.h:376-426).
+  int loadlowerorder(const char* filename, int loworder, Cplx* plaquette);
 };
 
 typedef struct {
@@ -99,6 +105,7 @@ typedef struct {
   char* normn;
   char* trun;
   char* logn;
+  char* name_nxm;
 } nspt_params_t;
 
 typedef struct {
@@ -125,8 +132,9 @@ extern act_params_t act_pars;
 extern mi355x_opts_t mi355x_opts;
 // Plaquette of plaquette_measure (PTORD+1, normalised), as the reference's global w.
 extern Cplx* w;
-// Output streams written once per Langevin step (nspt.cc:541-597).
-extern std::ofstream plaqfile, normfile, trufile;
+// Output streams written once per Langevin step (nspt.cc:541-597), and the
+// Wilson-loop stream written by ComputeLoops.
+extern std::ofstream plaqfile, normfile, trufile, loop_nxm;
 
 // Seeds the host noise stream as the reference seeds Rand[0]: srand(seed);
 // ranlxd(rand()) (Quenched.cc:301-306).
@@ -138,6 +146,10 @@ void NsptEvolve(ptGluon_fld& Umu);
 void plaquette_measure(ptGluon_fld& Umu, act_params_t& act);
 // 0 if equal to 1e-7 per order (nspt.cc:1048-1058).
 int plaquette_check(const Cplx* w1, const Cplx* w2);
+// 1x2, 2x2, 2x4 and 4x4 Wilson loops of the current field (nspt.cc:1156-1312):
+// one line on loop_nxm with the even orders 0, 2, .. <= PTORD of each, in
+// the reference's normalisation.
+void ComputeLoops(ptGluon_fld& Umu);
 void mi355x_sync_to_host(ptGluon_fld& Umu);
 void mi355x_sync_to_device(ptGluon_fld& Umu);
 void mi355x_release();

@@ -14,6 +14,7 @@
 // The gauge fixing is the Fourier-accelerated one (nspt.cc:866-948); it is
 // stable for alpha > 0 only, so the shipped Quench.cfg (alpha -0.05, written
 // for the sequential gauge fixing) needs "-a -0.05" (alpha = +0.05).
+#include <algorithm>
 #include <cmath>
 #include <ctime>
 #include <cstdlib>
@@ -28,6 +29,8 @@ namespace {
 
 std::ofstream logfile;
 int nInt = 0;
+int low_order = 0;      // -L: start from a configuration saved with this many orders
+std::string low_confn;  // its file name
 int config_counter = 0;
 Cplx* w1 = nullptr;
 
@@ -66,6 +69,7 @@ int initialize(int argc, char** argv) {
   nspt_pars.normn = new char[256];
   nspt_pars.logn = new char[256];
   nspt_pars.trun = new char[256];
+  nspt_pars.name_nxm = new char[256];
 
   FILE* fp = std::fopen("Quench.cfg", "r");
   if (!fp) {
@@ -89,7 +93,7 @@ int initialize(int argc, char** argv) {
   std::strcpy(nspt_pars.damon, "damocle.dag");
 
   int opt;
-  while ((opt = getopt(argc, argv, "l:s:b:d:r:g:a:t:p:c:n:I:F:Z:D:h")) != -1) {
+  while ((opt = getopt(argc, argv, "l:s:b:d:r:g:a:t:p:c:n:I:F:Z:D:L:h")) != -1) {
     switch (opt) {
       case 'l': act_pars.sz[0] = act_pars.sz[1] = act_pars.sz[2] = act_pars.sz[3] = std::atoi(optarg); break;
       case 's': nspt_pars.Sweep = std::atoi(optarg); break;
@@ -106,13 +110,15 @@ int initialize(int argc, char** argv) {
       case 'F': mi355x_opts.fa = std::atoi(optarg); break;
       case 'Z': mi355x_opts.zero_modes = std::atoi(optarg); break;
       case 'D': mi355x_opts.device_noise = std::atoi(optarg); break;
+      case 'L': low_order = std::atoi(optarg); break;
       case 'h':
         std::printf("\n-l\t\tlattice size (L0 = L1 = L2 = L3)\n-s\t\tnumber of iterations (SWEEP)\n"
                     "-b\t\titerations between damocle reads (BEAT)\n-r\t\trandom seed\n-g\t\ttau gluons\n"
                     "-a\t\talpha (alpha = -value)\n-t\t\tinitial state: 0 cold, otherwise from configuration\n"
                     "-p\t\tplaquette file name\n-c\t\tsaved configuration name\n-d\t\tdamocle file name\n"
                     "-n\t\tperturbative order\n-I\t\tintegrator: 0 Euler, 1 RK2\n-F\t\tFA gauge fixing: 1 on, 0 off\n"
-                    "-Z\t\tzero-mode subtraction: 1 on, 0 off\n-D\t\tnoise: 0 host ranlxd, 1 device Philox\n\n");
+                    "-Z\t\tzero-mode subtraction: 1 on, 0 off\n-D\t\tnoise: 0 host ranlxd, 1 device Philox\n"
+                    "-L\t\twith -t 1: start from the configuration saved with this lower allocORD\n\n");
         std::exit(0);
     }
   }
@@ -137,10 +143,18 @@ int initialize(int argc, char** argv) {
   const double a = std::fabs(act_pars.alpha), tg = act_pars.tau_g;
   const int* z = act_pars.sz;
   std::string plaq_stem = nspt_pars.plaqn, conf_stem = nspt_pars.confn;
+  std::snprintf(nspt_pars.name_nxm, 256, "%s_nxm_%s_nf0_%dx%dx%dx%d_a%2.2f_tg%3.3f_o%d.txt", plaq_stem.c_str(),
+                type_g(), z[0], z[1], z[2], z[3], a, tg, PTORD);
   std::snprintf(nspt_pars.plaqn, 256, "%s_%s_nf0_%dx%dx%dx%d_a%2.2f_tg%3.3f_o%d.txt", plaq_stem.c_str(), type_g(),
                 z[0], z[1], z[2], z[3], a, tg, PTORD);
   std::snprintf(nspt_pars.confn, 256, "%s%s_nf0_%dx%dx%dx%d_a%2.2f_tg%3.3f_o%d", conf_stem.c_str(), type_g(), z[0],
                 z[1], z[2], z[3], a, tg, allocORD);
+  if (low_order > 0) {
+    char nm[300];
+    std::snprintf(nm, sizeof(nm), "%s%s_nf0_%dx%dx%dx%d_a%2.2f_tg%3.3f_o%d", conf_stem.c_str(), type_g(), z[0], z[1],
+                  z[2], z[3], a, tg, low_order);
+    low_confn = nm;
+  }
   std::snprintf(nspt_pars.logn, 256, "%s_nf0_%dx%dx%dx%d_a%2.2f_tg%3.3f_o%d.log", type_g(), z[0], z[1], z[2], z[3],
                 a, tg, PTORD);
   std::snprintf(nspt_pars.normn, 256, "%s_nf0_%dx%dx%dx%d_a%2.2f_tg%3.3f_o%d.nor", type_g(), z[0], z[1], z[2],
@@ -151,6 +165,9 @@ int initialize(int argc, char** argv) {
   plaqfile.open(nspt_pars.plaqn, std::ios_base::app);
   plaqfile.precision(30);
   plaqfile << std::scientific;
+  loop_nxm.open(nspt_pars.name_nxm, std::ios_base::app);  // __MANY_LOOP__ (choices.h:76)
+  loop_nxm.precision(30);
+  loop_nxm << std::scientific;
   normfile.open(nspt_pars.normn);
   logfile.open(nspt_pars.logn);
   trufile.open(nspt_pars.trun);
@@ -171,6 +188,7 @@ int initialize(int argc, char** argv) {
           << "Ordine\t\t" << PTORD << std::endl
           << std::endl
           << "Placchetta\t" << nspt_pars.plaqn << std::endl
+          << "Altri loops\t" << nspt_pars.name_nxm << std::endl
           << "Configurazione\t" << nspt_pars.confn << std::endl
           << "Damocle\t\t" << nspt_pars.damon << std::endl
           << "Norma\t\t" << nspt_pars.normn << std::endl
@@ -194,12 +212,18 @@ int allocate(ptGluon_fld& Umu) {
     for (int i = 0; i < Umu.Z->TSize; i++)
       for (int mu = 0; mu < dim; mu++) Umu.W[i].U[mu].id();
   } else {
-    if (Umu.load(nspt_pars.confn, w1)) {
-      std::cout << "Error reading the configuration " << nspt_pars.confn << "." << std::endl;
+    const bool low = low_order > 0;
+    if (low ? Umu.loadlowerorder(low_confn.c_str(), low_order, w1) : Umu.load(nspt_pars.confn, w1)) {
+      std::cout << "Error reading the configuration " << (low ? low_confn.c_str() : nspt_pars.confn) << "."
+                << std::endl;
       return 1;
     }
     plaquette_measure(Umu, act_pars);
-    if (plaquette_check(w, w1)) {
+    // a lower-order configuration carries its plaquette up to its own order only
+    const int nchk = low ? std::min(PTORD, low_order) : PTORD;
+    bool differ = false;
+    for (int i = 0; i <= nchk; i++) differ = differ || (w[i] - w1[i]).mod() > 1e-7;
+    if (differ) {
       std::cout << "The plaquette does not match the one stored with the configuration." << std::endl;
       for (int i = 0; i <= PTORD; i++) std::printf("%e %e\t%e %e\n", w[i].re, w[i].im, w1[i].re, w1[i].im);
     }
@@ -210,8 +234,10 @@ int allocate(ptGluon_fld& Umu) {
   return 0;
 }
 
-// damocle.dag re-read after every BEAT steps (Quenched.cc:665-725)
+// After every BEAT steps: the Wilson loops, then damocle.dag re-read
+// (Quenched.cc:656-725)
 int update_parameters(ptGluon_fld& Umu) {
+  ComputeLoops(Umu);
   nInt += nspt_pars.Beat;
   logfile << "______________________________________" << std::endl << std::endl;
   stamp("Damocle Update\t\t");

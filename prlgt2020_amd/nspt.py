@@ -43,7 +43,8 @@ EXPORTS = ["nspt_create", "nspt_destroy", "nspt_last_error", "nspt_obs_len", "ns
            "nspt_fa_gauge_fixing", "nspt_drift", "nspt_rng_create_ranlxd", "nspt_rng_destroy",
            "nspt_rng_uniform", "nspt_rng_noise", "nspt_device_noise", "nspt_profile", "nspt_profile_read",
            "nspt_batch_create", "nspt_batch_destroy", "nspt_batch_last_error", "nspt_batch_evolve",
-           "nspt_batch_replica", "nspt_batch_tune", "nspt_adjoint_drift"]
+           "nspt_batch_replica", "nspt_batch_tune", "nspt_adjoint_drift", "nspt_loops",
+           "nspt_unitarize"]
 
 PROF_NAMES = ["drift", "update", "update_log", "zero_mode", "fa_div", "dft", "fa_gexp", "transform", "noise",
               "reduce"]
@@ -190,6 +191,17 @@ class Context:
         out = np.zeros(self.N + 1, np.complex128)
         self._check(lib().nspt_plaquette(self.h, _ptr(out)), "plaquette")
         return out
+
+    def loops(self):
+        """Tr of the Wilson-loop sums w21, w22, w42, w44 (nspt.cc:1156-1312):
+        array (4, N+1) complex, unnormalised."""
+        out = np.zeros(4 * (self.N + 1), np.complex128)
+        self._check(lib().nspt_loops(self.h, _ptr(out)), "loops")
+        return out.reshape(4, self.N + 1)
+
+    def unitarize(self):
+        """U <- exp(reH(log U)) for every link (QCDenvNODEpt_test.h:415-418)."""
+        self._check(lib().nspt_unitarize(self.h), "unitarize")
 
     def fa_gauge_fixing(self, alpha):
         self._check(lib().nspt_fa_gauge_fixing(self.h, alpha), "fa_gauge_fixing")

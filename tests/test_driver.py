@@ -30,7 +30,8 @@ def write_inputs(d, L=4, sweep=20, beat=10, tau=0.05, alpha=0.05, ptord=6, seed=
 
 def names(L, alpha, tau, ptord, alloc_ord, nc=3):
     stem = "SU%d_wil_nf0_%dx%dx%dx%d_a%2.2f_tg%3.3f" % (nc, L, L, L, L, abs(alpha), tau)
-    return {"plaq": "plaq_%s_o%d.txt" % (stem, ptord), "nor": "%s_o%d.nor" % (stem, ptord),
+    return {"plaq": "plaq_%s_o%d.txt" % (stem, ptord), "nxm": "plaq_nxm_%s_o%d.txt" % (stem, ptord),
+            "nor": "%s_o%d.nor" % (stem, ptord),
             "trU": "%s_o%d.trU" % (stem, ptord), "log": "%s_o%d.log" % (stem, ptord),
             "conf": "conf_%s_o%d" % (stem, alloc_ord)}
 
@@ -64,7 +65,7 @@ def test_driver_fails_loudly_without_gpu_and_names_outputs_like_the_reference(tm
     assert out.returncode != 0
     assert "MI355X engine: nspt_create" in out.stderr
     n = names(4, 0.05, 0.05, 6, 6)
-    for k in ("plaq", "nor", "trU", "log"):
+    for k in ("plaq", "nxm", "nor", "trU", "log"):
         assert (tmp_path / n[k]).exists(), n[k]
 
 
@@ -91,14 +92,18 @@ def test_driver_run_matches_oracle(tmp_path, nc):
     V = L ** 4
     U = O.cold_field(Lt, nc, N)
     rng = O.Ranlxd(O.glibc_first_rand(1))
-    plaq, nor, tru = [], [], []
-    for _ in range(steps):
+    plaq, nor, tru, nxm = [], [], [], []
+    for t in range(steps):
         eta = rng.noise(nc, 4 * V)
         obs = O.step(Lt, nc, N, U, eps, alpha, integrator=0, update=0, gauge_fix=1, zero_modes=1, eta=eta)
-        p, nr, t = O.split_obs(obs, N)
+        p, nr, tr = O.split_obs(obs, N)
         plaq.append(np.stack([p.real, p.imag], 1))
         nor.append(nr)
-        tru.append(np.concatenate([np.stack([t[mu].real, t[mu].imag], 1) for mu in range(4)], 1))
+        tru.append(np.concatenate([np.stack([tr[mu].real, tr[mu].imag], 1) for mu in range(4)], 1))
+        if (t + 1) % beat == 0:
+            # ComputeLoops after each beat: even orders, nspt.cc:1287-1311 normalisation
+            lo = O.compute_loops(Lt, nc, N, U).real / (18.0 * V)
+            nxm.append(np.concatenate([lo[w, 0::2] * h for w, h in enumerate((0.5, 1.0, 0.5, 1.0))]))
     plaq = np.concatenate(plaq)
     nor = np.concatenate(nor)
     tru = np.concatenate(tru)
@@ -113,6 +118,11 @@ def test_driver_run_matches_oracle(tmp_path, nc):
     # (relative rounding up to 5e-6)
     assert np.allclose(got_n, nor, rtol=1e-5, atol=1e-14)
     assert np.allclose(got_t, tru, rtol=1e-5, atol=1e-9)
+    # Wilson loops, written with precision 30 as the reference does
+    got_l = _read_cols(tmp_path / n["nxm"], 4 * (N // 2 + 1))
+    want_l = np.array(nxm)
+    assert got_l.shape == want_l.shape
+    assert np.abs(got_l - want_l).max() < 1e-12 * np.abs(want_l).max()
 
     # checkpoints: after each beat (save_config 1) and at the end, ptGluon[V] + plaquette
     S = 1 + nc * nc * 6
@@ -123,3 +133,30 @@ def test_driver_run_matches_oracle(tmp_path, nc):
     field = raw[:4 * V * S].reshape(4 * V, S)
     assert np.abs(field - U).max() < 1e-12 * np.abs(U).max()
     assert np.abs(raw[4 * V * S:] - O.plaquette_measure(Lt, nc, N, U)).max() < 1e-12
+
+
+@pytest.mark.gpu
+def test_driver_restarts_from_lower_order(tmp_path):
+    """A run saved at allocORD 6, loaded by the allocORD 8 build with -L 6
+    (loadlowerorder, QCDenvNODEpt_test.h:376-426): orders 7 and 8 zero, every
+    link re-unitarised as exp(reH(log U)); 0 sweeps, so the final checkpoint
+    holds the loaded field."""
+    from oracle import oracle as O
+    L, eps, alpha, nc = 4, 0.05, 0.05, 3
+    V = L ** 4
+    write_inputs(tmp_path, L=L, sweep=10, beat=10, tau=eps, alpha=alpha, ptord=6, seed=1)
+    out = subprocess.run([driver(nc, 6)], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    write_inputs(tmp_path, L=L, sweep=0, beat=10, tau=eps, alpha=alpha, ptord=8, seed=1, init=1)
+    out = subprocess.run([driver(nc, 8), "-L", "6"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    S6, S8 = 1 + 9 * 6, 1 + 9 * 8
+    f6 = np.fromfile(tmp_path / names(L, alpha, eps, 6, 6)["conf"], dtype=np.complex128)[:4 * V * S6]
+    f8 = np.fromfile(tmp_path / names(L, alpha, eps, 8, 8)["conf"], dtype=np.complex128)[:4 * V * S8]
+    f6, f8 = f6.reshape(4 * V, S6), f8.reshape(4 * V, S8)
+    pad = np.zeros((4 * V, S8), np.complex128)
+    pad[:, :S6] = f6
+    want = np.array([O.exp(nc, 8, O.reh(nc, 8, O.log(nc, 8, u))) for u in pad])
+    assert np.abs(f8 - want).max() < 1e-12 * np.abs(want).max()
+    assert np.abs(f8[:, :S6] - f6).max() < 1e-12
+    assert np.abs(f8[:, S6:]).max() > 1e-6  # the re-unitarised higher orders
