@@ -83,6 +83,15 @@ int nspt_step(nspt_ctx* ctx, const nspt_step_params* p, double* obs);
  * p->step is advanced by nsteps. */
 int nspt_evolve(nspt_ctx* ctx, nspt_step_params* p, int nsteps, double* obs);
 
+/* Gauge action of the drift (GAUGE_ACTION of choices.h:101-113, chosen at
+ * compile time by the reference): c1 = 0 Wilson (default), -1/12 TLSYM,
+ * -0.331 IWA, -1.4088 DBW2, c0 = 1 - 8 c1 (Quenched.cc:313-322).  With
+ * c1 != 0 the drift is tau_g reH(c0 U Sigma + c1 U Sigma_2x1) (nspt.cc:385-412,
+ * staple2x1 of QCDenvNODEpt_test.h:579-611) and the observable row gains the
+ * rectangle plaquette (ptord+1 complex, normalised as the plaquette) after
+ * trU: nspt_obs_len grows accordingly. */
+int nspt_set_action(nspt_ctx* ctx, double c1);
+
 /* plaquette_measure (nspt.cc:1316-1406): ptord+1 complex. */
 int nspt_plaquette(nspt_ctx* ctx, double* out);
 

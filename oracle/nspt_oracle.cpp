@@ -285,6 +285,28 @@ Ser staple(const Field& F, int n, int mu, int nu) {
   return ser_add(a, b);
 }
 
+// staple2x1(n, mu, nu) (QCDenvNODEpt_test.h:579-611): the six 5-link
+// staples of the 2x1 and 1x2 rectangles through link (n, mu) in plane
+// (mu, nu), with the reference's neighbour letters and product order.
+Ser staple2x1(const Field& F, int n, int mu, int nu) {
+  const Latt& Z = *F.lat;
+  auto up = [&](int s, int d) { return Z.up[s * 4 + d]; };
+  auto dn = [&](int s, int d) { return Z.dn[s * 4 + d]; };
+  auto U = [&](int s, int d) { return F.get(s, d); };
+  int a = up(n, mu), b = up(a, nu), c = up(n, nu), d = up(c, nu), e = up(a, mu), f = dn(n, mu), g = up(f, nu);
+  Ser t = ser_mul(ser_mul(U(a, nu), U(b, nu)), ser_dag(ser_mul(ser_mul(U(n, nu), U(c, nu)), U(d, mu))));
+  t = ser_add(t, ser_mul(ser_mul(U(a, mu), U(e, nu)), ser_dag(ser_mul(ser_mul(U(n, nu), U(c, mu)), U(b, mu)))));
+  t = ser_add(t, ser_mul(ser_mul(U(a, nu), ser_dag(ser_mul(ser_mul(U(f, nu), U(g, mu)), U(c, mu)))), U(f, mu)));
+  b = dn(a, nu);
+  c = dn(n, nu);
+  d = dn(c, nu);
+  const int e2 = dn(b, nu), f2 = up(b, mu), g2 = dn(c, mu), h = dn(n, mu);
+  t = ser_add(t, ser_mul(ser_mul(ser_dag(ser_mul(ser_mul(U(d, mu), U(e2, nu)), U(b, nu))), U(d, nu)), U(c, nu)));
+  t = ser_add(t, ser_mul(ser_mul(U(a, mu), ser_dag(ser_mul(ser_mul(U(c, mu), U(b, mu)), U(f2, nu)))), U(c, nu)));
+  t = ser_add(t, ser_mul(ser_mul(ser_dag(ser_mul(ser_mul(U(g2, mu), U(c, mu)), U(b, nu))), U(g2, nu)), U(h, mu)));
+  return t;
+}
+
 Ser staple_sum(const Field& F, int n, int mu) {
   Ser W(F.nc, F.N);  // zero(): flag 0
   for (int nu = 0; nu < 4; nu++)
@@ -301,10 +323,19 @@ struct Obs {
   std::vector<cx> w1;             // N+1
   std::vector<double> norm;       // N
   std::vector<cx> trU;            // 4*N
-  void init(int N) {
+  std::vector<cx> w2;             // N+1, improved actions only (nspt.cc:405-410)
+  bool improved = false;
+  void init(int N, bool impr = false) {
     w1.assign(N + 1, cx{0, 0});
     norm.assign(N, 0.0);
     trU.assign(4 * N, cx{0, 0});
+    w2.assign(N + 1, cx{0, 0});
+    improved = impr;
+  }
+  void accumulate2(const Ser& W2) {
+    std::vector<cx> w(W2.N + 1);
+    ser_tr(W2, w.data());
+    for (int k = 0; k <= W2.N; k++) w2[k] += w[k];
   }
   void accumulate(const Ser& U, const Ser& W, int mu) {
     const int N = U.N, nc = U.nc;
@@ -337,15 +368,34 @@ struct Obs {
       o[2 * q] = trU[q].re / tn;
       o[2 * q + 1] = trU[q].im / tn;
     }
+    if (improved) {
+      o += 8 * N;
+      for (int k = 0; k <= N; k++) {
+        o[2 * k] = w2[k].re / pn;
+        o[2 * k + 1] = w2[k].im / pn;
+      }
+    }
   }
 };
 
 // Drift W = tau_g * reH(U * sum of staples), tau_g = -eps/(2 nc)
 // (nspt.cc:383-417, Quenched.cc:584-589).  Optionally accumulates observables.
-Ser drift(const Field& F, int n, int mu, double tau_g, Obs* obs) {
+// c1 != 0: improved action (nspt.cc:385-412, Quenched.cc:313-322),
+// W = tau_g reH(c0 U Sigma + c1 U Sigma_2x1) with c0 = 1 - 8 c1.
+Ser drift(const Field& F, int n, int mu, double tau_g, Obs* obs, double c1 = 0.0) {
   Ser U = F.get(n, mu);
   Ser W = ser_mul(U, staple_sum(F, n, mu));
   if (obs) obs->accumulate(U, W, mu);
+  if (c1 != 0.0) {
+    Ser R(F.nc, F.N);
+    for (int nu = 0; nu < 4; nu++)
+      if (nu != mu) R = ser_add(R, staple2x1(F, n, mu, nu));
+    Ser W2 = ser_mul(U, R);
+    if (obs) obs->accumulate2(W2);
+    ser_scale(W, 1.0 - 8.0 * c1);
+    ser_scale(W2, c1);
+    W = ser_add(W, W2);
+  }
   ser_reH(W);
   ser_scale(W, tau_g);
   return W;
@@ -615,6 +665,7 @@ struct Params {
   int update;        // 0 Jacobi, 1 on-the-fly (sequential, in place)
   int gauge_fix;     // 0 none, 1 Fourier accelerated, 2 default in-place
   int zero_modes;    // 1: zero_modes_subtraction
+  double c1 = 0.0;   // rectangle coefficient: 0 Wilson, -1/12 TLSYM, -0.331 IWA, -1.4088 DBW2
 };
 
 // ---------------------------------------------------------------- adjoint
@@ -838,13 +889,13 @@ void step(const Geom& g, double* Udata, const Params& p, const double* eta, doub
   const double stau = std::sqrt(p.eps), tau_g = -p.eps / (2.0 * g.nc);
   std::vector<Ser> MM(4, Ser(g.nc, g.N));
   Obs obs;
-  obs.init(g.N);
+  obs.init(g.N, p.c1 != 0.0);
 
   if (p.update == 1) {
     // on the fly: in place, lexicographic site order, mu inner (nspt.cc:365-441)
     for (int s = 0; s < Z.V; s++)
       for (int mu = 0; mu < 4; mu++) {
-        Ser W = drift(U, s, mu, tau_g, &obs);
+        Ser W = drift(U, s, mu, tau_g, &obs, p.c1);
         sub_noise(W, stau, eta + (size_t)(4 * s + mu) * ne2);
         Ser Un = ser_mul(ser_exp(W), U.get(s, mu));
         U.set(s, mu, Un);
@@ -859,7 +910,7 @@ void step(const Geom& g, double* Udata, const Params& p, const double* eta, doub
       D1.resize(4 * Z.V);
       for (int s = 0; s < Z.V; s++)
         for (int mu = 0; mu < 4; mu++) {
-          D1[4 * s + mu] = drift(U, s, mu, tau_g, &obs);
+          D1[4 * s + mu] = drift(U, s, mu, tau_g, &obs, p.c1);
           Ser W = D1[4 * s + mu];
           sub_noise(W, stau, eta + (size_t)(4 * s + mu) * ne2);
           Fn.set(s, mu, ser_mul(ser_exp(W), U.get(s, mu)));
@@ -871,9 +922,9 @@ void step(const Geom& g, double* Udata, const Params& p, const double* eta, doub
         Ser W;
         if (p.integrator == 1) {
           // corrector: drift at U1, combined with D(U); applied to U
-          W = rk2_combine(D1[4 * s + mu], drift(Fn, s, mu, tau_g, nullptr), g.nc, p.eps);
+          W = rk2_combine(D1[4 * s + mu], drift(Fn, s, mu, tau_g, nullptr, p.c1), g.nc, p.eps);
         } else {
-          W = drift(U, s, mu, tau_g, &obs);
+          W = drift(U, s, mu, tau_g, &obs, p.c1);
         }
         sub_noise(W, stau, eta + (size_t)(4 * s + mu) * ne2);
         Ser Un = ser_mul(ser_exp(W), U.get(s, mu));
@@ -1022,6 +1073,22 @@ void orc_staple_sum(const int* gi, const double* U, int site, int mu, double* ou
 }
 
 // D[4V*S] = tau_g * reH(U * staples) for every link (no noise).
+void orc_staple2x1(const int* gi, const double* U, int site, int mu, int nu, double* out) {
+  Geom g = geom_from(gi);
+  Latt Z(g.L);
+  const int S = 1 + g.nc * g.nc * g.N;
+  Field F{g.nc, g.N, S, &Z, const_cast<double*>(U)};
+  to_ptr(staple2x1(F, site, mu, nu), out);
+}
+void orc_drift_c1(const int* gi, const double* U, double eps, double c1, double* D) {
+  Geom g = geom_from(gi);
+  Latt Z(g.L);
+  const int S = 1 + g.nc * g.nc * g.N;
+  Field F{g.nc, g.N, S, &Z, const_cast<double*>(U)};
+  for (int s = 0; s < Z.V; s++)
+    for (int mu = 0; mu < 4; mu++)
+      to_ptr(drift(F, s, mu, -eps / (2.0 * g.nc), nullptr, c1), D + 2 * (size_t)(4 * s + mu) * S);
+}
 void orc_drift(const int* gi, const double* U, double eps, double* D) {
   Geom g = geom_from(gi);
   Latt Z(g.L);
@@ -1033,7 +1100,7 @@ void orc_drift(const int* gi, const double* U, double eps, double* D) {
 }
 
 void orc_step(const int* gi, double* U, const double* pp, const double* eta, double* obs) {
-  Params p{pp[0], pp[1], (int)pp[2], (int)pp[3], (int)pp[4], (int)pp[5]};
+  Params p{pp[0], pp[1], (int)pp[2], (int)pp[3], (int)pp[4], (int)pp[5], pp[6]};
   step(geom_from(gi), U, p, eta, obs);
 }
 

@@ -120,34 +120,49 @@ def _geom(L, nc, N):
     return (ctypes.c_int * 6)(L[0], L[1], L[2], L[3], nc, N)
 
 
-def obs_len(N):
-    """plaq (N+1 complex) + norm (N real) + trU (4N complex), as doubles."""
-    return 2 * (N + 1) + N + 8 * N
+def obs_len(N, improved=False):
+    """plaq (N+1 complex) + norm (N real) + trU (4N complex), as doubles;
+    improved actions append the rectangle plaquette w2 (N+1 complex)."""
+    return 2 * (N + 1) + N + 8 * N + (2 * (N + 1) if improved else 0)
 
 
 def split_obs(obs, N):
+    """plaq, norm, trU (and w2 for an improved action's vector)."""
     plaq = obs[0:2 * (N + 1)].view(np.complex128)
     norm = obs[2 * (N + 1):2 * (N + 1) + N]
-    tru = obs[2 * (N + 1) + N:].view(np.complex128).reshape(4, N)
+    o = 2 * (N + 1) + N
+    tru = obs[o:o + 8 * N].view(np.complex128).reshape(4, N)
+    if len(obs) > o + 8 * N:
+        return plaq, norm, tru, obs[o + 8 * N:].view(np.complex128)
     return plaq, norm, tru
 
 
-def step(L, nc, N, U, eps, alpha=0.0, integrator=0, update=0, gauge_fix=0, zero_modes=1, eta=None):
-    """One NsptEvolve step in place on U; returns the observable vector."""
+def step(L, nc, N, U, eps, alpha=0.0, integrator=0, update=0, gauge_fix=0, zero_modes=1, eta=None, c1=0.0):
+    """One NsptEvolve step in place on U; returns the observable vector.
+    c1 != 0: improved gauge action (c0 = 1 - 8 c1)."""
     V = L[0] * L[1] * L[2] * L[3]
     assert U.shape == (4 * V, ser_len(nc, N)) and U.dtype == np.complex128 and U.flags.c_contiguous
     eta = np.ascontiguousarray(eta, dtype=np.complex128)
     assert eta.shape == (4 * V, nc * nc)
-    pp = (ctypes.c_double * 6)(eps, alpha, integrator, update, gauge_fix, zero_modes)
-    obs = np.zeros(obs_len(N))
+    pp = (ctypes.c_double * 7)(eps, alpha, integrator, update, gauge_fix, zero_modes, c1)
+    obs = np.zeros(obs_len(N, c1 != 0.0))
     lib().orc_step(_geom(L, nc, N), _ptr(U), pp, _ptr(eta), _ptr(obs))
     return obs
 
 
-def drift(L, nc, N, U, eps):
+def drift(L, nc, N, U, eps, c1=0.0):
     D = np.zeros_like(U)
-    lib().orc_drift(_geom(L, nc, N), _ptr(U), eps, _ptr(D))
+    if c1 == 0.0:
+        lib().orc_drift(_geom(L, nc, N), _ptr(U), eps, _ptr(D))
+    else:
+        lib().orc_drift_c1(_geom(L, nc, N), _ptr(U), ctypes.c_double(eps), ctypes.c_double(c1), _ptr(D))
     return D
+
+
+def staple2x1(L, nc, N, U, site, mu, nu):
+    out = np.zeros(ser_len(nc, N), np.complex128)
+    lib().orc_staple2x1(_geom(L, nc, N), _ptr(U), site, mu, nu, _ptr(out))
+    return out
 
 
 def gauge_wilson_jacobi(L, nc, N, U, eta, eps):

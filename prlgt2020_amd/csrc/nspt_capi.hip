@@ -169,7 +169,9 @@ __global__ void k_count(unsigned long long* ctr) {
 
 // slot_dev (may be null): a device counter selecting the output row, for
 // steps replayed from a captured graph.
-__global__ void k_obs_finalize(const double* obs_mu, int nrt, int nc, int V, double* out,
+// obs2_mu (may be null): per-direction sums of the rectangle plaquette of an
+// improved action, appended after trU (nspt.cc:575-578).
+__global__ void k_obs_finalize(const double* obs_mu, const double* obs2_mu, int nrt, int nc, int V, double* out,
                                const unsigned long long* slot_dev, int obs_len) {
   if (slot_dev) out += (long)(*slot_dev) * obs_len;
   const int nob = 2 * (nrt + 1) + 3 * nrt;
@@ -183,6 +185,12 @@ __global__ void k_obs_finalize(const double* obs_mu, int nrt, int nc, int V, dou
     const int mu = t / (2 * nrt), q = t % (2 * nrt);
     out[2 * (nrt + 1) + nrt + t] = obs_mu[mu * nob + 2 * (nrt + 1) + nrt + q] / tn;
   }
+  if (obs2_mu)
+    for (int t = threadIdx.x; t < 2 * (nrt + 1); t += blockDim.x) {
+      double s = 0.0;
+      for (int mu = 0; mu < 4; mu++) s += obs2_mu[mu * 2 * (nrt + 1) + t];
+      out[2 * (nrt + 1) + nrt + 8 * nrt + t] = s / pn;
+    }
 }
 
 // ------------------------------------------------------------ FA transform
@@ -611,6 +619,12 @@ struct nspt_ctx {
   // fields, per-block rectangle sums, reduction scratch, the six sums
   SerField L2{}, L4{};
   double *loop_part = nullptr, *loop_tmp = nullptr, *loop_res = nullptr;
+  // improved gauge action (nspt_set_action): rectangle coefficient, the
+  // rectangle-staple sum of every link, per-block and per-direction sums of
+  // the rectangle plaquette
+  double c1 = 0.0;
+  SerField R{};
+  double *obs2_part = nullptr, *obs2_mu = nullptr;
   double2* fb = nullptr;
   double2* eta = nullptr;
   double2* eta_tmp = nullptr;
@@ -755,11 +769,14 @@ bool reduce4(nspt_ctx* c, const double* in, long ld, int ncols, double* out, dou
   return reduce_groups(c, in, ld, c->g.nb_mu, 4, ncols, out, scale, c->red_tmp);
 }
 
-bool obs_reduce(nspt_ctx* c, double* out_dev) {
+// with_w2: the rectangle plaquette of an improved action is in obs2_part
+bool obs_reduce(nspt_ctx* c, double* out_dev, bool with_w2 = false) {
   if (!reduce4(c, c->obs_part, c->nob, c->nob, c->obs_mu, 1.0)) return false;
+  if (with_w2 && !reduce4(c, c->obs2_part, 2 * (c->nrt + 1), 2 * (c->nrt + 1), c->obs2_mu, 1.0)) return false;
   Prof pr(c, NSPT_PROF_REDUCE);
-  hipLaunchKernelGGL(k_obs_finalize, dim3(1), dim3(256), 0, c->stream, (const double*)c->obs_mu, c->nrt, c->nc, c->V,
-                     out_dev, c->graph ? (const unsigned long long*)(c->ctr_dev + 1) : nullptr, c->obs_len);
+  hipLaunchKernelGGL(k_obs_finalize, dim3(1), dim3(256), 0, c->stream, (const double*)c->obs_mu,
+                     with_w2 ? (const double*)c->obs2_mu : nullptr, c->nrt, c->nc, c->V, out_dev,
+                     c->graph ? (const unsigned long long*)(c->ctr_dev + 1) : nullptr, c->obs_len);
   return launch_ok(c, "obs_finalize");
 }
 
@@ -826,25 +843,37 @@ bool do_step(nspt_ctx* c, const nspt_step_params* p, double* obs_dev) {
   const bool need_log = p->zero_modes || p->gauge_fix == NSPT_GF_FA;
   SerField none{nullptr, nullptr, 0};
   double* obs_part = obs_dev ? c->obs_part : nullptr;
+  // an improved action first forms the rectangle staples of the field the
+  // drift reads (and, for the observables, their plaquette)
+  const bool impr = c->c1 != 0.0;
+  const SerField R = impr ? c->R : none;
+  auto rect = [&](const SerField& Uin, bool with_obs) {
+    if (!impr) return;
+    Prof pr(c, NSPT_PROF_DRIFT);
+    c->ks.rect(c->nblk_links, c->stream, c->g, Uin, c->R, with_obs && obs_part ? c->obs2_part : nullptr);
+  };
   if (p->integrator == NSPT_RK2) {
+    rect(c->U, true);
     {
       Prof pr(c, NSPT_PROF_DRIFT);
-      c->ks.drift(c->nblk_links, c->stream, c->g, c->U, c->D1, none, tau_g, 0.0, 0, obs_part);
+      c->ks.drift(c->nblk_links, c->stream, c->g, c->U, c->D1, none, tau_g, 0.0, 0, obs_part, R, c->c1);
     }
     {
       Prof pr(c, NSPT_PROF_UPDATE);
       c->ks.update(c->nblk_links, c->stream, c->g, c->D1, eta, stau, c->U, c->F, none, nullptr);
     }
+    rect(c->F, false);
     {
       Prof pr(c, NSPT_PROF_DRIFT);
-      c->ks.drift(c->nblk_links, c->stream, c->g, c->F, c->D, c->D1, tau_g, rk2c, 1, nullptr);
+      c->ks.drift(c->nblk_links, c->stream, c->g, c->F, c->D, c->D1, tau_g, rk2c, 1, nullptr, R, c->c1);
     }
   } else {
+    rect(c->U, true);
     Prof pr(c, NSPT_PROF_DRIFT);
-    c->ks.drift(c->nblk_links, c->stream, c->g, c->U, c->D, none, tau_g, 0.0, 0, obs_part);
+    c->ks.drift(c->nblk_links, c->stream, c->g, c->U, c->D, none, tau_g, 0.0, 0, obs_part, R, c->c1);
   }
   if (!launch_ok(c, "drift")) return false;
-  if (obs_dev && !obs_reduce(c, obs_dev)) return false;
+  if (obs_dev && !obs_reduce(c, obs_dev, impr)) return false;
   {
     Prof pr(c, need_log ? NSPT_PROF_UPDATE_LOG : NSPT_PROF_UPDATE);
     c->ks.update(c->nblk_links, c->stream, c->g, c->D, eta, stau, c->U, c->F, need_log ? c->L : none,
@@ -1045,7 +1074,7 @@ int nspt_evolve(nspt_ctx* c, nspt_step_params* p, int nsteps, double* obs) {
 int nspt_plaquette(nspt_ctx* c, double* out) {
   if (!c || !out) return -1;
   SerField none{nullptr, nullptr, 0};
-  c->ks.drift(c->nblk_links, c->stream, c->g, c->U, none, none, 0.0, 0.0, 2, c->obs_part);
+  c->ks.drift(c->nblk_links, c->stream, c->g, c->U, none, none, 0.0, 0.0, 2, c->obs_part, none, 0.0);
   if (!launch_ok(c, "plaquette") || !obs_reduce(c, c->obs_dev)) return -3;
   if (!c->hip_ok(hipMemcpyAsync(out, c->obs_dev, sizeof(double) * 2 * (c->nrt + 1), hipMemcpyDeviceToHost,
                                 c->stream),
@@ -1086,6 +1115,23 @@ int nspt_loops(nspt_ctx* c, double* out) {
   return 0;
 }
 
+int nspt_set_action(nspt_ctx* c, double c1) {
+  if (!c) return -1;
+  if (c1 != 0.0 && !c->R.data) {
+    const int ncols = 2 * (c->nrt + 1);
+    if (!(c->alloc_field(c->R, 4L * c->V, true) && c->alloc(&c->obs2_part, (size_t)c->nblk_links * ncols) &&
+          c->alloc(&c->obs2_mu, (size_t)4 * ncols))) {
+      c->R.data = nullptr;
+      return -2;
+    }
+  }
+  c->c1 = c1;
+  // the observable row gains the rectangle plaquette (N+1 complex)
+  c->obs_len = 2 * (c->nrt + 1) + c->nrt + 8 * c->nrt + (c1 != 0.0 ? 2 * (c->nrt + 1) : 0);
+  c->obs_cap = 0;
+  return ensure_obs(c, 1) ? 0 : -2;
+}
+
 int nspt_unitarize(nspt_ctx* c) {
   if (!c) return -1;
   // U <- exp(reH(log U)): the log kernel, then the zero-mode kernel with a zero mean
@@ -1108,7 +1154,9 @@ int nspt_fa_gauge_fixing(nspt_ctx* c, double alpha) {
 int nspt_drift(nspt_ctx* c, double eps, double* aos_out) {
   if (!c || !aos_out) return -1;
   SerField none{nullptr, nullptr, 0};
-  c->ks.drift(c->nblk_links, c->stream, c->g, c->U, c->D, none, -eps / (2.0 * c->nc), 0.0, 0, nullptr);
+  if (c->c1 != 0.0) c->ks.rect(c->nblk_links, c->stream, c->g, c->U, c->R, nullptr);
+  c->ks.drift(c->nblk_links, c->stream, c->g, c->U, c->D, none, -eps / (2.0 * c->nc), 0.0, 0, nullptr,
+              c->c1 != 0.0 ? c->R : none, c->c1);
   if (!launch_ok(c, "drift")) return -3;
   return download_field(c, c->D, aos_out);
 }

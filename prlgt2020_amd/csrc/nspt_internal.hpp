@@ -21,7 +21,10 @@ struct Geo {
 struct KernelSet {
   int nc = 0, nmax = 0, lpb = 0, qt = 0;  // qt: order-tile edge of the products
   void (*drift)(int nblk, hipStream_t, Geo, SerField U, SerField D, SerField D1, double tau_g, double rk2c, int mode,
-                double* obs) = nullptr;
+                double* obs, SerField R, double c1) = nullptr;
+  // sum of the rectangle staples of every link, and (obs2 != null) their
+  // per-block plaquette traces
+  void (*rect)(int nblk, hipStream_t, Geo, SerField U, SerField R, double* obs2) = nullptr;
   void (*update)(int nblk, hipStream_t, Geo, SerField D, const double2* eta, double stau, SerField Ubase,
                  SerField Uout, SerField L, double2* mm) = nullptr;
   void (*logf)(int nblk, hipStream_t, Geo, SerField U, SerField L) = nullptr;

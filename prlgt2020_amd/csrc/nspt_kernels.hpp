@@ -26,6 +26,15 @@ __device__ __forceinline__ int site_shift(const Geo& g, int s, int d, int delta)
   return s + (xn - x) * g.st[d];
 }
 
+// Shift by delta (either sign, |delta| <= L_d) along d.
+__device__ __forceinline__ int site_move(const Geo& g, int s, int d, int delta) {
+  const int x = (s / g.st[d]) % g.L[d];
+  int xn = x + delta;
+  xn += (xn < 0) ? g.L[d] : 0;
+  xn -= (xn >= g.L[d]) ? g.L[d] : 0;
+  return s + (xn - x) * g.st[d];
+}
+
 // Shift by delta >= 0 sites along d, any number of times around the lattice.
 __device__ __forceinline__ int site_shift_far(const Geo& g, int s, int d, int delta) {
   const int x = (s / g.st[d]) % g.L[d];
@@ -106,7 +115,7 @@ __device__ __forceinline__ double2 slot_sum(const double2* slotA0, int nvalid, i
 template <int NC, int N>
 __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_drift(Geo g, SerField U, SerField D, SerField D1,
                                                                  double tau_g, double rk2c, int mode,
-                                                                 double* obs_part) {
+                                                                 double* obs_part, SerField R, double c1) {
   constexpr int NE = NC * NC;
   using LN = Lane<NC, N>;
   __shared__ double2 lds[LN::LDS];
@@ -126,7 +135,7 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_drift(Geo g, SerField
   double2 Sig[N];
 #pragma unroll
   for (int k = 0; k < N; k++) Sig[k] = zero2();
-  double2 sigf = zero2(), t1f = zero2(), t2f = zero2(), wf = zero2();
+  double2 sigf = zero2(), t1f = zero2(), t2f = zero2(), wf = zero2(), uflag = zero2();
 
   // 12 staple products (4 per plane nu != mu) and W = U_mu(x) Sigma, one
   // product per iteration so that the product body is emitted once:
@@ -193,6 +202,7 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_drift(Geo g, SerField
     }
     if (kind == 4) ln.stage(ln.slotB, Sig);
     const double2 a0 = (kind == 3) ? c_conj(t2f) : paf;
+    if (kind == 4) uflag = a0;
     const double2 b0 = (kind == 1) ? c_conj(t1f) : (kind == 4) ? sigf : pbf;
     __syncthreads();
     if (i + 1 < 13) fetch(i + 1);
@@ -248,6 +258,26 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_drift(Geo g, SerField
     }
   }
   if (mode == 2) return;
+
+  if (R.data) {
+    // improved action (nspt.cc:405-412): W = c0 U Sigma + c1 U R, R the sum
+    // of the rectangle staples, c0 = 1 - 8 c1; U_mu(x) is still in slot A
+    double2 W1[N], T[N];
+    unstage(ln, ln.slotB, W1);
+    __syncthreads();
+    stage_global(ln, ln.slotB, R, idx, nrt);
+    const double2 rf = gflag(R, idx, ok);
+#pragma unroll
+    for (int k = 0; k < N; k++) T[k] = zero2();
+    __syncthreads();
+    mul<NC, N>(ln, uflag, rf, true, T, M, false, nullptr, false, 1.0);
+    __syncthreads();
+    const double c0 = 1.0 - 8.0 * c1;
+#pragma unroll
+    for (int k = 0; k < N; k++) W1[k] = c_add(c_scale(c0, W1[k]), c_scale(c1, T[k]));
+    ln.stage(ln.slotB, W1);
+    __syncthreads();
+  }
 
   // D = tau_g reH(W)
   double2 Out[N];
@@ -465,6 +495,113 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_gauge_transform(Geo g
   if (ok && ln.e == 0) U.flag[idx] = nf;
 }
 
+// ------------------------------------------------------------ rectangle staples
+// R_mu(x) = sum_{nu != mu} staple2x1(x, mu, nu) (QCDenvNODEpt_test.h:579-611):
+// per plane the six 5-link staples of the 2x1 and 1x2 rectangles through
+// the link, each a path product L1 L2 L3 L4 L5 of links or their daggers,
+// formed one link at a time in place in slot A (18 paths, 72 products per
+// link).  With obs2: per block, Tr(U_mu(x) R_mu(x)) per order (the
+// rectangle plaquette of nspt.cc:406-410), as the drift's plaquette rows.
+template <int NC, int N>
+__global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_rect(Geo g, SerField U, SerField R, double* obs2) {
+  constexpr int NE = NC * NC;
+  using LN = Lane<NC, N>;
+  __shared__ double2 lds[LN::LDS];
+  LN ln(lds);
+  const int nrt = g.nrt;
+  const int M = min(N, nrt) - 1;
+  const LinkBlock lb = link_block<LN::LPB>(g);
+  const int mu = lb.mu;
+  const int site0 = lb.site0;
+  const int nvalid = min(LN::LPB, g.V - site0);
+  const int site = min(site0 + ln.j, g.V - 1);
+  const bool ok = ln.lds_ok && (site0 + ln.j) < g.V;
+  const long V = g.V;
+  const long idx = mu * V + site;
+  // staple s, link k: offset (dm along mu, dn along nu), direction (0 mu,
+  // 1 nu), dagger -- the six expressions of QCDenvNODEpt_test.h:591-607
+  constexpr signed char SH[6][5][4] = {
+      {{1, 0, 1, 0}, {1, 1, 1, 0}, {0, 2, 0, 1}, {0, 1, 1, 1}, {0, 0, 1, 1}},
+      {{1, 0, 0, 0}, {2, 0, 1, 0}, {1, 1, 0, 1}, {0, 1, 0, 1}, {0, 0, 1, 1}},
+      {{1, 0, 1, 0}, {0, 1, 0, 1}, {-1, 1, 0, 1}, {-1, 0, 1, 1}, {-1, 0, 0, 0}},
+      {{1, -1, 1, 1}, {1, -2, 1, 1}, {0, -2, 0, 1}, {0, -2, 1, 0}, {0, -1, 1, 0}},
+      {{1, 0, 0, 0}, {2, -1, 1, 1}, {1, -1, 0, 1}, {0, -1, 0, 1}, {0, -1, 1, 0}},
+      {{1, -1, 1, 1}, {0, -1, 0, 1}, {-1, -1, 0, 1}, {-1, -1, 1, 0}, {-1, 0, 0, 0}}};
+  auto link_of = [&](int step, long& li, bool& dg) {
+    const int p = step / 30, r = step % 30, st = r / 5, k = r % 5;
+    const int nu = (p < mu) ? p : p + 1;
+    const signed char* e = SH[st][k];
+    const int x = site_move(g, site_move(g, site, mu, e[0]), nu, e[1]);
+    li = (long)(e[2] ? nu : mu) * V + x;
+    dg = e[3] != 0;
+  };
+  double2 Sig[N];
+#pragma unroll
+  for (int k = 0; k < N; k++) Sig[k] = zero2();
+  double2 sf = zero2(), tf = zero2();
+  // the next link is loaded into registers while the current product runs
+  constexpr bool PF = N <= 24;
+  constexpr int NP = PF ? N : 1;
+  double2 PB[NP];
+  long li;
+  bool dg;
+  link_of(0, li, dg);
+  if constexpr (PF) gload<NC, N>(U, li, ln.e, nrt, ok, PB);
+  for (int step = 0; step < 90; step++) {
+    const int k = step % 5;
+    double2 fl = gflag(U, li, ok);
+    if (dg) fl = c_conj(fl);
+    double2* slot = (k == 0) ? ln.slotA : ln.slotB;
+    if constexpr (PF)
+      ln.stage(slot, PB, dg);
+    else
+      stage_global(ln, slot, U, li, nrt, dg);
+    if (step + 1 < 90) {
+      link_of(step + 1, li, dg);
+      if constexpr (PF) gload<NC, N>(U, li, ln.e, nrt, ok, PB);
+    }
+    if (k == 0) {
+      tf = fl;
+      continue;
+    }
+    __syncthreads();
+    if (k < 4) {
+      tf = mul_lds<NC, N>(ln, tf, fl, true, M, ln.slotA, false);
+    } else {
+      sf = c_add(sf, mul<NC, N>(ln, tf, fl, true, Sig, M, false, nullptr, false, 1.0));
+    }
+    __syncthreads();
+  }
+  gstore<NC, N>(R, idx, ln.e, nrt, ok, Sig);
+  if (ok && ln.e == 0) R.flag[idx] = sf;
+  if (!obs2) return;
+  // Tr(U_mu(x) R_mu(x)) per order, summed over the block's links
+  __shared__ double2 flagbuf[LN::LPB];
+  stage_global(ln, ln.slotA, U, idx, nrt);
+  ln.stage(ln.slotB, Sig);
+  const double2 uf = gflag(U, idx, ok);
+  __syncthreads();
+  const double2 wf = mul_lds<NC, N>(ln, uf, sf, true, M, ln.slotB, false);
+  if (ln.lds_ok && ln.e == 0) flagbuf[ln.j] = wf;
+  __syncthreads();
+  const int lane = threadIdx.x;
+  if (lane <= nrt) {
+    double2 acc = zero2();
+    for (int jj = 0; jj < nvalid; jj++) {
+      if (lane == 0) {
+        acc = c_add(acc, c_scale((double)NC, flagbuf[jj]));
+      } else {
+        const double2* m = lds + LN::LPB * LN::SLOT + jj * LN::SLOT + (lane - 1) * NE;
+#pragma unroll
+        for (int q = 0; q < NC; q++) acc = c_add(acc, m[q * NC + q]);
+      }
+    }
+    double* op = obs2 + (long)lb.part * 2 * (nrt + 1);
+    op[2 * lane] = acc.x;
+    op[2 * lane + 1] = acc.y;
+  }
+}
+
 // ------------------------------------------------------------ Wilson loops
 // Long links out_mu(x) = A_mu(x) A'_mu(x + k mu) with A' = A: the 2-link
 // field (k = 1, A = U) and the 4-link field (k = 2, A = the 2-link field)
@@ -595,8 +732,11 @@ KernelSet make_kernel_set() {
   s.lpb = Lane<NC, N>::LPB;
   s.qt = NSPT_QT;
   s.drift = [](int nblk, hipStream_t st, Geo g, SerField U, SerField D, SerField D1, double tau_g, double rk2c,
-               int mode, double* obs) {
-    hipLaunchKernelGGL((k_drift<NC, N>), dim3(nblk), dim3(64), 0, st, g, U, D, D1, tau_g, rk2c, mode, obs);
+               int mode, double* obs, SerField R, double c1) {
+    hipLaunchKernelGGL((k_drift<NC, N>), dim3(nblk), dim3(64), 0, st, g, U, D, D1, tau_g, rk2c, mode, obs, R, c1);
+  };
+  s.rect = [](int nblk, hipStream_t st, Geo g, SerField U, SerField R, double* obs2) {
+    hipLaunchKernelGGL((k_rect<NC, N>), dim3(nblk), dim3(64), 0, st, g, U, R, obs2);
   };
   s.update = [](int nblk, hipStream_t st, Geo g, SerField D, const double2* eta, double stau, SerField Ub,
                 SerField Uo, SerField L, double2* mm) {

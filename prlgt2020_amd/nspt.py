@@ -44,7 +44,7 @@ EXPORTS = ["nspt_create", "nspt_destroy", "nspt_last_error", "nspt_obs_len", "ns
            "nspt_rng_uniform", "nspt_rng_noise", "nspt_device_noise", "nspt_profile", "nspt_profile_read",
            "nspt_batch_create", "nspt_batch_destroy", "nspt_batch_last_error", "nspt_batch_evolve",
            "nspt_batch_replica", "nspt_batch_tune", "nspt_adjoint_drift", "nspt_loops",
-           "nspt_unitarize"]
+           "nspt_unitarize", "nspt_set_action"]
 
 PROF_NAMES = ["drift", "update", "update_log", "zero_mode", "fa_div", "dft", "fa_gexp", "transform", "noise",
               "reduce"]
@@ -199,6 +199,13 @@ class Context:
         self._check(lib().nspt_loops(self.h, _ptr(out)), "loops")
         return out.reshape(4, self.N + 1)
 
+    def set_action(self, c1):
+        """Gauge action: c1 = 0 Wilson, -1/12 TLSYM, -0.331 IWA, -1.4088 DBW2
+        (c0 = 1 - 8 c1); the observable row then ends with the rectangle
+        plaquette."""
+        self._check(lib().nspt_set_action(self.h, ctypes.c_double(c1)), "set_action")
+        self.obs_len = lib().nspt_obs_len(self.h)
+
     def unitarize(self):
         """U <- exp(reH(log U)) for every link (QCDenvNODEpt_test.h:415-418)."""
         self._check(lib().nspt_unitarize(self.h), "unitarize")
@@ -311,8 +318,12 @@ def adjoint_drift(L, nc, N, U, eps):
 
 
 def split_obs(obs, N):
-    """plaq (N+1 complex), norm (N), trU (4 x N complex)."""
+    """plaq (N+1 complex), norm (N), trU (4 x N complex); with an improved
+    action also the rectangle plaquette (N+1 complex)."""
     plaq = obs[0:2 * (N + 1)].view(np.complex128)
     norm = obs[2 * (N + 1):2 * (N + 1) + N]
-    tru = obs[2 * (N + 1) + N:].view(np.complex128).reshape(4, N)
+    o = 2 * (N + 1) + N
+    tru = obs[o:o + 8 * N].view(np.complex128).reshape(4, N)
+    if len(obs) > o + 8 * N:
+        return plaq, norm, tru, obs[o + 8 * N:].view(np.complex128)
     return plaq, norm, tru

@@ -228,3 +228,32 @@ def test_evolve_device_noise_matches_oracle(relerr):
         obs_o = O.step(L, nc, N, Uo, 0.02, 0.05, integrator=1, update=0, gauge_fix=1, zero_modes=1, eta=eta)
         assert np.abs(obs[s] - obs_o).max() <= 1e-11
     assert relerr(ctx.download(), Uo, nc, N).max() <= 1e-11
+
+
+@pytest.mark.parametrize("nc,c1,integrator", [(3, -1.0 / 12.0, 0), (3, -1.4088, 1), (2, -0.331, 1)])
+def test_improved_action_step(nc, c1, integrator, relerr):
+    """TLSYM / DBW2 / IWA drift (rectangle staples, nspt.cc:385-412) with zero
+    modes and FA: fields and observables, including the rectangle
+    plaquette, against the oracle, two steps."""
+    nspt = _nspt()
+    L, N = (2, 2, 4, 4), 6
+    V = 64
+    U = O.random_field(L, nc, N, seed=3, scale=0.4)
+    ctx = nspt.Context(L, nc, N)
+    ctx.set_action(c1)
+    ctx.upload(U)
+    rng = nspt.Ranlxd(1234)
+    Uo = U.copy()
+    for s in range(2):
+        eta = rng.noise(nc, 4 * V)
+        ctx.set_noise(eta)
+        obs_g = ctx.step(nspt.Context.params(0.02, 0.05, integrator, nspt.GF_FA, 1, nspt.NOISE_HOST))
+        obs_o = O.step(L, nc, N, Uo, 0.02, 0.05, integrator=integrator, update=0, gauge_fix=1, zero_modes=1, eta=eta,
+                       c1=c1)
+        e = relerr(ctx.download(), Uo, nc, N)
+        print("step", s, "per-order rel err", e)
+        assert e.max() <= TOL, e
+        assert len(obs_g) == len(obs_o)
+        for a, b in zip(nspt.split_obs(obs_g, N), O.split_obs(obs_o, N)):
+            assert np.abs(a - b).max() <= TOL * max(1.0, np.abs(b).max())
+    ctx.close()
