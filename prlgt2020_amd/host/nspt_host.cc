@@ -97,6 +97,7 @@ void attach(ptGluon_fld& U) {
   mi355x_release();
   nspt_geom g = {{U.Z->Sz[0], U.Z->Sz[1], U.Z->Sz[2], U.Z->Sz[3]}, NC, PTORD, allocORD};
   if (nspt_create(&g, &E.ctx) != 0) fail("nspt_create");
+  if (nspt_set_action(E.ctx, action_c1(mi355x_opts.action)) != 0) fail("nspt_set_action");
   E.field = &U;
   if (nspt_upload(E.ctx, reinterpret_cast<const double*>(U.W)) != 0) fail("nspt_upload");
 }
@@ -106,7 +107,12 @@ void write_step(const double* o) {
   const double* plaq = o;                  // N+1 complex
   const double* nrm = o + 2 * (N + 1);     // N
   const double* tru = nrm + N;             // [mu][order] complex
-  for (int i = 0; i <= N; i++) plaqfile << plaq[2 * i] << "\t" << plaq[2 * i + 1] << std::endl;
+  const double* w2 = tru + 8 * N;          // improved actions: rectangle plaquette, N+1 complex
+  for (int i = 0; i <= N; i++) {
+    plaqfile << plaq[2 * i] << "\t" << plaq[2 * i + 1];
+    if (mi355x_opts.action != 0) plaqfile << "\t" << w2[2 * i] << "\t" << w2[2 * i + 1];  // nspt.cc:575-578
+    plaqfile << std::endl;
+  }
   for (int i = 0; i < N; i++) {
     normfile << nrm[i] << std::endl;
     for (int mu = 0; mu < dim; mu++)
@@ -116,6 +122,15 @@ void write_step(const double* o) {
 }
 
 }  // namespace
+
+double action_c1(int action) {
+  switch (action) {
+    case 1: return -1.0 / 12.0;  // TLSYM
+    case 2: return -0.331;       // IWA
+    case 3: return -1.4088;      // DBW2
+    default: return 0.0;         // WIL
+  }
+}
 
 void mi355x_seed_noise(unsigned long seed) {
   if (E.rng) nspt_rng_destroy(E.rng);

@@ -125,7 +125,11 @@ typedef struct {
   int zero_modes;    // 1: REMOVE_GLUON_ZEROMOM
   int device_noise;  // 0: ranlxd stream of the reference (Rand[0]); 1: device Philox
   unsigned long long noise_seed;
+  int action;        // GAUGE_ACTION (choices.h:101-113): 0 WIL, 1 TLSYM, 2 IWA, 3 DBW2
 } mi355x_opts_t;
+
+// c1 of an action (Quenched.cc:313-322); c0 = 1 - 8 c1.
+double action_c1(int action);
 
 extern nspt_params_t nspt_pars;
 extern act_params_t act_pars;

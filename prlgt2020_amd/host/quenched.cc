@@ -52,7 +52,13 @@ int get_val(FILE* fp, const char* str, const char* fmt, void* val) {
   return 0;
 }
 
-const char* type_g() { return NC == 3 ? "SU3_wil" : "SU2_wil"; }
+// file-name tag of the group and action (Quenched.cc:325-347)
+const char* type_g() {
+  static const char* t3[4] = {"SU3_wil", "SU3_tls", "SU3_iwa", "SU3_dbw"};
+  static const char* t2[4] = {"SU2_wil", "SU2_tls", "SU2_iwa", "SU2_dbw"};
+  const int a = (mi355x_opts.action >= 0 && mi355x_opts.action < 4) ? mi355x_opts.action : 0;
+  return NC == 3 ? t3[a] : t2[a];
+}
 
 void stamp(const char* what) {
   std::time_t t;
@@ -93,7 +99,7 @@ int initialize(int argc, char** argv) {
   std::strcpy(nspt_pars.damon, "damocle.dag");
 
   int opt;
-  while ((opt = getopt(argc, argv, "l:s:b:d:r:g:a:t:p:c:n:I:F:Z:D:L:h")) != -1) {
+  while ((opt = getopt(argc, argv, "l:s:b:d:r:g:a:t:p:c:n:I:F:Z:D:L:A:h")) != -1) {
     switch (opt) {
       case 'l': act_pars.sz[0] = act_pars.sz[1] = act_pars.sz[2] = act_pars.sz[3] = std::atoi(optarg); break;
       case 's': nspt_pars.Sweep = std::atoi(optarg); break;
@@ -111,6 +117,7 @@ int initialize(int argc, char** argv) {
       case 'Z': mi355x_opts.zero_modes = std::atoi(optarg); break;
       case 'D': mi355x_opts.device_noise = std::atoi(optarg); break;
       case 'L': low_order = std::atoi(optarg); break;
+      case 'A': mi355x_opts.action = std::atoi(optarg); break;
       case 'h':
         std::printf("\n-l\t\tlattice size (L0 = L1 = L2 = L3)\n-s\t\tnumber of iterations (SWEEP)\n"
                     "-b\t\titerations between damocle reads (BEAT)\n-r\t\trandom seed\n-g\t\ttau gluons\n"
@@ -118,7 +125,8 @@ int initialize(int argc, char** argv) {
                     "-p\t\tplaquette file name\n-c\t\tsaved configuration name\n-d\t\tdamocle file name\n"
                     "-n\t\tperturbative order\n-I\t\tintegrator: 0 Euler, 1 RK2\n-F\t\tFA gauge fixing: 1 on, 0 off\n"
                     "-Z\t\tzero-mode subtraction: 1 on, 0 off\n-D\t\tnoise: 0 host ranlxd, 1 device Philox\n"
-                    "-L\t\twith -t 1: start from the configuration saved with this lower allocORD\n\n");
+                    "-L\t\twith -t 1: start from the configuration saved with this lower allocORD\n"
+                    "-A\t\tgauge action: 0 Wilson, 1 TLSYM, 2 IWA, 3 DBW2\n\n");
         std::exit(0);
     }
   }
@@ -193,6 +201,7 @@ int initialize(int argc, char** argv) {
           << "Damocle\t\t" << nspt_pars.damon << std::endl
           << "Norma\t\t" << nspt_pars.normn << std::endl
           << "Tr(U)\t\t" << nspt_pars.trun << std::endl
+          << "Gauge action\t" << type_g() + 4 << " (c1 = " << action_c1(mi355x_opts.action) << ")" << std::endl
           << "Integrator\t" << (mi355x_opts.integrator ? "RK2" : "Euler") << std::endl
           << "Noise\t\t" << (mi355x_opts.device_noise ? "device Philox4x32-10" : "ranlxd (Rand[0])") << std::endl
           << std::endl;

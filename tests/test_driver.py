@@ -28,8 +28,8 @@ def write_inputs(d, L=4, sweep=20, beat=10, tau=0.05, alpha=0.05, ptord=6, seed=
                 % (sweep, beat, tau, alpha, save))
 
 
-def names(L, alpha, tau, ptord, alloc_ord, nc=3):
-    stem = "SU%d_wil_nf0_%dx%dx%dx%d_a%2.2f_tg%3.3f" % (nc, L, L, L, L, abs(alpha), tau)
+def names(L, alpha, tau, ptord, alloc_ord, nc=3, action="wil"):
+    stem = "SU%d_%s_nf0_%dx%dx%dx%d_a%2.2f_tg%3.3f" % (nc, action, L, L, L, L, abs(alpha), tau)
     return {"plaq": "plaq_%s_o%d.txt" % (stem, ptord), "nxm": "plaq_nxm_%s_o%d.txt" % (stem, ptord),
             "nor": "%s_o%d.nor" % (stem, ptord),
             "trU": "%s_o%d.trU" % (stem, ptord), "log": "%s_o%d.log" % (stem, ptord),
@@ -160,3 +160,31 @@ def test_driver_restarts_from_lower_order(tmp_path):
     assert np.abs(f8 - want).max() < 1e-12 * np.abs(want).max()
     assert np.abs(f8[:, :S6] - f6).max() < 1e-12
     assert np.abs(f8[:, S6:]).max() > 1e-6  # the re-unitarised higher orders
+
+
+@pytest.mark.gpu
+def test_driver_improved_action(tmp_path):
+    """-A 1 (TLSYM, c1 = -1/12): file names tagged SU3_tls and the plaquette
+    file's rectangle columns (nspt.cc:573-578), against the oracle under the
+    same ranlxd noise, 10 Euler steps."""
+    from oracle import oracle as O
+    L, N, eps, alpha, steps, nc, c1 = 4, 6, 0.05, 0.05, 10, 3, -1.0 / 12.0
+    write_inputs(tmp_path, L=L, sweep=steps, beat=steps, tau=eps, alpha=alpha, ptord=N, seed=1)
+    out = subprocess.run([driver(nc, 6), "-A", "1"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    n = names(L, alpha, eps, N, 6, nc, "tls")
+    Lt = (L, L, L, L)
+    V = L ** 4
+    U = O.cold_field(Lt, nc, N)
+    rng = O.Ranlxd(O.glibc_first_rand(1))
+    rows = []
+    for _ in range(steps):
+        eta = rng.noise(nc, 4 * V)
+        obs = O.step(Lt, nc, N, U, eps, alpha, integrator=0, update=0, gauge_fix=1, zero_modes=1, eta=eta, c1=c1)
+        p, _, _, w2 = O.split_obs(obs, N)
+        rows.append(np.stack([p.real, p.imag, w2.real, w2.imag], 1))
+    want = np.concatenate(rows)
+    got = _read_cols(tmp_path / n["plaq"], 4)
+    assert got.shape == want.shape
+    assert np.abs(got - want).max() < 1e-12 * np.abs(want).max()
+    assert (tmp_path / n["conf"]).exists()
