@@ -42,11 +42,17 @@ def ex_flops(N, nc=3):
     return 64 * (N - 1) * N * (N + 1) // 6 + 16 * N * (N - 1)
 
 
-def kernel_flops(name, N, V, nc=3):
-    """Algorithmic FP64 flops of one launch of a kernel (4V links or V sites)."""
+ACTIONS = {"wil": 0.0, "tls": -1.0 / 12.0, "iwa": -0.331, "dbw": -1.4088}  # c1, Quenched.cc:313-322
+
+
+def kernel_flops(name, N, V, nc=3, improved=False):
+    """Algorithmic FP64 flops of one launch of a kernel (4V links or V sites).
+    An improved action adds U R to the drift and 72 products per link (six
+    5-link staples in each of three planes, QCDenvNODEpt_test.h:591-607) in
+    the rectangle kernel."""
     SP, EX = sp_flops(N, nc), ex_flops(N, nc)
-    per_link = {"drift": 13 * SP, "update": EX + SP, "update_log": 2 * EX + SP, "zero_mode": EX,
-                "transform": 2 * SP}
+    per_link = {"drift": (14 if improved else 13) * SP, "update": EX + SP, "update_log": 2 * EX + SP,
+                "zero_mode": EX, "transform": 2 * SP, "rect": 72 * SP}
     if name in per_link:
         return per_link[name] * 4 * V
     if name == "fa_gexp":
@@ -54,9 +60,11 @@ def kernel_flops(name, N, V, nc=3):
     return 0
 
 
-def step_flops(N, V, integrator, nc=3):
+def step_flops(N, V, integrator, nc=3, improved=False):
     SP, EX = sp_flops(N, nc), ex_flops(N, nc)
     per_link = (30 * SP + 4.25 * EX) if integrator == 1 else (16 * SP + 3.25 * EX)
+    if improved:
+        per_link += (2 if integrator == 1 else 1) * 73 * SP
     return per_link * 4 * V
 
 
@@ -123,6 +131,8 @@ def main():
     ap.add_argument("--nc", type=int, choices=[3, 2], default=3)
     ap.add_argument("--eps", type=float, default=0.02)
     ap.add_argument("--integrator", choices=["rk2", "euler"], default="rk2")
+    ap.add_argument("--action", choices=list(ACTIONS), default="wil",
+                    help="gauge action: Wilson (the default workload) or an improved one")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched epsilon-replica sweep")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
@@ -148,6 +158,9 @@ def main():
     V = args.L ** 4
     integ = nspt.RK2 if args.integrator == "rk2" else nspt.EULER
     ctx = nspt.Context(L, nc, N)
+    improved = args.action != "wil"
+    if improved:
+        ctx.set_action(ACTIONS[args.action])
     p = nspt.Context.params(args.eps, 0.05, integ, nspt.GF_FA, 1, nspt.NOISE_DEVICE, seed=1, step=0)
     ctx.evolve(p, args.warmup, want_obs=False)
     import ctypes
@@ -165,18 +178,18 @@ def main():
     name = max(prof, key=lambda k: prof[k][0])
     tot_ms, cnt = prof[name]
     avg_s = tot_ms / cnt / 1e3
-    fl = kernel_flops(name, N, V, nc)
+    fl = kernel_flops(name, N, V, nc, improved)
     achieved = fl / avg_s / 1e12 if fl else 0.0
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_su3_16x4_N20_rk2.md")
-    if os.path.exists(tpath) and (args.L, N, args.integrator) == (16, 20, "rk2"):
+    if os.path.exists(tpath) and (args.L, N, args.integrator, args.action) == (16, 20, "rk2", "wil"):
         # HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE counter runs
         # recorded there (calibrated against a copy of known size)
         for line in open(tpath):
             f = line.split()
             if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == name:
                 traffic = float(f[2])
-    step_tf = step_flops(N, V, integ, nc) * args.steps / dt / 1e12
+    step_tf = step_flops(N, V, integ, nc, improved) * args.steps / dt / 1e12
 
     out = {
         "metric": "link-order updates/s per Langevin step (FP64), plain single replica",
@@ -191,8 +204,10 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic: cold start, device Philox noise",
-        "config": {"workload": "SU(%d) Wilson NSPT %d^4, %d orders, %s eps=%g, zero modes + FA gauge fixing "
-                               "(alpha=0.05) every step" % (nc, args.L, N, args.integrator.upper(), args.eps),
+        "config": {"workload": "SU(%d) %s NSPT %d^4, %d orders, %s eps=%g, zero modes + FA gauge fixing "
+                               "(alpha=0.05) every step"
+                               % (nc, {"wil": "Wilson", "tls": "tree-level Symanzik", "iwa": "Iwasaki",
+                                       "dbw": "DBW2"}[args.action], args.L, N, args.integrator.upper(), args.eps),
                    "lattice": list(L), "orders": N, "integrator": args.integrator, "replicas": 1},
         "roofline": {"bound": "mfma", "kernel": name, "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,

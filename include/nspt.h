@@ -43,16 +43,22 @@ typedef struct {
   int alloc_ord; /* allocORD: width of the host field image, >= ptord */
 } nspt_geom;
 
-enum { NSPT_EULER = 0, NSPT_RK2 = 1 };
-enum { NSPT_GF_NONE = 0, NSPT_GF_FA = 1 };
+/* NSPT_EULER: Jacobi Euler step (the reference built without UPDATE_ONTHEFLY).
+ * NSPT_EULER_CHECKERBOARD: the in-place on-the-fly sweep of nspt.cc:365-441
+ *   in checkerboard order (parity 0 then 1, mu = 0..3, each subset of links
+ *   at once); Wilson action, even extents.
+ * NSPT_GF_CHECKERBOARD: the sequential gauge fixing of nspt.cc:660-862 with
+ *   the sites of parity 0, then 1; even extents. */
+enum { NSPT_EULER = 0, NSPT_RK2 = 1, NSPT_EULER_CHECKERBOARD = 2 };
+enum { NSPT_GF_NONE = 0, NSPT_GF_FA = 1, NSPT_GF_CHECKERBOARD = 2 };
 enum { NSPT_NOISE_HOST = 0, NSPT_NOISE_DEVICE = 1, NSPT_NOISE_ZERO = 2 };
 
 /* One Langevin step (nspt.cc:963-1033, Jacobi update of nspt.cc:428-429). */
 typedef struct {
   double eps;        /* tau_g of Quench.cfg: Langevin time step */
   double alpha;      /* gauge-fixing alpha; FA is stable for alpha > 0 */
-  int integrator;    /* NSPT_EULER or NSPT_RK2 */
-  int gauge_fix;     /* NSPT_GF_NONE or NSPT_GF_FA */
+  int integrator;    /* NSPT_EULER, NSPT_RK2 or NSPT_EULER_CHECKERBOARD */
+  int gauge_fix;     /* NSPT_GF_NONE, NSPT_GF_FA or NSPT_GF_CHECKERBOARD */
   int zero_modes;    /* 1: zero_modes_subtraction (REMOVE_GLUON_ZEROMOM) */
   int noise;         /* NSPT_NOISE_HOST (set by nspt_set_noise), _DEVICE, _ZERO */
   unsigned long long seed; /* device noise: Philox4x32-10 key */
@@ -143,7 +149,8 @@ enum {
   NSPT_PROF_TRANSFORM = 7,  /* U <- G U G^dag */
   NSPT_PROF_NOISE = 8,      /* device noise */
   NSPT_PROF_REDUCE = 9,     /* deterministic reductions */
-  NSPT_PROF_N = 10
+  NSPT_PROF_RECT = 10,      /* rectangle staples of an improved action */
+  NSPT_PROF_N = 11
 };
 int nspt_profile(nspt_ctx* ctx, int enable);
 int nspt_profile_read(nspt_ctx* ctx, double* ms_total, long long* count);

@@ -662,8 +662,10 @@ struct Params {
   double eps;        // Langevin step epsilon (tau_g of Quench.cfg)
   double alpha;      // gauge-fixing alpha (reference sign conventions)
   int integrator;    // 0 Euler, 1 RK2
-  int update;        // 0 Jacobi, 1 on-the-fly (sequential, in place)
-  int gauge_fix;     // 0 none, 1 Fourier accelerated, 2 default in-place
+  int update;        // 0 Jacobi, 1 on-the-fly (sequential, in place),
+                     // 2 on the fly in checkerboard x direction order
+  int gauge_fix;     // 0 none, 1 Fourier accelerated, 2 default in-place,
+                     // 3 default in-place in checkerboard site order
   int zero_modes;    // 1: zero_modes_subtraction
   double c1 = 0.0;   // rectangle coefficient: 0 Wilson, -1/12 TLSYM, -0.331 IWA, -1.4088 DBW2
 };
@@ -867,6 +869,40 @@ void sgf_default(Field& F, double alpha) {
   }
 }
 
+// parity of a site, (x0+x1+x2+x3) mod 2
+int site_parity(const Latt& Z, int s) {
+  int x[4];
+  Z.coords(s, x);
+  return (x[0] + x[1] + x[2] + x[3]) & 1;
+}
+
+// The sequential gauge fixing of nspt.cc:660-862 with its sites taken in
+// checkerboard order: all sites of parity 0, then of parity 1.  Sites of one
+// parity share no link, so each pass is order-independent (the GPU form).
+void sgf_checkerboard(Field& F, double alpha) {
+  const Latt& Z = *F.lat;
+  for (int par = 0; par < 2; par++)
+    for (int s = 0; s < Z.V; s++) {
+      if (site_parity(Z, s) != par) continue;
+      Ser W(F.nc, F.N);
+      for (int mu = 0; mu < 4; mu++) {
+        Ser d = ser_sub(F.get(s, mu), F.get(Z.dn[s * 4 + mu], mu));
+        for (size_t q = 0; q < W.v.size(); q++) W.v[q] += d.v[q];
+      }
+      W = ser_sub(W, ser_dag(W));
+      ser_trless(W);
+      Ser Wa = W, Wm = W;
+      ser_scale(Wa, alpha);
+      ser_scale(Wm, -alpha);
+      Ser G1 = ser_exp(Wa), G2 = ser_exp(Wm);
+      for (int mu = 0; mu < 4; mu++) {
+        F.set(s, mu, ser_mul(G1, F.get(s, mu)));
+        const int b = Z.dn[s * 4 + mu];
+        F.set(b, mu, ser_mul(F.get(b, mu), G2));
+      }
+    }
+}
+
 // RK2 corrector drift: (D1 + D2)/2 times (1 + (nc eps/6) x^2), x = beta^-1/2.
 Ser rk2_combine(const Ser& D1, const Ser& D2, int nc, double eps) {
   Ser D = ser_add(D1, D2);
@@ -891,7 +927,21 @@ void step(const Geom& g, double* Udata, const Params& p, const double* eta, doub
   Obs obs;
   obs.init(g.N, p.c1 != 0.0);
 
-  if (p.update == 1) {
+  if (p.update == 2) {
+    // on the fly in checkerboard order: for parity 0 then 1, for mu = 0..3,
+    // every link (x, mu) of that parity at once (their Wilson staples use no
+    // link of the same direction and parity)
+    for (int par = 0; par < 2; par++)
+      for (int mu = 0; mu < 4; mu++)
+        for (int s = 0; s < Z.V; s++) {
+          if (site_parity(Z, s) != par) continue;
+          Ser W = drift(U, s, mu, tau_g, &obs, p.c1);
+          sub_noise(W, stau, eta + (size_t)(4 * s + mu) * ne2);
+          Ser Un = ser_mul(ser_exp(W), U.get(s, mu));
+          U.set(s, mu, Un);
+          MM[mu] = ser_add(MM[mu], ser_log(Un));
+        }
+  } else if (p.update == 1) {
     // on the fly: in place, lexicographic site order, mu inner (nspt.cc:365-441)
     for (int s = 0; s < Z.V; s++)
       for (int mu = 0; mu < 4; mu++) {
@@ -945,6 +995,7 @@ void step(const Geom& g, double* Udata, const Params& p, const double* eta, doub
   if (p.zero_modes) zero_modes(U, MM);
   if (p.gauge_fix == 1) fa_gauge_fixing(U, p.alpha);
   if (p.gauge_fix == 2) sgf_default(U, p.alpha);
+  if (p.gauge_fix == 3) sgf_checkerboard(U, p.alpha);
   if (obs_out) obs.write(g.nc, Z.V, obs_out);
 }
 
@@ -1154,6 +1205,13 @@ void orc_sgf_default(const int* gi, double* U, double alpha) {
 }
 
 void orc_compute_loops(const int* gi, const double* U, double* out) { compute_loops(geom_from(gi), U, out); }
+void orc_sgf_checkerboard(const int* gi, double* U, double alpha) {
+  Geom g = geom_from(gi);
+  Latt Z(g.L);
+  const int S = 1 + g.nc * g.nc * g.N;
+  Field F{g.nc, g.N, S, &Z, U};
+  sgf_checkerboard(F, alpha);
+}
 void orc_plaquette_measure(const int* gi, const double* U, double* out) {
   plaquette_measure(geom_from(gi), U, out);
 }

@@ -183,6 +183,11 @@ def fa_gauge_fixing(L, nc, N, U, alpha):
     lib().orc_fa_gauge_fixing(_geom(L, nc, N), _ptr(U), alpha)
 
 
+def sgf_checkerboard(L, nc, N, U, alpha):
+    """The sequential gauge fixing with sites in checkerboard order, in place."""
+    lib().orc_sgf_checkerboard(_geom(L, nc, N), _ptr(U), ctypes.c_double(alpha))
+
+
 def sgf_default(L, nc, N, U, alpha):
     lib().orc_sgf_default(_geom(L, nc, N), _ptr(U), alpha)
 

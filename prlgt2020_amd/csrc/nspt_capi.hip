@@ -769,9 +769,12 @@ bool reduce4(nspt_ctx* c, const double* in, long ld, int ncols, double* out, dou
   return reduce_groups(c, in, ld, c->g.nb_mu, 4, ncols, out, scale, c->red_tmp);
 }
 
-// with_w2: the rectangle plaquette of an improved action is in obs2_part
-bool obs_reduce(nspt_ctx* c, double* out_dev, bool with_w2 = false) {
-  if (!reduce4(c, c->obs_part, c->nob, c->nob, c->obs_mu, 1.0)) return false;
+// with_w2: the rectangle plaquette of an improved action is in obs2_part;
+// rows: per-direction rows of obs_part (nb_mu, or 2 nb_half after a
+// checkerboard sweep)
+bool obs_reduce(nspt_ctx* c, double* out_dev, bool with_w2 = false, long rows = 0) {
+  if (!reduce_groups(c, c->obs_part, c->nob, rows ? rows : c->g.nb_mu, 4, c->nob, c->obs_mu, 1.0, c->red_tmp))
+    return false;
   if (with_w2 && !reduce4(c, c->obs2_part, 2 * (c->nrt + 1), 2 * (c->nrt + 1), c->obs2_mu, 1.0)) return false;
   Prof pr(c, NSPT_PROF_REDUCE);
   hipLaunchKernelGGL(k_obs_finalize, dim3(1), dim3(256), 0, c->stream, (const double*)c->obs_mu,
@@ -849,10 +852,30 @@ bool do_step(nspt_ctx* c, const nspt_step_params* p, double* obs_dev) {
   const SerField R = impr ? c->R : none;
   auto rect = [&](const SerField& Uin, bool with_obs) {
     if (!impr) return;
-    Prof pr(c, NSPT_PROF_DRIFT);
+    Prof pr(c, NSPT_PROF_RECT);
     c->ks.rect(c->nblk_links, c->stream, c->g, Uin, c->R, with_obs && obs_part ? c->obs2_part : nullptr);
   };
-  if (p->integrator == NSPT_RK2) {
+  const bool cb = p->integrator == NSPT_EULER_CHECKERBOARD;
+  if (cb) {
+    // on the fly in checkerboard order (the reference's in-place sweep of
+    // nspt.cc:365-441 reordered): for parity 0 then 1, for mu = 0..3, every
+    // link (x, mu) of that parity at once -- their staples use no link of
+    // the same direction and parity -- drift, then the update in place
+    Geo gs = c->g;
+    for (int par = 0; par < 2; par++)
+      for (int mu = 0; mu < 4; mu++) {
+        gs.sub = 1 + 4 * par + mu;
+        {
+          Prof pr(c, NSPT_PROF_DRIFT);
+          c->ks.drift(c->g.nb_half, c->stream, gs, c->U, c->D, none, tau_g, 0.0, 0, obs_part, none, 0.0);
+        }
+        Prof pr(c, need_log ? NSPT_PROF_UPDATE_LOG : NSPT_PROF_UPDATE);
+        c->ks.update(c->g.nb_half, c->stream, gs, c->D, eta, stau, c->U, c->U, need_log ? c->L : none,
+                     p->zero_modes ? c->mm_part : nullptr);
+      }
+    if (!launch_ok(c, "checkerboard sweep")) return false;
+    if (obs_dev && !obs_reduce(c, obs_dev, false, 2L * c->g.nb_half)) return false;
+  } else if (p->integrator == NSPT_RK2) {
     rect(c->U, true);
     {
       Prof pr(c, NSPT_PROF_DRIFT);
@@ -872,33 +895,45 @@ bool do_step(nspt_ctx* c, const nspt_step_params* p, double* obs_dev) {
     Prof pr(c, NSPT_PROF_DRIFT);
     c->ks.drift(c->nblk_links, c->stream, c->g, c->U, c->D, none, tau_g, 0.0, 0, obs_part, R, c->c1);
   }
-  if (!launch_ok(c, "drift")) return false;
-  if (obs_dev && !obs_reduce(c, obs_dev, impr)) return false;
-  {
-    Prof pr(c, need_log ? NSPT_PROF_UPDATE_LOG : NSPT_PROF_UPDATE);
-    c->ks.update(c->nblk_links, c->stream, c->g, c->D, eta, stau, c->U, c->F, need_log ? c->L : none,
-                 p->zero_modes ? c->mm_part : nullptr);
+  if (!cb) {
+    if (!launch_ok(c, "drift")) return false;
+    if (obs_dev && !obs_reduce(c, obs_dev, impr)) return false;
+    {
+      Prof pr(c, need_log ? NSPT_PROF_UPDATE_LOG : NSPT_PROF_UPDATE);
+      c->ks.update(c->nblk_links, c->stream, c->g, c->D, eta, stau, c->U, c->F, need_log ? c->L : none,
+                   p->zero_modes ? c->mm_part : nullptr);
+    }
+    if (!launch_ok(c, "update")) return false;
+    std::swap(c->U, c->F);  // Umu = Fmu (nspt.cc:992)
   }
-  if (!launch_ok(c, "update")) return false;
-  std::swap(c->U, c->F);  // Umu = Fmu (nspt.cc:992)
   const SerField* xsrc = &c->L;
   if (p->zero_modes) {
     const int ncol = 2 * c->nrt * c->NE;
-    if (!reduce4(c, reinterpret_cast<const double*>(c->mm_part), ncol, ncol, reinterpret_cast<double*>(c->MM),
-                 1.0 / (double)c->V))
+    if (!reduce_groups(c, reinterpret_cast<const double*>(c->mm_part), ncol, cb ? 2L * c->g.nb_half : c->g.nb_mu,
+                       4, ncol, reinterpret_cast<double*>(c->MM), 1.0 / (double)c->V, c->red_tmp))
       return false;
     if (p->gauge_fix != NSPT_GF_FA) {
       Prof pr(c, NSPT_PROF_ZERO_MODE);
       c->ks.zero_mode(c->nblk_links, c->stream, c->g, c->L, c->MM, c->U, c->X);
-      return launch_ok(c, "zero_mode");
+      if (!launch_ok(c, "zero_mode")) return false;
     }
-    // FA follows: the divergence of L - MM is that of L, and the gauge
+    // with FA the divergence of L - MM is that of L, and the gauge
     // transform forms U = exp(L - MM) itself
   }
   if (p->gauge_fix == NSPT_GF_FA &&
       !fa_apply(c, *xsrc, p->alpha, p->zero_modes ? reinterpret_cast<const double2*>(c->MM) : nullptr,
                 p->zero_modes != 0))
     return false;
+  if (p->gauge_fix == NSPT_GF_CHECKERBOARD) {
+    // the sequential gauge fixing of nspt.cc:660-862 in checkerboard site order
+    Geo gs = c->g;
+    for (int par = 0; par < 2; par++) {
+      gs.sub = 1 + 4 * par;
+      Prof pr(c, NSPT_PROF_TRANSFORM);
+      c->ks.sgf(c->g.nb_half, c->stream, gs, c->U, p->alpha);
+    }
+    if (!launch_ok(c, "checkerboard gauge fixing")) return false;
+  }
   return true;
 }
 
@@ -947,6 +982,8 @@ int nspt_create(const nspt_geom* geom, nspt_ctx** out) {
   c->g.V = c->V;
   c->g.nrt = c->nrt;
   c->g.nb_mu = (c->V + c->ks.lpb - 1) / c->ks.lpb;
+  c->g.sub = 0;
+  c->g.nb_half = (c->V / 2 + c->ks.lpb - 1) / c->ks.lpb;
   c->nblk_links = 4 * c->g.nb_mu;
   c->nblk_sites = c->g.nb_mu;
   c->nob = 2 * (c->nrt + 1) + 3 * c->nrt;
@@ -954,15 +991,17 @@ int nspt_create(const nspt_geom* geom, nspt_ctx** out) {
   *out = c;
   if (!c->hip_ok(hipStreamCreate(&c->stream), "hipStreamCreate")) return -3;
   const long P = 4L * V;
+  // per-block partial sums: 4 nb_mu rows, or 8 nb_half for a checkerboard sweep
+  const long part_rows = std::max(4L * c->g.nb_mu, 8L * c->g.nb_half);
   bool ok = c->alloc_field(c->U, P, true) && c->alloc_field(c->F, P, true) && c->alloc_field(c->D, P, false) &&
             c->alloc_field(c->D1, P, false) && c->alloc_field(c->L, P, true) && c->alloc_field(c->X, P, false) &&
             c->alloc_field(c->G, V, true) &&
             c->alloc(&c->fb, (size_t)c->nrt * c->NE * V) && c->alloc(&c->eta, (size_t)P * c->NE) &&
             c->alloc(&c->eta_tmp, (size_t)P * c->NE) &&
-            c->alloc(&c->mm_part, (size_t)c->nblk_links * c->nrt * c->NE) &&
-            c->alloc(&c->MM, (size_t)4 * c->nrt * c->NE) && c->alloc(&c->obs_part, (size_t)c->nblk_links * c->nob) &&
+            c->alloc(&c->mm_part, (size_t)part_rows * c->nrt * c->NE) &&
+            c->alloc(&c->MM, (size_t)4 * c->nrt * c->NE) && c->alloc(&c->obs_part, (size_t)part_rows * c->nob) &&
             c->alloc(&c->obs_mu, (size_t)4 * c->nob) &&
-            c->alloc(&c->red_tmp, reduce_tmp_doubles(c->g.nb_mu, 4, std::max(c->nob, 2 * c->nrt * c->NE))) &&
+            c->alloc(&c->red_tmp, reduce_tmp_doubles(part_rows / 4, 4, std::max(c->nob, 2 * c->nrt * c->NE))) &&
             c->alloc(&c->aos_buf, (size_t)P * (1 + (size_t)c->NE * c->A)) && c->alloc(&c->ctr_dev, 2) &&
             ensure_obs(c, 1);
   if (!ok) return -3;
@@ -1036,8 +1075,16 @@ int nspt_set_noise(nspt_ctx* c, const double* eta) {
 static int check_params(nspt_ctx* c, const nspt_step_params* p) {
   if (!p) return c->fail("null step parameters");
   if (!(p->eps > 0.0)) return c->fail("eps must be > 0");
-  if (p->integrator != NSPT_EULER && p->integrator != NSPT_RK2) return c->fail("unknown integrator");
-  if (p->gauge_fix != NSPT_GF_NONE && p->gauge_fix != NSPT_GF_FA) return c->fail("unknown gauge fixing");
+  if (p->integrator != NSPT_EULER && p->integrator != NSPT_RK2 && p->integrator != NSPT_EULER_CHECKERBOARD)
+    return c->fail("unknown integrator");
+  if (p->gauge_fix != NSPT_GF_NONE && p->gauge_fix != NSPT_GF_FA && p->gauge_fix != NSPT_GF_CHECKERBOARD)
+    return c->fail("unknown gauge fixing");
+  if (p->integrator == NSPT_EULER_CHECKERBOARD || p->gauge_fix == NSPT_GF_CHECKERBOARD) {
+    for (int d = 0; d < 4; d++)
+      if (c->g.L[d] % 2) return c->fail("checkerboard passes need even lattice extents");
+  }
+  if (p->integrator == NSPT_EULER_CHECKERBOARD && c->c1 != 0.0)
+    return c->fail("the checkerboard sweep is for the Wilson action (rectangles couple links of one parity)");
   if (p->noise < NSPT_NOISE_HOST || p->noise > NSPT_NOISE_ZERO) return c->fail("unknown noise source");
   return 0;
 }

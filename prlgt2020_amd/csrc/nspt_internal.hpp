@@ -15,6 +15,11 @@ struct Geo {
   int V;
   int nrt;    // runtime order (PTORD)
   int nb_mu;  // link blocks per direction = ceil(V / links-per-block)
+  // Link subset of a checkerboard pass (0: every link): sub = 1 + 4 parity
+  // + mu selects the links (x, mu) with (x0+x1+x2+x3) mod 2 = parity, in
+  // nb_half blocks (ceil(V/2 / links-per-block)); needs L_3 even.
+  int sub;
+  int nb_half;
 };
 
 // Launchers of one (NC, N) instance.  nblk is the grid size in waves.
@@ -25,6 +30,8 @@ struct KernelSet {
   // sum of the rectangle staples of every link, and (obs2 != null) their
   // per-block plaquette traces
   void (*rect)(int nblk, hipStream_t, Geo, SerField U, SerField R, double* obs2) = nullptr;
+  // one checkerboard pass of the sequential gauge fixing (g.sub = 1 + 4 parity)
+  void (*sgf)(int nblk, hipStream_t, Geo, SerField U, double alpha) = nullptr;
   void (*update)(int nblk, hipStream_t, Geo, SerField D, const double2* eta, double stau, SerField Ubase,
                  SerField Uout, SerField L, double2* mm) = nullptr;
   void (*logf)(int nblk, hipStream_t, Geo, SerField U, SerField L) = nullptr;

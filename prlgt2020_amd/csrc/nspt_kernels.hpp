@@ -59,20 +59,41 @@ __device__ __forceinline__ int block_order() {
 #endif
 }
 
-// Link block of this workgroup: direction mu, first site site0, and the
-// index of its per-block partial sums (mu-major, as the reductions expect).
+// Link block of this workgroup: direction mu, first index site0 in the list
+// of its sites (the lattice, or a checkerboard subset of count = V/2 sites),
+// and the index of its per-block partial sums (mu-major, as the reductions
+// expect: a subset pass (mu, parity) owns rows (2 mu + parity) nb_half + b).
 struct LinkBlock {
-  int mu, site0, part;
+  int mu, site0, part, count;
 };
 template <int LPB>
 __device__ __forceinline__ LinkBlock link_block(const Geo& g) {
   const int b = block_order();
+  if (g.sub) {
+    const int mu = (g.sub - 1) & 3, par = (g.sub - 1) >> 2;
+    return {mu, b * LPB, (2 * mu + par) * g.nb_half + b, g.V / 2};
+  }
 #if NSPT_BLOCK_ORDER >= 1
   const int mu = b & 3, sb = b >> 2;
 #else
   const int mu = b / g.nb_mu, sb = b % g.nb_mu;
 #endif
-  return {mu, sb * LPB, mu * g.nb_mu + sb};
+  return {mu, sb * LPB, mu * g.nb_mu + sb, g.V};
+}
+
+// Site of entry k of a block's site list: k itself, or the k-th site of the
+// subset's parity (with L_3 even, sites 2k and 2k+1 share x0, x1, x2 and
+// have opposite parities).
+__device__ __forceinline__ int link_site(const Geo& g, int k) {
+  if (!g.sub) return k;
+  const int par = (g.sub - 1) >> 2;
+  const int s2 = 2 * k;
+  int r = s2 / g.L[3];
+  const int x2 = r % g.L[2];
+  r /= g.L[2];
+  const int x1 = r % g.L[1];
+  const int x0 = r / g.L[1];
+  return s2 + (((x0 + x1 + x2) ^ par) & 1);
 }
 
 // Lane-owned element of series idx of field f, all orders, into an LDS slot.
@@ -126,9 +147,9 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_drift(Geo g, SerField
   const LinkBlock lb = link_block<LN::LPB>(g);
   const int mu = lb.mu;
   const int site0 = lb.site0;
-  const int nvalid = min(LN::LPB, g.V - site0);
-  const int site = min(site0 + ln.j, g.V - 1);
-  const bool ok = ln.lds_ok && (site0 + ln.j) < g.V;
+  const int nvalid = min(LN::LPB, lb.count - site0);
+  const int site = link_site(g, min(site0 + ln.j, lb.count - 1));
+  const bool ok = ln.lds_ok && (site0 + ln.j) < lb.count;
   const long V = g.V;
   const long idx = mu * V + site;
 
@@ -313,9 +334,9 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_update(Geo g, SerFiel
   const LinkBlock lb = link_block<LN::LPB>(g);
   const int mu = lb.mu;
   const int site0 = lb.site0;
-  const int nvalid = min(LN::LPB, g.V - site0);
-  const int site = min(site0 + ln.j, g.V - 1);
-  const bool ok = ln.lds_ok && (site0 + ln.j) < g.V;
+  const int nvalid = min(LN::LPB, lb.count - site0);
+  const int site = link_site(g, min(site0 + ln.j, lb.count - 1));
+  const bool ok = ln.lds_ok && (site0 + ln.j) < lb.count;
   const long idx = (long)mu * g.V + site;
 
   // generator W = D - stau*eta (order 1) into slot A, Ubase into slot B
@@ -495,6 +516,90 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_gauge_transform(Geo g
   if (ok && ln.e == 0) U.flag[idx] = nf;
 }
 
+// ------------------------------------------------------------ checkerboard gauge fixing
+// One pass of the sequential gauge fixing of nspt.cc:660-862 over the sites
+// of one parity (g.sub = 1 + 4 parity), which share no link: per site
+// W = sum_mu U_mu(x) - U_mu(x-mu), W <- Trless(W - W^dag), G = exp(alpha W),
+// then U_mu(x) <- G U_mu(x) and U_mu(x-mu) <- U_mu(x-mu) G^dag
+// (exp(-alpha W) = G^dag, W antihermitian).
+template <int NC, int N>
+__global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_sgf(Geo g, SerField U, double alpha) {
+  constexpr int NE = NC * NC;
+  using LN = Lane<NC, N>;
+  __shared__ double2 lds[LN::LDS];
+  LN ln(lds);
+  const int nrt = g.nrt;
+  const int M = min(N, nrt) - 1;
+  const int cnt = g.V / 2;
+  const int k0 = block_order() * LN::LPB;
+  const int site = link_site(g, min(k0 + ln.j, cnt - 1));
+  const bool ok = ln.lds_ok && (k0 + ln.j) < cnt;
+  const long V = g.V;
+  double2 W[N], P[N];
+#pragma unroll
+  for (int k = 0; k < N; k++) W[k] = zero2();
+  for (int mu = 0; mu < 4; mu++) {
+    gload<NC, N>(U, mu * V + site, ln.e, nrt, ok, P);
+#pragma unroll
+    for (int k = 0; k < N; k++) W[k] = c_add(W[k], P[k]);
+    gload<NC, N>(U, mu * V + site_move(g, site, mu, -1), ln.e, nrt, ok, P);
+#pragma unroll
+    for (int k = 0; k < N; k++) W[k] = c_sub(W[k], P[k]);
+  }
+  // W - W^dag: this lane's element minus the conjugate of the transposed one
+  ln.stage(ln.slotA, W);
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < N; k++) W[k] = c_sub(ln.slotA[k * NE + ln.e], c_conj(ln.slotA[k * NE + ln.c * NC + ln.r]));
+  __syncthreads();
+  // Trless (QCDpt.cc:124-136): Tr/NC off the diagonal of every order
+  ln.stage(ln.slotA, W);
+  __syncthreads();
+  if (ln.r == ln.c) {
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+      double2 tr = zero2();
+#pragma unroll
+      for (int q = 0; q < NC; q++) tr = c_add(tr, ln.slotA[k * NE + q * NC + q]);
+      W[k] = c_sub(W[k], c_scale(1.0 / NC, tr));
+    }
+  }
+  __syncthreads();
+  // G = exp(alpha W)
+#pragma unroll
+  for (int k = 0; k < N; k++) W[k] = c_scale(alpha, W[k]);
+  ln.stage(ln.slotB, W);
+  horner<NC, N, true>(ln, M + 1);
+  unstage(ln, ln.slotA, W);  // W holds G from here on
+  __syncthreads();
+  for (int mu = 0; mu < 4; mu++) {
+    // U_mu(x) <- G U_mu(x), in place in slot B
+    const long i1 = mu * V + site;
+    ln.stage(ln.slotA, W);
+    stage_global(ln, ln.slotB, U, i1, nrt);
+    const double2 f1 = gflag(U, i1, ok);
+    __syncthreads();
+    const double2 n1 = mul_lds<NC, N>(ln, one2(), f1, true, M, ln.slotB, false);
+    __syncthreads();
+    unstage(ln, ln.slotB, P);
+    gstore<NC, N>(U, i1, ln.e, nrt, ok, P);
+    if (ok && ln.e == 0) U.flag[i1] = n1;
+    __syncthreads();
+    // U_mu(x-mu) <- U_mu(x-mu) G^dag, in place in slot A
+    const long i2 = mu * V + site_move(g, site, mu, -1);
+    stage_global(ln, ln.slotA, U, i2, nrt);
+    ln.stage(ln.slotB, W, true);
+    const double2 f2 = gflag(U, i2, ok);
+    __syncthreads();
+    const double2 n2 = mul_lds<NC, N>(ln, f2, one2(), true, M, ln.slotA, false);
+    __syncthreads();
+    unstage(ln, ln.slotA, P);
+    gstore<NC, N>(U, i2, ln.e, nrt, ok, P);
+    if (ok && ln.e == 0) U.flag[i2] = n2;
+    __syncthreads();
+  }
+}
+
 // ------------------------------------------------------------ rectangle staples
 // R_mu(x) = sum_{nu != mu} staple2x1(x, mu, nu) (QCDenvNODEpt_test.h:579-611):
 // per plane the six 5-link staples of the 2x1 and 1x2 rectangles through
@@ -513,9 +618,9 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_rect(Geo g, SerField 
   const LinkBlock lb = link_block<LN::LPB>(g);
   const int mu = lb.mu;
   const int site0 = lb.site0;
-  const int nvalid = min(LN::LPB, g.V - site0);
-  const int site = min(site0 + ln.j, g.V - 1);
-  const bool ok = ln.lds_ok && (site0 + ln.j) < g.V;
+  const int nvalid = min(LN::LPB, lb.count - site0);
+  const int site = link_site(g, min(site0 + ln.j, lb.count - 1));
+  const bool ok = ln.lds_ok && (site0 + ln.j) < lb.count;
   const long V = g.V;
   const long idx = mu * V + site;
   // staple s, link k: offset (dm along mu, dn along nu), direction (0 mu,
@@ -734,6 +839,9 @@ KernelSet make_kernel_set() {
   s.drift = [](int nblk, hipStream_t st, Geo g, SerField U, SerField D, SerField D1, double tau_g, double rk2c,
                int mode, double* obs, SerField R, double c1) {
     hipLaunchKernelGGL((k_drift<NC, N>), dim3(nblk), dim3(64), 0, st, g, U, D, D1, tau_g, rk2c, mode, obs, R, c1);
+  };
+  s.sgf = [](int nblk, hipStream_t st, Geo g, SerField U, double alpha) {
+    hipLaunchKernelGGL((k_sgf<NC, N>), dim3(nblk), dim3(64), 0, st, g, U, alpha);
   };
   s.rect = [](int nblk, hipStream_t st, Geo g, SerField U, SerField R, double* obs2) {
     hipLaunchKernelGGL((k_rect<NC, N>), dim3(nblk), dim3(64), 0, st, g, U, R, obs2);

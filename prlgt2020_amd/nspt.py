@@ -15,8 +15,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("NSPT_LIB") or os.path.join(HERE, "libnspt.so")
 
-EULER, RK2 = 0, 1
-GF_NONE, GF_FA = 0, 1
+EULER, RK2, EULER_CHECKERBOARD = 0, 1, 2
+GF_NONE, GF_FA, GF_CHECKERBOARD = 0, 1, 2
 NOISE_HOST, NOISE_DEVICE, NOISE_ZERO = 0, 1, 2
 
 
@@ -47,7 +47,7 @@ EXPORTS = ["nspt_create", "nspt_destroy", "nspt_last_error", "nspt_obs_len", "ns
            "nspt_unitarize", "nspt_set_action"]
 
 PROF_NAMES = ["drift", "update", "update_log", "zero_mode", "fa_div", "dft", "fa_gexp", "transform", "noise",
-              "reduce"]
+              "reduce", "rect"]
 
 
 def lib():

@@ -257,3 +257,45 @@ def test_improved_action_step(nc, c1, integrator, relerr):
         for a, b in zip(nspt.split_obs(obs_g, N), O.split_obs(obs_o, N)):
             assert np.abs(a - b).max() <= TOL * max(1.0, np.abs(b).max())
     ctx.close()
+
+
+@pytest.mark.parametrize("nc,gf", [(3, 0), (3, 1), (3, 2), (2, 2)])
+def test_checkerboard_sweep(nc, gf, relerr):
+    """The in-place on-the-fly sweep in checkerboard order (NSPT_EULER_CHECKERBOARD)
+    with zero modes and no / FA / checkerboard sequential gauge fixing, two
+    steps: fields and observables against the oracle (update=2; its
+    gauge_fix 3 is the checkerboard sequential gauge fixing)."""
+    nspt = _nspt()
+    L, N = (2, 2, 4, 4), 6
+    V = 64
+    alpha = 0.05 if gf == 1 else -0.05
+    U = O.random_field(L, nc, N, seed=3, scale=0.4)
+    ctx = nspt.Context(L, nc, N)
+    ctx.upload(U)
+    rng = nspt.Ranlxd(1234)
+    Uo = U.copy()
+    for s in range(2):
+        eta = rng.noise(nc, 4 * V)
+        ctx.set_noise(eta)
+        obs_g = ctx.step(nspt.Context.params(0.05, alpha, nspt.EULER_CHECKERBOARD, gf, 1, nspt.NOISE_HOST))
+        obs_o = O.step(L, nc, N, Uo, 0.05, alpha, integrator=0, update=2, gauge_fix={0: 0, 1: 1, 2: 3}[gf],
+                       zero_modes=1, eta=eta)
+        e = relerr(ctx.download(), Uo, nc, N)
+        print("step", s, "per-order rel err", e)
+        assert e.max() <= TOL, e
+        for a, b in zip(nspt.split_obs(obs_g, N), O.split_obs(obs_o, N)):
+            assert np.abs(a - b).max() <= TOL * max(1.0, np.abs(b).max())
+    ctx.close()
+
+
+def test_checkerboard_rejects_odd_extents_and_rectangles():
+    nspt = _nspt()
+    ctx = nspt.Context((2, 2, 3, 4), 3, 4)
+    with pytest.raises(nspt.NsptError, match="even lattice extents"):
+        ctx.step(nspt.Context.params(0.05, 0.0, nspt.EULER_CHECKERBOARD, nspt.GF_NONE, 0, nspt.NOISE_ZERO))
+    ctx.close()
+    ctx = nspt.Context((2, 2, 2, 4), 3, 4)
+    ctx.set_action(-1.0 / 12.0)
+    with pytest.raises(nspt.NsptError, match="Wilson action"):
+        ctx.step(nspt.Context.params(0.05, 0.0, nspt.EULER_CHECKERBOARD, nspt.GF_NONE, 0, nspt.NOISE_ZERO))
+    ctx.close()
