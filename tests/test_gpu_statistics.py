@@ -40,3 +40,47 @@ def test_order2_plaquette_equipartition(nc):
     assert abs(m1) < 5 * e1 + 1e-12
     assert abs(m3) < 5 * e3 + 1e-12
     assert np.allclose(plaq[:, 0], 1.0)
+
+
+def test_checkerboard_gauge_fixing_equipartition():
+    """RK2 with the checkerboard sequential gauge fixing (alpha = -0.05, the
+    reference's shipped value): the gauge-invariant order-2 plaquette takes
+    the equipartition value as with FA."""
+    from prlgt2020_amd import nspt
+    L, N, nc, eps = (4, 4, 4, 4), 4, 3, 0.02
+    want = -(nc * nc - 1) * (1 - 1 / 256) / 4
+    ctx = nspt.Context(L, nc, N)
+    p = nspt.Context.params(eps, -0.05, nspt.RK2, nspt.GF_CHECKERBOARD, 1, nspt.NOISE_DEVICE, seed=11, step=0)
+    ctx.evolve(p, 1000, want_obs=False)
+    plaq = ctx.evolve(p, 16000)[:, :2 * (N + 1)].view(np.complex128)
+    m2, e2 = blocked(plaq[:, 2].real)
+    m3, e3 = blocked(plaq[:, 3].real)
+    print("checkerboard gauge fixing: order 2 %.5f +- %.5f (expected %.5f)" % (m2, e2, want))
+    assert abs(m2 - want) < max(5 * e2, 0.005 * abs(want))
+    assert abs(m3) < 5 * e3 + 1e-12
+    ctx.close()
+
+
+def test_checkerboard_sweep_converges_to_equipartition():
+    """The on-the-fly Euler sweep in checkerboard order carries the Euler
+    O(eps) bias; extrapolated linearly from eps = 0.01 and 0.005 to eps = 0 the
+    order-2 plaquette is the equipartition value (within 3 sigma)."""
+    from prlgt2020_amd import nspt
+    L, N, nc = (4, 4, 4, 4), 2, 3
+    want = -(nc * nc - 1) * (1 - 1 / 256) / 4
+    res = []
+    for eps, n in ((0.01, 32000), (0.005, 64000)):
+        ctx = nspt.Context(L, nc, N)
+        p = nspt.Context.params(eps, -0.05, nspt.EULER_CHECKERBOARD, nspt.GF_CHECKERBOARD, 1, nspt.NOISE_DEVICE,
+                                seed=11, step=0)
+        ctx.evolve(p, n // 16, want_obs=False)
+        plaq = ctx.evolve(p, n)[:, :2 * (N + 1)].view(np.complex128)
+        res.append(blocked(plaq[:, 2].real))
+        ctx.close()
+    (a, ea), (b, eb) = res
+    p0 = 2 * b - a  # linear in eps through eps = 0.01 and 0.005
+    e0 = np.hypot(2 * eb, ea)
+    print("checkerboard sweep: order 2 at eps 0.01: %.5f, 0.005: %.5f, extrapolated %.5f +- %.5f (expected %.5f)"
+          % (a, b, p0, e0, want))
+    assert abs(p0 - want) < 3 * e0
+    assert a < b < 0  # the bias shrinks with eps
