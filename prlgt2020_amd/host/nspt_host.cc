@@ -145,8 +145,10 @@ void NsptEvolve(ptGluon_fld& Umu) {
   const int V = Umu.Z->Size;
   const int ol = nspt_obs_len(E.ctx);
   const double eps = act_pars.stau * act_pars.stau;
-  nspt_step_params p = {eps, act_pars.alpha, mi355x_opts.integrator ? NSPT_RK2 : NSPT_EULER,
-                        mi355x_opts.fa ? NSPT_GF_FA : NSPT_GF_NONE, mi355x_opts.zero_modes,
+  const int integ[3] = {NSPT_EULER, NSPT_RK2, NSPT_EULER_CHECKERBOARD};
+  const int gfix[3] = {NSPT_GF_NONE, NSPT_GF_FA, NSPT_GF_CHECKERBOARD};
+  nspt_step_params p = {eps, act_pars.alpha, integ[mi355x_opts.integrator], gfix[mi355x_opts.fa],
+                        mi355x_opts.zero_modes,
                         mi355x_opts.device_noise ? NSPT_NOISE_DEVICE : NSPT_NOISE_HOST,
                         mi355x_opts.noise_seed, E.step};
   const int beat = nspt_pars.Beat;

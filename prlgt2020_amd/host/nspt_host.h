@@ -120,8 +120,9 @@ typedef struct {
 
 // Engine choices the reference makes with compile-time switches.
 typedef struct {
-  int integrator;    // 0 Euler (the reference), 1 RK2
-  int fa;            // 1: Fourier-accelerated gauge fixing every step
+  int integrator;    // 0 Euler (Jacobi), 1 RK2, 2 Euler on the fly in checkerboard order
+  int fa;            // gauge fixing every step: 0 none, 1 Fourier accelerated,
+                     // 2 sequential in checkerboard site order
   int zero_modes;    // 1: REMOVE_GLUON_ZEROMOM
   int device_noise;  // 0: ranlxd stream of the reference (Rand[0]); 1: device Philox
   unsigned long long noise_seed;

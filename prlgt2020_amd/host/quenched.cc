@@ -8,12 +8,16 @@
 // -s SWEEP, -b BEAT, -r seed, -g tau_g, -a v (alpha = -v), -t init_status,
 // -p plaquette file stem, -c configuration stem, -d damocle file, -n PTORD,
 // -h.  Engine options (compile-time switches in the reference): -I 1 (RK2
-// integrator), -F 0 (no gauge fixing), -Z 0 (keep zero modes), -D 1
-// (device noise instead of the reference's ranlxd stream).
+// integrator) or 2 (the on-the-fly sweep in checkerboard order), -F 0 (no
+// gauge fixing) or 2 (the sequential gauge fixing in checkerboard order),
+// -Z 0 (keep zero modes), -D 1 (device noise instead of the reference's
+// ranlxd stream), -A (improved gauge action), -L (restart from a
+// configuration of fewer orders).  -I 2 -F 2 with the shipped Quench.cfg
+// (alpha -0.05) is the reference's default build, reordered.
 //
-// The gauge fixing is the Fourier-accelerated one (nspt.cc:866-948); it is
-// stable for alpha > 0 only, so the shipped Quench.cfg (alpha -0.05, written
-// for the sequential gauge fixing) needs "-a -0.05" (alpha = +0.05).
+// The default gauge fixing is the Fourier-accelerated one (nspt.cc:866-948);
+// it is stable for alpha > 0 only, so with it the shipped Quench.cfg (alpha
+// -0.05, written for the sequential gauge fixing) needs "-a -0.05".
 #include <algorithm>
 #include <cmath>
 #include <ctime>
@@ -123,7 +127,8 @@ int initialize(int argc, char** argv) {
                     "-b\t\titerations between damocle reads (BEAT)\n-r\t\trandom seed\n-g\t\ttau gluons\n"
                     "-a\t\talpha (alpha = -value)\n-t\t\tinitial state: 0 cold, otherwise from configuration\n"
                     "-p\t\tplaquette file name\n-c\t\tsaved configuration name\n-d\t\tdamocle file name\n"
-                    "-n\t\tperturbative order\n-I\t\tintegrator: 0 Euler, 1 RK2\n-F\t\tFA gauge fixing: 1 on, 0 off\n"
+                    "-n\t\tperturbative order\n-I\t\tintegrator: 0 Euler, 1 RK2, 2 Euler on the fly (checkerboard)\n"
+                    "-F\t\tgauge fixing: 0 off, 1 Fourier accelerated, 2 sequential (checkerboard)\n"
                     "-Z\t\tzero-mode subtraction: 1 on, 0 off\n-D\t\tnoise: 0 host ranlxd, 1 device Philox\n"
                     "-L\t\twith -t 1: start from the configuration saved with this lower allocORD\n"
                     "-A\t\tgauge action: 0 Wilson, 1 TLSYM, 2 IWA, 3 DBW2\n\n");
@@ -144,7 +149,11 @@ int initialize(int argc, char** argv) {
   }
   mi355x_opts.noise_seed = seed0;
   if (!mi355x_opts.device_noise) mi355x_seed_noise(seed0);
-  if (mi355x_opts.fa && act_pars.alpha <= 0.0)
+  if (mi355x_opts.integrator < 0 || mi355x_opts.integrator > 2 || mi355x_opts.fa < 0 || mi355x_opts.fa > 2) {
+    std::fprintf(stderr, "-I takes 0, 1 or 2 and -F 0, 1 or 2\n");
+    return 1;
+  }
+  if (mi355x_opts.fa == 1 && act_pars.alpha <= 0.0)
     std::fprintf(stderr, "warning: Fourier-accelerated gauge fixing needs alpha > 0 (alpha = %g)\n", act_pars.alpha);
 
   // file names, non-MPI form (Quenched.cc:388-423)
@@ -202,7 +211,13 @@ int initialize(int argc, char** argv) {
           << "Norma\t\t" << nspt_pars.normn << std::endl
           << "Tr(U)\t\t" << nspt_pars.trun << std::endl
           << "Gauge action\t" << type_g() + 4 << " (c1 = " << action_c1(mi355x_opts.action) << ")" << std::endl
-          << "Integrator\t" << (mi355x_opts.integrator ? "RK2" : "Euler") << std::endl
+          << "Integrator\t"
+          << (mi355x_opts.integrator == 1 ? "RK2" : mi355x_opts.integrator == 2 ? "Euler, on the fly (checkerboard)"
+                                                                                 : "Euler")
+          << std::endl
+          << "Gauge fixing\t"
+          << (mi355x_opts.fa == 1 ? "Fourier accelerated" : mi355x_opts.fa == 2 ? "sequential (checkerboard)" : "none")
+          << std::endl
           << "Noise\t\t" << (mi355x_opts.device_noise ? "device Philox4x32-10" : "ranlxd (Rand[0])") << std::endl
           << std::endl;
   return 0;

@@ -188,3 +188,33 @@ def test_driver_improved_action(tmp_path):
     assert got.shape == want.shape
     assert np.abs(got - want).max() < 1e-12 * np.abs(want).max()
     assert (tmp_path / n["conf"]).exists()
+
+
+@pytest.mark.gpu
+def test_driver_reference_default_build_reordered(tmp_path):
+    """-I 2 -F 2: the on-the-fly sweep and the sequential gauge fixing in
+    checkerboard order, with the shipped alpha sign (alpha = -0.05 read from
+    Quench.cfg), against the oracle (update 2, gauge_fix 3), 10 steps."""
+    from oracle import oracle as O
+    L, N, eps, alpha, steps, nc = 4, 6, 0.05, -0.05, 10, 3
+    write_inputs(tmp_path, L=L, sweep=steps, beat=steps, tau=eps, alpha=alpha, ptord=N, seed=1)
+    out = subprocess.run([driver(nc, 6), "-I", "2", "-F", "2"], cwd=tmp_path, capture_output=True, text=True,
+                         timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    n = names(L, alpha, eps, N, 6, nc)
+    Lt = (L, L, L, L)
+    V = L ** 4
+    U = O.cold_field(Lt, nc, N)
+    rng = O.Ranlxd(O.glibc_first_rand(1))
+    plaq = []
+    for _ in range(steps):
+        eta = rng.noise(nc, 4 * V)
+        obs = O.step(Lt, nc, N, U, eps, alpha, integrator=0, update=2, gauge_fix=3, zero_modes=1, eta=eta)
+        p = O.split_obs(obs, N)[0]
+        plaq.append(np.stack([p.real, p.imag], 1))
+    want = np.concatenate(plaq)
+    got = _read_cols(tmp_path / n["plaq"], 2)
+    assert got.shape == want.shape
+    assert np.abs(got - want).max() < 1e-12 * np.abs(want).max()
+    f = np.fromfile(tmp_path / n["conf"], dtype=np.complex128)[:4 * V * (1 + 9 * N)].reshape(4 * V, -1)
+    assert np.abs(f - U).max() < 1e-12 * np.abs(U).max()
