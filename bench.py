@@ -130,7 +130,10 @@ def main():
     ap.add_argument("--N", type=int, default=20)
     ap.add_argument("--nc", type=int, choices=[3, 2], default=3)
     ap.add_argument("--eps", type=float, default=0.02)
-    ap.add_argument("--integrator", choices=["rk2", "euler"], default="rk2")
+    ap.add_argument("--integrator", choices=["rk2", "euler", "checkerboard"], default="rk2",
+                    help="checkerboard: the on-the-fly Euler sweep in checkerboard order")
+    ap.add_argument("--gauge", choices=["fa", "checkerboard"], default="fa",
+                    help="gauge fixing: Fourier accelerated, or the sequential one in checkerboard order")
     ap.add_argument("--action", choices=list(ACTIONS), default="wil",
                     help="gauge action: Wilson (the default workload) or an improved one")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -156,12 +159,14 @@ def main():
     L = (args.L,) * 4
     N, nc = args.N, args.nc
     V = args.L ** 4
-    integ = nspt.RK2 if args.integrator == "rk2" else nspt.EULER
+    integ = {"rk2": nspt.RK2, "euler": nspt.EULER, "checkerboard": nspt.EULER_CHECKERBOARD}[args.integrator]
+    gfix = nspt.GF_FA if args.gauge == "fa" else nspt.GF_CHECKERBOARD
+    alpha = 0.05 if args.gauge == "fa" else -0.05  # FA needs alpha > 0; -0.05 is the shipped value
     ctx = nspt.Context(L, nc, N)
     improved = args.action != "wil"
     if improved:
         ctx.set_action(ACTIONS[args.action])
-    p = nspt.Context.params(args.eps, 0.05, integ, nspt.GF_FA, 1, nspt.NOISE_DEVICE, seed=1, step=0)
+    p = nspt.Context.params(args.eps, alpha, integ, gfix, 1, nspt.NOISE_DEVICE, seed=1, step=0)
     ctx.evolve(p, args.warmup, want_obs=False)
     import ctypes
     lib = nspt.lib()
@@ -179,6 +184,8 @@ def main():
     tot_ms, cnt = prof[name]
     avg_s = tot_ms / cnt / 1e3
     fl = kernel_flops(name, N, V, nc, improved)
+    if integ == nspt.EULER_CHECKERBOARD and name in ("drift", "update", "update_log"):
+        fl /= 8  # one launch per (parity, direction): V/2 of the 4V links
     achieved = fl / avg_s / 1e12 if fl else 0.0
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_su3_16x4_N20_rk2.md")
@@ -207,7 +214,11 @@ def main():
         "config": {"workload": "SU(%d) %s NSPT %d^4, %d orders, %s eps=%g, zero modes + FA gauge fixing "
                                "(alpha=0.05) every step"
                                % (nc, {"wil": "Wilson", "tls": "tree-level Symanzik", "iwa": "Iwasaki",
-                                       "dbw": "DBW2"}[args.action], args.L, N, args.integrator.upper(), args.eps),
+                                       "dbw": "DBW2"}[args.action], args.L, N, args.integrator.upper(), args.eps)
+                               if args.gauge == "fa" else
+                               "SU(%d) Wilson NSPT %d^4, %d orders, %s eps=%g, zero modes + sequential gauge fixing "
+                               "in checkerboard order (alpha=-0.05) every step"
+                               % (nc, args.L, N, args.integrator.upper(), args.eps),
                    "lattice": list(L), "orders": N, "integrator": args.integrator, "replicas": 1},
         "roofline": {"bound": "mfma", "kernel": name, "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
