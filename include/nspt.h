@@ -37,7 +37,7 @@ typedef struct nspt_ctx nspt_ctx;
 
 /* Lattice and series geometry (Quenched.cc:160-170, choices.h:30, 86-90). */
 typedef struct {
-  int sz[4];     /* taglia L0..L3, periodic boundary conditions */
+  int sz[4];     /* taglia L0..L3 (periodic; see nspt_set_twist) */
   int nc;        /* 3 (SU(3)) or 2 (SU(2)) */
   int ptord;     /* PTORD: orders 1..ptord are evolved */
   int alloc_ord; /* allocORD: width of the host field image, >= ptord */
@@ -97,6 +97,19 @@ int nspt_evolve(nspt_ctx* ctx, nspt_step_params* p, int nsteps, double* obs);
  * rectangle plaquette (ptord+1 complex, normalised as the plaquette) after
  * trU: nspt_obs_len grows accordingly. */
 int nspt_set_action(nspt_ctx* ctx, double c1);
+
+/* Twisted boundary conditions (TWISTED_BC of choices.h:43-55, chosen at
+ * compile time by the reference): bit d of mask twists direction d,
+ * U(x + L_d e_d) = W_d U(x) W_d^dag, with W_d = omega + 2 nc^2 d (nc x nc,
+ * row-major, interleaved complex; untwisted directions are not read).  The
+ * reference keeps twisted copies of the boundary links in halo sites
+ * (twist_boundary, nspt.cc:163-256; lattice.h:156-1053); here the links
+ * read across a twisted boundary are conjugated as they are loaded by the
+ * drift, rectangle, gauge-fixing and loop kernels.  Each W_d must be
+ * monomial (one entry of modulus 1 per row and column), as all matrices of
+ * twistmatrices.h are.  mask 0 restores periodic conditions.  The FA gauge
+ * fixing needs periodic conditions and is refused on a twisted lattice. */
+int nspt_set_twist(nspt_ctx* ctx, int mask, const double* omega);
 
 /* plaquette_measure (nspt.cc:1316-1406): ptord+1 complex. */
 int nspt_plaquette(nspt_ctx* ctx, double* out);

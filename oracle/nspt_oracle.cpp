@@ -9,6 +9,9 @@
 //   * series algebra of ptSU3 / ptSU2          QCDpt.h:92-114, 176-220; QCDpt.cc:37-209, 318-537
 //   * 3x3 / 2x2 complex products, dag, Tr      MyMath.h:791-849, 1001, 1064-1080
 //   * periodic lattice indexing + neighbours   lattice.h:37, 139-233, 1194-1222
+//   * twisted boundary conditions (as the      nspt.cc:97-256, lattice.h:156-1053,
+//     conjugation W U W^dag of links read      twistmatrices.h
+//     across a twisted boundary)
 //   * staple                                   QCDenvNODEpt_test.h:554-557
 //   * gauge_wilson (Jacobi and on-the-fly)     nspt.cc:258-600
 //   * zero_modes_subtraction                   nspt.cc:605-654
@@ -30,7 +33,8 @@
 // oracle/_ref (the reference's own algebra + RNG sources compiled here), the
 // ranlxd known-answer test of ranlxd/test/test.cc:47-102, numpy's FFT for the
 // DFT, and closed forms / invariants (log(exp X) = X, unitarity, gauge
-// covariance, cold-start first step, brute-force plaquette).
+// covariance, cold-start first step, brute-force plaquette); the twisted
+// lattice against the periodic one on the unwrapped covering lattice.
 //
 // Series layout: (1 + nc*nc*N) complex, index 0 = flag (order-0 scalar),
 // 1 + (k-1)*nc*nc + r*nc + c = element (r,c) of order k.  This is the memory
@@ -226,12 +230,57 @@ Ser ser_log(const Ser& U) {
   return res;
 }
 
+// ---------------------------------------------------------------- twist
+// Twisted boundary conditions (TWISTED_BC, choices.h:43-55; nspt.cc:97-256;
+// lattice.h:156-1053): the field obeys U(x + L_d e_d) = W_d U(x) W_d^dag for
+// every twisted direction d, W_d the twist matrix of twistmatrices.h.  The
+// reference stores twisted copies of the boundary links in halo sites
+// (twist_boundary, nspt.cc:163-256: forward copies W A W^dag, backward copies
+// W^dag A W); here a link reached across the boundary of d, n_d times
+// (n_d = +-1, +-2), is conjugated by W_d^(n_d) when it is read.  Twist
+// matrices of different directions commute up to a centre phase, so the
+// order of the conjugations does not matter.
+struct TwistState {
+  int mask = 0;  // bit d: direction d twisted
+  int nc = 0;
+  cx om[4][9];
+};
+TwistState g_tw;
+
 // ---------------------------------------------------------------- lattice
 struct Latt {
   int L[4];
   int V;
   std::vector<int> up, dn;  // [site*4 + mu]
+  // Twisted lattices name the sites reached by neighbour steps with extended
+  // indices e = site + V*code, code digit d (base 5) = n_d mod 5, n_d the
+  // net number of crossings of the boundary of twisted direction d.
+  int tw = 0;
+  static constexpr int P5[4] = {1, 5, 25, 125};
+  int fwd(int e, int d) const {
+    if (!tw) return up[e * 4 + d];
+    const int s = e % V;
+    int code = e / V;
+    int x[4];
+    coords(s, x);
+    if (((tw >> d) & 1) && x[d] == L[d] - 1) code = bump(code, d, 1);
+    return up[s * 4 + d] + V * code;
+  }
+  int bwd(int e, int d) const {
+    if (!tw) return dn[e * 4 + d];
+    const int s = e % V;
+    int code = e / V;
+    int x[4];
+    coords(s, x);
+    if (((tw >> d) & 1) && x[d] == 0) code = bump(code, d, -1);
+    return dn[s * 4 + d] + V * code;
+  }
+  static int bump(int code, int d, int by) {
+    const int dig = (code / P5[d]) % 5;
+    return code + (((dig + by + 5) % 5) - dig) * P5[d];
+  }
   explicit Latt(const int* sz) {
+    tw = g_tw.mask;
     for (int d = 0; d < 4; d++) L[d] = sz[d];
     V = L[0] * L[1] * L[2] * L[3];
     up.resize(4 * V);
@@ -260,17 +309,62 @@ struct Latt {
   }
 };
 
+// Conjugation of the twisted copy reached with crossing code `code`:
+// M = prod_d W_d^(n_d) (W^-1 = W^dag).
+void twist_matrix(int nc, int code, cx* M) {
+  for (int q = 0; q < nc * nc; q++) M[q] = cx{(q / nc == q % nc) ? 1.0 : 0.0, 0.0};
+  for (int d = 0; d < 4; d++) {
+    int n = (code / Latt::P5[d]) % 5;
+    if (n > 2) n -= 5;
+    cx Wd[9], T[9];
+    if (n < 0)
+      mat_dag(nc, g_tw.om[d], Wd);
+    else
+      std::memcpy(Wd, g_tw.om[d], sizeof(cx) * nc * nc);
+    for (int i = 0; i < (n < 0 ? -n : n); i++) {
+      mat_mul(nc, M, Wd, T);
+      std::memcpy(M, T, sizeof(cx) * nc * nc);
+    }
+  }
+}
+
 struct Field {
   int nc, N, S;
   const Latt* lat;
   double* data;  // interleaved complex, [(4*site+mu)*S + j]
-  Ser get(int site, int mu) const {
+  // e: a site or an extended index (twisted lattice): the twisted copy
+  // M U M^dag of the stored link
+  Ser get(int e, int mu) const {
+    const int V = lat->V, site = e % V, code = e / V;
     Ser s(nc, N);
     std::memcpy(s.v.data(), data + 2 * (size_t)(4 * site + mu) * S, sizeof(cx) * S);
+    if (code) {
+      cx M[9], Md[9], T[9];
+      twist_matrix(nc, code, M);
+      mat_dag(nc, M, Md);
+      for (int k = 1; k <= N; k++) {
+        mat_mul(nc, M, s.ord(k), T);
+        mat_mul(nc, T, Md, s.ord(k));
+      }
+    }
     return s;
   }
-  void set(int site, int mu, const Ser& s) {
-    std::memcpy(data + 2 * (size_t)(4 * site + mu) * S, s.v.data(), sizeof(cx) * S);
+  // a twisted copy is written back as the stored link M^dag T M
+  void set(int e, int mu, const Ser& s) {
+    const int V = lat->V, site = e % V, code = e / V;
+    double* dst = data + 2 * (size_t)(4 * site + mu) * S;
+    std::memcpy(dst, s.v.data(), sizeof(cx) * S);
+    if (code) {
+      Ser u = s;
+      cx M[9], Md[9], T[9];
+      twist_matrix(nc, code, M);
+      mat_dag(nc, M, Md);
+      for (int k = 1; k <= N; k++) {
+        mat_mul(nc, Md, s.ord(k), T);
+        mat_mul(nc, T, M, u.ord(k));
+      }
+      std::memcpy(dst, u.v.data(), sizeof(cx) * S);
+    }
   }
 };
 
@@ -278,8 +372,8 @@ struct Field {
 //                   + [U_mu(n-nu) U_nu(n-nu+mu)]^dag U_nu(n-nu)   (QCDenvNODEpt_test.h:554-557)
 Ser staple(const Field& F, int n, int mu, int nu) {
   const Latt& Z = *F.lat;
-  const int n_pmu = Z.up[n * 4 + mu], n_pnu = Z.up[n * 4 + nu], n_mnu = Z.dn[n * 4 + nu];
-  const int n_mnu_pmu = Z.up[n_mnu * 4 + mu];
+  const int n_pmu = Z.fwd(n, mu), n_pnu = Z.fwd(n, nu), n_mnu = Z.bwd(n, nu);
+  const int n_mnu_pmu = Z.fwd(n_mnu, mu);
   Ser a = ser_mul(F.get(n_pmu, nu), ser_dag(ser_mul(F.get(n, nu), F.get(n_pnu, mu))));
   Ser b = ser_mul(ser_dag(ser_mul(F.get(n_mnu, mu), F.get(n_mnu_pmu, nu))), F.get(n_mnu, nu));
   return ser_add(a, b);
@@ -290,8 +384,8 @@ Ser staple(const Field& F, int n, int mu, int nu) {
 // (mu, nu), with the reference's neighbour letters and product order.
 Ser staple2x1(const Field& F, int n, int mu, int nu) {
   const Latt& Z = *F.lat;
-  auto up = [&](int s, int d) { return Z.up[s * 4 + d]; };
-  auto dn = [&](int s, int d) { return Z.dn[s * 4 + d]; };
+  auto up = [&](int s, int d) { return Z.fwd(s, d); };
+  auto dn = [&](int s, int d) { return Z.bwd(s, d); };
   auto U = [&](int s, int d) { return F.get(s, d); };
   int a = up(n, mu), b = up(a, nu), c = up(n, nu), d = up(c, nu), e = up(a, mu), f = dn(n, mu), g = up(f, nu);
   Ser t = ser_mul(ser_mul(U(a, nu), U(b, nu)), ser_dag(ser_mul(ser_mul(U(n, nu), U(c, nu)), U(d, mu))));
@@ -852,7 +946,7 @@ void sgf_default(Field& F, double alpha) {
   for (int s = 0; s < Z.V; s++) {
     Ser W(F.nc, F.N);
     for (int mu = 0; mu < 4; mu++) {
-      Ser d = ser_sub(F.get(s, mu), F.get(Z.dn[s * 4 + mu], mu));
+      Ser d = ser_sub(F.get(s, mu), F.get(Z.bwd(s, mu), mu));
       for (size_t q = 0; q < W.v.size(); q++) W.v[q] += d.v[q];
     }
     W = ser_sub(W, ser_dag(W));
@@ -863,7 +957,7 @@ void sgf_default(Field& F, double alpha) {
     Ser G1 = ser_exp(Wa), G2 = ser_exp(Wm);
     for (int mu = 0; mu < 4; mu++) {
       F.set(s, mu, ser_mul(G1, F.get(s, mu)));
-      const int b = Z.dn[s * 4 + mu];
+      const int b = Z.bwd(s, mu);
       F.set(b, mu, ser_mul(F.get(b, mu), G2));
     }
   }
@@ -886,7 +980,7 @@ void sgf_checkerboard(Field& F, double alpha) {
       if (site_parity(Z, s) != par) continue;
       Ser W(F.nc, F.N);
       for (int mu = 0; mu < 4; mu++) {
-        Ser d = ser_sub(F.get(s, mu), F.get(Z.dn[s * 4 + mu], mu));
+        Ser d = ser_sub(F.get(s, mu), F.get(Z.bwd(s, mu), mu));
         for (size_t q = 0; q < W.v.size(); q++) W.v[q] += d.v[q];
       }
       W = ser_sub(W, ser_dag(W));
@@ -897,7 +991,7 @@ void sgf_checkerboard(Field& F, double alpha) {
       Ser G1 = ser_exp(Wa), G2 = ser_exp(Wm);
       for (int mu = 0; mu < 4; mu++) {
         F.set(s, mu, ser_mul(G1, F.get(s, mu)));
-        const int b = Z.dn[s * 4 + mu];
+        const int b = Z.bwd(s, mu);
         F.set(b, mu, ser_mul(F.get(b, mu), G2));
       }
     }
@@ -1029,7 +1123,7 @@ void compute_loops(const Geom& gm, const double* Udata, double* out) {
   const int S = 1 + nc * nc * N;
   Field F{nc, N, S, &Z, const_cast<double*>(Udata)};
   auto U = [&](int site, int mu) { return F.get(site, mu); };
-  auto up = [&](int site, int mu) { return Z.up[site * 4 + mu]; };
+  auto up = [&](int site, int mu) { return Z.fwd(site, mu); };
   Ser w21(nc, N), w22(nc, N), w42(nc, N), w44(nc, N);
   for (int mu = 0; mu < 3; mu++)
     for (int nu = mu + 1; nu < 4; nu++)
@@ -1211,6 +1305,20 @@ void orc_sgf_checkerboard(const int* gi, double* U, double alpha) {
   const int S = 1 + g.nc * g.nc * g.N;
   Field F{g.nc, g.N, S, &Z, U};
   sgf_checkerboard(F, alpha);
+}
+// Twisted boundary conditions for the calls that follow: mask bit d twists
+// direction d with the nc x nc matrix om + 2 nc^2 d (row-major, interleaved
+// complex); mask 0 restores periodic conditions.  The FA gauge fixing and
+// the adjoint functions stay periodic.
+void orc_set_twist(int nc, int mask, const double* om) {
+  g_tw.mask = mask;
+  g_tw.nc = nc;
+  for (int d = 0; d < 4; d++)
+    for (int q = 0; q < nc * nc; q++) {
+      const bool on = (mask >> d) & 1;
+      g_tw.om[d][q] = on ? cx{om[2 * (d * nc * nc + q)], om[2 * (d * nc * nc + q) + 1]}
+                         : cx{(q / nc == q % nc) ? 1.0 : 0.0, 0.0};
+    }
 }
 void orc_plaquette_measure(const int* gi, const double* U, double* out) {
   plaquette_measure(geom_from(gi), U, out);

@@ -192,6 +192,45 @@ def sgf_default(L, nc, N, U, alpha):
     lib().orc_sgf_default(_geom(L, nc, N), _ptr(U), alpha)
 
 
+def reference_twist(nc):
+    """The twist matrices of twistmatrices.h (not IDENTITY_TWIST), 4 x nc x nc:
+    SU(3) y = diag(e^{-2 pi i/3}, 1, e^{2 pi i/3}), z the cyclic shift, t the
+    shift with phases; SU(2) y = diag(-i, i), z = [[0, 1], [-1, 0]],
+    t = [[0, -i], [-i, 0]].  x is the identity (the reference's SU(2) x twist
+    is the identity and its SU(3) one is left undefined)."""
+    w = np.exp(2j * np.pi / 3)
+    if nc == 3:
+        om = [np.eye(3), np.diag([np.conj(w), 1.0, w]),
+              np.array([[0, 1, 0], [0, 0, 1], [1, 0, 0]]),
+              np.array([[0, np.conj(w), 0], [0, 0, 1], [w, 0, 0]])]
+    else:
+        om = [np.eye(2), np.diag([-1j, 1j]), np.array([[0, 1], [-1, 0]]), np.array([[0, -1j], [-1j, 0]])]
+    return np.array(om, np.complex128)
+
+
+def set_twist(nc, mask, omegas=None):
+    """Twisted boundary conditions for the oracle calls that follow (mask bit d:
+    U(x + L_d e_d) = W_d U(x) W_d^dag); mask 0 restores periodic ones."""
+    om = np.ascontiguousarray(omegas if omegas is not None else np.zeros((4, nc, nc)), np.complex128)
+    lib().orc_set_twist(nc, mask, _ptr(om))
+
+
+class twisted:
+    """with twisted(nc, mask[, omegas]): ... -- oracle calls with twisted
+    boundary conditions, periodic again on exit."""
+
+    def __init__(self, nc, mask, omegas=None):
+        self.args = (nc, mask, reference_twist(nc) if omegas is None else omegas)
+
+    def __enter__(self):
+        set_twist(*self.args)
+        return self
+
+    def __exit__(self, *exc):
+        set_twist(self.args[0], 0)
+        return False
+
+
 def plaquette_measure(L, nc, N, U):
     out = np.zeros(N + 1, np.complex128)
     lib().orc_plaquette_measure(_geom(L, nc, N), _ptr(U), _ptr(out))

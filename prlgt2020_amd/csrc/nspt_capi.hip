@@ -19,6 +19,7 @@
 using nspt::Geo;
 using nspt::KernelSet;
 using nspt::SerField;
+using nspt::TwistTab;
 
 namespace {
 
@@ -624,6 +625,8 @@ struct nspt_ctx {
   // the rectangle plaquette
   double c1 = 0.0;
   SerField R{};
+  // twisted boundary conditions (nspt_set_twist): the twist matrices
+  TwistTab* twist_tab = nullptr;
   double *obs2_part = nullptr, *obs2_mu = nullptr;
   double2* fb = nullptr;
   double2* eta = nullptr;
@@ -1083,6 +1086,8 @@ static int check_params(nspt_ctx* c, const nspt_step_params* p) {
     for (int d = 0; d < 4; d++)
       if (c->g.L[d] % 2) return c->fail("checkerboard passes need even lattice extents");
   }
+  if (c->g.twist && p->gauge_fix == NSPT_GF_FA)
+    return c->fail("the FA gauge fixing needs periodic boundary conditions (its Fourier modes are periodic)");
   if (p->integrator == NSPT_EULER_CHECKERBOARD && c->c1 != 0.0)
     return c->fail("the checkerboard sweep is for the Wilson action (rectangles couple links of one parity)");
   if (p->noise < NSPT_NOISE_HOST || p->noise > NSPT_NOISE_ZERO) return c->fail("unknown noise source");
@@ -1179,6 +1184,48 @@ int nspt_set_action(nspt_ctx* c, double c1) {
   return ensure_obs(c, 1) ? 0 : -2;
 }
 
+int nspt_set_twist(nspt_ctx* c, int mask, const double* omega) {
+  if (!c) return -1;
+  if (mask < 0 || mask > 15) return c->fail("twist mask must be in 0..15") - 1;
+  if (mask && !omega) return c->fail("null twist matrices") - 1;
+  const int nc = c->nc;
+  TwistTab t{};
+  for (int d = 0; d < 4; d++) {
+    if (!((mask >> d) & 1)) {
+      for (int i = 0; i < nc; i++) {
+        t.ph[d][i] = make_double2(1.0, 0.0);
+        t.perm[d] |= (unsigned)i << (2 * i) | (unsigned)i << (8 + 2 * i);
+      }
+      continue;
+    }
+    // one entry of modulus 1 per row and per column, zeros elsewhere
+    int seen = 0;
+    for (int i = 0; i < nc; i++) {
+      int col = -1;
+      for (int k = 0; k < nc; k++) {
+        const double re = omega[2 * ((d * nc + i) * nc + k)], im = omega[2 * ((d * nc + i) * nc + k) + 1];
+        const double m2 = re * re + im * im;
+        if (m2 == 0.0) continue;
+        if (col >= 0 || std::fabs(m2 - 1.0) > 1e-12) col = -2;
+        if (col == -2) break;
+        col = k;
+        t.ph[d][i] = make_double2(re, im);
+      }
+      if (col < 0 || ((seen >> col) & 1))
+        return c->fail("twist matrices must have one entry of modulus 1 per row and column (as twistmatrices.h)") - 1;
+      seen |= 1 << col;
+      t.perm[d] |= (unsigned)col << (2 * i) | (unsigned)i << (8 + 2 * col);
+    }
+  }
+  if (!c->twist_tab && !c->alloc(&c->twist_tab, 1)) return -2;
+  if (!c->hip_ok(hipMemcpyAsync(c->twist_tab, &t, sizeof(t), hipMemcpyHostToDevice, c->stream), "twist") ||
+      !c->hip_ok(hipStreamSynchronize(c->stream), "sync"))
+    return -3;
+  c->g.twist = mask;
+  c->g.tw = c->twist_tab;
+  return 0;
+}
+
 int nspt_unitarize(nspt_ctx* c) {
   if (!c) return -1;
   // U <- exp(reH(log U)): the log kernel, then the zero-mode kernel with a zero mean
@@ -1193,6 +1240,7 @@ int nspt_unitarize(nspt_ctx* c) {
 
 int nspt_fa_gauge_fixing(nspt_ctx* c, double alpha) {
   if (!c) return -1;
+  if (c->g.twist) return c->fail("the FA gauge fixing needs periodic boundary conditions") - 1;
   c->ks.logf(c->nblk_links, c->stream, c->g, c->U, c->L);
   if (!launch_ok(c, "log") || !fa_apply(c, c->L, alpha, nullptr, false)) return -3;
   return c->hip_ok(hipStreamSynchronize(c->stream), "sync") ? 0 : -3;

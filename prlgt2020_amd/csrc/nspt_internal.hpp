@@ -20,6 +20,18 @@ struct Geo {
   // nb_half blocks (ceil(V/2 / links-per-block)); needs L_3 even.
   int sub;
   int nb_half;
+  // Twisted boundary conditions: bit d of twist set means
+  // U(x + L_d e_d) = W_d U(x) W_d^dag, W_d described by tw (device memory).
+  int twist;
+  const struct TwistTab* tw;
+};
+
+// Monomial twist matrices: row i of W_d holds its one nonzero entry
+// ph[d][i] in column (perm[d] >> 2i) & 3; bits 8 + 2i of perm[d] hold the
+// inverse permutation.
+struct TwistTab {
+  double2 ph[4][3];
+  unsigned perm[4];
 };
 
 // Launchers of one (NC, N) instance.  nblk is the grid size in waves.

@@ -41,6 +41,67 @@ __device__ __forceinline__ int site_shift_far(const Geo& g, int s, int d, int de
   return s + ((x + delta) % g.L[d] - x) * g.st[d];
 }
 
+// ------------------------------------------------------------ twisted boundaries
+// A neighbour reached from a site carries a crossing code: 3 bits per
+// direction d holding n_d + 2, n_d the net number of times the step crossed
+// the boundary of d (|n_d| <= 2).  TW0 = no crossing.  On a twisted lattice
+// the link read there is the twisted copy W U W^dag, W = prod_d W_d^(n_d)
+// (the halo copies of twist_boundary, nspt.cc:163-256: forward W A W^dag,
+// backward W^dag A W).
+constexpr int TW0 = 2 | (2 << 3) | (2 << 6) | (2 << 9);
+
+// site_move / site_shift_far that also count the boundary crossings.
+__device__ __forceinline__ int site_move_tw(const Geo& g, int s, int d, int delta, int& code) {
+  const int x = (s / g.st[d]) % g.L[d];
+  int xn = x + delta, n = 0;
+  if (xn < 0) {
+    xn += g.L[d];
+    n = -1;
+  } else if (xn >= g.L[d]) {
+    xn -= g.L[d];
+    n = 1;
+  }
+  code += n * (1 << (3 * d));
+  return s + (xn - x) * g.st[d];
+}
+__device__ __forceinline__ int site_shift_far_tw(const Geo& g, int s, int d, int delta, int& code) {
+  const int x = (s / g.st[d]) % g.L[d];
+  code += ((x + delta) / g.L[d]) << (3 * d);
+  return s + ((x + delta) % g.L[d] - x) * g.st[d];
+}
+
+// One factor of a row walk: row i of M times W_d (inv: W_d^dag) is row
+// i' of the product with phase m.
+__device__ __forceinline__ void tw_step(const TwistTab* tw, int d, bool inv, int& i, double2& m) {
+  const unsigned perm = tw->perm[d];
+  if (!inv) {
+    m = c_mul(m, tw->ph[d][i]);
+    i = (perm >> (2 * i)) & 3;
+  } else {
+    const int k = (perm >> (8 + 2 * i)) & 3;
+    m = c_mul(m, c_conj(tw->ph[d][k]));
+    i = k;
+  }
+}
+// Element (r, c) of the twisted copy W U W^dag is ph times element
+// (pe / NC, pe % NC) of U: W monomial, W[r][pi(r)] = m_r, so
+// (W U W^dag)[r][c] = m_r conj(m_c) U[pi(r)][pi(c)].
+template <int NC>
+__device__ __forceinline__ void twist_elem(const Geo& g, int code, int r, int c, int& pe, double2& ph) {
+  double2 mr = one2(), mc = one2();
+  if (code != TW0) {
+    for (int d = 0; d < 4; d++) {
+      const int n = ((code >> (3 * d)) & 7) - 2;
+      for (int t = 0; t < n || t < -n; t++) {
+        tw_step(g.tw, d, n < 0, r, mr);
+        tw_step(g.tw, d, n < 0, c, mc);
+      }
+    }
+  }
+  pe = r * NC + c;
+  ph = c_mul(mr, c_conj(mc));
+}
+
 // Block order.  The dispatcher hands consecutive workgroups to the 8 XCDs in
 // turn; block_order() renumbers them so that each XCD runs one contiguous
 // range of logical blocks (neighbour links then meet in that XCD's L2), and
@@ -117,6 +178,47 @@ __device__ __forceinline__ void stage_global(const Lane<NC, N>& ln, double2* slo
   }
 }
 
+// Loads and stores of a link reached with crossing code `code`: on a
+// twisted lattice the lane reads (writes) the element of the stored link
+// that its element of the twisted copy comes from (goes to), times the
+// phase (its conjugate).
+template <int NC, int N>
+__device__ __forceinline__ void gload_tw(const Geo& g, const Lane<NC, N>& ln, const SerField& f, long idx, int code,
+                                         int nrt, bool ok, double2 (&X)[N]) {
+  int pe = ln.e;
+  double2 ph = one2();
+  if (g.twist) twist_elem<NC>(g, code, ln.r, ln.c, pe, ph);
+  gload<NC, N>(f, idx, pe, nrt, ok, X);
+  if (g.twist) {
+#pragma unroll
+    for (int k = 0; k < N; k++) X[k] = c_mul(ph, X[k]);
+  }
+}
+template <int NC, int N>
+__device__ __forceinline__ void gstore_tw(const Geo& g, const Lane<NC, N>& ln, const SerField& f, long idx, int code,
+                                          int nrt, bool ok, double2 (&X)[N]) {
+  int pe = ln.e;
+  double2 ph = one2();
+  if (g.twist) {
+    twist_elem<NC>(g, code, ln.r, ln.c, pe, ph);
+#pragma unroll
+    for (int k = 0; k < N; k++) X[k] = c_mul(c_conj(ph), X[k]);
+  }
+  gstore<NC, N>(f, idx, pe, nrt, ok, X);
+}
+// stage_global of a link reached with crossing code `code`.
+template <int NC, int N>
+__device__ __forceinline__ void stage_global_tw(const Geo& g, const Lane<NC, N>& ln, double2* slot, const SerField& f,
+                                                long idx, int code, int nrt, bool dag = false) {
+  if (!g.twist) {
+    stage_global(ln, slot, f, idx, nrt, dag);
+    return;
+  }
+  double2 X[N];
+  gload_tw(g, ln, f, idx, code, nrt, true, X);
+  ln.stage(slot, X, dag);
+}
+
 // Sum over the valid link slots of the block of element (k, e) of slot A.
 // Deterministic (ascending slots).
 template <int NC, int N>
@@ -168,25 +270,35 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_drift(Geo g, SerField
   // T1^dag and T2^dag are formed in place in the slot that the next product
   // reads.  The links of product i+1 are loaded into PA/PB while product i
   // runs from LDS.
-  const int xpmu = site_shift(g, site, mu, 1);
+  int cpmu = TW0;
+  const int xpmu = site_move_tw(g, site, mu, 1, cpmu);
   auto kind_of = [](int i) { return (i == 12) ? 4 : (i & 3); };
-  auto links_of = [&](int i, long& ia, long& ib) {
+  // links of product i and their boundary-crossing codes
+  auto links_of = [&](int i, long& ia, long& ib, int& ca, int& cb) {
     const int p = i >> 2, kind = kind_of(i);
     const int nu = (p < mu) ? p : p + 1;
-    const int xpnu = site_shift(g, site, nu, 1);
-    const int xmnu = site_shift(g, site, nu, -1);
+    int cpnu = TW0, cmnu = TW0;
+    const int xpnu = site_move_tw(g, site, nu, 1, cpnu);
+    const int xmnu = site_move_tw(g, site, nu, -1, cmnu);
     ia = idx;
     ib = idx;
+    ca = TW0;
+    cb = TW0;
     if (kind == 0) {
       ia = nu * V + site;
       ib = mu * V + xpnu;
+      cb = cpnu;
     } else if (kind == 1) {
       ia = nu * V + xpmu;
+      ca = cpmu;
     } else if (kind == 2) {
       ia = mu * V + xmnu;
-      ib = nu * V + site_shift(g, xmnu, mu, 1);
+      ca = cmnu;
+      cb = cmnu;
+      ib = nu * V + site_move_tw(g, xmnu, mu, 1, cb);
     } else if (kind == 3) {
       ib = nu * V + xmnu;
+      cb = cmnu;
     }
   };
   // Up to N = 24 the next product's links are prefetched into registers;
@@ -197,14 +309,15 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_drift(Geo g, SerField
   double2 PA[NP], PB[NP], paf = zero2(), pbf = zero2();
   auto fetch = [&](int i) {
     long ia, ib;
-    links_of(i, ia, ib);
+    int ca, cb;
+    links_of(i, ia, ib, ca, cb);
     const int kind = kind_of(i);
     if (kind != 3) {
-      if constexpr (PF) gload<NC, N>(U, ia, ln.e, nrt, ok, PA);
+      if constexpr (PF) gload_tw<NC, N>(g, ln, U, ia, ca, nrt, ok, PA);
       paf = gflag(U, ia, ok);
     }
     if (kind == 0 || kind == 2 || kind == 3) {
-      if constexpr (PF) gload<NC, N>(U, ib, ln.e, nrt, ok, PB);
+      if constexpr (PF) gload_tw<NC, N>(g, ln, U, ib, cb, nrt, ok, PB);
       pbf = gflag(U, ib, ok);
     }
   };
@@ -217,9 +330,10 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_drift(Geo g, SerField
       if (kind == 0 || kind == 2 || kind == 3) ln.stage(ln.slotB, PB);
     } else {
       long ia, ib;
-      links_of(i, ia, ib);
-      if (kind != 3) stage_global(ln, ln.slotA, U, ia, nrt);
-      if (kind == 0 || kind == 2 || kind == 3) stage_global(ln, ln.slotB, U, ib, nrt);
+      int ca, cb;
+      links_of(i, ia, ib, ca, cb);
+      if (kind != 3) stage_global_tw(g, ln, ln.slotA, U, ia, ca, nrt);
+      if (kind == 0 || kind == 2 || kind == 3) stage_global_tw(g, ln, ln.slotB, U, ib, cb, nrt);
     }
     if (kind == 4) ln.stage(ln.slotB, Sig);
     const double2 a0 = (kind == 3) ? c_conj(t2f) : paf;
@@ -542,7 +656,9 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_sgf(Geo g, SerField U
     gload<NC, N>(U, mu * V + site, ln.e, nrt, ok, P);
 #pragma unroll
     for (int k = 0; k < N; k++) W[k] = c_add(W[k], P[k]);
-    gload<NC, N>(U, mu * V + site_move(g, site, mu, -1), ln.e, nrt, ok, P);
+    int cm = TW0;
+    const int xm = site_move_tw(g, site, mu, -1, cm);
+    gload_tw<NC, N>(g, ln, U, mu * V + xm, cm, nrt, ok, P);
 #pragma unroll
     for (int k = 0; k < N; k++) W[k] = c_sub(W[k], P[k]);
   }
@@ -585,16 +701,18 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_sgf(Geo g, SerField U
     gstore<NC, N>(U, i1, ln.e, nrt, ok, P);
     if (ok && ln.e == 0) U.flag[i1] = n1;
     __syncthreads();
-    // U_mu(x-mu) <- U_mu(x-mu) G^dag, in place in slot A
-    const long i2 = mu * V + site_move(g, site, mu, -1);
-    stage_global(ln, ln.slotA, U, i2, nrt);
+    // U_mu(x-mu) <- U_mu(x-mu) G^dag, in place in slot A (on a twisted
+    // lattice: the twisted copy times G^dag, stored back untwisted)
+    int c2 = TW0;
+    const long i2 = mu * V + site_move_tw(g, site, mu, -1, c2);
+    stage_global_tw(g, ln, ln.slotA, U, i2, c2, nrt);
     ln.stage(ln.slotB, W, true);
     const double2 f2 = gflag(U, i2, ok);
     __syncthreads();
     const double2 n2 = mul_lds<NC, N>(ln, f2, one2(), true, M, ln.slotA, false);
     __syncthreads();
     unstage(ln, ln.slotA, P);
-    gstore<NC, N>(U, i2, ln.e, nrt, ok, P);
+    gstore_tw<NC, N>(g, ln, U, i2, c2, nrt, ok, P);
     if (ok && ln.e == 0) U.flag[i2] = n2;
     __syncthreads();
   }
@@ -632,11 +750,12 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_rect(Geo g, SerField 
       {{1, -1, 1, 1}, {1, -2, 1, 1}, {0, -2, 0, 1}, {0, -2, 1, 0}, {0, -1, 1, 0}},
       {{1, 0, 0, 0}, {2, -1, 1, 1}, {1, -1, 0, 1}, {0, -1, 0, 1}, {0, -1, 1, 0}},
       {{1, -1, 1, 1}, {0, -1, 0, 1}, {-1, -1, 0, 1}, {-1, -1, 1, 0}, {-1, 0, 0, 0}}};
-  auto link_of = [&](int step, long& li, bool& dg) {
+  auto link_of = [&](int step, long& li, bool& dg, int& cl) {
     const int p = step / 30, r = step % 30, st = r / 5, k = r % 5;
     const int nu = (p < mu) ? p : p + 1;
     const signed char* e = SH[st][k];
-    const int x = site_move(g, site_move(g, site, mu, e[0]), nu, e[1]);
+    cl = TW0;
+    const int x = site_move_tw(g, site_move_tw(g, site, mu, e[0], cl), nu, e[1], cl);
     li = (long)(e[2] ? nu : mu) * V + x;
     dg = e[3] != 0;
   };
@@ -650,8 +769,9 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_rect(Geo g, SerField 
   double2 PB[NP];
   long li;
   bool dg;
-  link_of(0, li, dg);
-  if constexpr (PF) gload<NC, N>(U, li, ln.e, nrt, ok, PB);
+  int cl;
+  link_of(0, li, dg, cl);
+  if constexpr (PF) gload_tw<NC, N>(g, ln, U, li, cl, nrt, ok, PB);
   for (int step = 0; step < 90; step++) {
     const int k = step % 5;
     double2 fl = gflag(U, li, ok);
@@ -660,10 +780,10 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_rect(Geo g, SerField 
     if constexpr (PF)
       ln.stage(slot, PB, dg);
     else
-      stage_global(ln, slot, U, li, nrt, dg);
+      stage_global_tw(g, ln, slot, U, li, cl, nrt, dg);
     if (step + 1 < 90) {
-      link_of(step + 1, li, dg);
-      if constexpr (PF) gload<NC, N>(U, li, ln.e, nrt, ok, PB);
+      link_of(step + 1, li, dg, cl);
+      if constexpr (PF) gload_tw<NC, N>(g, ln, U, li, cl, nrt, ok, PB);
     }
     if (k == 0) {
       tf = fl;
@@ -722,9 +842,10 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_link_pair(Geo g, SerF
   const int site = min(lb.site0 + ln.j, g.V - 1);
   const bool ok = ln.lds_ok && (lb.site0 + ln.j) < g.V;
   const long idx = (long)mu * g.V + site;
-  const long idx2 = (long)mu * g.V + site_shift_far(g, site, mu, k);
+  int c2 = TW0;
+  const long idx2 = (long)mu * g.V + site_shift_far_tw(g, site, mu, k, c2);
   stage_global(ln, ln.slotA, A, idx, nrt);
-  stage_global(ln, ln.slotB, A, idx2, nrt);
+  stage_global_tw(g, ln, ln.slotB, A, idx2, c2, nrt);
   const double2 fa = gflag(A, idx, ok), fb = gflag(A, idx2, ok);
   __syncthreads();
   const double2 f = mul_lds<NC, N>(ln, fa, fb, true, min(N, nrt) - 1, ln.slotB, false);
@@ -772,12 +893,13 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_loops(Geo g, SerField
   const int site = min(site0 + ln.j, g.V - 1);
   const bool ok = ln.lds_ok && (site0 + ln.j) < g.V;
   const long V = g.V;
-  const long i1 = mu * V + site, i2 = nu * V + site_shift_far(g, site, mu, lm);
-  const long i3 = nu * V + site, i4 = mu * V + site_shift_far(g, site, nu, lnu);
+  int c2 = TW0, c4 = TW0;
+  const long i1 = mu * V + site, i2 = nu * V + site_shift_far_tw(g, site, mu, lm, c2);
+  const long i3 = nu * V + site, i4 = mu * V + site_shift_far_tw(g, site, nu, lnu, c4);
   double2 P[N], Q[N];
   // T1
   stage_global(ln, ln.slotA, F(lm), i1, nrt);
-  stage_global(ln, ln.slotB, F(lnu), i2, nrt);
+  stage_global_tw(g, ln, ln.slotB, F(lnu), i2, c2, nrt);
   const double2 fa = gflag(F(lm), i1, ok), fb = gflag(F(lnu), i2, ok);
   __syncthreads();
   const double2 f1 = mul_lds<NC, N>(ln, fa, fb, true, M, ln.slotA, false);
@@ -786,7 +908,7 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_loops(Geo g, SerField
   __syncthreads();
   // T2
   stage_global(ln, ln.slotA, F(lnu), i3, nrt);
-  stage_global(ln, ln.slotB, F(lm), i4, nrt);
+  stage_global_tw(g, ln, ln.slotB, F(lm), i4, c4, nrt);
   const double2 fc = gflag(F(lnu), i3, ok), fd = gflag(F(lm), i4, ok);
   __syncthreads();
   const double2 f2 = mul_lds<NC, N>(ln, fc, fd, true, M, ln.slotA, false);

@@ -24,6 +24,26 @@ class NsptError(RuntimeError):
     pass
 
 
+def reference_twist(nc):
+    """Twist matrices W_x, W_y, W_z, W_t of twistmatrices.h (not
+    IDENTITY_TWIST), shape (4, nc, nc).  SU(3): W_y = diag(w*, 1, w) with
+    w = exp(2 pi i/3), W_z the cyclic shift (rows (0 1 0), (0 0 1), (1 0 0)),
+    W_t the shift with phases (rows (0 w* 0), (0 0 1), (w 0 0)); SU(2):
+    W_y = diag(-i, i), W_z = ((0, 1), (-1, 0)), W_t = ((0, -i), (-i, 0)).
+    W_x is the identity (the reference's SU(2) x twist is the identity, its
+    SU(3) one is left undefined); the reference twists y and z by default
+    (choices.h:45-50)."""
+    w = np.exp(2j * np.pi / 3)
+    if nc == 3:
+        m = [np.eye(3), np.diag([w.conjugate(), 1.0, w]), [[0, 1, 0], [0, 0, 1], [1, 0, 0]],
+             [[0, w.conjugate(), 0], [0, 0, 1], [w, 0, 0]]]
+    elif nc == 2:
+        m = [np.eye(2), np.diag([-1j, 1j]), [[0, 1], [-1, 0]], [[0, -1j], [-1j, 0]]]
+    else:
+        raise ValueError("nc must be 2 or 3")
+    return np.array(m, np.complex128)
+
+
 class _Geom(ctypes.Structure):
     _fields_ = [("sz", ctypes.c_int * 4), ("nc", ctypes.c_int), ("ptord", ctypes.c_int),
                 ("alloc_ord", ctypes.c_int)]
@@ -44,7 +64,7 @@ EXPORTS = ["nspt_create", "nspt_destroy", "nspt_last_error", "nspt_obs_len", "ns
            "nspt_rng_uniform", "nspt_rng_noise", "nspt_device_noise", "nspt_profile", "nspt_profile_read",
            "nspt_batch_create", "nspt_batch_destroy", "nspt_batch_last_error", "nspt_batch_evolve",
            "nspt_batch_replica", "nspt_batch_tune", "nspt_adjoint_drift", "nspt_loops",
-           "nspt_unitarize", "nspt_set_action"]
+           "nspt_unitarize", "nspt_set_action", "nspt_set_twist"]
 
 PROF_NAMES = ["drift", "update", "update_log", "zero_mode", "fa_div", "dft", "fa_gexp", "transform", "noise",
               "reduce", "rect"]
@@ -205,6 +225,15 @@ class Context:
         plaquette."""
         self._check(lib().nspt_set_action(self.h, ctypes.c_double(c1)), "set_action")
         self.obs_len = lib().nspt_obs_len(self.h)
+
+    def set_twist(self, mask, omegas=None):
+        """Twisted boundary conditions: bit d of mask twists direction d,
+        U(x + L_d e_d) = W_d U(x) W_d^dag (TWISTED_BC, nspt.cc:97-256).
+        omegas: (4, nc, nc) monomial matrices, default reference_twist(nc);
+        mask 0 restores periodic conditions."""
+        om = reference_twist(self.nc) if omegas is None else np.asarray(omegas, np.complex128)
+        om = np.ascontiguousarray(om.reshape(4, self.nc, self.nc))
+        self._check(lib().nspt_set_twist(self.h, int(mask), _ptr(om)), "set_twist")
 
     def unitarize(self):
         """U <- exp(reH(log U)) for every link (QCDenvNODEpt_test.h:415-418)."""
