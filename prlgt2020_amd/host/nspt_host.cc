@@ -98,6 +98,11 @@ void attach(ptGluon_fld& U) {
   nspt_geom g = {{U.Z->Sz[0], U.Z->Sz[1], U.Z->Sz[2], U.Z->Sz[3]}, NC, PTORD, allocORD};
   if (nspt_create(&g, &E.ctx) != 0) fail("nspt_create");
   if (nspt_set_action(E.ctx, action_c1(mi355x_opts.action)) != 0) fail("nspt_set_action");
+  if (mi355x_opts.twist) {
+    double om[4 * 2 * NC * NC];
+    reference_twist(om);
+    if (nspt_set_twist(E.ctx, mi355x_opts.twist, om) != 0) fail("nspt_set_twist");
+  }
   E.field = &U;
   if (nspt_upload(E.ctx, reinterpret_cast<const double*>(U.W)) != 0) fail("nspt_upload");
 }
@@ -122,6 +127,35 @@ void write_step(const double* o) {
 }
 
 }  // namespace
+
+void reference_twist(double* om) {
+  const double c = std::cos(2.0 * M_PI / 3.0), s = std::sin(2.0 * M_PI / 3.0);
+  // (re, im) of the nonzero entry of each row and its column, per direction
+  struct Row {
+    int col;
+    double re, im;
+  };
+#if NC == 3
+  const Row rows[4][3] = {{{0, 1, 0}, {1, 1, 0}, {2, 1, 0}},
+                          {{0, c, -s}, {1, 1, 0}, {2, c, s}},
+                          {{1, 1, 0}, {2, 1, 0}, {0, 1, 0}},
+                          {{1, c, -s}, {2, 1, 0}, {0, c, s}}};
+#else
+  (void)c;
+  (void)s;
+  const Row rows[4][2] = {{{0, 1, 0}, {1, 1, 0}},
+                          {{0, 0, -1}, {1, 0, 1}},
+                          {{1, 1, 0}, {0, -1, 0}},
+                          {{1, 0, -1}, {0, 0, -1}}};
+#endif
+  for (int q = 0; q < 4 * 2 * NC * NC; q++) om[q] = 0.0;
+  for (int d = 0; d < 4; d++)
+    for (int i = 0; i < NC; i++) {
+      const Row& r = rows[d][i];
+      om[2 * ((d * NC + i) * NC + r.col)] = r.re;
+      om[2 * ((d * NC + i) * NC + r.col) + 1] = r.im;
+    }
+}
 
 double action_c1(int action) {
   switch (action) {

@@ -127,7 +127,13 @@ typedef struct {
   int device_noise;  // 0: ranlxd stream of the reference (Rand[0]); 1: device Philox
   unsigned long long noise_seed;
   int action;        // GAUGE_ACTION (choices.h:101-113): 0 WIL, 1 TLSYM, 2 IWA, 3 DBW2
+  int twist;         // TWISTED_BC (choices.h:43-55): bit d twists direction d; 0 periodic
 } mi355x_opts_t;
+
+// The twist matrices of twistmatrices.h (not IDENTITY_TWIST), 4 x NC x NC
+// row-major interleaved complex, x first; x is the identity (the
+// reference's SU(2) x twist is the identity, its SU(3) one is undefined).
+void reference_twist(double* om);
 
 // c1 of an action (Quenched.cc:313-322); c0 = 1 - 8 c1.
 double action_c1(int action);

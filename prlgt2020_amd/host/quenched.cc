@@ -103,7 +103,7 @@ int initialize(int argc, char** argv) {
   std::strcpy(nspt_pars.damon, "damocle.dag");
 
   int opt;
-  while ((opt = getopt(argc, argv, "l:s:b:d:r:g:a:t:p:c:n:I:F:Z:D:L:A:h")) != -1) {
+  while ((opt = getopt(argc, argv, "l:s:b:d:r:g:a:t:p:c:n:I:F:Z:D:L:A:T:h")) != -1) {
     switch (opt) {
       case 'l': act_pars.sz[0] = act_pars.sz[1] = act_pars.sz[2] = act_pars.sz[3] = std::atoi(optarg); break;
       case 's': nspt_pars.Sweep = std::atoi(optarg); break;
@@ -122,6 +122,7 @@ int initialize(int argc, char** argv) {
       case 'D': mi355x_opts.device_noise = std::atoi(optarg); break;
       case 'L': low_order = std::atoi(optarg); break;
       case 'A': mi355x_opts.action = std::atoi(optarg); break;
+      case 'T': mi355x_opts.twist = std::atoi(optarg); break;
       case 'h':
         std::printf("\n-l\t\tlattice size (L0 = L1 = L2 = L3)\n-s\t\tnumber of iterations (SWEEP)\n"
                     "-b\t\titerations between damocle reads (BEAT)\n-r\t\trandom seed\n-g\t\ttau gluons\n"
@@ -131,7 +132,8 @@ int initialize(int argc, char** argv) {
                     "-F\t\tgauge fixing: 0 off, 1 Fourier accelerated, 2 sequential (checkerboard)\n"
                     "-Z\t\tzero-mode subtraction: 1 on, 0 off\n-D\t\tnoise: 0 host ranlxd, 1 device Philox\n"
                     "-L\t\twith -t 1: start from the configuration saved with this lower allocORD\n"
-                    "-A\t\tgauge action: 0 Wilson, 1 TLSYM, 2 IWA, 3 DBW2\n\n");
+                    "-A\t\tgauge action: 0 Wilson, 1 TLSYM, 2 IWA, 3 DBW2\n"
+                    "-T\t\ttwisted boundary conditions: bit 0 x, 1 y, 2 z, 3 t (6: y and z)\n\n");
         std::exit(0);
     }
   }
@@ -151,6 +153,10 @@ int initialize(int argc, char** argv) {
   if (!mi355x_opts.device_noise) mi355x_seed_noise(seed0);
   if (mi355x_opts.integrator < 0 || mi355x_opts.integrator > 2 || mi355x_opts.fa < 0 || mi355x_opts.fa > 2) {
     std::fprintf(stderr, "-I takes 0, 1 or 2 and -F 0, 1 or 2\n");
+    return 1;
+  }
+  if (mi355x_opts.twist < 0 || mi355x_opts.twist > 15 || (mi355x_opts.twist && mi355x_opts.fa == 1)) {
+    std::fprintf(stderr, "-T takes 0..15, and twisted lattices need -F 0 or -F 2\n");
     return 1;
   }
   if (mi355x_opts.fa == 1 && act_pars.alpha <= 0.0)
@@ -194,8 +200,12 @@ int initialize(int argc, char** argv) {
   }
 
   logfile << "Gauge group\t\t" << (NC == 3 ? "SU3" : "SU2") << std::endl
-          << "\tgauge field update not \"on the fly\"" << std::endl
-          << "Gauge field boundary conditions\t\tPERIODIC" << std::endl
+          << (mi355x_opts.integrator == 2 ? "\tgauge field update \"on the fly\"" : "\tgauge field update not \"on the fly\"")
+          << std::endl
+          << "Gauge field boundary conditions\t\t" << (mi355x_opts.twist ? "TWISTED" : "PERIODIC") << std::endl;
+  for (int d = 0; d < 4; d++)
+    if ((mi355x_opts.twist >> d) & 1) logfile << "\t\t\t\t\t- " << "XYZT"[d] << " direction is twisted" << std::endl;
+  logfile
           << "Taglia\t\t" << z[0] << " " << z[1] << " " << z[2] << " " << z[3] << std::endl
           << "Sweep\t\t" << nspt_pars.Sweep << std::endl
           << "Beat\t\t" << nspt_pars.Beat << std::endl

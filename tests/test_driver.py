@@ -218,3 +218,49 @@ def test_driver_reference_default_build_reordered(tmp_path):
     assert np.abs(got - want).max() < 1e-12 * np.abs(want).max()
     f = np.fromfile(tmp_path / n["conf"], dtype=np.complex128)[:4 * V * (1 + 9 * N)].reshape(4 * V, -1)
     assert np.abs(f - U).max() < 1e-12 * np.abs(U).max()
+
+
+def test_driver_refuses_fa_on_a_twisted_lattice(tmp_path):
+    """-T with the FA gauge fixing (the default -F 1) stops before any GPU
+    work: FA needs periodic boundary conditions."""
+    write_inputs(tmp_path)
+    out = subprocess.run([driver(3, 6), "-T", "6"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert out.returncode != 0 and "-F 0 or -F 2" in out.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nc", [3, 2])
+def test_driver_twisted_reference_default(tmp_path, nc):
+    """-T 6 -I 2 -F 2: the reference's TWISTED_BC default (y and z twisted,
+    choices.h:45-50, the matrices of twistmatrices.h) with its default
+    on-the-fly sweep and sequential gauge fixing in checkerboard order,
+    against the twisted oracle under the same ranlxd noise, 10 steps; the
+    log names the twisted directions (Quenched.cc:456-468)."""
+    from oracle import oracle as O
+    L, N, eps, alpha, steps = 4, 6, 0.05, -0.05, 10
+    write_inputs(tmp_path, L=L, sweep=steps, beat=steps, tau=eps, alpha=alpha, ptord=N, seed=1)
+    out = subprocess.run([driver(nc, 6), "-T", "6", "-I", "2", "-F", "2"], cwd=tmp_path, capture_output=True,
+                         text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    n = names(L, alpha, eps, N, 6, nc)
+    log = (tmp_path / n["log"]).read_text()
+    assert "Gauge field boundary conditions\t\tTWISTED" in log
+    assert "- Y direction is twisted" in log and "- Z direction is twisted" in log
+    assert "- X direction" not in log and "- T direction" not in log
+    Lt = (L, L, L, L)
+    V = L ** 4
+    U = O.cold_field(Lt, nc, N)
+    rng = O.Ranlxd(O.glibc_first_rand(1))
+    plaq = []
+    with O.twisted(nc, 6):
+        for _ in range(steps):
+            eta = rng.noise(nc, 4 * V)
+            obs = O.step(Lt, nc, N, U, eps, alpha, integrator=0, update=2, gauge_fix=3, zero_modes=1, eta=eta)
+            p = O.split_obs(obs, N)[0]
+            plaq.append(np.stack([p.real, p.imag], 1))
+    want = np.concatenate(plaq)
+    got = _read_cols(tmp_path / n["plaq"], 2)
+    assert got.shape == want.shape
+    assert np.abs(got - want).max() < 1e-12 * np.abs(want).max()
+    f = np.fromfile(tmp_path / n["conf"], dtype=np.complex128)[:4 * V * (1 + nc * nc * N)].reshape(4 * V, -1)
+    assert np.abs(f - U).max() < 1e-12 * np.abs(U).max()
