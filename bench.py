@@ -136,6 +136,9 @@ def main():
                     help="gauge fixing: Fourier accelerated, or the sequential one in checkerboard order")
     ap.add_argument("--action", choices=list(ACTIONS), default="wil",
                     help="gauge action: Wilson (the default workload) or an improved one")
+    ap.add_argument("--twist", type=int, default=0,
+                    help="twisted boundary conditions: bit d twists direction d with the reference's twist "
+                         "matrices (6: y and z, its default); needs --gauge checkerboard")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched epsilon-replica sweep")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
@@ -163,6 +166,8 @@ def main():
     gfix = nspt.GF_FA if args.gauge == "fa" else nspt.GF_CHECKERBOARD
     alpha = 0.05 if args.gauge == "fa" else -0.05  # FA needs alpha > 0; -0.05 is the shipped value
     ctx = nspt.Context(L, nc, N)
+    if args.twist:
+        ctx.set_twist(args.twist)
     improved = args.action != "wil"
     if improved:
         ctx.set_action(ACTIONS[args.action])
@@ -217,8 +222,10 @@ def main():
                                        "dbw": "DBW2"}[args.action], args.L, N, args.integrator.upper(), args.eps)
                                if args.gauge == "fa" else
                                "SU(%d) Wilson NSPT %d^4, %d orders, %s eps=%g, zero modes + sequential gauge fixing "
-                               "in checkerboard order (alpha=-0.05) every step"
-                               % (nc, args.L, N, args.integrator.upper(), args.eps),
+                               "in checkerboard order (alpha=-0.05) every step%s"
+                               % (nc, args.L, N, args.integrator.upper(), args.eps,
+                                  ", twisted boundary conditions in %s" % "".join(
+                                      "xyzt"[d] for d in range(4) if (args.twist >> d) & 1) if args.twist else ""),
                    "lattice": list(L), "orders": N, "integrator": args.integrator, "replicas": 1},
         "roofline": {"bound": "mfma", "kernel": name, "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,

@@ -72,16 +72,17 @@ __device__ __forceinline__ int site_shift_far_tw(const Geo& g, int s, int d, int
 
 // One factor of a row walk: row i of M times W_d (inv: W_d^dag) is row
 // i' of the product with phase m.
+// The table is wave-uniform: its words are read with scalar loads and the
+// lane's entry picked by selects.
+template <int NC>
 __device__ __forceinline__ void tw_step(const TwistTab* tw, int d, bool inv, int& i, double2& m) {
+  d = uniform(d);
   const unsigned perm = tw->perm[d];
-  if (!inv) {
-    m = c_mul(m, tw->ph[d][i]);
-    i = (perm >> (2 * i)) & 3;
-  } else {
-    const int k = (perm >> (8 + 2 * i)) & 3;
-    m = c_mul(m, c_conj(tw->ph[d][k]));
-    i = k;
-  }
+  const double2 p0 = tw->ph[d][0], p1 = tw->ph[d][1], p2 = NC > 2 ? tw->ph[d][2] : p1;
+  const int k = inv ? (int)((perm >> (8 + 2 * i)) & 3) : i;
+  const double2 w = k == 0 ? p0 : (k == 1 ? p1 : p2);
+  m = c_mul(m, inv ? c_conj(w) : w);
+  i = inv ? k : (int)((perm >> (2 * i)) & 3);
 }
 // Element (r, c) of the twisted copy W U W^dag is ph times element
 // (pe / NC, pe % NC) of U: W monomial, W[r][pi(r)] = m_r, so
@@ -93,8 +94,8 @@ __device__ __forceinline__ void twist_elem(const Geo& g, int code, int r, int c,
     for (int d = 0; d < 4; d++) {
       const int n = ((code >> (3 * d)) & 7) - 2;
       for (int t = 0; t < n || t < -n; t++) {
-        tw_step(g.tw, d, n < 0, r, mr);
-        tw_step(g.tw, d, n < 0, c, mc);
+        tw_step<NC>(g.tw, d, n < 0, r, mr);
+        tw_step<NC>(g.tw, d, n < 0, c, mc);
       }
     }
   }
@@ -185,11 +186,13 @@ __device__ __forceinline__ void stage_global(const Lane<NC, N>& ln, double2* slo
 template <int NC, int N>
 __device__ __forceinline__ void gload_tw(const Geo& g, const Lane<NC, N>& ln, const SerField& f, long idx, int code,
                                          int nrt, bool ok, double2 (&X)[N]) {
+  // (only waves with a lane across a twisted boundary take the twisted path)
+  const bool tw = g.twist && __any(code != TW0);
   int pe = ln.e;
   double2 ph = one2();
-  if (g.twist) twist_elem<NC>(g, code, ln.r, ln.c, pe, ph);
+  if (tw) twist_elem<NC>(g, code, ln.r, ln.c, pe, ph);
   gload<NC, N>(f, idx, pe, nrt, ok, X);
-  if (g.twist) {
+  if (tw) {
 #pragma unroll
     for (int k = 0; k < N; k++) X[k] = c_mul(ph, X[k]);
   }
@@ -199,7 +202,7 @@ __device__ __forceinline__ void gstore_tw(const Geo& g, const Lane<NC, N>& ln, c
                                           int nrt, bool ok, double2 (&X)[N]) {
   int pe = ln.e;
   double2 ph = one2();
-  if (g.twist) {
+  if (g.twist && __any(code != TW0)) {
     twist_elem<NC>(g, code, ln.r, ln.c, pe, ph);
 #pragma unroll
     for (int k = 0; k < N; k++) X[k] = c_mul(c_conj(ph), X[k]);
@@ -210,7 +213,7 @@ __device__ __forceinline__ void gstore_tw(const Geo& g, const Lane<NC, N>& ln, c
 template <int NC, int N>
 __device__ __forceinline__ void stage_global_tw(const Geo& g, const Lane<NC, N>& ln, double2* slot, const SerField& f,
                                                 long idx, int code, int nrt, bool dag = false) {
-  if (!g.twist) {
+  if (!g.twist || !__any(code != TW0)) {
     stage_global(ln, slot, f, idx, nrt, dag);
     return;
   }
