@@ -617,7 +617,8 @@ struct nspt_ctx {
   std::vector<void*> allocs;
   SerField U{}, F{}, D{}, D1{}, L{}, X{}, G{};
   // Wilson-loop measurement, allocated on first use: 2-link and 4-link
-  // fields, per-block rectangle sums, reduction scratch, the six sums
+  // fields (the 2-link field also feeds the rectangles of an improved
+  // action), per-block rectangle sums, reduction scratch, the six sums
   SerField L2{}, L4{};
   double *loop_part = nullptr, *loop_tmp = nullptr, *loop_res = nullptr;
   // improved gauge action (nspt_set_action): rectangle coefficient, the
@@ -856,7 +857,8 @@ bool do_step(nspt_ctx* c, const nspt_step_params* p, double* obs_dev) {
   auto rect = [&](const SerField& Uin, bool with_obs) {
     if (!impr) return;
     Prof pr(c, NSPT_PROF_RECT);
-    c->ks.rect(c->nblk_links, c->stream, c->g, Uin, c->R, with_obs && obs_part ? c->obs2_part : nullptr);
+    c->ks.link_pair(c->nblk_links, c->stream, c->g, Uin, c->L2, 1);
+    c->ks.rect(c->nblk_links, c->stream, c->g, Uin, c->L2, c->R, with_obs && obs_part ? c->obs2_part : nullptr);
   };
   const bool cb = p->integrator == NSPT_EULER_CHECKERBOARD;
   if (cb) {
@@ -1141,7 +1143,7 @@ int nspt_loops(nspt_ctx* c, double* out) {
   const long rows = 6L * c->g.nb_mu;  // per rectangle type: 6 planes x site blocks
   if (!c->loop_part) {
     const long P = 4L * c->V;
-    if (!(c->alloc_field(c->L2, P, true) && c->alloc_field(c->L4, P, true) &&
+    if (!((c->L2.data || c->alloc_field(c->L2, P, true)) && c->alloc_field(c->L4, P, true) &&
           c->alloc(&c->loop_part, (size_t)6 * rows * ncols) &&
           c->alloc(&c->loop_tmp, reduce_tmp_doubles(rows, 6, ncols)) && c->alloc(&c->loop_res, (size_t)6 * ncols))) {
       c->loop_part = nullptr;
@@ -1171,8 +1173,9 @@ int nspt_set_action(nspt_ctx* c, double c1) {
   if (!c) return -1;
   if (c1 != 0.0 && !c->R.data) {
     const int ncols = 2 * (c->nrt + 1);
+    // R, and the 2-link field the rectangles are formed from
     if (!(c->alloc_field(c->R, 4L * c->V, true) && c->alloc(&c->obs2_part, (size_t)c->nblk_links * ncols) &&
-          c->alloc(&c->obs2_mu, (size_t)4 * ncols))) {
+          c->alloc(&c->obs2_mu, (size_t)4 * ncols) && (c->L2.data || c->alloc_field(c->L2, 4L * c->V, true)))) {
       c->R.data = nullptr;
       return -2;
     }
@@ -1249,7 +1252,10 @@ int nspt_fa_gauge_fixing(nspt_ctx* c, double alpha) {
 int nspt_drift(nspt_ctx* c, double eps, double* aos_out) {
   if (!c || !aos_out) return -1;
   SerField none{nullptr, nullptr, 0};
-  if (c->c1 != 0.0) c->ks.rect(c->nblk_links, c->stream, c->g, c->U, c->R, nullptr);
+  if (c->c1 != 0.0) {
+    c->ks.link_pair(c->nblk_links, c->stream, c->g, c->U, c->L2, 1);
+    c->ks.rect(c->nblk_links, c->stream, c->g, c->U, c->L2, c->R, nullptr);
+  }
   c->ks.drift(c->nblk_links, c->stream, c->g, c->U, c->D, none, -eps / (2.0 * c->nc), 0.0, 0, nullptr,
               c->c1 != 0.0 ? c->R : none, c->c1);
   if (!launch_ok(c, "drift")) return -3;

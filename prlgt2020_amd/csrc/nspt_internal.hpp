@@ -39,9 +39,9 @@ struct KernelSet {
   int nc = 0, nmax = 0, lpb = 0, qt = 0;  // qt: order-tile edge of the products
   void (*drift)(int nblk, hipStream_t, Geo, SerField U, SerField D, SerField D1, double tau_g, double rk2c, int mode,
                 double* obs, SerField R, double c1) = nullptr;
-  // sum of the rectangle staples of every link, and (obs2 != null) their
-  // per-block plaquette traces
-  void (*rect)(int nblk, hipStream_t, Geo, SerField U, SerField R, double* obs2) = nullptr;
+  // sum of the rectangle staples of every link from U and its 2-link field
+  // L2, and (obs2 != null) their per-block plaquette traces
+  void (*rect)(int nblk, hipStream_t, Geo, SerField U, SerField L2, SerField R, double* obs2) = nullptr;
   // one checkerboard pass of the sequential gauge fixing (g.sub = 1 + 4 parity)
   void (*sgf)(int nblk, hipStream_t, Geo, SerField U, double alpha) = nullptr;
   void (*update)(int nblk, hipStream_t, Geo, SerField D, const double2* eta, double stau, SerField Ubase,

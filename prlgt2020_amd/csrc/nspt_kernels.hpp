@@ -724,12 +724,15 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_sgf(Geo g, SerField U
 // ------------------------------------------------------------ rectangle staples
 // R_mu(x) = sum_{nu != mu} staple2x1(x, mu, nu) (QCDenvNODEpt_test.h:579-611):
 // per plane the six 5-link staples of the 2x1 and 1x2 rectangles through
-// the link, each a path product L1 L2 L3 L4 L5 of links or their daggers,
-// formed one link at a time in place in slot A (18 paths, 72 products per
-// link).  With obs2: per block, Tr(U_mu(x) R_mu(x)) per order (the
-// rectangle plaquette of nspt.cc:406-410), as the drift's plaquette rows.
+// the link.  Every rectangle has a long side of two collinear links outside
+// the link itself, so with the 2-link field L2_d(x) = U_d(x) U_d(x+d) each
+// staple is a path of 3 or 4 factors (48 products per link instead of 72),
+// formed one factor at a time in place in slot A.  With obs2: per block,
+// Tr(U_mu(x) R_mu(x)) per order (the rectangle plaquette of
+// nspt.cc:406-410), as the drift's plaquette rows.
 template <int NC, int N>
-__global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_rect(Geo g, SerField U, SerField R, double* obs2) {
+__global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_rect(Geo g, SerField U, SerField L2, SerField R,
+                                                                double* obs2) {
   constexpr int NE = NC * NC;
   using LN = Lane<NC, N>;
   __shared__ double2 lds[LN::LDS];
@@ -744,56 +747,82 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_rect(Geo g, SerField 
   const bool ok = ln.lds_ok && (site0 + ln.j) < lb.count;
   const long V = g.V;
   const long idx = mu * V + site;
-  // staple s, link k: offset (dm along mu, dn along nu), direction (0 mu,
-  // 1 nu), dagger -- the six expressions of QCDenvNODEpt_test.h:591-607
-  constexpr signed char SH[6][5][4] = {
-      {{1, 0, 1, 0}, {1, 1, 1, 0}, {0, 2, 0, 1}, {0, 1, 1, 1}, {0, 0, 1, 1}},
-      {{1, 0, 0, 0}, {2, 0, 1, 0}, {1, 1, 0, 1}, {0, 1, 0, 1}, {0, 0, 1, 1}},
-      {{1, 0, 1, 0}, {0, 1, 0, 1}, {-1, 1, 0, 1}, {-1, 0, 1, 1}, {-1, 0, 0, 0}},
-      {{1, -1, 1, 1}, {1, -2, 1, 1}, {0, -2, 0, 1}, {0, -2, 1, 0}, {0, -1, 1, 0}},
-      {{1, 0, 0, 0}, {2, -1, 1, 1}, {1, -1, 0, 1}, {0, -1, 0, 1}, {0, -1, 1, 0}},
-      {{1, -1, 1, 1}, {0, -1, 0, 1}, {-1, -1, 0, 1}, {-1, -1, 1, 0}, {-1, 0, 0, 0}}};
-  auto link_of = [&](int step, long& li, bool& dg, int& cl) {
-    const int p = step / 30, r = step % 30, st = r / 5, k = r % 5;
+  // factor f of a plane's six paths: offset (dm along mu, dn along nu),
+  // direction (0 mu, 1 nu), dagger, field (0 U, 1 L2), position in its path
+  // (0 first, 1 middle, 2 last).  The six expressions of
+  // QCDenvNODEpt_test.h:591-607 with U_d(y) U_d(y+d) written L2_d(y):
+  //   L2_nu(x+mu) U_mu(x+2nu)^+ L2_nu(x)^+
+  //   U_mu(x+mu) U_nu(x+2mu) L2_mu(x+nu)^+ U_nu(x)^+
+  //   U_nu(x+mu) L2_mu(x-mu+nu)^+ U_nu(x-mu)^+ U_mu(x-mu)
+  //   L2_nu(x+mu-2nu)^+ U_mu(x-2nu)^+ L2_nu(x-2nu)
+  //   U_mu(x+mu) U_nu(x+2mu-nu)^+ L2_mu(x-nu)^+ U_nu(x-nu)
+  //   U_nu(x+mu-nu)^+ L2_mu(x-mu-nu)^+ U_nu(x-mu-nu) U_mu(x-mu)
+  constexpr int NF = 22;
+  constexpr signed char SH[NF][6] = {
+      {1, 0, 1, 0, 1, 0},   {0, 2, 0, 1, 0, 1},  {0, 0, 1, 1, 1, 2},                        //
+      {1, 0, 0, 0, 0, 0},   {2, 0, 1, 0, 0, 1},  {0, 1, 0, 1, 1, 1},  {0, 0, 1, 1, 0, 2},  //
+      {1, 0, 1, 0, 0, 0},   {-1, 1, 0, 1, 1, 1}, {-1, 0, 1, 1, 0, 1}, {-1, 0, 0, 0, 0, 2},  //
+      {1, -2, 1, 1, 1, 0},  {0, -2, 0, 1, 0, 1}, {0, -2, 1, 0, 1, 2},                       //
+      {1, 0, 0, 0, 0, 0},   {2, -1, 1, 1, 0, 1}, {0, -1, 0, 1, 1, 1}, {0, -1, 1, 0, 0, 2},  //
+      {1, -1, 1, 1, 0, 0},  {-1, -1, 0, 1, 1, 1}, {-1, -1, 1, 0, 0, 1}, {-1, 0, 0, 0, 0, 2}};
+  constexpr int NSTEP = 3 * NF;
+  struct Fac {
+    long li;
+    int cl, pos;
+    bool dg, lng;
+  };
+  auto fac_of = [&](int step) {
+    const int p = step / NF;
+    const signed char* e = SH[step % NF];
     const int nu = (p < mu) ? p : p + 1;
-    const signed char* e = SH[st][k];
-    cl = TW0;
-    const int x = site_move_tw(g, site_move_tw(g, site, mu, e[0], cl), nu, e[1], cl);
-    li = (long)(e[2] ? nu : mu) * V + x;
-    dg = e[3] != 0;
+    Fac f;
+    f.cl = TW0;
+    const int x = site_move_tw(g, site_move_tw(g, site, mu, e[0], f.cl), nu, e[1], f.cl);
+    f.li = (long)(e[2] ? nu : mu) * V + x;
+    f.dg = e[3] != 0;
+    f.lng = e[4] != 0;
+    f.pos = e[5];
+    return f;
+  };
+  // the field of a factor (pointers picked one by one: a runtime pick of a
+  // whole struct goes through scratch)
+  auto field = [&](bool lng) {
+    SerField f;
+    f.data = lng ? L2.data : U.data;
+    f.flag = lng ? L2.flag : U.flag;
+    f.P = U.P;
+    return f;
   };
   double2 Sig[N];
 #pragma unroll
   for (int k = 0; k < N; k++) Sig[k] = zero2();
   double2 sf = zero2(), tf = zero2();
-  // the next link is loaded into registers while the current product runs
+  // the next factor is loaded into registers while the current product runs
   constexpr bool PF = N <= 24;
   constexpr int NP = PF ? N : 1;
   double2 PB[NP];
-  long li;
-  bool dg;
-  int cl;
-  link_of(0, li, dg, cl);
-  if constexpr (PF) gload_tw<NC, N>(g, ln, U, li, cl, nrt, ok, PB);
-  for (int step = 0; step < 90; step++) {
-    const int k = step % 5;
-    double2 fl = gflag(U, li, ok);
-    if (dg) fl = c_conj(fl);
-    double2* slot = (k == 0) ? ln.slotA : ln.slotB;
+  Fac cur = fac_of(0);
+  if constexpr (PF) gload_tw<NC, N>(g, ln, field(cur.lng), cur.li, cur.cl, nrt, ok, PB);
+  for (int step = 0; step < NSTEP; step++) {
+    const SerField fc = field(cur.lng);
+    double2 fl = gflag(fc, cur.li, ok);
+    if (cur.dg) fl = c_conj(fl);
+    double2* slot = (cur.pos == 0) ? ln.slotA : ln.slotB;
     if constexpr (PF)
-      ln.stage(slot, PB, dg);
+      ln.stage(slot, PB, cur.dg);
     else
-      stage_global_tw(g, ln, slot, U, li, cl, nrt, dg);
-    if (step + 1 < 90) {
-      link_of(step + 1, li, dg, cl);
-      if constexpr (PF) gload_tw<NC, N>(g, ln, U, li, cl, nrt, ok, PB);
+      stage_global_tw(g, ln, slot, fc, cur.li, cur.cl, nrt, cur.dg);
+    const int pos = cur.pos;
+    if (step + 1 < NSTEP) {
+      cur = fac_of(step + 1);
+      if constexpr (PF) gload_tw<NC, N>(g, ln, field(cur.lng), cur.li, cur.cl, nrt, ok, PB);
     }
-    if (k == 0) {
+    if (pos == 0) {
       tf = fl;
       continue;
     }
     __syncthreads();
-    if (k < 4) {
+    if (pos == 1) {
       tf = mul_lds<NC, N>(ln, tf, fl, true, M, ln.slotA, false);
     } else {
       sf = c_add(sf, mul<NC, N>(ln, tf, fl, true, Sig, M, false, nullptr, false, 1.0));
@@ -968,8 +997,8 @@ KernelSet make_kernel_set() {
   s.sgf = [](int nblk, hipStream_t st, Geo g, SerField U, double alpha) {
     hipLaunchKernelGGL((k_sgf<NC, N>), dim3(nblk), dim3(64), 0, st, g, U, alpha);
   };
-  s.rect = [](int nblk, hipStream_t st, Geo g, SerField U, SerField R, double* obs2) {
-    hipLaunchKernelGGL((k_rect<NC, N>), dim3(nblk), dim3(64), 0, st, g, U, R, obs2);
+  s.rect = [](int nblk, hipStream_t st, Geo g, SerField U, SerField L2, SerField R, double* obs2) {
+    hipLaunchKernelGGL((k_rect<NC, N>), dim3(nblk), dim3(64), 0, st, g, U, L2, R, obs2);
   };
   s.update = [](int nblk, hipStream_t st, Geo g, SerField D, const double2* eta, double stau, SerField Ub,
                 SerField Uo, SerField L, double2* mm) {
