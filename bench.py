@@ -45,13 +45,17 @@ def ex_flops(N, nc=3):
 ACTIONS = {"wil": 0.0, "tls": -1.0 / 12.0, "iwa": -0.331, "dbw": -1.4088}  # c1, Quenched.cc:313-322
 
 
-def kernel_flops(name, N, V, nc=3, improved=False):
+def kernel_flops(name, N, V, nc=3, improved=False, staples=False):
     """Algorithmic FP64 flops of one launch of a kernel (4V links or V sites).
-    An improved action adds U R to the drift and 72 products per link (six
-    5-link staples in each of three planes, QCDenvNODEpt_test.h:591-607) in
-    the rectangle kernel."""
+    The Jacobi drift is the plaquette field (3 products per plane and site,
+    4.5 per link) plus 6 products per link; the checkerboard sweep's drift
+    (staples=True) forms the staple sum: 13 products per link.  An improved
+    action adds U R to the drift and 72 products per link (six 5-link staples
+    in each of three planes, QCDenvNODEpt_test.h:591-607) in the rectangle
+    kernel, counted as the reference forms them."""
     SP, EX = sp_flops(N, nc), ex_flops(N, nc)
-    per_link = {"drift": (14 if improved else 13) * SP, "update": EX + SP, "update_log": 2 * EX + SP,
+    drift = (13 if staples else 6) + (1 if improved else 0)
+    per_link = {"drift": drift * SP, "plaq_field": 4.5 * SP, "update": EX + SP, "update_log": 2 * EX + SP,
                 "zero_mode": EX, "transform": 2 * SP, "rect": 72 * SP}
     if name in per_link:
         return per_link[name] * 4 * V
@@ -188,7 +192,7 @@ def main():
     name = max(prof, key=lambda k: prof[k][0])
     tot_ms, cnt = prof[name]
     avg_s = tot_ms / cnt / 1e3
-    fl = kernel_flops(name, N, V, nc, improved)
+    fl = kernel_flops(name, N, V, nc, improved, staples=integ == nspt.EULER_CHECKERBOARD)
     if integ == nspt.EULER_CHECKERBOARD and name in ("drift", "update", "update_log"):
         fl /= 8  # one launch per (parity, direction): V/2 of the 4V links
     achieved = fl / avg_s / 1e12 if fl else 0.0

@@ -163,7 +163,8 @@ enum {
   NSPT_PROF_NOISE = 8,      /* device noise */
   NSPT_PROF_REDUCE = 9,     /* deterministic reductions */
   NSPT_PROF_RECT = 10,      /* rectangle staples of an improved action */
-  NSPT_PROF_N = 11
+  NSPT_PROF_PLAQ_FIELD = 11, /* plaquette field of a Jacobi drift */
+  NSPT_PROF_N = 12
 };
 int nspt_profile(nspt_ctx* ctx, int enable);
 int nspt_profile_read(nspt_ctx* ctx, double* ms_total, long long* count);

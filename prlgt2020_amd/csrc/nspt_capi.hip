@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -616,6 +617,10 @@ struct nspt_ctx {
   hipStream_t stream = nullptr;
   std::vector<void*> allocs;
   SerField U{}, F{}, D{}, D1{}, L{}, X{}, G{};
+  // Jacobi drift form: from the plaquette field Pf (6 per site, plane-major)
+  // or from the staple sum as the reference forms it
+  bool plaq_form = false;
+  SerField Pf{};
   // Wilson-loop measurement, allocated on first use: 2-link and 4-link
   // fields (the 2-link field also feeds the rectangles of an improved
   // action), per-block rectangle sums, reduction scratch, the six sums
@@ -840,6 +845,23 @@ bool make_noise(nspt_ctx* c, const nspt_step_params* p, const double2** eta) {
   return true;
 }
 
+// Drift of every link of Uin (Jacobi passes), from its plaquette field or
+// from the staple sums.
+void drift_p(nspt_ctx* c, const SerField& Uin, const SerField& D, const SerField& D1, double tau_g, double rk2c,
+             int mode, double* obs, const SerField& R) {
+  if (!c->plaq_form) {
+    Prof pr(c, NSPT_PROF_DRIFT);
+    c->ks.drift(c->nblk_links, c->stream, c->g, Uin, D, D1, tau_g, rk2c, mode, obs, R, c->c1);
+    return;
+  }
+  {
+    Prof pr(c, NSPT_PROF_PLAQ_FIELD);
+    c->ks.plaq_field(6 * c->g.nb_mu, c->stream, c->g, Uin, c->Pf);
+  }
+  Prof pr(c, NSPT_PROF_DRIFT);
+  c->ks.drift_p(c->nblk_links, c->stream, c->g, Uin, c->Pf, D, D1, tau_g, rk2c, mode, obs, R, c->c1);
+}
+
 // One step; observables to obs_dev (device) if non-null.
 bool do_step(nspt_ctx* c, const nspt_step_params* p, double* obs_dev) {
   const double stau = std::sqrt(p->eps);
@@ -882,23 +904,16 @@ bool do_step(nspt_ctx* c, const nspt_step_params* p, double* obs_dev) {
     if (obs_dev && !obs_reduce(c, obs_dev, false, 2L * c->g.nb_half)) return false;
   } else if (p->integrator == NSPT_RK2) {
     rect(c->U, true);
-    {
-      Prof pr(c, NSPT_PROF_DRIFT);
-      c->ks.drift(c->nblk_links, c->stream, c->g, c->U, c->D1, none, tau_g, 0.0, 0, obs_part, R, c->c1);
-    }
+    drift_p(c, c->U, c->D1, none, tau_g, 0.0, 0, obs_part, R);
     {
       Prof pr(c, NSPT_PROF_UPDATE);
       c->ks.update(c->nblk_links, c->stream, c->g, c->D1, eta, stau, c->U, c->F, none, nullptr);
     }
     rect(c->F, false);
-    {
-      Prof pr(c, NSPT_PROF_DRIFT);
-      c->ks.drift(c->nblk_links, c->stream, c->g, c->F, c->D, c->D1, tau_g, rk2c, 1, nullptr, R, c->c1);
-    }
+    drift_p(c, c->F, c->D, c->D1, tau_g, rk2c, 1, nullptr, R);
   } else {
     rect(c->U, true);
-    Prof pr(c, NSPT_PROF_DRIFT);
-    c->ks.drift(c->nblk_links, c->stream, c->g, c->U, c->D, none, tau_g, 0.0, 0, obs_part, R, c->c1);
+    drift_p(c, c->U, c->D, none, tau_g, 0.0, 0, obs_part, R);
   }
   if (!cb) {
     if (!launch_ok(c, "drift")) return false;
@@ -993,6 +1008,16 @@ int nspt_create(const nspt_geom* geom, nspt_ctx** out) {
   c->nblk_sites = c->g.nb_mu;
   c->nob = 2 * (c->nrt + 1) + 3 * c->nrt;
   c->obs_len = 2 * (c->nrt + 1) + c->nrt + 8 * c->nrt;
+  // The plaquette form of the drift does fewer products but sums them in
+  // another association than the reference; above 16 orders a step is
+  // ill-conditioned in FP64 anyway (the spread of the step under a rounding
+  // of its input exceeds 1e-12), up to 16 the staple form keeps the
+  // reference's association.  NSPT_DRIFT_FORM=staples|plaquettes forces one.
+  c->plaq_form = c->nrt > 16;
+  if (const char* f = std::getenv("NSPT_DRIFT_FORM")) {
+    if (!std::strcmp(f, "staples")) c->plaq_form = false;
+    if (!std::strcmp(f, "plaquettes")) c->plaq_form = true;
+  }
   *out = c;
   if (!c->hip_ok(hipStreamCreate(&c->stream), "hipStreamCreate")) return -3;
   const long P = 4L * V;
@@ -1000,7 +1025,7 @@ int nspt_create(const nspt_geom* geom, nspt_ctx** out) {
   const long part_rows = std::max(4L * c->g.nb_mu, 8L * c->g.nb_half);
   bool ok = c->alloc_field(c->U, P, true) && c->alloc_field(c->F, P, true) && c->alloc_field(c->D, P, false) &&
             c->alloc_field(c->D1, P, false) && c->alloc_field(c->L, P, true) && c->alloc_field(c->X, P, false) &&
-            c->alloc_field(c->G, V, true) &&
+            c->alloc_field(c->G, V, true) && (!c->plaq_form || c->alloc_field(c->Pf, 6L * V, true)) &&
             c->alloc(&c->fb, (size_t)c->nrt * c->NE * V) && c->alloc(&c->eta, (size_t)P * c->NE) &&
             c->alloc(&c->eta_tmp, (size_t)P * c->NE) &&
             c->alloc(&c->mm_part, (size_t)part_rows * c->nrt * c->NE) &&
@@ -1256,8 +1281,7 @@ int nspt_drift(nspt_ctx* c, double eps, double* aos_out) {
     c->ks.link_pair(c->nblk_links, c->stream, c->g, c->U, c->L2, 1);
     c->ks.rect(c->nblk_links, c->stream, c->g, c->U, c->L2, c->R, nullptr);
   }
-  c->ks.drift(c->nblk_links, c->stream, c->g, c->U, c->D, none, -eps / (2.0 * c->nc), 0.0, 0, nullptr,
-              c->c1 != 0.0 ? c->R : none, c->c1);
+  drift_p(c, c->U, c->D, none, -eps / (2.0 * c->nc), 0.0, 0, nullptr, c->c1 != 0.0 ? c->R : none);
   if (!launch_ok(c, "drift")) return -3;
   return download_field(c, c->D, aos_out);
 }

@@ -39,6 +39,11 @@ struct KernelSet {
   int nc = 0, nmax = 0, lpb = 0, qt = 0;  // qt: order-tile edge of the products
   void (*drift)(int nblk, hipStream_t, Geo, SerField U, SerField D, SerField D1, double tau_g, double rk2c, int mode,
                 double* obs, SerField R, double c1) = nullptr;
+  // the plaquette field of U (6 per site, plane-major), and the drift of
+  // `drift` formed from it (Jacobi passes only: g.sub = 0)
+  void (*plaq_field)(int nblk, hipStream_t, Geo, SerField U, SerField P) = nullptr;
+  void (*drift_p)(int nblk, hipStream_t, Geo, SerField U, SerField P, SerField D, SerField D1, double tau_g,
+                  double rk2c, int mode, double* obs, SerField R, double c1) = nullptr;
   // sum of the rectangle staples of every link from U and its 2-link field
   // L2, and (obs2 != null) their per-block plaquette traces
   void (*rect)(int nblk, hipStream_t, Geo, SerField U, SerField L2, SerField R, double* obs2) = nullptr;

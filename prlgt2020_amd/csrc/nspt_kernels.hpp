@@ -232,6 +232,90 @@ __device__ __forceinline__ double2 slot_sum(const double2* slotA0, int nvalid, i
   return s;
 }
 
+// End of a drift evaluation, with U_mu(x) in slot A and W in slot B (flag
+// wf; U's flag uflag): the observables of the block (op, may be null: plaquette
+// sums Tr W per order, norm and Tr U of U_mu(x)), the improved-action term
+// W = c0 W + c1 U R, and D = tau_g reH(W) (mode 1: the RK2 combination with
+// D1).  mode 2 stops after the observables.
+template <int NC, int N>
+__device__ __forceinline__ void drift_finish(const Lane<NC, N>& ln, double2* lds, double2* flagbuf, int nvalid,
+                                             int nrt, int M, double* op, double2 wf, double2 uflag, const SerField& R,
+                                             double c1, long idx, bool ok, double tau_g, int mode, const SerField& D,
+                                             const SerField& D1, double rk2c) {
+  constexpr int NE = NC * NC;
+  using LN = Lane<NC, N>;
+  if (op) {
+    const int lane = threadIdx.x;
+    // norm_k = sum Re Tr(U_k U_k^dag), trU_k = sum Tr U_k over U_mu(x)
+    if (lane < nrt) {
+      double nr = 0.0;
+      double2 tr = zero2();
+      for (int jj = 0; jj < nvalid; jj++) {
+        const double2* m = lds + jj * LN::SLOT + lane * NE;
+#pragma unroll
+        for (int q = 0; q < NE; q++) nr += m[q].x * m[q].x + m[q].y * m[q].y;
+#pragma unroll
+        for (int q = 0; q < NC; q++) tr = c_add(tr, m[q * NC + q]);
+      }
+      op[2 * (nrt + 1) + lane] = nr;
+      op[3 * nrt + 2 + 2 * lane] = tr.x;
+      op[3 * nrt + 2 + 2 * lane + 1] = tr.y;
+    }
+    if (ln.lds_ok && ln.e == 0) flagbuf[ln.j] = wf;
+    __syncthreads();
+    // plaquette sums Tr W_k (k = 0 from the flags)
+    if (lane <= nrt) {
+      double2 s = zero2();
+      for (int jj = 0; jj < nvalid; jj++) {
+        if (lane == 0) {
+          s = c_add(s, c_scale((double)NC, flagbuf[jj]));
+        } else {
+          const double2* m = lds + LN::LPB * LN::SLOT + jj * LN::SLOT + (lane - 1) * NE;
+#pragma unroll
+          for (int q = 0; q < NC; q++) s = c_add(s, m[q * NC + q]);
+        }
+      }
+      op[2 * lane] = s.x;
+      op[2 * lane + 1] = s.y;
+    }
+  }
+  if (mode == 2) return;
+
+  if (R.data) {
+    // improved action (nspt.cc:405-412): W = c0 U Sigma + c1 U R, R the sum
+    // of the rectangle staples, c0 = 1 - 8 c1; U_mu(x) is still in slot A
+    double2 W1[N], T[N];
+    unstage(ln, ln.slotB, W1);
+    __syncthreads();
+    stage_global(ln, ln.slotB, R, idx, nrt);
+    const double2 rf = gflag(R, idx, ok);
+#pragma unroll
+    for (int k = 0; k < N; k++) T[k] = zero2();
+    __syncthreads();
+    mul<NC, N>(ln, uflag, rf, true, T, M, false, nullptr, false, 1.0);
+    __syncthreads();
+    const double c0 = 1.0 - 8.0 * c1;
+#pragma unroll
+    for (int k = 0; k < N; k++) W1[k] = c_add(c_scale(c0, W1[k]), c_scale(c1, T[k]));
+    ln.stage(ln.slotB, W1);
+    __syncthreads();
+  }
+
+  // D = tau_g reH(W)
+  double2 Out[N];
+#pragma unroll
+  for (int k = 0; k < N; k++) Out[k] = c_scale(tau_g, reh_elem(ln, ln.slotB, k));
+  if (mode == 1) {
+    double2 D1v[N];
+    gload<NC, N>(D1, idx, ln.e, nrt, ok, D1v);
+#pragma unroll
+    for (int k = 0; k < N; k++) Out[k] = c_scale(0.5, c_add(D1v[k], Out[k]));
+#pragma unroll
+    for (int k = N - 1; k >= 2; k--) Out[k] = c_add(Out[k], c_scale(rk2c, Out[k - 2]));
+  }
+  gstore<NC, N>(D, idx, ln.e, nrt, ok, Out);
+}
+
 // ------------------------------------------------------------ drift
 // D = tau_g * reH(U_mu(x) * sum_{nu != mu} staple_{mu nu}(x))
 // (nspt.cc:383-417, staple QCDenvNODEpt_test.h:554-557).
@@ -358,78 +442,228 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_drift(Geo g, SerField
     __syncthreads();
   }
   // slot A: U_mu(x); slot B: W
+  drift_finish<NC, N>(ln, lds, flagbuf, nvalid, nrt, M,
+                      obs_part ? obs_part + (long)lb.part * (2 * (nrt + 1) + 3 * nrt) : nullptr, wf, uflag, R, c1,
+                      idx, ok, tau_g, mode, D, D1, rk2c);
+}
 
-  double* op = obs_part ? obs_part + (long)lb.part * (2 * (nrt + 1) + 3 * nrt) : nullptr;
-  if (op) {
-    const int lane = threadIdx.x;
-    // norm_k = sum Re Tr(U_k U_k^dag), trU_k = sum Tr U_k over U_mu(x)
-    if (lane < nrt) {
-      double nr = 0.0;
-      double2 tr = zero2();
-      for (int jj = 0; jj < nvalid; jj++) {
-        const double2* m = lds + jj * LN::SLOT + lane * NE;
+// ------------------------------------------------------------ drift from the plaquette field
+// The same drift without forming the staple sum: W = U_mu(x) Sigma_mu(x) is
+// a sum over nu != mu of the two plaquettes through the link, rotated to
+// start there,
+//   W = sum_nu P_mu,nu(x) + U_nu(x-nu)^dag P_mu,nu(x-nu)^dag U_nu(x-nu),
+//   P_mu,nu(x) = U_mu(x) U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag,
+// and P_nu,mu = P_mu,nu^dag, so the six plaquettes of a site (3 products
+// each) serve all four of its links: 4.5 + 6 products per link instead of
+// the 13 of k_drift (no unitarity is assumed).
+//
+// k_plaq_field: P of plane p = (mu < nu) at every site, stored at series
+// index p V + site, planes (0,1) (0,2) (0,3) (1,2) (1,3) (2,3).
+__device__ __forceinline__ int plane_index(int a, int b) {
+  // a < b
+  return a == 0 ? b - 1 : (a == 1 ? b + 1 : 5);
+}
+template <int NC, int N>
+__global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_plaq_field(Geo g, SerField U, SerField P) {
+  using LN = Lane<NC, N>;
+  __shared__ double2 lds[LN::LDS];
+  LN ln(lds);
+  const int nrt = g.nrt;
+  const int M = min(N, nrt) - 1;
+  const int b = block_order();
+  const int plane = b % 6, sb = b / 6;
+  const int mu = plane < 3 ? 0 : (plane < 5 ? 1 : 2);
+  const int nu = plane < 3 ? plane + 1 : (plane < 5 ? plane - 1 : 3);
+  const int site0 = sb * LN::LPB;
+  const int site = min(site0 + ln.j, g.V - 1);
+  const bool ok = ln.lds_ok && (site0 + ln.j) < g.V;
+  const long V = g.V;
+  int cpn = TW0, cpm = TW0;
+  const int xpnu = site_move_tw(g, site, nu, 1, cpn);
+  const int xpmu = site_move_tw(g, site, mu, 1, cpm);
+  // product 0: C = U_nu(x) U_mu(x+nu) -> C^dag in place in slot B
+  // product 1: T = U_nu(x+mu) C^dag   -> in place in slot B
+  // product 2: P = U_mu(x) T          -> registers (PB, free after staging)
+  // one product per iteration (the product body is emitted once); the next
+  // left factor is loaded into PA while the current product runs
+  double2 PA[N], PB[N];
+  gload<NC, N>(U, nu * V + site, ln.e, nrt, ok, PA);
+  gload_tw<NC, N>(g, ln, U, mu * V + xpnu, cpn, nrt, ok, PB);
+  double2 fa = gflag(U, nu * V + site, ok), fb = gflag(U, mu * V + xpnu, ok);
+  ln.stage(ln.slotB, PB);
 #pragma unroll
-        for (int q = 0; q < NE; q++) nr += m[q].x * m[q].x + m[q].y * m[q].y;
-#pragma unroll
-        for (int q = 0; q < NC; q++) tr = c_add(tr, m[q * NC + q]);
-      }
-      op[2 * (nrt + 1) + lane] = nr;
-      op[3 * nrt + 2 + 2 * lane] = tr.x;
-      op[3 * nrt + 2 + 2 * lane + 1] = tr.y;
-    }
-    if (ln.lds_ok && ln.e == 0) flagbuf[ln.j] = wf;
+  for (int k = 0; k < N; k++) PB[k] = zero2();
+  for (int i = 0; i < 3; i++) {
+    asm volatile("" ::: "memory");
+    ln.stage(ln.slotA, PA);
+    const double2 a0 = fa;
     __syncthreads();
-    // plaquette sums Tr W_k (k = 0 from the flags)
-    if (lane <= nrt) {
-      double2 s = zero2();
-      for (int jj = 0; jj < nvalid; jj++) {
-        if (lane == 0) {
-          s = c_add(s, c_scale((double)NC, flagbuf[jj]));
-        } else {
-          const double2* m = lds + LN::LPB * LN::SLOT + jj * LN::SLOT + (lane - 1) * NE;
+    if (i < 2) {
+      const long il = (i == 0) ? nu * V + xpmu : mu * V + site;
+      gload_tw<NC, N>(g, ln, U, il, i == 0 ? cpm : TW0, nrt, ok, PA);
+      fa = gflag(U, il, ok);
+    }
+    const double2 f = mul<NC, N, 1, false, NSPT_QT_DRIFT>(ln, a0, fb, true, PB, M, i < 2, ln.slotB, i == 0, 1.0);
+    fb = (i == 0) ? c_conj(f) : f;
+    __syncthreads();
+  }
+  const double2 pf = fb;
+  const long ip = (long)plane * V + site;
+  gstore<NC, N>(P, ip, ln.e, nrt, ok, PB);
+  if (ok && ln.e == 0) P.flag[ip] = pf;
+}
+
+// k_drift_p: the drift of k_drift (same modes, observables and improved
+// action) from the plaquette field of U.  Per nu: t = U_nu(x-nu)^dag X
+// (X = P_mu,nu(x-nu)^dag) in place in slot A, then Sigma += t U_nu(x-nu);
+// the plaquettes at x are added as loaded.
+template <int NC, int N>
+__global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_drift_p(Geo g, SerField U, SerField P, SerField D,
+                                                                   SerField D1, double tau_g, double rk2c, int mode,
+                                                                   double* obs_part, SerField R, double c1) {
+  constexpr int NE = NC * NC;
+  using LN = Lane<NC, N>;
+  __shared__ double2 lds[LN::LDS];
+  __shared__ double2 flagbuf[LN::LPB];
+  LN ln(lds);
+  const int nrt = g.nrt;
+  const int M = min(N, nrt) - 1;
+  const LinkBlock lb = link_block<LN::LPB>(g);
+  const int mu = lb.mu;
+  const int site0 = lb.site0;
+  const int nvalid = min(LN::LPB, lb.count - site0);
+  const int site = min(site0 + ln.j, lb.count - 1);
+  const bool ok = ln.lds_ok && (site0 + ln.j) < lb.count;
+  const long V = g.V;
+  const long idx = mu * V + site;
+  const int etr = ln.c * NC + ln.r;  // element of the transpose
+
+  // the plaquettes at x: P_mu,nu(x), or P_nu,mu(x)^dag for nu < mu (the
+  // first loaded straight into Sigma, the other two after the products,
+  // into the prefetch registers; staging them through LDS before the
+  // products measured 24% slower)
+  auto at_x = [&](int p, bool& dg) {
+    const int nu = (p < mu) ? p : p + 1;
+    dg = nu < mu;
+    return (long)plane_index(min(mu, nu), max(mu, nu)) * V + site;
+  };
+  double2 Sig[N];
+  double2 sigf;
+  {
+    bool dg;
+    const long ip = at_x(0, dg);
+    gload<NC, N>(P, ip, dg ? etr : ln.e, nrt, ok, Sig);
+    sigf = gflag(P, ip, ok);
+    if (dg) {
+      sigf = c_conj(sigf);
 #pragma unroll
-          for (int q = 0; q < NC; q++) s = c_add(s, m[q * NC + q]);
+      for (int k = 0; k < N; k++) Sig[k] = c_conj(Sig[k]);
+    }
+  }
+  // the plaquettes at x - nu through the link.  Up to N = 20 the next
+  // plaquette and link are prefetched into registers; above that the
+  // registers are needed for Sigma and the product, and they are staged
+  // straight from HBM.
+  constexpr bool PF = N <= 20;
+  constexpr int NP = PF ? N : 1;
+  double2 PX[NP], PU[NP];
+  int cm = TW0;
+  long ix, iu;
+  bool xdag;
+  auto locate = [&](int p) {
+    const int nu = (p < mu) ? p : p + 1;
+    cm = TW0;
+    const int xm = site_move_tw(g, site, nu, -1, cm);
+    // P_mu,nu(x-nu)^dag: the stored P_mu,nu daggered (nu > mu), or the
+    // stored P_nu,mu itself (nu < mu)
+    xdag = nu > mu;
+    ix = (long)plane_index(min(mu, nu), max(mu, nu)) * V + xm;
+    iu = (long)nu * V + xm;
+  };
+  locate(0);
+  if constexpr (PF) {
+    gload_tw<NC, N>(g, ln, P, ix, cm, nrt, ok, PX);
+    gload_tw<NC, N>(g, ln, U, iu, cm, nrt, ok, PU);
+  }
+  double2 xf = gflag(P, ix, ok), uf = gflag(U, iu, ok);
+  bool xd = xdag;
+  double2 tf = zero2(), ufc = zero2();
+  // one product per iteration (the product body is emitted once):
+  //   even i: slot A <- X^dag = P_mu,nu(x-nu) (the stored P, or the dagger
+  //           of the stored P_nu,mu), slot B <- U_nu(x-nu); X^dag U is
+  //           retired daggered in place into slot A: t = U^dag X
+  //   odd i:  Sigma += t U_nu(x-nu)
+  // the next plaquette and link (last: U_mu(x)) load during the even products
+  for (int i = 0; i < 6; i++) {
+    asm volatile("" ::: "memory");
+    const bool even = (i & 1) == 0;
+    double2 a0 = tf;
+    if (even) {
+      if constexpr (PF) {
+        ln.stage(ln.slotA, PX, !xd);
+        ln.stage(ln.slotB, PU);
+      } else {
+        stage_global_tw(g, ln, ln.slotA, P, ix, cm, nrt, !xd);
+        stage_global_tw(g, ln, ln.slotB, U, iu, cm, nrt);
+      }
+      ufc = uf;
+      a0 = xd ? xf : c_conj(xf);
+    }
+    __syncthreads();
+    if (even) {
+      if (i + 2 < 6) {
+        locate(i / 2 + 1);
+        if constexpr (PF) {
+          gload_tw<NC, N>(g, ln, P, ix, cm, nrt, ok, PX);
+          gload_tw<NC, N>(g, ln, U, iu, cm, nrt, ok, PU);
         }
+        xf = gflag(P, ix, ok);
+        uf = gflag(U, iu, ok);
+        xd = xdag;
+      } else {
+        if constexpr (PF) gload<NC, N>(U, idx, ln.e, nrt, ok, PU);  // U_mu(x), for the observables
+        uf = gflag(U, idx, ok);
       }
-      op[2 * lane] = s.x;
-      op[2 * lane + 1] = s.y;
     }
-  }
-  if (mode == 2) return;
-
-  if (R.data) {
-    // improved action (nspt.cc:405-412): W = c0 U Sigma + c1 U R, R the sum
-    // of the rectangle staples, c0 = 1 - 8 c1; U_mu(x) is still in slot A
-    double2 W1[N], T[N];
-    unstage(ln, ln.slotB, W1);
-    __syncthreads();
-    stage_global(ln, ln.slotB, R, idx, nrt);
-    const double2 rf = gflag(R, idx, ok);
-#pragma unroll
-    for (int k = 0; k < N; k++) T[k] = zero2();
-    __syncthreads();
-    mul<NC, N>(ln, uflag, rf, true, T, M, false, nullptr, false, 1.0);
-    __syncthreads();
-    const double c0 = 1.0 - 8.0 * c1;
-#pragma unroll
-    for (int k = 0; k < N; k++) W1[k] = c_add(c_scale(c0, W1[k]), c_scale(c1, T[k]));
-    ln.stage(ln.slotB, W1);
+    const double2 f = mul<NC, N, 1, false, NSPT_QT_DRIFT>(ln, a0, ufc, true, Sig, M, even, ln.slotA, even, 1.0);
+    if (even)
+      tf = c_conj(f);
+    else
+      sigf = c_add(sigf, f);
     __syncthreads();
   }
-
-  // D = tau_g reH(W)
-  double2 Out[N];
+  // U_mu(x) into slot A; the other two plaquettes at x into Sigma
+  if constexpr (PF) {
+    ln.stage(ln.slotA, PU);
+    bool d1, d2;
+    const long i1 = at_x(1, d1), i2 = at_x(2, d2);
+    gload<NC, N>(P, i1, d1 ? etr : ln.e, nrt, ok, PX);
+    gload<NC, N>(P, i2, d2 ? etr : ln.e, nrt, ok, PU);
+    const double2 f1 = gflag(P, i1, ok), f2 = gflag(P, i2, ok);
+    sigf = c_add(sigf, c_add(d1 ? c_conj(f1) : f1, d2 ? c_conj(f2) : f2));
 #pragma unroll
-  for (int k = 0; k < N; k++) Out[k] = c_scale(tau_g, reh_elem(ln, ln.slotB, k));
-  if (mode == 1) {
-    double2 D1v[N];
-    gload<NC, N>(D1, idx, ln.e, nrt, ok, D1v);
+    for (int k = 0; k < N; k++)
+      Sig[k] = c_add(Sig[k], c_add(d1 ? c_conj(PX[k]) : PX[k], d2 ? c_conj(PU[k]) : PU[k]));
+  } else {
+    for (int q = 1; q < 3; q++) {
+      bool dq;
+      const long iq = at_x(q, dq);
+      stage_global(ln, ln.slotB, P, iq, nrt, dq);
+      const double2 fq = gflag(P, iq, ok);
+      sigf = c_add(sigf, dq ? c_conj(fq) : fq);
+      __syncthreads();
 #pragma unroll
-    for (int k = 0; k < N; k++) Out[k] = c_scale(0.5, c_add(D1v[k], Out[k]));
-#pragma unroll
-    for (int k = N - 1; k >= 2; k--) Out[k] = c_add(Out[k], c_scale(rk2c, Out[k - 2]));
+      for (int k = 0; k < N; k++) Sig[k] = c_add(Sig[k], ln.slotB[k * NE + ln.e]);
+      __syncthreads();
+    }
+    stage_global(ln, ln.slotA, U, idx, nrt);
   }
-  gstore<NC, N>(D, idx, ln.e, nrt, ok, Out);
+  ln.stage(ln.slotB, Sig);
+  __syncthreads();
+  // slot A: U_mu(x); slot B: W
+  drift_finish<NC, N>(ln, lds, flagbuf, nvalid, nrt, M,
+                      obs_part ? obs_part + (long)lb.part * (2 * (nrt + 1) + 3 * nrt) : nullptr, sigf, uf, R, c1,
+                      idx, ok, tau_g, mode, D, D1, rk2c);
 }
 
 // ------------------------------------------------------------ update
@@ -993,6 +1227,14 @@ KernelSet make_kernel_set() {
   s.drift = [](int nblk, hipStream_t st, Geo g, SerField U, SerField D, SerField D1, double tau_g, double rk2c,
                int mode, double* obs, SerField R, double c1) {
     hipLaunchKernelGGL((k_drift<NC, N>), dim3(nblk), dim3(64), 0, st, g, U, D, D1, tau_g, rk2c, mode, obs, R, c1);
+  };
+  s.plaq_field = [](int nblk, hipStream_t st, Geo g, SerField U, SerField P) {
+    hipLaunchKernelGGL((k_plaq_field<NC, N>), dim3(nblk), dim3(64), 0, st, g, U, P);
+  };
+  s.drift_p = [](int nblk, hipStream_t st, Geo g, SerField U, SerField P, SerField D, SerField D1, double tau_g,
+                 double rk2c, int mode, double* obs, SerField R, double c1) {
+    hipLaunchKernelGGL((k_drift_p<NC, N>), dim3(nblk), dim3(64), 0, st, g, U, P, D, D1, tau_g, rk2c, mode, obs, R,
+                       c1);
   };
   s.sgf = [](int nblk, hipStream_t st, Geo g, SerField U, double alpha) {
     hipLaunchKernelGGL((k_sgf<NC, N>), dim3(nblk), dim3(64), 0, st, g, U, alpha);

@@ -67,7 +67,7 @@ EXPORTS = ["nspt_create", "nspt_destroy", "nspt_last_error", "nspt_obs_len", "ns
            "nspt_unitarize", "nspt_set_action", "nspt_set_twist"]
 
 PROF_NAMES = ["drift", "update", "update_log", "zero_mode", "fa_div", "dft", "fa_gexp", "transform", "noise",
-              "reduce", "rect"]
+              "reduce", "rect", "plaq_field"]
 
 
 def lib():
