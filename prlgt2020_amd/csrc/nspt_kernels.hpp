@@ -560,11 +560,11 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_drift_p(Geo g, SerFie
       for (int k = 0; k < N; k++) Sig[k] = c_conj(Sig[k]);
     }
   }
-  // the plaquettes at x - nu through the link.  Up to N = 20 the next
-  // plaquette and link are prefetched into registers; above that the
-  // registers are needed for Sigma and the product, and they are staged
-  // straight from HBM.
-  constexpr bool PF = N <= 20;
+  // the plaquettes at x - nu through the link.  Up to N = 20 (24 for
+  // SU(2), whose smaller products leave more registers) the next plaquette
+  // and link are prefetched into registers; above that the registers are
+  // needed for Sigma and the product, and they are staged straight from HBM.
+  constexpr bool PF = N <= (NC == 2 ? 24 : 20);
   constexpr int NP = PF ? N : 1;
   double2 PX[NP], PU[NP];
   int cm = TW0;
