@@ -91,14 +91,15 @@ def cpu_baseline(N, integrator, budget_s=12.0):
                       % (N, "RK2" if integrator == 1 else "Euler", done, dt)}
 
 
-def batched_sweep(nsteps, warmup):
-    """Epsilon-replica sweep: SU(3) 4 x 12^4 + 4 x 16^4, N = 16, RK2 + zero
-    modes + FA, eps in {0.01, 0.015, 0.02, 0.03} per volume, distinct seeds.
-    Plain: each replica stepped alone (nspt_evolve, its own launches).
-    Batched: one graph-replayed step of all replicas, tiles autotuned."""
+def batched_sweep(nsteps, warmup, sizes=(12, 16)):
+    """Epsilon-replica sweep: SU(3) 4 x a^4 + 4 x b^4 for sizes (a, b) (the
+    workload is 12, 16), N = 16, RK2 + zero modes + FA, eps in {0.01, 0.015,
+    0.02, 0.03} per volume, distinct seeds.  Plain: each replica stepped
+    alone (nspt_evolve, its own launches).  Batched: one graph-replayed step
+    of all replicas, tiles autotuned."""
     from prlgt2020_amd import nspt
     N = 16
-    Ls = [(12,) * 4] * 4 + [(16,) * 4] * 4
+    Ls = [(sizes[0],) * 4] * 4 + [(sizes[1],) * 4] * 4
     eps = [0.01, 0.015, 0.02, 0.03] * 2
     ps = [nspt.Context.params(e, 0.05, nspt.RK2, nspt.GF_FA, 1, nspt.NOISE_DEVICE, seed=100 + r, step=0)
           for r, e in enumerate(eps)]
@@ -118,8 +119,8 @@ def batched_sweep(nsteps, warmup):
     b.evolve(nsteps, want_obs=True)
     t_batch = time.perf_counter() - t0
     b.close()
-    return {"workload": "SU(3) 4x12^4 + 4x16^4, 16 orders, RK2 + zero modes + FA, eps {0.01,0.015,0.02,0.03} "
-                        "per volume, distinct seeds, device noise",
+    return {"workload": "SU(3) 4x%d^4 + 4x%d^4, 16 orders, RK2 + zero modes + FA, eps {0.01,0.015,0.02,0.03} "
+                        "per volume, distinct seeds, device noise" % sizes,
             "steps": nsteps, "plain_value": work * nsteps / t_plain, "batched_value": work * nsteps / t_batch,
             "unit": "link-order updates/s", "speedup": t_plain / t_batch, "tile_edge": qt,
             "tune_ms_per_step": [round(x, 3) for x in tune_ms if x > 0]}
@@ -241,6 +242,9 @@ def main():
     ctx.close()
     if not args.no_batched:
         out["batched"] = batched_sweep(max(4, args.steps), 2)
+        # replicas too small to fill the GPU alone, where the graph and the
+        # concurrent branches pay
+        out["batched_small"] = batched_sweep(max(20, 4 * args.steps), 4, sizes=(4, 6))
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(N, 1 if integ == nspt.RK2 else 0)
     print(json.dumps(out))
