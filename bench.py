@@ -207,6 +207,14 @@ def main():
             if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == name:
                 traffic = float(f[2])
     step_tf = step_flops(N, V, integ, nc, improved) * args.steps / dt / 1e12
+    # a Jacobi drift evaluation is k_plaq_field + k_drift_p: at the canonical
+    # 13 products per link (14 with an improved action) over both kernels
+    drift_eval = None
+    if prof.get("plaq_field", (0, 0))[1] and prof["drift"][1]:
+        ev_ms = (prof["plaq_field"][0] + prof["drift"][0]) / prof["drift"][1]
+        ev_fl = kernel_flops("drift", N, V, nc, improved, staples=True)
+        drift_eval = {"kernels": ["k_plaq_field", "k_drift_p"], "ms": ev_ms, "canonical_flops": ev_fl,
+                      "achieved": ev_fl / ev_ms / 1e9, "frac": ev_fl / ev_ms / 1e9 / PEAK_FP64_TFLOPS}
 
     out = {
         "metric": "link-order updates/s per Langevin step (FP64), plain single replica",
@@ -235,7 +243,8 @@ def main():
         "roofline": {"bound": "mfma", "kernel": name, "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
                      "avg_launch_ms": avg_s * 1e3, "flops_per_launch": fl,
-                     "step_fp64_tflops": step_tf, "step_frac": step_tf / PEAK_FP64_TFLOPS},
+                     "step_fp64_tflops": step_tf, "step_frac": step_tf / PEAK_FP64_TFLOPS,
+                     "drift_evaluation": drift_eval},
         "kernels_ms_per_step": {k: v[0] / args.steps for k, v in prof.items() if v[1]},
         "plaquette_order2": float(plaq[2].real),
     }
