@@ -84,3 +84,28 @@ def test_checkerboard_sweep_converges_to_equipartition():
           % (a, b, p0, e0, want))
     assert abs(p0 - want) < 3 * e0
     assert a < b < 0  # the bias shrinks with eps
+
+
+def test_twisted_lattice_has_no_zero_modes():
+    """Twisted boundary conditions in y and z with the reference's SU(3)
+    twist matrices (an irreducible pair: W_y W_z = w W_z W_y): no colour
+    component is constant on the lattice, so all 3 V (NC^2-1) transverse
+    modes are nonzero and equipartition gives <P_2> = -(NC^2-1)/4 = -2
+    exactly, without any zero-mode subtraction -- against -(NC^2-1)(1-1/V)/4
+    = -1.99219 on the periodic lattice with its zero modes removed.  RK2 with
+    the checkerboard sequential gauge fixing (FA needs periodic conditions)."""
+    from prlgt2020_amd import nspt
+    L, N, nc, eps = (4, 4, 4, 4), 4, 3, 0.02
+    want, periodic = -(nc * nc - 1) / 4, -(nc * nc - 1) * (1 - 1 / 256) / 4
+    print("twisted equipartition run: SU(3) 4^4 N=4 RK2 eps=%g, y and z twisted" % eps, flush=True)
+    ctx = nspt.Context(L, nc, N)
+    ctx.set_twist(0b0110)
+    p = nspt.Context.params(eps, -0.05, nspt.RK2, nspt.GF_CHECKERBOARD, 0, nspt.NOISE_DEVICE, seed=23, step=0)
+    ctx.evolve(p, 1000, want_obs=False)
+    plaq = ctx.evolve(p, 48000)[:, :2 * (N + 1)].view(np.complex128)
+    m2, e2 = blocked(plaq[:, 2].real)
+    m3, e3 = blocked(plaq[:, 3].real)
+    print("twisted: order 2 %.5f +- %.5f (expected %.5f; periodic with zero modes removed %.5f)"
+          % (m2, e2, want, periodic))
+    assert abs(m2 - want) < 4 * e2
+    assert abs(m3) < 5 * e3 + 1e-12
