@@ -91,6 +91,45 @@ def cpu_baseline(N, integrator, budget_s=12.0):
                       % (N, "RK2" if integrator == 1 else "Euler", done, dt)}
 
 
+_FARM_CHILD = """
+import sys, time
+sys.path.insert(0, %r)
+from oracle import oracle as O
+N, integrator, budget, seed = %d, %d, %f, int(sys.argv[1])
+L, nc, V = (4, 4, 4, 4), 3, 256
+U = O.cold_field(L, nc, N)
+done, t0 = 0, time.time()
+while time.time() - t0 < budget:
+    eta = O.noise_philox(nc, seed, done, 4 * V)
+    O.step(L, nc, N, U, 0.02, 0.05, integrator=integrator, update=0, gauge_fix=1, zero_modes=1, eta=eta)
+    done += 1
+print(done, time.time() - t0)
+"""
+
+
+def cpu_farm(N, integrator, budget_s=8.0, procs=None):
+    """The reference's farm model (one independent replica per core,
+    Quenched.cc:262): the oracle step of cpu_baseline in `procs` child
+    processes at once for about budget_s seconds; throughput summed.  Run
+    before the GPU is touched (the children are separate programs)."""
+    import subprocess
+    procs = procs or min(16, os.cpu_count() or 1)
+    code = _FARM_CHILD % (ROOT, N, integrator, budget_s)
+    ps = [subprocess.Popen([sys.executable, "-c", code, str(r + 1)], stdout=subprocess.PIPE, text=True)
+          for r in range(procs)]
+    V, total, steps = 256, 0.0, 0
+    for p in ps:
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            return None
+        n, dt = out.split()
+        total += 4 * V * N * int(n) / float(dt)
+        steps += int(n)
+    return {"value": total, "unit": "link-order updates/s", "cores": procs, "kind": "port",
+            "sample": "%d processes x oracle SU(3) 4^4, N=%d, %s + zero modes + FA, %.0f s each, %d steps in all"
+                      % (procs, N, "RK2" if integrator == 1 else "Euler", budget_s, steps)}
+
+
 def batched_sweep(nsteps, warmup, sizes=(12, 16)):
     """Epsilon-replica sweep: SU(3) 4 x a^4 + 4 x b^4 for sizes (a, b) (the
     workload is 12, 16), N = 16, RK2 + zero modes + FA, eps in {0.01, 0.015,
@@ -154,6 +193,9 @@ def main():
         print("bench.py: this build runs on one GPU only; the multi-GPU layout is described in DESIGN.md "
               "(--gpus %d not supported)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+
+    integ_cpu = {"rk2": 1, "euler": 0, "checkerboard": 0}[args.integrator]
+    farm = None if args.no_cpu else cpu_farm(args.N, integ_cpu)
 
     from prlgt2020_amd import nspt
     if args.copy_calib_mib:
@@ -256,6 +298,7 @@ def main():
         out["batched_small"] = batched_sweep(max(20, 4 * args.steps), 4, sizes=(4, 6))
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(N, 1 if integ == nspt.RK2 else 0)
+        out["cpu_baseline"]["farm"] = farm
     print(json.dumps(out))
 
 
