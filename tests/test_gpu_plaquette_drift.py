@@ -65,3 +65,18 @@ def test_plaquette_drift_steps(nspt, nc, integrator, c1, mask, relerr):
         for a, b in zip(nspt.split_obs(obs_g, N), O.split_obs(obs_o, N)):
             assert np.abs(a - b).max() <= TOL * max(1.0, np.abs(b).max())
     ctx.close()
+
+
+@pytest.mark.parametrize("nc,N", [(3, 5), (3, 18), (2, 22)])
+def test_plaquette_drift_runtime_order_below_instance(nspt, nc, N, relerr):
+    """A runtime order below the compiled one (5 on the N = 6 instance, 18
+    on 20, 22 on SU(2)'s 24): the products truncate at the runtime order.
+    Drift of a random field, against the oracle at 1e-12 per order."""
+    L = (2, 2, 2, 4)
+    U = O.random_field(L, nc, N, seed=19, scale=0.3)
+    ctx = nspt.Context(L, nc, N)
+    ctx.upload(U)
+    e = relerr(ctx.drift(0.05), O.drift(L, nc, N, U, 0.05), nc, N)
+    print("per-order rel err", e)
+    assert e.max() <= TOL
+    ctx.close()
