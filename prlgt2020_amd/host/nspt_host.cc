@@ -14,7 +14,7 @@ nspt_params_t nspt_pars;
 act_params_t act_pars;
 mi355x_opts_t mi355x_opts = {0, 1, 1, 0, 1};
 Cplx* w = nullptr;
-std::ofstream plaqfile, normfile, trufile, loop_nxm;
+std::ofstream plaqfile, normfile, trufile, loop_nxm, loop2x2;
 
 double Cplx::mod() const { return std::sqrt(re * re + im * im); }
 
@@ -274,6 +274,24 @@ void ComputeLoops(ptGluon_fld& Umu) {
   for (int w = 0; w < 4; w++)
     for (int i1 = 0; i1 <= PTORD; i1 += 2) loop_nxm << l[2 * ((size_t)w * n1 + i1)] * half[w] * loopNorm << "\t";
   loop_nxm << std::endl;
+}
+
+void WL2x2(ptGluon_fld& Umu) {
+  attach(Umu);
+  const int n1 = PTORD + 1;
+  std::vector<double> l((size_t)4 * 2 * n1);
+  if (nspt_loops(E.ctx, l.data()) != 0) fail("nspt_loops");
+  // the reference walks each 2x2 loop the other way round (mu > nu, along mu
+  // first), which conjugates its trace: the real parts are those of l22.
+  // Its loop prints ptU[i1] (order i1 + 1) for odd i1 <= PTORD (OpenMP form,
+  // the default build).
+  const double D18 = 1.0 / 18.0;  // MyMath.h:48
+  for (int i1 = 1; i1 <= PTORD; i1 += 2) {
+    const int k = i1 + 1;
+    const double v = k <= PTORD ? l[2 * ((size_t)n1 + k)] : 0.0;
+    loop2x2 << v / Umu.Z->Size * D18 << "\t";
+  }
+  loop2x2 << std::endl;
 }
 
 int plaquette_check(const Cplx* w1, const Cplx* w2) {

@@ -106,6 +106,7 @@ typedef struct {
   char* trun;
   char* logn;
   char* name_nxm;
+  char* name2x2;
 } nspt_params_t;
 
 typedef struct {
@@ -128,6 +129,7 @@ typedef struct {
   unsigned long long noise_seed;
   int action;        // GAUGE_ACTION (choices.h:101-113): 0 WIL, 1 TLSYM, 2 IWA, 3 DBW2
   int twist;         // TWISTED_BC (choices.h:43-55): bit d twists direction d; 0 periodic
+  int wl2x2;         // __WILLOOP_2x2__ (choices.h:77): write the 2x2 loop file
 } mi355x_opts_t;
 
 // The twist matrices of twistmatrices.h (not IDENTITY_TWIST), 4 x NC x NC
@@ -145,7 +147,7 @@ extern mi355x_opts_t mi355x_opts;
 extern Cplx* w;
 // Output streams written once per Langevin step (nspt.cc:541-597), and the
 // Wilson-loop stream written by ComputeLoops.
-extern std::ofstream plaqfile, normfile, trufile, loop_nxm;
+extern std::ofstream plaqfile, normfile, trufile, loop_nxm, loop2x2;
 
 // Seeds the host noise stream as the reference seeds Rand[0]: srand(seed);
 // ranlxd(rand()) (Quenched.cc:301-306).
@@ -161,6 +163,11 @@ int plaquette_check(const Cplx* w1, const Cplx* w2);
 // one line on loop_nxm with the even orders 0, 2, .. <= PTORD of each, in
 // the reference's normalisation.
 void ComputeLoops(ptGluon_fld& Umu);
+// 2x2 Wilson loops (nspt.cc:1064-1145, __WILLOOP_2x2__): one line on loop2x2
+// with Re Tr of their sum over sites and planes at orders 2, 4, .. (PTORD+1
+// when PTORD is odd, which the truncated products leave zero), times
+// 1/(18 V).
+void WL2x2(ptGluon_fld& Umu);
 void mi355x_sync_to_host(ptGluon_fld& Umu);
 void mi355x_sync_to_device(ptGluon_fld& Umu);
 void mi355x_release();

@@ -12,7 +12,8 @@
 // gauge fixing) or 2 (the sequential gauge fixing in checkerboard order),
 // -Z 0 (keep zero modes), -D 1 (device noise instead of the reference's
 // ranlxd stream), -A (improved gauge action), -L (restart from a
-// configuration of fewer orders).  -I 2 -F 2 with the shipped Quench.cfg
+// configuration of fewer orders), -T (twisted boundary conditions), -W 1
+// (the 2x2 Wilson loop file).  -I 2 -F 2 with the shipped Quench.cfg
 // (alpha -0.05) is the reference's default build, reordered.
 //
 // The default gauge fixing is the Fourier-accelerated one (nspt.cc:866-948);
@@ -80,6 +81,7 @@ int initialize(int argc, char** argv) {
   nspt_pars.logn = new char[256];
   nspt_pars.trun = new char[256];
   nspt_pars.name_nxm = new char[256];
+  nspt_pars.name2x2 = new char[256];
 
   FILE* fp = std::fopen("Quench.cfg", "r");
   if (!fp) {
@@ -103,7 +105,7 @@ int initialize(int argc, char** argv) {
   std::strcpy(nspt_pars.damon, "damocle.dag");
 
   int opt;
-  while ((opt = getopt(argc, argv, "l:s:b:d:r:g:a:t:p:c:n:I:F:Z:D:L:A:T:h")) != -1) {
+  while ((opt = getopt(argc, argv, "l:s:b:d:r:g:a:t:p:c:n:I:F:Z:D:L:A:T:W:h")) != -1) {
     switch (opt) {
       case 'l': act_pars.sz[0] = act_pars.sz[1] = act_pars.sz[2] = act_pars.sz[3] = std::atoi(optarg); break;
       case 's': nspt_pars.Sweep = std::atoi(optarg); break;
@@ -123,6 +125,7 @@ int initialize(int argc, char** argv) {
       case 'L': low_order = std::atoi(optarg); break;
       case 'A': mi355x_opts.action = std::atoi(optarg); break;
       case 'T': mi355x_opts.twist = std::atoi(optarg); break;
+      case 'W': mi355x_opts.wl2x2 = std::atoi(optarg); break;
       case 'h':
         std::printf("\n-l\t\tlattice size (L0 = L1 = L2 = L3)\n-s\t\tnumber of iterations (SWEEP)\n"
                     "-b\t\titerations between damocle reads (BEAT)\n-r\t\trandom seed\n-g\t\ttau gluons\n"
@@ -133,7 +136,8 @@ int initialize(int argc, char** argv) {
                     "-Z\t\tzero-mode subtraction: 1 on, 0 off\n-D\t\tnoise: 0 host ranlxd, 1 device Philox\n"
                     "-L\t\twith -t 1: start from the configuration saved with this lower allocORD\n"
                     "-A\t\tgauge action: 0 Wilson, 1 TLSYM, 2 IWA, 3 DBW2\n"
-                    "-T\t\ttwisted boundary conditions: bit 0 x, 1 y, 2 z, 3 t (6: y and z)\n\n");
+                    "-T\t\ttwisted boundary conditions: bit 0 x, 1 y, 2 z, 3 t (6: y and z)\n"
+                    "-W\t\t1: write the 2x2 Wilson loop file (__WILLOOP_2x2__)\n\n");
         std::exit(0);
     }
   }
@@ -168,6 +172,8 @@ int initialize(int argc, char** argv) {
   std::string plaq_stem = nspt_pars.plaqn, conf_stem = nspt_pars.confn;
   std::snprintf(nspt_pars.name_nxm, 256, "%s_nxm_%s_nf0_%dx%dx%dx%d_a%2.2f_tg%3.3f_o%d.txt", plaq_stem.c_str(),
                 type_g(), z[0], z[1], z[2], z[3], a, tg, PTORD);
+  std::snprintf(nspt_pars.name2x2, 256, "%s2x2_%s_nf0_%dx%dx%dx%d_a%2.2f_tg%3.3f_o%d.txt", plaq_stem.c_str(),
+                type_g(), z[0], z[1], z[2], z[3], a, tg, PTORD);
   std::snprintf(nspt_pars.plaqn, 256, "%s_%s_nf0_%dx%dx%dx%d_a%2.2f_tg%3.3f_o%d.txt", plaq_stem.c_str(), type_g(),
                 z[0], z[1], z[2], z[3], a, tg, PTORD);
   std::snprintf(nspt_pars.confn, 256, "%s%s_nf0_%dx%dx%dx%d_a%2.2f_tg%3.3f_o%d", conf_stem.c_str(), type_g(), z[0],
@@ -188,6 +194,11 @@ int initialize(int argc, char** argv) {
   plaqfile.open(nspt_pars.plaqn, std::ios_base::app);
   plaqfile.precision(30);
   plaqfile << std::scientific;
+  if (mi355x_opts.wl2x2) {  // __WILLOOP_2x2__ (Quenched.cc:434-437)
+    loop2x2.open(nspt_pars.name2x2, std::ios_base::app);
+    loop2x2.precision(30);
+    loop2x2 << std::scientific;
+  }
   loop_nxm.open(nspt_pars.name_nxm, std::ios_base::app);  // __MANY_LOOP__ (choices.h:76)
   loop_nxm.precision(30);
   loop_nxm << std::scientific;
@@ -215,6 +226,7 @@ int initialize(int argc, char** argv) {
           << "Ordine\t\t" << PTORD << std::endl
           << std::endl
           << "Placchetta\t" << nspt_pars.plaqn << std::endl
+          << (mi355x_opts.wl2x2 ? std::string("Altri loops\t") + nspt_pars.name2x2 + "\n" : std::string())
           << "Altri loops\t" << nspt_pars.name_nxm << std::endl
           << "Configurazione\t" << nspt_pars.confn << std::endl
           << "Damocle\t\t" << nspt_pars.damon << std::endl
@@ -271,6 +283,7 @@ int allocate(ptGluon_fld& Umu) {
 // After every BEAT steps: the Wilson loops, then damocle.dag re-read
 // (Quenched.cc:656-725)
 int update_parameters(ptGluon_fld& Umu) {
+  if (mi355x_opts.wl2x2) WL2x2(Umu);
   ComputeLoops(Umu);
   nInt += nspt_pars.Beat;
   logfile << "______________________________________" << std::endl << std::endl;
@@ -309,6 +322,10 @@ int finalize(ptGluon_fld& Umu, int t) {
   std::printf("Plaquette:\n");
   for (int i = 0; i <= PTORD; i++) std::printf("%e\t%e\n", w[i].re, w[i].im);
   Umu.save(nspt_pars.confn, w);
+  if (mi355x_opts.wl2x2) {
+    WL2x2(Umu);
+    loop2x2.close();
+  }
   return 0;
 }
 

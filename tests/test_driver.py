@@ -264,3 +264,32 @@ def test_driver_twisted_reference_default(tmp_path, nc):
     assert np.abs(got - want).max() < 1e-12 * np.abs(want).max()
     f = np.fromfile(tmp_path / n["conf"], dtype=np.complex128)[:4 * V * (1 + nc * nc * N)].reshape(4 * V, -1)
     assert np.abs(f - U).max() < 1e-12 * np.abs(U).max()
+
+
+@pytest.mark.gpu
+def test_driver_wl2x2(tmp_path):
+    """-W 1 (__WILLOOP_2x2__): the 2x2 loop file <plaq>2x2_*.txt, one line
+    per beat and one at the end, Re Tr of the 2x2 loop sum at orders 2, 4, 6
+    over 18 V (nspt.cc:1064-1145), against the oracle's loops."""
+    from oracle import oracle as O
+    L, N, eps, alpha, steps, beat, nc = 4, 6, 0.05, 0.05, 20, 10, 3
+    write_inputs(tmp_path, L=L, sweep=steps, beat=beat, tau=eps, alpha=alpha, ptord=N, seed=1)
+    out = subprocess.run([driver(nc, 6), "-W", "1"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    stem = "SU%d_wil_nf0_%dx%dx%dx%d_a%2.2f_tg%3.3f" % (nc, L, L, L, L, alpha, eps)
+    log = (tmp_path / ("%s_o%d.log" % (stem, N))).read_text()
+    assert "Altri loops\t./plaq2x2_%s_o%d.txt" % (stem, N) in log
+    Lt = (L, L, L, L)
+    V = L ** 4
+    U = O.cold_field(Lt, nc, N)
+    rng = O.Ranlxd(O.glibc_first_rand(1))
+    want = []
+    for t in range(steps):
+        eta = rng.noise(nc, 4 * V)
+        O.step(Lt, nc, N, U, eps, alpha, integrator=0, update=0, gauge_fix=1, zero_modes=1, eta=eta)
+        if (t + 1) % beat == 0:
+            want.append(O.compute_loops(Lt, nc, N, U)[1, 2::2].real / (18.0 * V))
+    want.append(want[-1])  # the final call, on the same field
+    got = _read_cols(tmp_path / ("plaq2x2_%s_o%d.txt" % (stem, N)), 3)
+    assert got.shape == (3, 3)
+    assert np.abs(got - np.array(want)).max() < 1e-12 * np.abs(want).max()
