@@ -650,6 +650,9 @@ struct nspt_ctx {
   // ({step offset, row}), advanced by k_count at the end of each step.
   bool graph = false;
   unsigned long long* ctr_dev = nullptr;
+  // a replica of a batch: its step is replayed from captured graphs, so the
+  // action and the boundary conditions are fixed at creation
+  bool in_batch = false;
   std::string err;
   // per-kernel HIP-event timing (nspt_profile)
   bool prof = false;
@@ -1196,6 +1199,7 @@ int nspt_loops(nspt_ctx* c, double* out) {
 
 int nspt_set_action(nspt_ctx* c, double c1) {
   if (!c) return -1;
+  if (c->in_batch) return c->fail("batch replicas run the Wilson action") - 1;
   if (c1 != 0.0 && !c->R.data) {
     const int ncols = 2 * (c->nrt + 1);
     // R, and the 2-link field the rectangles are formed from
@@ -1214,6 +1218,7 @@ int nspt_set_action(nspt_ctx* c, double c1) {
 
 int nspt_set_twist(nspt_ctx* c, int mask, const double* omega) {
   if (!c) return -1;
+  if (c->in_batch) return c->fail("batch replicas run on periodic lattices") - 1;
   if (mask < 0 || mask > 15) return c->fail("twist mask must be in 0..15") - 1;
   if (mask && !omega) return c->fail("null twist matrices") - 1;
   const int nc = c->nc;
@@ -1458,6 +1463,7 @@ int nspt_batch_create(const nspt_geom* geoms, const nspt_step_params* params, in
       if (c) nspt_destroy(c);
       return -2;
     }
+    c->in_batch = true;
     b->reps.push_back(c);
     b->params.push_back(params[r]);
   }

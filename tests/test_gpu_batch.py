@@ -76,3 +76,17 @@ def test_tuner_restores_the_run():
     assert np.abs(obs[0] - o).max() < 1e-12
     b.close()
     ref.close()
+
+
+def test_batch_replicas_refuse_action_and_twist():
+    """A replica's step is replayed from graphs captured at its first
+    evolve: the action and the boundary conditions cannot change under it."""
+    from prlgt2020_amd import nspt
+    Ls, ps = _replicas(nspt)
+    b = nspt.Batch(Ls[:1], 3, 4, ps[:1])
+    rep = b.replicas[0]
+    with pytest.raises(nspt.NsptError, match="Wilson action"):
+        rep.set_action(-1.0 / 12)
+    with pytest.raises(nspt.NsptError, match="periodic"):
+        rep.set_twist(0b0110)
+    b.close()
