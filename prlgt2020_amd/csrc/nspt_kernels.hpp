@@ -960,9 +960,21 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_sgf(Geo g, SerField U
 // per plane the six 5-link staples of the 2x1 and 1x2 rectangles through
 // the link.  Every rectangle has a long side of two collinear links outside
 // the link itself, so with the 2-link field L2_d(x) = U_d(x) U_d(x+d) each
-// staple is a path of 3 or 4 factors (48 products per link instead of 72),
-// formed one factor at a time in place in slot A.  With obs2: per block,
-// Tr(U_mu(x) R_mu(x)) per order (the rectangle plaquette of
+// staple is a path of 3 or 4 factors; and two shapes of every plane start
+// with U_mu(x+mu), two end with U_mu(x-mu), so those are summed over the
+// planes first:
+//   R = U_mu(x+mu) sum_nu (B1 + B4) + sum_nu (A2 + A5) U_mu(x-mu)
+//       + sum_nu (S0 + S3),
+//   B1 = U_nu(x+2mu) L2_mu(x+nu)^+ U_nu(x)^+,
+//   B4 = U_nu(x+2mu-nu)^+ L2_mu(x-nu)^+ U_nu(x-nu),
+//   A2 = U_nu(x+mu) L2_mu(x-mu+nu)^+ U_nu(x-mu)^+,
+//   A5 = U_nu(x+mu-nu)^+ L2_mu(x-mu-nu)^+ U_nu(x-mu-nu),
+//   S0 = L2_nu(x+mu) U_mu(x+2nu)^+ L2_nu(x)^+,
+//   S3 = L2_nu(x+mu-2nu)^+ U_mu(x-2nu)^+ L2_nu(x-2nu):
+// 38 products per link (72 as the reference forms them).  Paths are formed
+// one factor at a time in place in slot A and summed into Acc (registers);
+// the three partial sums of R go through the R field itself.  With obs2: per
+// block, Tr(U_mu(x) R_mu(x)) per order (the rectangle plaquette of
 // nspt.cc:406-410), as the drift's plaquette rows.
 template <int NC, int N>
 __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_rect(Geo g, SerField U, SerField L2, SerField R,
@@ -981,75 +993,101 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_rect(Geo g, SerField 
   const bool ok = ln.lds_ok && (site0 + ln.j) < lb.count;
   const long V = g.V;
   const long idx = mu * V + site;
-  // factor f of a plane's six paths: offset (dm along mu, dn along nu),
-  // direction (0 mu, 1 nu), dagger, field (0 U, 1 L2), position in its path
-  // (0 first, 1 middle, 2 last).  The six expressions of
-  // QCDenvNODEpt_test.h:591-607 with U_d(y) U_d(y+d) written L2_d(y):
-  //   L2_nu(x+mu) U_mu(x+2nu)^+ L2_nu(x)^+
-  //   U_mu(x+mu) U_nu(x+2mu) L2_mu(x+nu)^+ U_nu(x)^+
-  //   U_nu(x+mu) L2_mu(x-mu+nu)^+ U_nu(x-mu)^+ U_mu(x-mu)
-  //   L2_nu(x+mu-2nu)^+ U_mu(x-2nu)^+ L2_nu(x-2nu)
-  //   U_mu(x+mu) U_nu(x+2mu-nu)^+ L2_mu(x-nu)^+ U_nu(x-nu)
-  //   U_nu(x+mu-nu)^+ L2_mu(x-mu-nu)^+ U_nu(x-mu-nu) U_mu(x-mu)
-  constexpr int NF = 22;
-  constexpr signed char SH[NF][6] = {
-      {1, 0, 1, 0, 1, 0},   {0, 2, 0, 1, 0, 1},  {0, 0, 1, 1, 1, 2},                        //
-      {1, 0, 0, 0, 0, 0},   {2, 0, 1, 0, 0, 1},  {0, 1, 0, 1, 1, 1},  {0, 0, 1, 1, 0, 2},  //
-      {1, 0, 1, 0, 0, 0},   {-1, 1, 0, 1, 1, 1}, {-1, 0, 1, 1, 0, 1}, {-1, 0, 0, 0, 0, 2},  //
-      {1, -2, 1, 1, 1, 0},  {0, -2, 0, 1, 0, 1}, {0, -2, 1, 0, 1, 2},                       //
-      {1, 0, 0, 0, 0, 0},   {2, -1, 1, 1, 0, 1}, {0, -1, 0, 1, 1, 1}, {0, -1, 1, 0, 0, 2},  //
-      {1, -1, 1, 1, 0, 0},  {-1, -1, 0, 1, 1, 1}, {-1, -1, 1, 0, 0, 1}, {-1, 0, 0, 0, 0, 2}};
-  constexpr int NSTEP = 3 * NF;
+  // A step stages one factor: offset (dm along mu, dn along nu), direction
+  // (0 mu, 1 nu), dagger, source (0 U, 1 L2, 2 the sum Acc), position in its
+  // path (0 first, 1 middle, 2 last), and what follows a last factor (0 keep
+  // summing into Acc, 1 R = Acc, 2 R += Acc).  Plane steps come 6 per plane
+  // (nu = the p-th direction other than mu), three planes per phase.
+  constexpr int NPL = 18;  // factors of one phase: 3 planes x 2 paths x 3
+  constexpr signed char PH[3][6][5] = {
+      {{2, 0, 1, 0, 0}, {0, 1, 0, 1, 1}, {0, 0, 1, 1, 0}, {2, -1, 1, 1, 0}, {0, -1, 0, 1, 1}, {0, -1, 1, 0, 0}},
+      {{1, 0, 1, 0, 0}, {-1, 1, 0, 1, 1}, {-1, 0, 1, 1, 0}, {1, -1, 1, 1, 0}, {-1, -1, 0, 1, 1}, {-1, -1, 1, 0, 0}},
+      {{1, 0, 1, 0, 1}, {0, 2, 0, 1, 0}, {0, 0, 1, 1, 1}, {1, -2, 1, 1, 1}, {0, -2, 0, 1, 0}, {0, -2, 1, 0, 1}}};
+  constexpr int NSTEP = 3 * NPL + 4;
   struct Fac {
     long li;
-    int cl, pos;
-    bool dg, lng;
+    int cl, pos, src, post;
+    bool dg;
   };
+  // steps: phase B (0..17), U_mu(x+mu) and Acc (18, 19), phase A (20..37),
+  // Acc and U_mu(x-mu) (38, 39), phase S (40..57)
   auto fac_of = [&](int step) {
-    const int p = step / NF;
-    const signed char* e = SH[step % NF];
-    const int nu = (p < mu) ? p : p + 1;
     Fac f;
     f.cl = TW0;
-    const int x = site_move_tw(g, site_move_tw(g, site, mu, e[0], f.cl), nu, e[1], f.cl);
-    f.li = (long)(e[2] ? nu : mu) * V + x;
-    f.dg = e[3] != 0;
-    f.lng = e[4] != 0;
-    f.pos = e[5];
+    f.dg = false;
+    f.post = 0;
+    f.src = 0;
+    int dm = 0, dn = 0, dir = 0, nu = mu;
+    if (step == 18 || step == 39) {
+      dm = step == 18 ? 1 : -1;
+      f.pos = step == 18 ? 0 : 2;
+      f.post = step == 39 ? 2 : 0;
+    } else if (step == 19 || step == 38) {
+      f.src = 2;
+      f.pos = step == 19 ? 2 : 0;
+      f.post = step == 19 ? 1 : 0;
+    } else {
+      const int ph = step < 18 ? 0 : (step < 38 ? 1 : 2);
+      const int r = step - (ph == 0 ? 0 : (ph == 1 ? 20 : 40));
+      const int p = r / 6, k = r % 6;
+      nu = (p < mu) ? p : p + 1;
+      const signed char* e = PH[ph][k];
+      dm = e[0];
+      dn = e[1];
+      dir = e[2];
+      f.dg = e[3] != 0;
+      f.src = e[4];
+      f.pos = (k % 3 == 0) ? 0 : (k % 3 == 1 ? 1 : 2);
+      f.post = (ph == 2 && r == NPL - 1) ? 2 : 0;
+    }
+    const int x = site_move_tw(g, site_move_tw(g, site, mu, dm, f.cl), nu, dn, f.cl);
+    f.li = (long)(dir ? nu : mu) * V + x;
     return f;
   };
   // the field of a factor (pointers picked one by one: a runtime pick of a
   // whole struct goes through scratch)
-  auto field = [&](bool lng) {
+  auto field = [&](int src) {
     SerField f;
-    f.data = lng ? L2.data : U.data;
-    f.flag = lng ? L2.flag : U.flag;
+    f.data = src == 1 ? L2.data : U.data;
+    f.flag = src == 1 ? L2.flag : U.flag;
     f.P = U.P;
     return f;
   };
-  double2 Sig[N];
+  double2 Acc[N];
 #pragma unroll
-  for (int k = 0; k < N; k++) Sig[k] = zero2();
-  double2 sf = zero2(), tf = zero2();
+  for (int k = 0; k < N; k++) Acc[k] = zero2();
+  double2 af = zero2(), rf = zero2(), tf = zero2();
   // the next factor is loaded into registers while the current product runs
   constexpr bool PF = N <= 24;
   constexpr int NP = PF ? N : 1;
   double2 PB[NP];
   Fac cur = fac_of(0);
-  if constexpr (PF) gload_tw<NC, N>(g, ln, field(cur.lng), cur.li, cur.cl, nrt, ok, PB);
+  if constexpr (PF) gload_tw<NC, N>(g, ln, field(cur.src), cur.li, cur.cl, nrt, ok, PB);
   for (int step = 0; step < NSTEP; step++) {
-    const SerField fc = field(cur.lng);
-    double2 fl = gflag(fc, cur.li, ok);
-    if (cur.dg) fl = c_conj(fl);
+    asm volatile("" ::: "memory");
     double2* slot = (cur.pos == 0) ? ln.slotA : ln.slotB;
-    if constexpr (PF)
-      ln.stage(slot, PB, cur.dg);
-    else
-      stage_global_tw(g, ln, slot, fc, cur.li, cur.cl, nrt, cur.dg);
-    const int pos = cur.pos;
+    double2 fl;
+    if (cur.src == 2) {
+      // the sum Acc as a factor; Acc restarts from zero
+      ln.stage(slot, Acc);
+      fl = af;
+#pragma unroll
+      for (int k = 0; k < N; k++) Acc[k] = zero2();
+      af = zero2();
+    } else {
+      const SerField fc = field(cur.src);
+      fl = gflag(fc, cur.li, ok);
+      if (cur.dg) fl = c_conj(fl);
+      if constexpr (PF)
+        ln.stage(slot, PB, cur.dg);
+      else
+        stage_global_tw(g, ln, slot, fc, cur.li, cur.cl, nrt, cur.dg);
+    }
+    const int pos = cur.pos, post = cur.post;
     if (step + 1 < NSTEP) {
       cur = fac_of(step + 1);
-      if constexpr (PF) gload_tw<NC, N>(g, ln, field(cur.lng), cur.li, cur.cl, nrt, ok, PB);
+      if constexpr (PF)
+        if (cur.src != 2) gload_tw<NC, N>(g, ln, field(cur.src), cur.li, cur.cl, nrt, ok, PB);
     }
     if (pos == 0) {
       tf = fl;
@@ -1059,20 +1097,27 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_rect(Geo g, SerField 
     if (pos == 1) {
       tf = mul_lds<NC, N>(ln, tf, fl, true, M, ln.slotA, false);
     } else {
-      sf = c_add(sf, mul<NC, N>(ln, tf, fl, true, Sig, M, false, nullptr, false, 1.0));
+      af = c_add(af, mul<NC, N>(ln, tf, fl, true, Acc, M, false, nullptr, false, 1.0));
     }
     __syncthreads();
+    if (post) {
+      // R = Acc or R += Acc; Acc restarts from zero
+      gaccum<NC, N>(R, idx, ln.e, nrt, ok, Acc, post == 1);
+      rf = post == 1 ? af : c_add(rf, af);
+#pragma unroll
+      for (int k = 0; k < N; k++) Acc[k] = zero2();
+      af = zero2();
+    }
   }
-  gstore<NC, N>(R, idx, ln.e, nrt, ok, Sig);
-  if (ok && ln.e == 0) R.flag[idx] = sf;
+  if (ok && ln.e == 0) R.flag[idx] = rf;
   if (!obs2) return;
   // Tr(U_mu(x) R_mu(x)) per order, summed over the block's links
   __shared__ double2 flagbuf[LN::LPB];
   stage_global(ln, ln.slotA, U, idx, nrt);
-  ln.stage(ln.slotB, Sig);
+  stage_global(ln, ln.slotB, R, idx, nrt);
   const double2 uf = gflag(U, idx, ok);
   __syncthreads();
-  const double2 wf = mul_lds<NC, N>(ln, uf, sf, true, M, ln.slotB, false);
+  const double2 wf = mul_lds<NC, N>(ln, uf, rf, true, M, ln.slotB, false);
   if (ln.lds_ok && ln.e == 0) flagbuf[ln.j] = wf;
   __syncthreads();
   const int lane = threadIdx.x;
