@@ -3,7 +3,8 @@
     python -m prlgt2020_amd.build [--force] [-j N]
 
 Each (NC, N) kernel instance is one translation unit (nspt_inst.hip compiled
-with -DINST_NC/-DINST_N) so they compile in parallel; the host side
+with -DINST_NC/-DINST_N, once more with -DINST_BIG=1 for fields past 2 GiB)
+so they compile in parallel; the host side
 (nspt_capi.hip, nspt_rng.cpp) is linked with them into
 prlgt2020_amd/libnspt.so.  The instance list mirrors NSPT_INSTANCES of
 csrc/nspt_internal.hpp.
@@ -64,10 +65,11 @@ def build(force=False, jobs=None, verbose=False):
     tasks = []
     for nc, n, q in instances():
         src = os.path.join(CSRC, "nspt_inst.hip")
-        obj = os.path.join(OBJ, "inst_%d_%d_q%d.o" % (nc, n, q))
-        if force or _newer(obj, [src] + hdrs):
-            tasks.append([HIPCC] + HIPFLAGS + ["-DINST_NC=%d" % nc, "-DINST_N=%d" % n, "-DINST_QT=%d" % q, "-c", src,
-                          "-o", obj])
+        for big in (0, 1):
+            obj = os.path.join(OBJ, "inst_%d_%d_q%d%s.o" % (nc, n, q, "_big" if big else ""))
+            if force or _newer(obj, [src] + hdrs):
+                tasks.append([HIPCC] + HIPFLAGS + ["-DINST_NC=%d" % nc, "-DINST_N=%d" % n, "-DINST_QT=%d" % q,
+                                                   "-DINST_BIG=%d" % big, "-c", src, "-o", obj])
     capi_obj = os.path.join(OBJ, "nspt_capi.o")
     if force or _newer(capi_obj, [os.path.join(CSRC, "nspt_capi.hip")] + hdrs):
         tasks.append([HIPCC] + HIPFLAGS + ["-c", os.path.join(CSRC, "nspt_capi.hip"), "-o", capi_obj])
@@ -81,7 +83,8 @@ def build(force=False, jobs=None, verbose=False):
         for r in ex.map(_run, tasks):
             if verbose and r.stderr:
                 print(r.stderr, file=sys.stderr)
-    objs = [os.path.join(OBJ, "inst_%d_%d_q%d.o" % inst) for inst in instances()] + [capi_obj, adj_obj, rng_obj]
+    objs = [os.path.join(OBJ, "inst_%d_%d_q%d%s.o" % (nc, n, q, sfx)) for nc, n, q in instances()
+            for sfx in ("", "_big")] + [capi_obj, adj_obj, rng_obj]
     relinked = False
     if force or tasks or _newer(LIB, objs):
         _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", LIB + ".tmp"] + objs)

@@ -571,26 +571,30 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fa_fft(Geo g, int pass, const d
 
 // Smallest compiled N >= n; among those the tile edge qt if compiled, else
 // the default edge 4.
-KernelSet find_set(int nc, int n, int qt = 4) {
+KernelSet find_set(int nc, int n, int qt = 4, bool big = false) {
   KernelSet best;
   auto consider = [&](KernelSet s) {
-    if (s.nc != nc || s.nmax < n) return;
+    if (s.nc != nc || s.nmax < n || s.big != big) return;
     if (best.nmax == 0 || s.nmax < best.nmax) {
       best = s;
     } else if (s.nmax == best.nmax && s.qt == qt && best.qt != qt) {
       best = s;
     }
   };
-#define NSPT_CONSIDER(a, b, q) consider(nspt::kernel_set_##a##_##b##_q##q());
+#define NSPT_CONSIDER(a, b, q)                  \
+  consider(nspt::kernel_set_##a##_##b##_q##q()); \
+  consider(nspt::kernel_set_##a##_##b##_q##q##_big());
   NSPT_INSTANCES(NSPT_CONSIDER)
 #undef NSPT_CONSIDER
   if (best.nmax && best.qt != qt) {
     KernelSet dflt = best;
     best = KernelSet();
     auto prefer4 = [&](KernelSet s) {
-      if (s.nc == nc && s.nmax == dflt.nmax && s.qt == 4) best = s;
+      if (s.nc == nc && s.nmax == dflt.nmax && s.qt == 4 && s.big == big) best = s;
     };
-#define NSPT_CONSIDER4(a, b, q) prefer4(nspt::kernel_set_##a##_##b##_q##q());
+#define NSPT_CONSIDER4(a, b, q)                 \
+  prefer4(nspt::kernel_set_##a##_##b##_q##q()); \
+  prefer4(nspt::kernel_set_##a##_##b##_q##q##_big());
     NSPT_INSTANCES(NSPT_CONSIDER4)
 #undef NSPT_CONSIDER4
     if (!best.nmax) best = dflt;
@@ -611,6 +615,7 @@ int grid_for(long n, int block) {
 struct nspt_ctx {
   nspt_geom geom{};
   int nc = 3, NE = 9, nrt = 0, A = 0, V = 0;
+  bool big = false;  // kernels built for fields past 2 GiB
   KernelSet ks;
   Geo g{};
   int nblk_links = 0, nblk_sites = 0, nob = 0, obs_len = 0;
@@ -997,6 +1002,10 @@ int nspt_create(const nspt_geom* geom, nspt_ctx** out) {
   c->V = (int)V;
   c->ks = find_set(c->nc, c->nrt);
   if (!c->ks.drift) return bad("nspt_create: no compiled kernel instance for this order (SU(3) up to 32, SU(2) up to 24)");
+  // fields whose order planes reach past 2 GiB (the plaquette field, 6V
+  // series, is the widest) take the instance built for them
+  c->big = (long)(c->ks.nmax - 1) * c->NE * 6L * V * 16 >= 0x80000000L;
+  if (c->big) c->ks = find_set(c->nc, c->nrt, 4, true);
   for (int d = 0; d < 4; d++) c->g.L[d] = geom->sz[d];
   c->g.st[3] = 1;
   c->g.st[2] = c->g.L[3];
@@ -1542,7 +1551,7 @@ int nspt_batch_tune(nspt_batch* b, int steps, int* qt_chosen, double* ms_per_ste
   // candidate tile edges compiled for the replicas' instance
   std::vector<int> cand;
   for (int q : {4, 3, 5, 2}) {
-    const KernelSet ks = find_set(c0->nc, c0->nrt, q);
+    const KernelSet ks = find_set(c0->nc, c0->nrt, q, c0->big);
     if (ks.qt == q && ks.nmax == c0->ks.nmax) cand.push_back(q);
   }
   // snapshot fields and counters: tuning steps are not part of the run
@@ -1565,7 +1574,7 @@ int nspt_batch_tune(nspt_batch* b, int steps, int* qt_chosen, double* ms_per_ste
   (void)hipEventCreate(&e0);
   (void)hipEventCreate(&e1);
   for (size_t i = 0; i < cand.size(); i++) {
-    for (auto* c : b->reps) c->ks = find_set(c->nc, c->nrt, cand[i]);
+    for (auto* c : b->reps) c->ks = find_set(c->nc, c->nrt, cand[i], c->big);
     if (!batch_build(b, false) || !batch_replay(b, 1)) return -3;
     (void)hipEventRecord(e0, b->master);
     if (!batch_replay(b, steps)) return -3;
@@ -1582,7 +1591,7 @@ int nspt_batch_tune(nspt_batch* b, int steps, int* qt_chosen, double* ms_per_ste
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  for (auto* c : b->reps) c->ks = find_set(c->nc, c->nrt, best_q);
+  for (auto* c : b->reps) c->ks = find_set(c->nc, c->nrt, best_q, c->big);
   batch_drop_graphs(b);
   for (size_t r = 0; r < b->reps.size(); r++) {
     nspt_ctx* c = b->reps[r];

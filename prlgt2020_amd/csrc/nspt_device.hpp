@@ -33,6 +33,22 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+// The order planes of a large field (32^4 at N = 32: 0.6 GB apart) lie
+// beyond the 32-bit byte offset of one buffer descriptor: every kernel
+// instance is also compiled with NSPT_BIG_FIELDS = 1, which takes a
+// descriptor per plane and which the host picks for such fields (the
+// per-plane descriptors cost scalar registers the common case should not
+// pay).  The two builds live in different inline namespaces, so that their
+// kernels and helpers are distinct symbols.
+#ifndef NSPT_BIG_FIELDS
+#define NSPT_BIG_FIELDS 0
+#endif
+#if NSPT_BIG_FIELDS
+#define NSPT_VARIANT big_fields
+#else
+#define NSPT_VARIANT std_fields
+#endif
+
 namespace nspt {
 
 struct SerField {
@@ -40,6 +56,8 @@ struct SerField {
   double2* flag;  // may be null for fields whose flag is implied
   long P;         // number of series (plane stride)
 };
+
+inline namespace NSPT_VARIANT {
 
 __device__ __forceinline__ double2 c_add(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 c_sub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
@@ -384,6 +402,10 @@ __device__ __forceinline__ void unstage(const Lane<NC, N>& ln, const double2* sl
   for (int k = 0; k < N; k++) X[k] = slot[k * NE + ln.e];
 }
 
+// Order plane k of a field starts k * plane elements after order 0.  While
+// the last plane starts within 2 GiB of the field, one buffer descriptor
+// and a scalar byte offset per order reach every plane; NSPT_BIG_FIELDS
+// builds take a descriptor per plane (see the top of this file).
 // Global loads/stores of a lane-owned series.  Orders >= nrt read as zero and
 // are not stored.  Loads are unmasked (lanes of invalid links hold clamped,
 // valid indices); stores are masked by ok.
@@ -393,9 +415,14 @@ __device__ __forceinline__ void gload(const SerField& f, long idx, int e, int nr
   const unsigned off = (unsigned)(e * f.P + idx);
   const long plane = (long)NE * f.P;
   (void)ok;
+#if NSPT_BIG_FIELDS
+#pragma unroll
+  for (int k = 0; k < N; k++) X[k] = (k < nrt) ? ld_r(rsrc(f.data + k * plane), off, 0u) : zero2();
+#else
   const __amdgpu_buffer_rsrc_t r = rsrc(f.data);
 #pragma unroll
   for (int k = 0; k < N; k++) X[k] = (k < nrt) ? ld_r(r, off, (unsigned)(k * plane * 16)) : zero2();
+#endif
 }
 template <int NC, int N>
 __device__ __forceinline__ void gstore(const SerField& f, long idx, int e, int nrt, bool ok, const double2 (&X)[N]) {
@@ -426,4 +453,5 @@ __device__ __forceinline__ double2 gflag(const SerField& f, long idx, bool ok) {
   return (ok && f.flag) ? f.flag[idx] : zero2();
 }
 
+}  // inline namespace NSPT_VARIANT
 }  // namespace nspt

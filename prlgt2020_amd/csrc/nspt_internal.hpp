@@ -37,6 +37,7 @@ struct TwistTab {
 // Launchers of one (NC, N) instance.  nblk is the grid size in waves.
 struct KernelSet {
   int nc = 0, nmax = 0, lpb = 0, qt = 0;  // qt: order-tile edge of the products
+  bool big = false;                       // built for fields past 2 GiB (NSPT_BIG_FIELDS)
   void (*drift)(int nblk, hipStream_t, Geo, SerField U, SerField D, SerField D1, double tau_g, double rk2c, int mode,
                 double* obs, SerField R, double c1) = nullptr;
   // the plaquette field of U (6 per site, plane-major), and the drift of
@@ -71,7 +72,11 @@ struct KernelSet {
   X(2, 4, 4) X(2, 6, 4) X(2, 8, 4) X(2, 12, 4) X(2, 16, 4) X(2, 20, 4) X(2, 24, 4)
 #endif
 
-#define NSPT_DECL_SET(nc, n, q) KernelSet kernel_set_##nc##_##n##_q##q();
+// Each instance comes in two builds: for fields whose order planes lie
+// within 2 GiB of the field's base, and (_big) for larger ones.
+#define NSPT_DECL_SET(nc, n, q) \
+  KernelSet kernel_set_##nc##_##n##_q##q();  \
+  KernelSet kernel_set_##nc##_##n##_q##q##_big();
 NSPT_INSTANCES(NSPT_DECL_SET)
 #undef NSPT_DECL_SET
 

@@ -17,6 +17,7 @@
 #endif
 
 namespace nspt {
+inline namespace NSPT_VARIANT {
 
 __device__ __forceinline__ int site_shift(const Geo& g, int s, int d, int delta) {
   const int x = (s / g.st[d]) % g.L[d];
@@ -169,14 +170,22 @@ __device__ __forceinline__ void stage_global(const Lane<NC, N>& ln, double2* slo
   if (!ln.lds_ok) return;
   const unsigned off = (unsigned)(ln.e * f.P + idx);
   const long plane = (long)NE * f.P;
-  const __amdgpu_buffer_rsrc_t r = rsrc(f.data);
   const int pos = dag ? (ln.c * NC + ln.r) : ln.e;
   const double sy = dag ? -1.0 : 1.0;
+#if NSPT_BIG_FIELDS
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    const double2 v = (k < nrt) ? ld_r(rsrc(f.data + k * plane), off, 0u) : zero2();
+    slot[k * NE + pos] = make_double2(v.x, sy * v.y);
+  }
+#else
+  const __amdgpu_buffer_rsrc_t r = rsrc(f.data);
 #pragma unroll
   for (int k = 0; k < N; k++) {
     const double2 v = (k < nrt) ? ld_r(r, off, (unsigned)(k * plane * 16)) : zero2();
     slot[k * NE + pos] = make_double2(v.x, sy * v.y);
   }
+#endif
 }
 
 // Loads and stores of a link reached with crossing code `code`: on a
@@ -993,55 +1002,61 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_rect(Geo g, SerField 
   const bool ok = ln.lds_ok && (site0 + ln.j) < lb.count;
   const long V = g.V;
   const long idx = mu * V + site;
-  // A step stages one factor: offset (dm along mu, dn along nu), direction
-  // (0 mu, 1 nu), dagger, source (0 U, 1 L2, 2 the sum Acc), position in its
-  // path (0 first, 1 middle, 2 last), and what follows a last factor (0 keep
-  // summing into Acc, 1 R = Acc, 2 R += Acc).  Plane steps come 6 per plane
-  // (nu = the p-th direction other than mu), three planes per phase.
-  constexpr int NPL = 18;  // factors of one phase: 3 planes x 2 paths x 3
-  constexpr signed char PH[3][6][5] = {
+  // A step stages one factor: offset (dm along mu, dn along nu, plane
+  // p: nu = the p-th direction other than mu), direction (0 mu, 1 nu),
+  // dagger, source (0 U, 1 L2, 2 the sum Acc), position in its path (0
+  // first, 1 middle, 2 last), and what follows a last factor (0 keep
+  // summing into Acc, 1 R = Acc, 2 R += Acc).  Phase B (steps 0-17), then
+  // U_mu(x+mu) and Acc (18, 19), phase A (20-37), Acc and U_mu(x-mu) (38,
+  // 39), phase S (40-57); a phase takes 6 factors per plane.
+  constexpr int NSTEP = 58;
+  struct Step {
+    signed char dm, dn, dir, dag, src, pos, post, p;
+  };
+  static constexpr signed char PH[3][6][5] = {
       {{2, 0, 1, 0, 0}, {0, 1, 0, 1, 1}, {0, 0, 1, 1, 0}, {2, -1, 1, 1, 0}, {0, -1, 0, 1, 1}, {0, -1, 1, 0, 0}},
       {{1, 0, 1, 0, 0}, {-1, 1, 0, 1, 1}, {-1, 0, 1, 1, 0}, {1, -1, 1, 1, 0}, {-1, -1, 0, 1, 1}, {-1, -1, 1, 0, 0}},
       {{1, 0, 1, 0, 1}, {0, 2, 0, 1, 0}, {0, 0, 1, 1, 1}, {1, -2, 1, 1, 1}, {0, -2, 0, 1, 0}, {0, -2, 1, 0, 1}}};
-  constexpr int NSTEP = 3 * NPL + 4;
+  struct Table {
+    Step s[NSTEP];
+  };
+  // the step list, built at compile time and packed into one word per step
+  static constexpr auto table = [] {
+    Table t{};
+    int n = 0;
+    for (int ph = 0; ph < 3; ph++) {
+      for (int r = 0; r < 18; r++) {
+        const signed char* e = PH[ph][r % 6];
+        const int k = r % 6;
+        t.s[n++] = {e[0], e[1], e[2], e[3], e[4], (signed char)(k % 3), (signed char)(ph == 2 && r == 17 ? 2 : 0),
+                    (signed char)(r / 6)};
+      }
+      if (ph == 0) {
+        t.s[n++] = {1, 0, 0, 0, 0, 0, 0, 0};  // U_mu(x+mu)
+        t.s[n++] = {0, 0, 0, 0, 2, 2, 1, 0};  // times sum_nu (B1 + B4) -> R
+      } else if (ph == 1) {
+        t.s[n++] = {0, 0, 0, 0, 2, 0, 0, 0};   // sum_nu (A2 + A5)
+        t.s[n++] = {-1, 0, 0, 0, 0, 2, 2, 0};  // times U_mu(x-mu) -> R +=
+      }
+    }
+    return t;
+  }();
   struct Fac {
     long li;
     int cl, pos, src, post;
     bool dg;
   };
-  // steps: phase B (0..17), U_mu(x+mu) and Acc (18, 19), phase A (20..37),
-  // Acc and U_mu(x-mu) (38, 39), phase S (40..57)
   auto fac_of = [&](int step) {
+    const Step st = table.s[uniform(step)];
     Fac f;
     f.cl = TW0;
-    f.dg = false;
-    f.post = 0;
-    f.src = 0;
-    int dm = 0, dn = 0, dir = 0, nu = mu;
-    if (step == 18 || step == 39) {
-      dm = step == 18 ? 1 : -1;
-      f.pos = step == 18 ? 0 : 2;
-      f.post = step == 39 ? 2 : 0;
-    } else if (step == 19 || step == 38) {
-      f.src = 2;
-      f.pos = step == 19 ? 2 : 0;
-      f.post = step == 19 ? 1 : 0;
-    } else {
-      const int ph = step < 18 ? 0 : (step < 38 ? 1 : 2);
-      const int r = step - (ph == 0 ? 0 : (ph == 1 ? 20 : 40));
-      const int p = r / 6, k = r % 6;
-      nu = (p < mu) ? p : p + 1;
-      const signed char* e = PH[ph][k];
-      dm = e[0];
-      dn = e[1];
-      dir = e[2];
-      f.dg = e[3] != 0;
-      f.src = e[4];
-      f.pos = (k % 3 == 0) ? 0 : (k % 3 == 1 ? 1 : 2);
-      f.post = (ph == 2 && r == NPL - 1) ? 2 : 0;
-    }
-    const int x = site_move_tw(g, site_move_tw(g, site, mu, dm, f.cl), nu, dn, f.cl);
-    f.li = (long)(dir ? nu : mu) * V + x;
+    const int nu = (st.p < mu) ? st.p : st.p + 1;
+    const int x = site_move_tw(g, site_move_tw(g, site, mu, st.dm, f.cl), nu, st.dn, f.cl);
+    f.li = (long)(st.dir ? nu : mu) * V + x;
+    f.dg = st.dag != 0;
+    f.src = st.src;
+    f.pos = st.pos;
+    f.post = st.post;
     return f;
   };
   // the field of a factor (pointers picked one by one: a runtime pick of a
@@ -1267,6 +1282,7 @@ KernelSet make_kernel_set() {
   KernelSet s;
   s.nc = NC;
   s.nmax = N;
+  s.big = NSPT_BIG_FIELDS != 0;
   s.lpb = Lane<NC, N>::LPB;
   s.qt = NSPT_QT;
   s.drift = [](int nblk, hipStream_t st, Geo g, SerField U, SerField D, SerField D1, double tau_g, double rk2c,
@@ -1312,4 +1328,5 @@ KernelSet make_kernel_set() {
   return s;
 }
 
+}  // inline namespace NSPT_VARIANT
 }  // namespace nspt
