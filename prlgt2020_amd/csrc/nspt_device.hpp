@@ -35,11 +35,11 @@
 
 // The order planes of a large field (32^4 at N = 32: 0.6 GB apart) lie
 // beyond the 32-bit byte offset of one buffer descriptor: every kernel
-// instance is also compiled with NSPT_BIG_FIELDS = 1, which takes a
-// descriptor per plane and which the host picks for such fields (the
-// per-plane descriptors cost scalar registers the common case should not
-// pay).  The two builds live in different inline namespaces, so that their
-// kernels and helpers are distinct symbols.
+// instance is also compiled with NSPT_BIG_FIELDS = 1, which reads each plane
+// at a scalar 64-bit address plus a 32-bit lane offset and which the host
+// picks for such fields (the per-plane addresses cost scalar registers the
+// common case should not pay).  The two builds live in different inline
+// namespaces, so that their kernels and helpers are distinct symbols.
 #ifndef NSPT_BIG_FIELDS
 #define NSPT_BIG_FIELDS 0
 #endif
@@ -182,6 +182,13 @@ __device__ __forceinline__ double2 ld_r(__amdgpu_buffer_rsrc_t r, unsigned off, 
   return d;
 }
 __device__ __forceinline__ double2 ld_u(const double2* base, unsigned off) { return ld_r(rsrc(base), off, 0u); }
+// 16-byte load at element `off` (per lane) of the plane starting `pofs`
+// elements (wave-uniform, any size) after `base`: a scalar 64-bit plane
+// address plus a 32-bit per-lane byte offset.
+__device__ __forceinline__ double2 ld_plane(const double2* base, long pofs, unsigned off) {
+  const char* p = reinterpret_cast<const char*>(uniform_ptr(base + pofs));
+  return *reinterpret_cast<const double2*>(p + (off * 16u));
+}
 __device__ __forceinline__ void st_u(double2* base, unsigned off, double2 v) {
   *reinterpret_cast<double2*>(reinterpret_cast<char*>(base) + (size_t)(off * 16u)) = v;
 }
@@ -405,7 +412,7 @@ __device__ __forceinline__ void unstage(const Lane<NC, N>& ln, const double2* sl
 // Order plane k of a field starts k * plane elements after order 0.  While
 // the last plane starts within 2 GiB of the field, one buffer descriptor
 // and a scalar byte offset per order reach every plane; NSPT_BIG_FIELDS
-// builds take a descriptor per plane (see the top of this file).
+// builds address each plane on its own (see the top of this file).
 // Global loads/stores of a lane-owned series.  Orders >= nrt read as zero and
 // are not stored.  Loads are unmasked (lanes of invalid links hold clamped,
 // valid indices); stores are masked by ok.
@@ -417,7 +424,7 @@ __device__ __forceinline__ void gload(const SerField& f, long idx, int e, int nr
   (void)ok;
 #if NSPT_BIG_FIELDS
 #pragma unroll
-  for (int k = 0; k < N; k++) X[k] = (k < nrt) ? ld_r(rsrc(f.data + k * plane), off, 0u) : zero2();
+  for (int k = 0; k < N; k++) X[k] = (k < nrt) ? ld_plane(f.data, k * plane, off) : zero2();
 #else
   const __amdgpu_buffer_rsrc_t r = rsrc(f.data);
 #pragma unroll

@@ -175,7 +175,7 @@ __device__ __forceinline__ void stage_global(const Lane<NC, N>& ln, double2* slo
 #if NSPT_BIG_FIELDS
 #pragma unroll
   for (int k = 0; k < N; k++) {
-    const double2 v = (k < nrt) ? ld_r(rsrc(f.data + k * plane), off, 0u) : zero2();
+    const double2 v = (k < nrt) ? ld_plane(f.data, k * plane, off) : zero2();
     slot[k * NE + pos] = make_double2(v.x, sy * v.y);
   }
 #else
