@@ -64,6 +64,16 @@ def kernel_flops(name, N, V, nc=3, improved=False, staples=False):
     return 0
 
 
+def kernel_function(pname, plaquette_drift, checkerboard_gauge=False):
+    """The kernel function(s) behind one of the library's profile names."""
+    if pname == "drift":
+        return "k_drift_p" if plaquette_drift else "k_drift"
+    if pname == "transform":
+        return "k_sgf" if checkerboard_gauge else "k_gauge_transform"
+    return {"update_log": "k_update", "fa_div": "k_fa_fft", "dft": "k_fa_fft", "rect": "k_link_pair+k_rect",
+            "noise": "k_noise_philox", "reduce": "k_reduce_rows"}.get(pname, "k_" + pname)
+
+
 def step_flops(N, V, integrator, nc=3, improved=False):
     SP, EX = sp_flops(N, nc), ex_flops(N, nc)
     per_link = (30 * SP + 4.25 * EX) if integrator == 1 else (16 * SP + 3.25 * EX)
@@ -231,17 +241,31 @@ def main():
     plaq = ctx.plaquette()
     value = 4 * V * N * args.steps / dt
 
-    # dominant kernel by total time
-    name = max(prof, key=lambda k: prof[k][0])
-    tot_ms, cnt = prof[name]
+    # dominant kernel function by total time: the library's profile names
+    # split some functions by what a launch does (k_update with and without
+    # the log), so launches, time and flops are summed per function first
+    uses_pf = bool(prof.get("plaq_field", (0, 0))[1])
+    groups = {}
+    for pname, (ms, c) in prof.items():
+        if not c:
+            continue
+        fname = kernel_function(pname, uses_pf, args.gauge != "fa")
+        fl1 = kernel_flops(pname, N, V, nc, improved, staples=integ == nspt.EULER_CHECKERBOARD)
+        if integ == nspt.EULER_CHECKERBOARD and pname in ("drift", "update", "update_log"):
+            fl1 /= 8  # one launch per (parity, direction): V/2 of the 4V links
+        g = groups.setdefault(fname, [0.0, 0, 0.0, []])
+        g[0] += ms
+        g[1] += c
+        g[2] += fl1 * c
+        g[3].append(pname)
+    name = max(groups, key=lambda k: groups[k][0])
+    tot_ms, cnt, fl_tot, pnames = groups[name]
     avg_s = tot_ms / cnt / 1e3
-    fl = kernel_flops(name, N, V, nc, improved, staples=integ == nspt.EULER_CHECKERBOARD)
-    if integ == nspt.EULER_CHECKERBOARD and name in ("drift", "update", "update_log"):
-        fl /= 8  # one launch per (parity, direction): V/2 of the 4V links
+    fl = fl_tot / cnt
     achieved = fl / avg_s / 1e12 if fl else 0.0
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_su3_16x4_N20_rk2.md")
-    if os.path.exists(tpath) and (args.L, N, args.integrator, args.action) == (16, 20, "rk2", "wil"):
+    if os.path.exists(tpath) and (args.L, N, args.integrator, args.action, args.twist) == (16, 20, "rk2", "wil", 0):
         # HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE counter runs
         # recorded there (calibrated against a copy of known size)
         for line in open(tpath):
@@ -282,7 +306,8 @@ def main():
                                   ", twisted boundary conditions in %s" % "".join(
                                       "xyzt"[d] for d in range(4) if (args.twist >> d) & 1) if args.twist else ""),
                    "lattice": list(L), "orders": N, "integrator": args.integrator, "replicas": 1},
-        "roofline": {"bound": "mfma", "kernel": name, "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+        "roofline": {"bound": "mfma", "kernel": name, "profile_names": pnames, "achieved": achieved,
+                     "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
                      "avg_launch_ms": avg_s * 1e3, "flops_per_launch": fl,
                      "step_fp64_tflops": step_tf, "step_frac": step_tf / PEAK_FP64_TFLOPS,
