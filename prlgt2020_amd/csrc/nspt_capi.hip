@@ -215,6 +215,7 @@ __global__ void k_obs_finalize(const double* obs_mu, const double* obs2_mu, int 
 constexpr int FFT_THREADS = 64;
 constexpr int FFT_MAXE = 2048;  // tile sites: L_da * L_db * C
 constexpr int FFT_MAXN = 32;    // lattice extents are at most 32
+constexpr int FFT_LOAD_BATCH = 8;  // tile loads in flight per thread
 
 #define NSPT_FFT_SIZES(X) X(2) X(3) X(4) X(5) X(6) X(8) X(10) X(12) X(16) X(20) X(24) X(32)
 
@@ -506,36 +507,64 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fa_fft(Geo g, int pass, const d
 
   double2* line = fb + comp * V;
   const double2* xc = Xsrc + comp * 4 * V;
-  for (int l = threadIdx.x; l < t.E; l += FFT_THREADS) {
-    int ia, ib, ic, site, off;
-    if (pass == 1) {
-      ic = l % t.C;
-      ib = (l / t.C) % t.nb;
-      ia = l / (t.C * t.nb);
-      site = (int)(base + (long)ia * g.st[0] + (long)ib * g.st[1] + ic);
-      off = ia * t.ra + ib * t.C + ic;
-    } else {
-      ib = l % t.nb;
-      ia = (l / t.nb) % t.na;
-      ic = l / (t.na * t.nb);
-      site = (int)(base + (long)ic * g.st[1] + (long)ia * g.st[2] + ib);
-      off = ic * t.cs + ia * t.ra + ib;
-    }
-    double2 w;
-    if (pass == 0) {
-      // divergence at x = (o0, o1 + ic, ia, ib)
-      const int x[4] = {o0, o1 + ic, ia, ib};
-      w = make_double2(0.0, 0.0);
+  // FFT_LOAD_BATCH sites per thread are loaded before any is stored to LDS,
+  // so that their global-load latencies overlap (the pass is a constant
+  // inside, so no branch separates the loads)
+  auto load_tile = [&](auto pass_c) {
+    constexpr int PASS = decltype(pass_c)::value;
+    // the divergence already has eight loads per site
+    constexpr int NB = PASS == 0 ? 2 : FFT_LOAD_BATCH;
+    for (int l0 = threadIdx.x; l0 < t.E; l0 += NB * FFT_THREADS) {
+      double wx[NB], wy[NB];  // plain doubles: an array of double2 is kept in scratch
+      int offs[NB];
 #pragma unroll
-      for (int mu = 0; mu < 4; mu++) {
-        const int xm = (x[mu] == 0) ? site + (g.L[mu] - 1) * g.st[mu] : site - g.st[mu];
-        w = nspt::c_add(w, nspt::c_sub(xc[mu * V + site], xc[mu * V + xm]));
+      for (int u = 0; u < NB; u++) {
+        // past the tile's end a lane reloads the tile's last site and stores
+        // the same value there as the lane that owns it (no branch, so the
+        // compiler keeps the loads together)
+        const int l = min(l0 + u * FFT_THREADS, t.E - 1);
+        int ia, ib, ic, site, off;
+        if constexpr (PASS == 1) {
+          ic = l % t.C;
+          ib = (l / t.C) % t.nb;
+          ia = l / (t.C * t.nb);
+          site = (int)(base + (long)ia * g.st[0] + (long)ib * g.st[1] + ic);
+          off = ia * t.ra + ib * t.C + ic;
+        } else {
+          ib = l % t.nb;
+          ia = (l / t.nb) % t.na;
+          ic = l / (t.na * t.nb);
+          site = (int)(base + (long)ic * g.st[1] + (long)ia * g.st[2] + ib);
+          off = ic * t.cs + ia * t.ra + ib;
+        }
+        offs[u] = off;
+        if constexpr (PASS == 0) {
+          // divergence at x = (o0, o1 + ic, ia, ib)
+          const int x[4] = {o0, o1 + ic, ia, ib};
+          double2 w = make_double2(0.0, 0.0);
+#pragma unroll
+          for (int mu = 0; mu < 4; mu++) {
+            const int xm = (x[mu] == 0) ? site + (g.L[mu] - 1) * g.st[mu] : site - g.st[mu];
+            w = nspt::c_add(w, nspt::c_sub(xc[mu * V + site], xc[mu * V + xm]));
+          }
+          wx[u] = w.x;
+          wy[u] = w.y;
+        } else {
+          const double2 w = line[site];
+          wx[u] = w.x;
+          wy[u] = w.y;
+        }
       }
-    } else {
-      w = line[site];
+#pragma unroll
+      for (int u = 0; u < NB; u++) sh.tile[offs[u]] = make_double2(wx[u], wy[u]);
     }
-    sh.tile[off] = w;
-  }
+  };
+  if (pass == 0)
+    load_tile(std::integral_constant<int, 0>());
+  else if (pass == 1)
+    load_tile(std::integral_constant<int, 1>());
+  else
+    load_tile(std::integral_constant<int, 2>());
   __syncthreads();
   FftAxis axa, axb;  // lines (ib, ic) along a, (ia, ic) along b
   if (pass == 1) {
