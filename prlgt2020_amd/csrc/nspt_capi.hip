@@ -415,7 +415,7 @@ __device__ void axis_transform(const FftShared& sh, const double2* tw, const Fft
 // ra = L_1*C + 1; passes 0, 2: ic*cs + ia*ra + ib with ra = L_3 + 1,
 // cs = L_2*ra.  The extra site per row spreads the rows over the LDS banks.
 struct FftTile {
-  int da, db, na, nb, C, E, tiles, ra, cs;
+  int da, db, na, nb, C, E, tiles, ra, cs, words;
 };
 
 // C: the largest divisor of the batch direction's extent that keeps the
@@ -436,15 +436,17 @@ __host__ __device__ inline FftTile fft_tile(const Geo& g, int pass) {
   if (pass == 1) {
     t.ra = t.nb * t.C + 1;
     t.cs = 1;
+    t.words = t.na * t.ra;  // the last offset is na*ra - 2
   } else {
     t.ra = t.nb + 1;
     t.cs = t.na * t.ra;
+    t.words = t.C * t.cs;  // the last offset is C*na*ra - 2
   }
   return t;
 }
 
 __host__ __device__ inline int fft_tile_words(const FftTile& t) {  // double2 per tile buffer
-  return t.C * t.na * t.ra;
+  return t.words;
 }
 
 __host__ inline size_t fft_shm(const FftTile& t, const Geo& g, bool direct) {
