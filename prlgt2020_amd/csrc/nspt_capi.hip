@@ -515,7 +515,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fa_fft(Geo g, int pass, const d
   auto load_tile = [&](auto pass_c) {
     constexpr int PASS = decltype(pass_c)::value;
     // the divergence already has eight loads per site
-    constexpr int NB = PASS == 0 ? 2 : FFT_LOAD_BATCH;
+    constexpr int NB = PASS == 0 ? 4 : FFT_LOAD_BATCH;
     for (int l0 = threadIdx.x; l0 < t.E; l0 += NB * FFT_THREADS) {
       double wx[NB], wy[NB];  // plain doubles: an array of double2 is kept in scratch
       int offs[NB];
