@@ -846,9 +846,16 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_gauge_transform(Geo g
     ln.stage(ln.slotB, Q, true);
     __syncthreads();
     unstage(ln, ln.slotB, Q);  // this lane's element of G(x+mu)^dag
-    horner_exp_times<NC, N>(ln, M + 1, gpf, Q);
-    stage_global(ln, ln.slotA, G, site, nrt);
+    // G(x) is loaded into the free registers while the exponential runs
+    // (above 24 orders those registers are not free: it is loaded after)
+    constexpr bool PF = N <= 24;
+    if constexpr (PF) gload<NC, N>(G, site, ln.e, nrt, ok, P);
     const double2 gf = gflag(G, site, ok);
+    horner_exp_times<NC, N>(ln, M + 1, gpf, Q);
+    if constexpr (PF)
+      ln.stage(ln.slotA, P);
+    else
+      stage_global(ln, ln.slotA, G, site, nrt);
     __syncthreads();
     const double2 nf = mul_lds<NC, N>(ln, gf, gpf, true, M, ln.slotB, false);
     __syncthreads();
