@@ -140,6 +140,36 @@ def cpu_farm(N, integrator, budget_s=8.0, procs=None):
                       % (procs, N, "RK2" if integrator == 1 else "Euler", budget_s, steps)}
 
 
+def host_noise_path(nspt, L, nc, N, eps, p_dev, steps=3):
+    """The reference's noise path beside `value`: the su(N) noise of every
+    link drawn on the host by ranlxd (the reference's stream, nspt_rng_noise)
+    and uploaded from pageable memory before each step, then the same step.
+    Wall time per phase; the upload completes inside the step call, which
+    ends with a stream synchronisation."""
+    V = int(np.prod(L))
+    ctx = nspt.Context(L, nc, N)
+    rng = nspt.Ranlxd(1)
+    p = nspt.Context.params(eps, p_dev.alpha, p_dev.integrator, p_dev.gauge_fix, p_dev.zero_modes,
+                            nspt.NOISE_HOST)
+    t_gen = t_set = t_step = 0.0
+    for s in range(steps + 1):  # the first step is a warmup
+        t0 = time.perf_counter()
+        eta = rng.noise(nc, 4 * V)
+        t1 = time.perf_counter()
+        ctx.set_noise(eta)
+        t2 = time.perf_counter()
+        ctx.step(p, want_obs=False)
+        t3 = time.perf_counter()
+        if s:
+            t_gen, t_set, t_step = t_gen + t1 - t0, t_set + t2 - t1, t_step + t3 - t2
+    ctx.close()
+    tot = t_gen + t_set + t_step
+    return {"value": 4 * V * N * steps / tot, "unit": "link-order updates/s", "ms_per_step": tot / steps * 1e3,
+            "ms_ranlxd_host": t_gen / steps * 1e3, "ms_set_noise": t_set / steps * 1e3,
+            "ms_step_with_upload": t_step / steps * 1e3, "noise_bytes_per_step": 4 * V * nc * nc * 16,
+            "steps": steps, "host_threads": 1}
+
+
 def batched_sweep(nsteps, warmup, sizes=(12, 16)):
     """Epsilon-replica sweep: SU(3) 4 x a^4 + 4 x b^4 for sizes (a, b) (the
     workload is 12, 16), N = 16, RK2 + zero modes + FA, eps in {0.01, 0.015,
@@ -195,6 +225,8 @@ def main():
                          "matrices (6: y and z, its default); needs --gauge checkerboard")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched epsilon-replica sweep")
+    ap.add_argument("--no-host-noise", action="store_true",
+                    help="skip the host-noise path (ranlxd on the host, upload before each step)")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="before the step, copy a buffer of this many MiB device to device (a plain copy kernel "
                          "of known size that scales the FETCH_SIZE / WRITE_SIZE counters of a --pmc run)")
@@ -316,6 +348,8 @@ def main():
         "plaquette_order2": float(plaq[2].real),
     }
     ctx.close()
+    if not args.no_host_noise and not args.twist and not improved:
+        out["host_noise"] = host_noise_path(nspt, L, nc, N, args.eps, p)
     if not args.no_batched:
         out["batched"] = batched_sweep(max(4, args.steps), 2)
         # replicas too small to fill the GPU alone, where the graph and the
