@@ -29,7 +29,8 @@ PEAK_HBM_GBS = 8000.0
 
 
 def sp_flops(N, nc=3):
-    """Canonical flops of one truncated series product (survey count)."""
+    """Canonical flops of one truncated series product: 216 per 3x3 complex
+    matrix product of two orders (64 for 2x2), plus the flag terms."""
     if nc == 3:
         return 216 * N * (N - 1) // 2 + 144 * N
     return 64 * N * (N - 1) // 2 + 64 * N
@@ -232,8 +233,9 @@ def main():
                          "of known size that scales the FETCH_SIZE / WRITE_SIZE counters of a --pmc run)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only; the multi-GPU layout is described in DESIGN.md "
-              "(--gpus %d not supported)" % args.gpus, file=sys.stderr)
+        print("bench.py: measures one GPU; --gpus %d is not supported (independent replicas would run one "
+              "or more per GPU; one lattice would need halo exchanges not implemented here)" % args.gpus,
+              file=sys.stderr)
         sys.exit(2)
 
     integ_cpu = {"rk2": 1, "euler": 0, "checkerboard": 0}[args.integrator]
