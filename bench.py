@@ -344,6 +344,8 @@ def main():
                      "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
                      "avg_launch_ms": avg_s * 1e3, "flops_per_launch": fl,
+                     "traffic_gbs": traffic / avg_s / 1e9 if traffic else None,
+                     "traffic_frac_of_hbm_peak": traffic / avg_s / 1e9 / PEAK_HBM_GBS if traffic else None,
                      "step_fp64_tflops": step_tf, "step_frac": step_tf / PEAK_FP64_TFLOPS,
                      "drift_evaluation": drift_eval},
         "kernels_ms_per_step": {k: v[0] / args.steps for k, v in prof.items() if v[1]},
