@@ -168,11 +168,12 @@ def test_high_orders_within_fp64_conditioning(nc, N, relerr):
 
 
 @pytest.mark.parametrize("L", [(4, 2, 4, 2), (6, 3, 2, 5), (3, 4, 5, 2), (2, 6, 2, 6), (7, 2, 3, 7), (2, 2, 32, 2),
-                               (2, 8, 2, 12)])
+                               (2, 8, 2, 12), (2, 2, 2, 32), (5, 2, 3, 24)])
 def test_fa_alone(relerr, L):
-    """FA gauge fixing: the unrolled line FFTs (extents 2..6, 8, 12, 32; the
-    32-site line runs in the wide kernel variant), the direct DFT of other
-    extents (7), and the batch of consecutive sites in the middle pass."""
+    """FA gauge fixing: the unrolled line FFTs (extents 2..6, 8, 12, 24, 32;
+    the 32-site line runs in the wide kernel variant), the direct DFT of
+    other extents (7), and the batch of consecutive sites in the middle pass
+    (up to 32 of them: its tile rows hold L_1 * C + 1 sites)."""
     nspt = _nspt()
     nc, N = 3, 6
     U = O.random_field(L, nc, N, seed=21)
