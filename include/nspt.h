@@ -78,7 +78,9 @@ int nspt_upload(nspt_ctx* ctx, const double* aos);
 int nspt_download(nspt_ctx* ctx, double* aos);
 int nspt_cold_start(nspt_ctx* ctx);
 
-/* Host-generated noise for the next step (NSPT_NOISE_HOST). */
+/* Host-generated noise for the next step (NSPT_NOISE_HOST), 4V x NE complex
+ * in the reference's link order [4*site + mu][NE].  Copied before the call
+ * returns: the caller may reuse eta at once. */
 int nspt_set_noise(nspt_ctx* ctx, const double* eta);
 
 /* NsptEvolve for one step; obs may be null. */

@@ -1141,6 +1141,8 @@ int nspt_set_noise(nspt_ctx* c, const double* eta) {
   hipLaunchKernelGGL(k_eta_permute, dim3(grid_for((long)n, 256)), dim3(256), 0, c->stream,
                      (const double2*)c->eta_tmp, c->eta, c->NE, c->V, 1);
   if (!launch_ok(c, "eta_permute")) return -3;
+  // the caller may reuse eta as soon as this returns
+  if (!c->hip_ok(hipStreamSynchronize(c->stream), "sync")) return -3;
   c->noise_ready = true;
   return 0;
 }
