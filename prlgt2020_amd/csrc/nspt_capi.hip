@@ -1473,14 +1473,17 @@ bool batch_replay(nspt_batch* b, int n) {
 }
 
 bool batch_set_counters(nspt_batch* b, unsigned long long step, unsigned long long row) {
+  // the host source stays alive until the copies have completed
+  const unsigned long long v[2] = {step, row};
+  bool ok = true;
   for (auto* c : b->reps) {
-    const unsigned long long v[2] = {step, row};
     if (!c->hip_ok(hipMemcpyAsync(c->ctr_dev, v, sizeof(v), hipMemcpyHostToDevice, b->master), "counters")) {
       b->err = c->err;
-      return false;
+      ok = false;
+      break;
     }
   }
-  return batch_sync(b);
+  return batch_sync(b) && ok;
 }
 
 }  // namespace
@@ -1557,8 +1560,8 @@ int nspt_batch_evolve(nspt_batch* b, int nsteps, double* obs) {
     const int n = want ? std::min(chunk, nsteps - done) : nsteps;
     if (want) {
       // the row counter restarts for each chunk; the noise step counter runs on
+      static const unsigned long long zero = 0;  // outlives the queued copies
       for (auto* c : b->reps) {
-        const unsigned long long zero = 0;
         if (hipMemcpyAsync(c->ctr_dev + 1, &zero, sizeof(zero), hipMemcpyHostToDevice, b->master) != hipSuccess)
           return b->fail("counter reset failed");
       }
