@@ -130,8 +130,7 @@ struct LinkBlock {
   int mu, site0, part, count;
 };
 template <int LPB>
-__device__ __forceinline__ LinkBlock link_block(const Geo& g) {
-  const int b = block_order();
+__device__ __forceinline__ LinkBlock link_block_at(const Geo& g, int b) {
   if (g.sub) {
     const int mu = (g.sub - 1) & 3, par = (g.sub - 1) >> 2;
     return {mu, b * LPB, (2 * mu + par) * g.nb_half + b, g.V / 2};
@@ -142,6 +141,10 @@ __device__ __forceinline__ LinkBlock link_block(const Geo& g) {
   const int mu = b / g.nb_mu, sb = b % g.nb_mu;
 #endif
   return {mu, sb * LPB, mu * g.nb_mu + sb, g.V};
+}
+template <int LPB>
+__device__ __forceinline__ LinkBlock link_block(const Geo& g) {
+  return link_block_at<LPB>(g, block_order());
 }
 
 // Site of entry k of a block's site list: k itself, or the k-th site of the
@@ -1164,29 +1167,57 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_rect(Geo g, SerField 
 // Long links out_mu(x) = A_mu(x) A'_mu(x + k mu) with A' = A: the 2-link
 // field (k = 1, A = U) and the 4-link field (k = 2, A = the 2-link field)
 // of the loop measurement.
+// Link blocks per workgroup (above 24 orders the registers of a prefetch
+// are not free: one block, as many workgroups as blocks).
+template <int N>
+constexpr int link_pair_blocks() {
+  return N <= 24 ? 4 : 1;
+}
 template <int NC, int N>
-__global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_link_pair(Geo g, SerField A, SerField out, int k) {
+__global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_link_pair(Geo g, SerField A, SerField out, int k,
+                                                                     int nblk) {
   using LN = Lane<NC, N>;
   __shared__ double2 lds[LN::LDS];
   LN ln(lds);
   const int nrt = g.nrt;
-  const LinkBlock lb = link_block<LN::LPB>(g);
-  const int mu = lb.mu;
-  const int site = min(lb.site0 + ln.j, g.V - 1);
-  const bool ok = ln.lds_ok && (lb.site0 + ln.j) < g.V;
-  const long idx = (long)mu * g.V + site;
-  int c2 = TW0;
-  const long idx2 = (long)mu * g.V + site_shift_far_tw(g, site, mu, k, c2);
-  stage_global(ln, ln.slotA, A, idx, nrt);
-  stage_global_tw(g, ln, ln.slotB, A, idx2, c2, nrt);
-  const double2 fa = gflag(A, idx, ok), fb = gflag(A, idx2, ok);
-  __syncthreads();
-  const double2 f = mul_lds<NC, N>(ln, fa, fb, true, min(N, nrt) - 1, ln.slotB, false);
-  __syncthreads();
-  double2 P[N];
-  unstage(ln, ln.slotB, P);
-  gstore<NC, N>(out, idx, ln.e, nrt, ok, P);
-  if (ok && ln.e == 0) out.flag[idx] = f;
+  const int M = min(N, nrt) - 1;
+  // NB link blocks per workgroup, one after the other; the next block's two
+  // links are loaded while the current product runs (a single product per
+  // block leaves the loads' latency exposed otherwise)
+  constexpr int NB = link_pair_blocks<N>();
+  const int b0 = block_order() * NB;
+  double2 PA[N], PB[N], P[N];
+  double2 fa, fb;
+  auto load = [&](int b) {
+    const LinkBlock lb = link_block_at<LN::LPB>(g, b);
+    const int site = min(lb.site0 + ln.j, g.V - 1);
+    const long ix = (long)lb.mu * g.V + site;
+    int c2 = TW0;
+    const long ix2 = (long)lb.mu * g.V + site_shift_far_tw(g, site, lb.mu, k, c2);
+    const bool okb = ln.lds_ok && (lb.site0 + ln.j) < g.V;
+    gload<NC, N>(A, ix, ln.e, nrt, okb, PA);
+    gload_tw<NC, N>(g, ln, A, ix2, c2, nrt, okb, PB);
+    fa = gflag(A, ix, okb);
+    fb = gflag(A, ix2, okb);
+  };
+  load(b0);
+  for (int i = 0; i < NB && b0 + i < nblk; i++) {
+    const LinkBlock lb = link_block_at<LN::LPB>(g, b0 + i);
+    const int site = min(lb.site0 + ln.j, g.V - 1);
+    const bool ok = ln.lds_ok && (lb.site0 + ln.j) < g.V;
+    const long idx = (long)lb.mu * g.V + site;
+    const double2 a0 = fa, bf = fb;
+    ln.stage(ln.slotA, PA);
+    ln.stage(ln.slotB, PB);
+    __syncthreads();
+    if (i + 1 < NB && b0 + i + 1 < nblk) load(b0 + i + 1);
+    const double2 f = mul_lds<NC, N>(ln, a0, bf, true, M, ln.slotB, false);
+    __syncthreads();
+    unstage(ln, ln.slotB, P);
+    gstore<NC, N>(out, idx, ln.e, nrt, ok, P);
+    if (ok && ln.e == 0) out.flag[idx] = f;
+    __syncthreads();  // the slots are restaged for the next block
+  }
 }
 
 // One rectangle type of one plane per block (ComputeLoops, nspt.cc:1156-1312):
@@ -1327,7 +1358,8 @@ KernelSet make_kernel_set() {
     hipLaunchKernelGGL((k_gauge_transform<NC, N>), dim3(nblk), dim3(64), 0, st, g, G, U, Xexp, MM);
   };
   s.link_pair = [](int nblk, hipStream_t st, Geo g, SerField A, SerField out, int k) {
-    hipLaunchKernelGGL((k_link_pair<NC, N>), dim3(nblk), dim3(64), 0, st, g, A, out, k);
+    constexpr int nb = link_pair_blocks<N>();
+    hipLaunchKernelGGL((k_link_pair<NC, N>), dim3((nblk + nb - 1) / nb), dim3(64), 0, st, g, A, out, k, nblk);
   };
   s.loops = [](int nblk, hipStream_t st, Geo g, SerField U, SerField L2, SerField L4, double* part) {
     hipLaunchKernelGGL((k_loops<NC, N>), dim3(nblk), dim3(64), 0, st, g, U, L2, L4, part);
