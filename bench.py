@@ -187,14 +187,19 @@ def batched_sweep(nsteps, warmup, sizes=(12, 16)):
     b = nspt.Batch(Ls, 3, N, ps)
     b.evolve(warmup, want_obs=False)
     qt, tune_ms = b.tune(steps=3)
-    # plain: the same replicas, one after the other, each with its own launches
-    lib = nspt.lib()
+    # plain: the same replicas as contexts of their own, one after the other,
+    # each with its own launches
+    plain = [nspt.Context(L, 3, N) for L in Ls]
+    pp = [nspt.Context.params(e, 0.05, nspt.RK2, nspt.GF_FA, 1, nspt.NOISE_DEVICE, seed=100 + r, step=0)
+          for r, e in enumerate(eps)]
+    for c, p in zip(plain, pp):
+        c.evolve(p, warmup, want_obs=False)
     t0 = time.perf_counter()
-    for r, rep in enumerate(b.replicas):
-        p = nspt.Context.params(eps[r], 0.05, nspt.RK2, nspt.GF_FA, 1, nspt.NOISE_DEVICE, seed=100 + r,
-                                step=10 ** 6)
-        rep.evolve(p, nsteps, want_obs=True)
+    for c, p in zip(plain, pp):
+        c.evolve(p, nsteps, want_obs=True)
     t_plain = time.perf_counter() - t0
+    for c in plain:
+        c.close()
     t0 = time.perf_counter()
     b.evolve(nsteps, want_obs=True)
     t_batch = time.perf_counter() - t0

@@ -83,7 +83,9 @@ int nspt_cold_start(nspt_ctx* ctx);
  * returns: the caller may reuse eta at once. */
 int nspt_set_noise(nspt_ctx* ctx, const double* eta);
 
-/* NsptEvolve for one step; obs may be null. */
+/* NsptEvolve for one step; obs may be null.  nspt_step, nspt_evolve,
+ * nspt_gauge_wilson and nspt_set_noise refuse a replica of a batch (its
+ * steps are nspt_batch_evolve's). */
 int nspt_step(nspt_ctx* ctx, const nspt_step_params* p, double* obs);
 
 /* nsteps steps with device noise (p->noise must be NSPT_NOISE_DEVICE or
@@ -127,6 +129,32 @@ int nspt_loops(nspt_ctx* ctx, double* out);
  * fewer orders (QCDenvNODEpt_test.h:415-418). */
 int nspt_unitarize(nspt_ctx* ctx);
 
+/* The stages of NsptEvolve (nspt.cc:963-1033) one by one, in the order the
+ * reference calls them: gauge_wilson, zero_modes_subtraction, then
+ * stochastic_gauge_fixing or FAstochastic_gauge_fixing.  Together they are
+ * nspt_step (which fuses them and is faster).
+ *
+ * gauge_wilson (nspt.cc:258-600): the Langevin update of every link with the
+ * noise of p (integrator as in nspt_step; the zero_modes and gauge_fix
+ * fields of p are ignored); obs as in nspt_step.  The Jacobi form writes the
+ * new field into F and the reference then copies it back (Umu = Fmu,
+ * nspt.cc:991-992): here the new field simply becomes the context's field. */
+int nspt_gauge_wilson(nspt_ctx* ctx, const nspt_step_params* p, double* obs);
+
+/* zero_modes_subtraction (nspt.cc:605-654): MM_mu = (1/V) sum_x log U_mu(x),
+ * U <- exp(reH(log U - MM_mu)).  The reference accumulates MM over the links
+ * gauge_wilson has just written; that is the current field, so the mean is
+ * formed here from it (deterministic fixed-order sums). */
+int nspt_zero_modes_subtraction(nspt_ctx* ctx);
+
+/* stochastic_gauge_fixing (nspt.cc:660-862): per site W = sum_mu U_mu(x) -
+ * U_mu(x-mu), W <- Trless(W - W^dag), U_mu(x) <- e^{alpha W} U_mu(x),
+ * U_mu(x-mu) <- U_mu(x-mu) e^{-alpha W}.  The reference sweeps the sites
+ * lexicographically in place; here the sites of parity 0, then 1 (sites of
+ * one parity share no link), even extents only.  alpha in the reference's
+ * convention (Quench.cfg alpha; the shipped value is -0.05). */
+int nspt_stochastic_gauge_fixing(nspt_ctx* ctx, double alpha);
+
 /* Stand-alone FAstochastic_gauge_fixing of the current field
  * (nspt.cc:866-948). */
 int nspt_fa_gauge_fixing(nspt_ctx* ctx, double alpha);
@@ -136,11 +164,19 @@ int nspt_fa_gauge_fixing(nspt_ctx* ctx, double alpha);
  * (nspt.cc:383-417). */
 int nspt_drift(nspt_ctx* ctx, double eps, double* aos_out);
 
-/* Host noise source: ranlxd (ranlxd/ranlxd.cc) seeded as the reference
- * seeds Rand[0] (Quenched.cc:301-306) and mapped to su(nc) matrices
- * (MyMath.cc:295-322).  SU(2) draws four uniforms per matrix. */
+/* Host noise sources, drawn in the reference's link order.
+ * ranlxd (ranlxd/ranlxd.cc), the reference built without USE_RAND55: seeded
+ * as the reference seeds Rand[0] (the first glibc rand() after srand(seed),
+ * Quenched.cc:301-306) and mapped to su(nc) matrices by MyMath.cc:295-322;
+ * SU(2) draws four uniforms per matrix.
+ * RAND55 (MyRand, MyRand.cc:30-130), the generator of the shipped build
+ * (Makefile:2 -DUSE_RAND55): seeded with Quench.cfg's seed as
+ * Rand[0].init(seed) (Quenched.cc:296-299), Gaussian pairs of
+ * MyRand::generate_gauss mapped by SU3rand / SU2rand (MyMath.cc:256-292,
+ * 326-346). */
 typedef struct nspt_rng nspt_rng;
 int nspt_rng_create_ranlxd(int seed, int level, nspt_rng** out);
+int nspt_rng_create_rand55(long seed, nspt_rng** out);
 int nspt_rng_destroy(nspt_rng* rng);
 int nspt_rng_uniform(nspt_rng* rng, int n, double* out);
 int nspt_rng_noise(nspt_rng* rng, int nc, int nlinks, double* eta);

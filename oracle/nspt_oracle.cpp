@@ -50,14 +50,24 @@
 
 namespace {
 
+// Scalar type of the series arithmetic: double, or (built with
+// -DORC_EXTENDED=1) the x87 80-bit long double, whose 64-bit significand
+// gives an accuracy reference for FP64 steps at high orders.
+#if ORC_EXTENDED
+using real = long double;
+#else
+using real = double;
+#endif
+constexpr real kPi = 3.14159265358979323846264338327950288L;
+
 struct cx {
-  double re, im;
+  real re, im;
 };
-inline cx mk(double r, double i) { return cx{r, i}; }
+inline cx mk(real r, real i) { return cx{r, i}; }
 inline cx operator+(cx a, cx b) { return cx{a.re + b.re, a.im + b.im}; }
 inline cx operator-(cx a, cx b) { return cx{a.re - b.re, a.im - b.im}; }
 inline cx operator*(cx a, cx b) { return cx{a.re * b.re - a.im * b.im, a.im * b.re + a.re * b.im}; }
-inline cx operator*(double x, cx b) { return cx{x * b.re, x * b.im}; }
+inline cx operator*(real x, cx b) { return cx{x * b.re, x * b.im}; }
 inline cx& operator+=(cx& a, cx b) { a.re += b.re; a.im += b.im; return a; }
 inline cx& operator-=(cx& a, cx b) { a.re -= b.re; a.im -= b.im; return a; }
 inline cx conj(cx a) { return cx{a.re, -a.im}; }
@@ -70,7 +80,7 @@ inline bool is_zero(cx a) { return a.re == 0.0 && a.im == 0.0; }
 inline void mat_mul(int nc, const cx* A, const cx* B, cx* C) {
   for (int r = 0; r < nc; r++)
     for (int c = 0; c < nc; c++) {
-      double re = 0.0, im = 0.0;
+      real re = 0, im = 0;
       for (int m = 0; m < nc; m++) {
         const cx a = A[r * nc + m], b = B[m * nc + c];
         re += a.re * b.re - a.im * b.im;
@@ -143,10 +153,10 @@ Ser ser_sub(const Ser& A, const Ser& B) {
   for (size_t i = 0; i < C.v.size(); i++) C.v[i] -= B.v[i];
   return C;
 }
-void ser_scale(Ser& A, double x) {
+void ser_scale(Ser& A, real x) {
   for (auto& z : A.v) z = x * z;
 }
-void ser_div(Ser& A, double x) {
+void ser_div(Ser& A, real x) {
   for (auto& z : A.v) z = cx{z.re / x, z.im / x};
 }
 
@@ -160,7 +170,7 @@ Ser ser_dag(const Ser& A) {
 // reH (QCDpt.cc:197-209, SU(2): 510-521): per order (M - M^dag)/2 minus Tr/nc.
 void ser_reH(Ser& A) {
   const int nc = A.nc;
-  const double inv = (nc == 3) ? (1.0 / 3.0) : 0.5;
+  const real inv = (nc == 3) ? real(1) / 3 : real(1) / 2;
   for (int k = 1; k <= A.N; k++) {
     cx* m = A.ord(k);
     cx d[9];
@@ -175,7 +185,7 @@ void ser_reH(Ser& A) {
 // Trless (QCDpt.cc:124-136)
 void ser_trless(Ser& A) {
   const int nc = A.nc;
-  const double inv = (nc == 3) ? (1.0 / 3.0) : 0.5;
+  const real inv = (nc == 3) ? real(1) / 3 : real(1) / 2;
   for (int k = 1; k <= A.N; k++) {
     cx* m = A.ord(k);
     const cx z = inv * mat_tr(nc, m);
@@ -187,7 +197,7 @@ void ser_trless(Ser& A) {
 // Tr (QCDpt.cc:115-122): tt[0] = nc*flag, tt[k] = Tr of order k.
 void ser_tr(const Ser& A, cx* tt) {
   for (int k = 1; k <= A.N; k++) tt[k] = mat_tr(A.nc, A.ord(k));
-  tt[0] = (double)A.nc * A.flag();
+  tt[0] = (real)A.nc * A.flag();
 }
 
 // Powers step shared by exp and log: Bnew_{iA+iB} += B_iB * A_iA for iB >= i-1
@@ -204,7 +214,7 @@ Ser ser_exp(const Ser& A) {
   Ser B = A, Bnew(A.nc, A.N), res = A;
   for (int i = 2; i <= A.N; i++) {
     power_step(B, A, i, Bnew);
-    const double den = 1.0 / (double)i;
+    const real den = real(1) / i;
     B = Bnew;
     ser_scale(B, den);
     for (size_t q = 0; q < res.v.size(); q++) res.v[q] += B.v[q];
@@ -218,11 +228,11 @@ Ser ser_exp(const Ser& A) {
 Ser ser_log(const Ser& U) {
   Ser B = U, Bnew(U.nc, U.N), res = U;
   B.flag() = cx{0, 0};
-  double segno = -1.0;
+  real segno = -1;
   for (int i = 2; i <= U.N; i++) {
     power_step(B, U, i, Bnew);
     for (int q = 1; q < (int)B.v.size(); q++) B.v[q] = Bnew.v[q];
-    const double aux = segno / (double)i;
+    const real aux = segno / i;
     for (size_t q = 0; q < res.v.size(); q++) res.v[q] += aux * B.v[q];
     segno = -segno;
   }
@@ -331,7 +341,7 @@ void twist_matrix(int nc, int code, cx* M) {
 struct Field {
   int nc, N, S;
   const Latt* lat;
-  double* data;  // interleaved complex, [(4*site+mu)*S + j]
+  real* data;  // interleaved complex, [(4*site+mu)*S + j]
   // e: a site or an extended index (twisted lattice): the twisted copy
   // M U M^dag of the stored link
   Ser get(int e, int mu) const {
@@ -352,7 +362,7 @@ struct Field {
   // a twisted copy is written back as the stored link M^dag T M
   void set(int e, int mu, const Ser& s) {
     const int V = lat->V, site = e % V, code = e / V;
-    double* dst = data + 2 * (size_t)(4 * site + mu) * S;
+    real* dst = data + 2 * (size_t)(4 * site + mu) * S;
     std::memcpy(dst, s.v.data(), sizeof(cx) * S);
     if (code) {
       Ser u = s;
@@ -415,7 +425,7 @@ Ser staple_sum(const Field& F, int n, int mu) {
 // normalisation (nspt.cc:570-597).
 struct Obs {
   std::vector<cx> w1;             // N+1
-  std::vector<double> norm;       // N
+  std::vector<real> norm;         // N
   std::vector<cx> trU;            // 4*N
   std::vector<cx> w2;             // N+1, improved actions only (nspt.cc:405-410)
   bool improved = false;
@@ -448,9 +458,9 @@ struct Obs {
   // out: plaq[2(N+1)], norm[N], trU[2*4N] normalised as written to file
   void write(int nc, int V, double* out) const {
     const int N = (int)norm.size();
-    const double pn = (nc == 3) ? 72.0 * V : 48.0 * V;
-    const double nn = (nc == 3) ? 12.0 * V : 8.0 * V;
-    const double tn = (nc == 3) ? 3.0 * V : 2.0 * V;
+    const real pn = (nc == 3) ? real(72) * V : real(48) * V;
+    const real nn = (nc == 3) ? real(12) * V : real(8) * V;
+    const real tn = (nc == 3) ? real(3) * V : real(2) * V;
     for (int k = 0; k <= N; k++) {
       out[2 * k] = w1[k].re / pn;
       out[2 * k + 1] = w1[k].im / pn;
@@ -476,7 +486,7 @@ struct Obs {
 // (nspt.cc:383-417, Quenched.cc:584-589).  Optionally accumulates observables.
 // c1 != 0: improved action (nspt.cc:385-412, Quenched.cc:313-322),
 // W = tau_g reH(c0 U Sigma + c1 U Sigma_2x1) with c0 = 1 - 8 c1.
-Ser drift(const Field& F, int n, int mu, double tau_g, Obs* obs, double c1 = 0.0) {
+Ser drift(const Field& F, int n, int mu, real tau_g, Obs* obs, double c1 = 0.0) {
   Ser U = F.get(n, mu);
   Ser W = ser_mul(U, staple_sum(F, n, mu));
   if (obs) obs->accumulate(U, W, mu);
@@ -495,7 +505,7 @@ Ser drift(const Field& F, int n, int mu, double tau_g, Obs* obs, double c1 = 0.0
   return W;
 }
 
-void sub_noise(Ser& W, double stau, const double* eta) {
+void sub_noise(Ser& W, real stau, const double* eta) {
   cx* o1 = W.ord(1);
   for (int e = 0; e < W.ne(); e++) {
     o1[e].re -= stau * eta[2 * e];
@@ -515,7 +525,7 @@ void dft_many(const int* L, int howmany, int stride, cx* data, int sign) {
     const int n = L[d];
     std::vector<cx> tw(n);
     for (int j = 0; j < n; j++) {
-      const double ang = 2.0 * M_PI * (double)j / (double)n;
+      const real ang = 2 * kPi * j / n;
       tw[j] = cx{std::cos(ang), sign * std::sin(ang)};
     }
     line.resize(n);
@@ -536,6 +546,7 @@ void dft_many(const int* L, int howmany, int stride, cx* data, int sign) {
   }
 }
 
+#if !ORC_EXTENDED
 // ---------------------------------------------------------------- RNGs
 // ranlxd, portable path (ranlxd/ranlxd.cc:332-531): four interleaved 24-bit
 // subtract-with-borrow lanes, 12 vectors of (low word, high word) x 4 lanes,
@@ -715,7 +726,10 @@ void noise_rand55(int nc, Rand55& R, double* out) {
     g = R.gauss();
     P[0].im = g; P[3].im = -g;
   }
-  std::memcpy(out, P, sizeof(cx) * nc * nc);
+  for (int e = 0; e < nc * nc; e++) {
+    out[2 * e] = (double)P[e].re;
+    out[2 * e + 1] = (double)P[e].im;
+  }
 }
 
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).
@@ -751,6 +765,21 @@ void noise_philox_link(int nc, uint64_t seed, uint64_t step, uint32_t link, doub
   noise_from_uniforms(nc, g, out);
 }
 
+#endif  // !ORC_EXTENDED
+
+// ---------------------------------------------------------------- planted mistakes
+// Mutation switches for the tests that pin this oracle: a test that claims
+// to pin a formula shows that it fails on an oracle carrying a plausible
+// mistake in it.  Off (0, 1.0) everywhere except inside those tests.
+//   fa:  1 alpha with the wrong sign, 2 the 1/V of the inverse DFT dropped,
+//        3 p^2 = 4 sin^2(p) instead of 4 sin^2(p/2) (lattice.h:1144-1168)
+//   rk2: factor on the nc*eps/6 coefficient of the RK2 corrector
+struct Plant {
+  int fa = 0;
+  double rk2 = 1.0;
+};
+Plant g_plant;
+
 // ---------------------------------------------------------------- step
 struct Params {
   double eps;        // Langevin step epsilon (tau_g of Quench.cfg)
@@ -764,6 +793,7 @@ struct Params {
   double c1 = 0.0;   // rectangle coefficient: 0 Wilson, -1/12 TLSYM, -0.331 IWA, -1.4088 DBW2
 };
 
+#if !ORC_EXTENDED
 // ---------------------------------------------------------------- adjoint
 // Generalised series with a matrix at order 0 (random configurations with a
 // Haar-random order 0): orders k = 0..N, each NC x NC, [(k*NE + e)].  The
@@ -879,13 +909,15 @@ GSer gstaple_sum(const GField& U, int x, int mu) {
   return S;
 }
 
+#endif  // !ORC_EXTENDED
+
 struct Geom {
   int L[4], nc, N;
 };
 
 void zero_modes(Field& F, std::vector<Ser>& MM) {
   const Latt& Z = *F.lat;
-  for (int mu = 0; mu < 4; mu++) ser_scale(MM[mu], 1.0 / (double)Z.V);  // MM *= rVol
+  for (int mu = 0; mu < 4; mu++) ser_scale(MM[mu], real(1) / Z.V);  // MM *= rVol
   for (int s = 0; s < Z.V; s++)
     for (int mu = 0; mu < 4; mu++) {
       Ser W = ser_sub(ser_log(F.get(s, mu)), MM[mu]);
@@ -909,12 +941,12 @@ void fa_gauge_fixing(Field& F, double alpha) {
   // plan howmany = nc*nc*PTORD + 1, stride = S, dist = 1 (Quenched.cc:827-838)
   dft_many(Z.L, S, S, Wg.data(), -1);
   for (int q = 0; q < S; q++) Wg[q] = 0.0 * Wg[q];  // PBC: k = 0 mode removed
-  std::vector<double> p2[4];
+  std::vector<real> p2[4];
   for (int d = 0; d < 4; d++) {
     p2[d].resize(Z.L[d]);
     for (int i = 0; i < Z.L[d]; i++) {
-      const double p = 2 * M_PI * i / Z.L[d];
-      double h = std::sin(p * 0.5);
+      const real p = 2 * kPi * i / Z.L[d];
+      real h = std::sin(g_plant.fa == 3 ? p : p / 2);
       h *= 4 * h;
       p2[d][i] = h;  // lattice.h:1144-1168
     }
@@ -922,7 +954,8 @@ void fa_gauge_fixing(Field& F, double alpha) {
   for (int x = 1; x < Z.V; x++) {
     int c[4];
     Z.coords(x, c);
-    const double f = -(alpha / (p2[0][c[0]] + p2[1][c[1]] + p2[2][c[2]] + p2[3][c[3]]));
+    const real a = g_plant.fa == 1 ? -alpha : alpha;
+    const real f = -(a / (p2[0][c[0]] + p2[1][c[1]] + p2[2][c[2]] + p2[3][c[3]]));
     for (int q = 0; q < S; q++) Wg[(size_t)x * S + q] = f * Wg[(size_t)x * S + q];
   }
   dft_many(Z.L, S, S, Wg.data(), +1);
@@ -930,7 +963,7 @@ void fa_gauge_fixing(Field& F, double alpha) {
   for (int x = 0; x < Z.V; x++) {
     Ser W(nc, N);
     std::memcpy(W.v.data(), &Wg[(size_t)x * S], sizeof(cx) * S);
-    ser_div(W, (double)Z.V);
+    if (g_plant.fa != 2) ser_div(W, (real)Z.V);
     ser_reH(W);
     G[x] = ser_exp(W);
   }
@@ -1001,7 +1034,7 @@ void sgf_checkerboard(Field& F, double alpha) {
 Ser rk2_combine(const Ser& D1, const Ser& D2, int nc, double eps) {
   Ser D = ser_add(D1, D2);
   ser_scale(D, 0.5);
-  const double c = (double)nc * eps / 6.0;
+  const real c = (real)nc * eps / 6 * g_plant.rk2;
   for (int k = D.N; k >= 3; k--) {
     cx* o = D.ord(k);
     const cx* p = D.ord(k - 2);
@@ -1012,11 +1045,11 @@ Ser rk2_combine(const Ser& D1, const Ser& D2, int nc, double eps) {
 
 // One NsptEvolve step (nspt.cc:963-1033).  eta: 4V noise matrices in link
 // order.  obs: plaq[2(N+1)], norm[N], trU[8N] (may be null).
-void step(const Geom& g, double* Udata, const Params& p, const double* eta, double* obs_out) {
+void step(const Geom& g, real* Udata, const Params& p, const double* eta, double* obs_out) {
   Latt Z(g.L);
   const int S = 1 + g.nc * g.nc * g.N, ne2 = 2 * g.nc * g.nc;
   Field U{g.nc, g.N, S, &Z, Udata};
-  const double stau = std::sqrt(p.eps), tau_g = -p.eps / (2.0 * g.nc);
+  const real stau = std::sqrt((real)p.eps), tau_g = -(real)p.eps / (2 * g.nc);
   std::vector<Ser> MM(4, Ser(g.nc, g.N));
   Obs obs;
   obs.init(g.N, p.c1 != 0.0);
@@ -1046,7 +1079,7 @@ void step(const Geom& g, double* Udata, const Params& p, const double* eta, doub
         MM[mu] = ser_add(MM[mu], ser_log(Un));
       }
   } else {
-    std::vector<double> Fbuf((size_t)8 * Z.V * S);
+    std::vector<real> Fbuf((size_t)8 * Z.V * S);
     Field Fn{g.nc, g.N, S, &Z, Fbuf.data()};
     std::vector<Ser> D1;
     if (p.integrator == 1) {
@@ -1060,7 +1093,7 @@ void step(const Geom& g, double* Udata, const Params& p, const double* eta, doub
           Fn.set(s, mu, ser_mul(ser_exp(W), U.get(s, mu)));
         }
     }
-    std::vector<double> Gbuf;
+    std::vector<real> Gbuf;
     for (int s = 0; s < Z.V; s++)
       for (int mu = 0; mu < 4; mu++) {
         Ser W;
@@ -1082,9 +1115,9 @@ void step(const Geom& g, double* Udata, const Params& p, const double* eta, doub
       }
     // Umu = Fmu (nspt.cc:992)
     if (p.integrator == 1)
-      std::memcpy(Udata, Gbuf.data(), sizeof(double) * Gbuf.size());
+      std::memcpy(Udata, Gbuf.data(), sizeof(real) * Gbuf.size());
     else
-      std::memcpy(Udata, Fbuf.data(), sizeof(double) * Fbuf.size());
+      std::memcpy(Udata, Fbuf.data(), sizeof(real) * Fbuf.size());
   }
   if (p.zero_modes) zero_modes(U, MM);
   if (p.gauge_fix == 1) fa_gauge_fixing(U, p.alpha);
@@ -1093,6 +1126,7 @@ void step(const Geom& g, double* Udata, const Params& p, const double* eta, doub
   if (obs_out) obs.write(g.nc, Z.V, obs_out);
 }
 
+#if !ORC_EXTENDED
 // plaquette_measure, OpenMP form (nspt.cc:1355-1402): sum over mu != nu of
 // Tr(U_mu * staple) / (72 V) (SU(3)) or (48 V) (SU(2)).
 void plaquette_measure(const Geom& g, const double* Udata, double* out) {
@@ -1175,6 +1209,8 @@ Ser from_ptr(int nc, int N, const double* p) {
 }
 void to_ptr(const Ser& s, double* p) { std::memcpy(p, s.v.data(), sizeof(cx) * s.v.size()); }
 
+#endif  // !ORC_EXTENDED
+
 Geom geom_from(const int* gi) {
   Geom g;
   for (int d = 0; d < 4; d++) g.L[d] = gi[d];
@@ -1183,12 +1219,54 @@ Geom geom_from(const int* gi) {
   return g;
 }
 
+#if !ORC_EXTENDED
+// Long run for statistics: nsteps steps of p from U (in place), the noise of
+// each step drawn from one ranlxd stream (level 1) in link order.  bins
+// receives, per bin of nsteps / nbin consecutive steps, the mean over the
+// bin of the real part of every plaquette order 0..N (the observable of
+// each step, measured on the field entering it, nspt.cc:392-402, 570-597).
+void run_stats(const Geom& g, double* U, const Params& p, int seed, long nsteps, int nbin, double* bins) {
+  Ranlxd R(seed, 1);
+  const int V = g.L[0] * g.L[1] * g.L[2] * g.L[3];
+  const int ne2 = 2 * g.nc * g.nc, ng = (g.nc == 3) ? 8 : 4;
+  const int olen = 2 * (g.N + 1) + g.N + 8 * g.N + (p.c1 != 0.0 ? 2 * (g.N + 1) : 0);
+  std::vector<double> eta((size_t)4 * V * ne2), obs(olen);
+  const long per = nsteps / nbin;
+  for (int b = 0; b < nbin; b++) {
+    double* acc = bins + (size_t)b * (g.N + 1);
+    for (int k = 0; k <= g.N; k++) acc[k] = 0.0;
+    for (long s = 0; s < per; s++) {
+      for (int l = 0; l < 4 * V; l++) {
+        double u[8];
+        for (int i = 0; i < ng; i++) u[i] = R.next();
+        noise_from_uniforms(g.nc, u, eta.data() + (size_t)l * ne2);
+      }
+      step(g, U, p, eta.data(), obs.data());
+      for (int k = 0; k <= g.N; k++) acc[k] += obs[2 * k];
+    }
+    for (int k = 0; k <= g.N; k++) acc[k] /= (double)per;
+  }
+}
+#endif  // !ORC_EXTENDED
+
 }  // namespace
 
+#if !ORC_EXTENDED
 // ================================================================ C ABI
 // geometry arrays are int[6] = {L0, L1, L2, L3, nc, N}; params arrays are
-// double[6] = {eps, alpha, integrator, update, gauge_fix, zero_modes}.
+// double[7] = {eps, alpha, integrator, update, gauge_fix, zero_modes, c1}.
 extern "C" {
+
+// planted mistakes (see Plant): fa variant, factor on the RK2 coefficient
+void orc_set_plant(int fa, double rk2) {
+  g_plant.fa = fa;
+  g_plant.rk2 = rk2;
+}
+
+void orc_run_stats(const int* gi, double* U, const double* pp, int seed, long nsteps, int nbin, double* bins) {
+  Params p{pp[0], pp[1], (int)pp[2], (int)pp[3], (int)pp[4], (int)pp[5], pp[6]};
+  run_stats(geom_from(gi), U, p, seed, nsteps, nbin, bins);
+}
 
 void orc_ser_mul(int nc, int N, const double* a, const double* b, double* c) {
   to_ptr(ser_mul(from_ptr(nc, N, a), from_ptr(nc, N, b)), c);
@@ -1411,3 +1489,24 @@ void orc_gstaple_sum(const int* gi, const double* U, int site, int mu, double* o
 }
 
 }  // extern "C"
+#else
+// ================================================================ extended C ABI
+// The step in long double, as the accuracy reference of FP64 steps: the field
+// (4V series, link order) comes and goes as a double-double pair, value =
+// hi + lo, so nothing of the extended result is lost on return; the noise and
+// the parameters are the double inputs a FP64 step reads.  obs may be null.
+extern "C" {
+void orcx_step(const int* gi, double* Uhi, double* Ulo, const double* pp, const double* eta, double* obs) {
+  const Geom g = geom_from(gi);
+  const size_t n = (size_t)8 * g.L[0] * g.L[1] * g.L[2] * g.L[3] * (1 + g.nc * g.nc * g.N);
+  std::vector<real> U(n);
+  for (size_t i = 0; i < n; i++) U[i] = (real)Uhi[i] + (real)Ulo[i];
+  Params p{pp[0], pp[1], (int)pp[2], (int)pp[3], (int)pp[4], (int)pp[5], pp[6]};
+  step(g, U.data(), p, eta, obs);
+  for (size_t i = 0; i < n; i++) {
+    Uhi[i] = (double)U[i];
+    Ulo[i] = (double)(U[i] - (real)Uhi[i]);
+  }
+}
+}  // extern "C"
+#endif  // ORC_EXTENDED

@@ -17,6 +17,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+EXT_PATH = os.path.join(HERE, "_build", "liboracle_ext.so")
 REF_PATH = os.path.join(HERE, "_ref", "libprlgt_ref.so")
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -67,7 +68,23 @@ def lib():
         _lib.orc_gaction.argtypes = [_ip, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         _lib.orc_gaction_grad.argtypes = [_ip, _dp, ctypes.c_int, _dp]
         _lib.orc_gstaple_sum.argtypes = [_ip, _dp, ctypes.c_int, ctypes.c_int, _dp]
+        _lib.orc_set_plant.argtypes = [ctypes.c_int, ctypes.c_double]
+        _lib.orc_run_stats.argtypes = [_ip, _dp, _dp, ctypes.c_int, ctypes.c_long, ctypes.c_int, _dp]
     return _lib
+
+
+_ext = None
+
+
+def ext_lib():
+    """The oracle built in x87 long double (64-bit significand)."""
+    global _ext
+    if _ext is None:
+        if not os.path.exists(EXT_PATH):
+            build()
+        _ext = ctypes.CDLL(EXT_PATH)
+        _ext.orcx_step.argtypes = [_ip, _dp, _dp, _dp, _dp, _dp]
+    return _ext
 
 
 def ser_len(nc, N):
@@ -148,6 +165,48 @@ def step(L, nc, N, U, eps, alpha=0.0, integrator=0, update=0, gauge_fix=0, zero_
     obs = np.zeros(obs_len(N, c1 != 0.0))
     lib().orc_step(_geom(L, nc, N), _ptr(U), pp, _ptr(eta), _ptr(obs))
     return obs
+
+
+def step_extended(L, nc, N, Uhi, Ulo, eps, alpha=0.0, integrator=0, gauge_fix=0, zero_modes=1, eta=None):
+    """One Jacobi step of the oracle in long double on the field hi + lo (two
+    complex128 arrays, updated in place): an accuracy reference for FP64
+    steps.  The noise and parameters are the double inputs; returns obs."""
+    V = L[0] * L[1] * L[2] * L[3]
+    for a in (Uhi, Ulo):
+        assert a.shape == (4 * V, ser_len(nc, N)) and a.dtype == np.complex128 and a.flags.c_contiguous
+    eta = np.ascontiguousarray(eta, dtype=np.complex128)
+    pp = (ctypes.c_double * 7)(eps, alpha, integrator, 0, gauge_fix, zero_modes, 0.0)
+    obs = np.zeros(obs_len(N))
+    ext_lib().orcx_step(_geom(L, nc, N), _ptr(Uhi), _ptr(Ulo), pp, _ptr(eta), _ptr(obs))
+    return obs
+
+
+class planted:
+    """with planted(fa=..., rk2=...): oracle calls carry a planted mistake
+    (fa 1: alpha with the wrong sign, 2: the 1/V dropped, 3: p^2 = 4 sin^2(p);
+    rk2: factor on the nc*eps/6 coefficient).  For the tests that show a pin
+    of the oracle fails on a wrong oracle; restored on exit."""
+
+    def __init__(self, fa=0, rk2=1.0):
+        self.fa, self.rk2 = fa, rk2
+
+    def __enter__(self):
+        lib().orc_set_plant(self.fa, self.rk2)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_plant(0, 1.0)
+        return False
+
+
+def run_stats(L, nc, N, U, eps, alpha, integrator, gauge_fix, zero_modes, seed, nsteps, nbin):
+    """nsteps Jacobi steps from U (in place) with ranlxd(seed) noise; returns
+    the (nbin, N+1) array of per-bin means of the real plaquette per order."""
+    assert nsteps % nbin == 0
+    pp = (ctypes.c_double * 7)(eps, alpha, integrator, 0, gauge_fix, zero_modes, 0.0)
+    bins = np.zeros((nbin, N + 1))
+    lib().orc_run_stats(_geom(L, nc, N), _ptr(U), pp, seed, nsteps, nbin, _ptr(bins))
+    return bins
 
 
 def drift(L, nc, N, U, eps, c1=0.0):

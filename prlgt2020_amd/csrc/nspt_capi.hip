@@ -159,6 +159,28 @@ __global__ void k_reduce_rows(const double* in, long ld, long in_mu, long rows, 
   out[(mu * out_mu + blockIdx.y) * out_ld + col] = s * scale;
 }
 
+// Mean over the sites of every (direction, order, element) column of a
+// link field: out[(mu*nrt + k)*NE + e] = (1/V) sum_x f[(k*NE + e)*P + mu*V + x]
+// (the layout of the zero-mode means MM).  One block per column; each
+// thread sums a fixed stride of sites, then a fixed tree: deterministic.
+constexpr int MEAN_THREADS = 256;
+__global__ __launch_bounds__(MEAN_THREADS) void k_field_mean(const double2* f, long P, int V, int nrt, int NE,
+                                                             double2* out) {
+  const int col = blockIdx.x;
+  const int e = col % NE, k = (col / NE) % nrt, mu = col / (NE * nrt);
+  const double2* src = f + ((long)k * NE + e) * P + (long)mu * V;
+  double2 s = make_double2(0.0, 0.0);
+  for (int x = threadIdx.x; x < V; x += MEAN_THREADS) s = nspt::c_add(s, src[x]);
+  __shared__ double2 red[MEAN_THREADS];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = MEAN_THREADS / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] = nspt::c_add(red[threadIdx.x], red[threadIdx.x + h]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[col] = nspt::c_scale(1.0 / (double)V, red[0]);
+}
+
 // Per-step observables from the per-mu block sums (layout per row:
 // plaq (N+1) cplx, norm N, trU N cplx), normalised as nspt.cc:570-597.
 // Advances the noise step and observable row counters of a captured step.
@@ -1133,8 +1155,16 @@ int nspt_download(nspt_ctx* c, double* aos) {
   return download_field(c, c->U, aos);
 }
 
+// A replica's step is replayed from the batch's captured graphs, which
+// track its U/F buffer roles and its observable rows: stepping it on its own
+// would desynchronise them.
+static int refuse_batch_step(nspt_ctx* c, const char* what) {
+  return c->fail(std::string(what) + ": the context is a replica of a batch (step it with nspt_batch_evolve)") - 1;
+}
+
 int nspt_set_noise(nspt_ctx* c, const double* eta) {
   if (!c || !eta) return -1;
+  if (c->in_batch) return refuse_batch_step(c, "nspt_set_noise");
   const size_t n = 4UL * c->V * c->NE;
   if (!c->hip_ok(hipMemcpyAsync(c->eta_tmp, eta, n * sizeof(double2), hipMemcpyHostToDevice, c->stream), "noise"))
     return -3;
@@ -1168,6 +1198,7 @@ static int check_params(nspt_ctx* c, const nspt_step_params* p) {
 
 int nspt_step(nspt_ctx* c, const nspt_step_params* p, double* obs) {
   if (!c) return -1;
+  if (c->in_batch) return refuse_batch_step(c, "nspt_step");
   if (check_params(c, p)) return -2;
   if (!do_step(c, p, obs ? c->obs_dev : nullptr)) return -3;
   if (obs && !c->hip_ok(hipMemcpyAsync(obs, c->obs_dev, sizeof(double) * c->obs_len, hipMemcpyDeviceToHost,
@@ -1179,6 +1210,7 @@ int nspt_step(nspt_ctx* c, const nspt_step_params* p, double* obs) {
 
 int nspt_evolve(nspt_ctx* c, nspt_step_params* p, int nsteps, double* obs) {
   if (!c) return -1;
+  if (c->in_batch) return refuse_batch_step(c, "nspt_evolve");
   if (check_params(c, p)) return -2;
   if (p->noise == NSPT_NOISE_HOST) return c->fail("nspt_evolve needs device or zero noise") - 1;
   if (nsteps < 0) return c->fail("nsteps < 0") - 1;
@@ -1310,6 +1342,41 @@ int nspt_unitarize(nspt_ctx* c) {
     return -3;
   c->ks.zero_mode(c->nblk_links, c->stream, c->g, c->L, c->MM, c->U, c->X);
   if (!launch_ok(c, "unitarize")) return -3;
+  return c->hip_ok(hipStreamSynchronize(c->stream), "sync") ? 0 : -3;
+}
+
+int nspt_gauge_wilson(nspt_ctx* c, const nspt_step_params* p, double* obs) {
+  if (!c) return -1;
+  if (!p) return c->fail("null step parameters") - 1;
+  nspt_step_params q = *p;
+  q.zero_modes = 0;
+  q.gauge_fix = NSPT_GF_NONE;
+  return nspt_step(c, &q, obs);
+}
+
+int nspt_zero_modes_subtraction(nspt_ctx* c) {
+  if (!c) return -1;
+  // MM_mu = (1/V) sum_x log U_mu(x), then U <- exp(reH(log U - MM_mu))
+  c->ks.logf(c->nblk_links, c->stream, c->g, c->U, c->L);
+  if (!launch_ok(c, "log")) return -3;
+  hipLaunchKernelGGL(k_field_mean, dim3(4 * c->nrt * c->NE), dim3(MEAN_THREADS), 0, c->stream,
+                     (const double2*)c->L.data, c->L.P, c->V, c->nrt, c->NE, reinterpret_cast<double2*>(c->MM));
+  if (!launch_ok(c, "field mean")) return -3;
+  c->ks.zero_mode(c->nblk_links, c->stream, c->g, c->L, c->MM, c->U, c->X);
+  if (!launch_ok(c, "zero_mode")) return -3;
+  return c->hip_ok(hipStreamSynchronize(c->stream), "sync") ? 0 : -3;
+}
+
+int nspt_stochastic_gauge_fixing(nspt_ctx* c, double alpha) {
+  if (!c) return -1;
+  for (int d = 0; d < 4; d++)
+    if (c->g.L[d] % 2) return c->fail("the checkerboard gauge fixing needs even lattice extents") - 1;
+  Geo gs = c->g;
+  for (int par = 0; par < 2; par++) {
+    gs.sub = 1 + 4 * par;
+    c->ks.sgf(c->g.nb_half, c->stream, gs, c->U, alpha);
+  }
+  if (!launch_ok(c, "checkerboard gauge fixing")) return -3;
   return c->hip_ok(hipStreamSynchronize(c->stream), "sync") ? 0 : -3;
 }
 

@@ -169,34 +169,105 @@ double action_c1(int action) {
 void mi355x_seed_noise(unsigned long seed) {
   if (E.rng) nspt_rng_destroy(E.rng);
   E.rng = nullptr;
-  std::srand((unsigned)seed);
-  if (nspt_rng_create_ranlxd(std::rand(), 1, &E.rng) != 0) fail("nspt_rng_create_ranlxd");
+  if (mi355x_opts.device_noise == 2) {
+    if (nspt_rng_create_rand55((long)seed, &E.rng) != 0) fail("nspt_rng_create_rand55");
+  } else {
+    std::srand((unsigned)seed);
+    if (nspt_rng_create_ranlxd(std::rand(), 1, &E.rng) != 0) fail("nspt_rng_create_ranlxd");
+  }
   mi355x_opts.noise_seed = seed;
+}
+
+namespace {
+
+// Step parameters of the current options; noise drawn on the host (and
+// uploaded) unless device noise is chosen.
+nspt_step_params step_params() {
+  const int integ[3] = {NSPT_EULER, NSPT_RK2, NSPT_EULER_CHECKERBOARD};
+  const int gfix[3] = {NSPT_GF_NONE, NSPT_GF_FA, NSPT_GF_CHECKERBOARD};
+  const bool dev = mi355x_opts.device_noise == 1;
+  return {act_pars.stau * act_pars.stau, act_pars.alpha, integ[mi355x_opts.integrator], gfix[mi355x_opts.fa],
+          mi355x_opts.zero_modes, dev ? NSPT_NOISE_DEVICE : NSPT_NOISE_HOST, mi355x_opts.noise_seed, E.step};
+}
+
+void host_noise(int V) {
+  if (!E.rng) mi355x_seed_noise(mi355x_opts.noise_seed);
+  E.eta.resize((size_t)4 * V * NC * NC * 2);
+  nspt_rng_noise(E.rng, NC, 4 * V, E.eta.data());
+  if (nspt_set_noise(E.ctx, E.eta.data()) != 0) fail("nspt_set_noise");
+}
+
+}  // namespace
+
+void gauge_wilson(ptGluon_fld& Umu, ptGluon_fld& Fmu) {
+  attach(Umu);
+  if (mi355x_opts.integrator == 2) {
+    std::fprintf(stderr, "MI355X engine: gauge_wilson(U, F) is the Jacobi update (-I 0 or 1)\n");
+    std::exit(1);
+  }
+  // Umu keeps its value: its host image is brought up to date first, then
+  // the engine's field becomes Fmu's
+  mi355x_sync_to_host(Umu);
+  nspt_step_params p = step_params();
+  if (p.noise == NSPT_NOISE_HOST) host_noise(Umu.Z->Size);
+  E.obs.resize(nspt_obs_len(E.ctx));
+  if (nspt_gauge_wilson(E.ctx, &p, E.obs.data()) != 0) fail("nspt_gauge_wilson");
+  E.step++;
+  E.field = &Fmu;
+  write_step(E.obs.data());
+}
+
+ptGluon_fld& ptGluon_fld::operator=(const ptGluon_fld& src) {
+  if (this == &src) return *this;
+  ptGluon_fld& s = const_cast<ptGluon_fld&>(src);
+  mi355x_sync_to_host(s);
+  std::memcpy(W, src.W, sizeof(ptGluon) * Z->Size);
+  if (E.ctx && E.field == &s)
+    E.field = this;  // the engine's field is this content
+  else if (E.ctx && E.field == this)
+    mi355x_sync_to_device(*this);
+  return *this;
+}
+
+void zero_modes_subtraction(ptGluon_fld& Umu) {
+  attach(Umu);
+  if (nspt_zero_modes_subtraction(E.ctx) != 0) fail("nspt_zero_modes_subtraction");
+}
+
+void stochastic_gauge_fixing(ptGluon_fld& Umu) {
+  attach(Umu);
+  if (nspt_stochastic_gauge_fixing(E.ctx, act_pars.alpha) != 0) fail("nspt_stochastic_gauge_fixing");
+}
+
+void FAstochastic_gauge_fixing(ptGluon_fld& Umu) {
+  attach(Umu);
+  if (nspt_fa_gauge_fixing(E.ctx, act_pars.alpha) != 0) fail("nspt_fa_gauge_fixing");
+}
+
+void NsptEvolve(ptGluon_fld& Umu, ptGluon_fld& Fmu) {
+  for (int t = 0; t < nspt_pars.Beat; t++) {
+    gauge_wilson(Umu, Fmu);
+    Umu = Fmu;  // nspt.cc:991-992
+    if (mi355x_opts.zero_modes) zero_modes_subtraction(Umu);
+    if (mi355x_opts.fa == 1) FAstochastic_gauge_fixing(Umu);
+    if (mi355x_opts.fa == 2) stochastic_gauge_fixing(Umu);
+  }
 }
 
 void NsptEvolve(ptGluon_fld& Umu) {
   attach(Umu);
-  const int V = Umu.Z->Size;
   const int ol = nspt_obs_len(E.ctx);
-  const double eps = act_pars.stau * act_pars.stau;
-  const int integ[3] = {NSPT_EULER, NSPT_RK2, NSPT_EULER_CHECKERBOARD};
-  const int gfix[3] = {NSPT_GF_NONE, NSPT_GF_FA, NSPT_GF_CHECKERBOARD};
-  nspt_step_params p = {eps, act_pars.alpha, integ[mi355x_opts.integrator], gfix[mi355x_opts.fa],
-                        mi355x_opts.zero_modes,
-                        mi355x_opts.device_noise ? NSPT_NOISE_DEVICE : NSPT_NOISE_HOST,
-                        mi355x_opts.noise_seed, E.step};
+  nspt_step_params p = step_params();
   const int beat = nspt_pars.Beat;
   E.obs.resize((size_t)beat * ol);
-  if (mi355x_opts.device_noise) {
+  if (p.noise == NSPT_NOISE_DEVICE) {
     if (nspt_evolve(E.ctx, &p, beat, E.obs.data()) != 0) fail("nspt_evolve");
     E.step = p.step;
   } else {
-    if (!E.rng) mi355x_seed_noise(mi355x_opts.noise_seed);
-    E.eta.resize((size_t)4 * V * NC * NC * 2);
     for (int t = 0; t < beat; t++) {
-      nspt_rng_noise(E.rng, NC, 4 * V, E.eta.data());
-      if (nspt_set_noise(E.ctx, E.eta.data()) != 0) fail("nspt_set_noise");
+      host_noise(Umu.Z->Size);
       if (nspt_step(E.ctx, &p, E.obs.data() + (size_t)t * ol) != 0) fail("nspt_step");
+      E.step++;
     }
   }
   for (int t = 0; t < beat; t++) write_step(E.obs.data() + (size_t)t * ol);

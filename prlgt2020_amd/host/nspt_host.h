@@ -8,7 +8,10 @@
 //   latt (sizes, site index)      lattice.h:37, 81-233
 //   ptGluon_fld (W, handle, get, save/load)   QCDenvNODEpt_test.h:331-426
 //   nspt_params_t, act_params_t   input.h:35-70
-//   NsptEvolve, plaquette_measure, plaquette_check   nspt.h:55-94, nspt.cc:1048-1058
+//   NsptEvolve (one-field and Jacobi two-field forms), gauge_wilson,
+//   zero_modes_subtraction, stochastic_gauge_fixing,
+//   FAstochastic_gauge_fixing, plaquette_measure, plaquette_check
+//                                 nspt.h:55-94, nspt.cc:952-1058
 //   ComputeLoops                  nspt.h:94, nspt.cc:1156-1312
 // The series types are data containers; the arithmetic of the step runs on
 // the GPU.  The field lives in GPU memory between NsptEvolve calls: the host
@@ -76,7 +79,10 @@ class ptGluon_fld {
   explicit ptGluon_fld(latt* z);
   ~ptGluon_fld();
   ptGluon_fld(const ptGluon_fld&) = delete;
-  ptGluon_fld& operator=(const ptGluon_fld&) = delete;
+  // Umu = Fmu of the Jacobi update (nspt.cc:991-992): a copy of the field
+  // (fetched from the GPU first when the engine holds it); the engine then
+  // holds this field, with the same content and no upload.
+  ptGluon_fld& operator=(const ptGluon_fld& src);
   ptSU3* handle() { return W->U; }
   ptSU3& get(int n, int mu) { return W[n].U[mu]; }
   ptGluon& get(int* n) { return W[Z->get(n)]; }
@@ -125,7 +131,8 @@ typedef struct {
   int fa;            // gauge fixing every step: 0 none, 1 Fourier accelerated,
                      // 2 sequential in checkerboard site order
   int zero_modes;    // 1: REMOVE_GLUON_ZEROMOM
-  int device_noise;  // 0: ranlxd stream of the reference (Rand[0]); 1: device Philox
+  int device_noise;  // 0: ranlxd stream of the reference (Rand[0]); 1: device Philox;
+                     // 2: RAND55 stream of the shipped build (Rand[0].init(seed))
   unsigned long long noise_seed;
   int action;        // GAUGE_ACTION (choices.h:101-113): 0 WIL, 1 TLSYM, 2 IWA, 3 DBW2
   int twist;         // TWISTED_BC (choices.h:43-55): bit d twists direction d; 0 periodic
@@ -150,11 +157,30 @@ extern Cplx* w;
 extern std::ofstream plaqfile, normfile, trufile, loop_nxm, loop2x2;
 
 // Seeds the host noise stream as the reference seeds Rand[0]: srand(seed);
-// ranlxd(rand()) (Quenched.cc:301-306).
+// ranlxd(rand()) (Quenched.cc:301-306), or, with device_noise 2, RAND55
+// Rand[0].init(seed) (Quenched.cc:296-299).
 void mi355x_seed_noise(unsigned long seed);
 // nspt_pars.Beat Langevin steps (nspt.cc:952-1039); observables of every
-// step go to plaqfile / normfile / trufile.
+// step go to plaqfile / normfile / trufile.  The field stays on the GPU for
+// the whole beat (the stages fused, nspt_step).
 void NsptEvolve(ptGluon_fld& Umu);
+// The Jacobi form (UPDATE_ONTHEFLY off): per step gauge_wilson(Umu, Fmu);
+// Umu = Fmu; zero_modes_subtraction(Umu); the gauge fixing of
+// mi355x_opts.fa (nspt.cc:952-1033), each stage a call of the engine.
+// Euler or RK2.  The results equal NsptEvolve(Umu)'s up to rounding; it
+// copies the field to the host twice per step (the two Jacobi copies).
+void NsptEvolve(ptGluon_fld& Umu, ptGluon_fld& Fmu);
+// Fmu = the Langevin update of Umu with this step's noise, Umu unchanged;
+// observables of the step to the output streams (nspt.cc:258-600).
+void gauge_wilson(ptGluon_fld& Umu, ptGluon_fld& Fmu);
+// U <- exp(reH(log U - MM_mu)), MM_mu the mean of log U_mu (nspt.cc:605-654).
+void zero_modes_subtraction(ptGluon_fld& Umu);
+// The sequential gauge fixing with alpha = act_pars.alpha (nspt.cc:660-862),
+// sites of parity 0, then 1.
+void stochastic_gauge_fixing(ptGluon_fld& Umu);
+// The Fourier-accelerated gauge fixing with alpha = act_pars.alpha
+// (nspt.cc:866-948).
+void FAstochastic_gauge_fixing(ptGluon_fld& Umu);
 // Plaquette per order into w (nspt.cc:1316-1406); refreshes the host image.
 void plaquette_measure(ptGluon_fld& Umu, act_params_t& act);
 // 0 if equal to 1e-7 per order (nspt.cc:1048-1058).

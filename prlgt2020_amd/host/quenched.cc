@@ -11,7 +11,9 @@
 // integrator) or 2 (the on-the-fly sweep in checkerboard order), -F 0 (no
 // gauge fixing) or 2 (the sequential gauge fixing in checkerboard order),
 // -Z 0 (keep zero modes), -D 1 (device noise instead of the reference's
-// ranlxd stream), -A (improved gauge action), -L (restart from a
+// ranlxd stream) or 2 (the RAND55 stream of the shipped build, USE_RAND55),
+// -J 1 (the Jacobi two-field NsptEvolve(Umu, Fmu), stage by stage),
+// -A (improved gauge action), -L (restart from a
 // configuration of fewer orders), -T (twisted boundary conditions), -W 1
 // (the 2x2 Wilson loop file).  -I 2 -F 2 with the shipped Quench.cfg
 // (alpha -0.05) is the reference's default build, reordered.
@@ -36,6 +38,7 @@ std::ofstream logfile;
 int nInt = 0;
 int low_order = 0;      // -L: start from a configuration saved with this many orders
 std::string low_confn;  // its file name
+bool two_field = false;  // -J 1: NsptEvolve(Umu, Fmu), the Jacobi stage sequence
 int config_counter = 0;
 Cplx* w1 = nullptr;
 
@@ -105,7 +108,7 @@ int initialize(int argc, char** argv) {
   std::strcpy(nspt_pars.damon, "damocle.dag");
 
   int opt;
-  while ((opt = getopt(argc, argv, "l:s:b:d:r:g:a:t:p:c:n:I:F:Z:D:L:A:T:W:h")) != -1) {
+  while ((opt = getopt(argc, argv, "l:s:b:d:r:g:a:t:p:c:n:I:F:Z:D:J:L:A:T:W:h")) != -1) {
     switch (opt) {
       case 'l': act_pars.sz[0] = act_pars.sz[1] = act_pars.sz[2] = act_pars.sz[3] = std::atoi(optarg); break;
       case 's': nspt_pars.Sweep = std::atoi(optarg); break;
@@ -122,6 +125,7 @@ int initialize(int argc, char** argv) {
       case 'F': mi355x_opts.fa = std::atoi(optarg); break;
       case 'Z': mi355x_opts.zero_modes = std::atoi(optarg); break;
       case 'D': mi355x_opts.device_noise = std::atoi(optarg); break;
+      case 'J': two_field = std::atoi(optarg) != 0; break;
       case 'L': low_order = std::atoi(optarg); break;
       case 'A': mi355x_opts.action = std::atoi(optarg); break;
       case 'T': mi355x_opts.twist = std::atoi(optarg); break;
@@ -133,7 +137,8 @@ int initialize(int argc, char** argv) {
                     "-p\t\tplaquette file name\n-c\t\tsaved configuration name\n-d\t\tdamocle file name\n"
                     "-n\t\tperturbative order\n-I\t\tintegrator: 0 Euler, 1 RK2, 2 Euler on the fly (checkerboard)\n"
                     "-F\t\tgauge fixing: 0 off, 1 Fourier accelerated, 2 sequential (checkerboard)\n"
-                    "-Z\t\tzero-mode subtraction: 1 on, 0 off\n-D\t\tnoise: 0 host ranlxd, 1 device Philox\n"
+                    "-Z\t\tzero-mode subtraction: 1 on, 0 off\n-D\t\tnoise: 0 host ranlxd, 1 device Philox, 2 host RAND55\n"
+                    "-J\t\t1: Jacobi two-field NsptEvolve(Umu, Fmu), stage by stage\n"
                     "-L\t\twith -t 1: start from the configuration saved with this lower allocORD\n"
                     "-A\t\tgauge action: 0 Wilson, 1 TLSYM, 2 IWA, 3 DBW2\n"
                     "-T\t\ttwisted boundary conditions: bit 0 x, 1 y, 2 z, 3 t (6: y and z)\n"
@@ -154,7 +159,15 @@ int initialize(int argc, char** argv) {
     std::fclose(df);
   }
   mi355x_opts.noise_seed = seed0;
-  if (!mi355x_opts.device_noise) mi355x_seed_noise(seed0);
+  if (mi355x_opts.device_noise < 0 || mi355x_opts.device_noise > 2) {
+    std::fprintf(stderr, "-D takes 0, 1 or 2\n");
+    return 1;
+  }
+  if (two_field && mi355x_opts.integrator == 2) {
+    std::fprintf(stderr, "-J 1 is the Jacobi update: -I 0 or 1\n");
+    return 1;
+  }
+  if (mi355x_opts.device_noise != 1) mi355x_seed_noise(seed0);
   if (mi355x_opts.integrator < 0 || mi355x_opts.integrator > 2 || mi355x_opts.fa < 0 || mi355x_opts.fa > 2) {
     std::fprintf(stderr, "-I takes 0, 1 or 2 and -F 0, 1 or 2\n");
     return 1;
@@ -240,7 +253,10 @@ int initialize(int argc, char** argv) {
           << "Gauge fixing\t"
           << (mi355x_opts.fa == 1 ? "Fourier accelerated" : mi355x_opts.fa == 2 ? "sequential (checkerboard)" : "none")
           << std::endl
-          << "Noise\t\t" << (mi355x_opts.device_noise ? "device Philox4x32-10" : "ranlxd (Rand[0])") << std::endl
+          << "Noise\t\t"
+          << (mi355x_opts.device_noise == 1 ? "device Philox4x32-10"
+                                             : mi355x_opts.device_noise == 2 ? "RAND55 (Rand[0])" : "ranlxd (Rand[0])")
+          << std::endl
           << std::endl;
   return 0;
 }
@@ -337,11 +353,15 @@ int main(int argc, char** argv) {
   if (initialize(argc, argv)) return 1;
   latt LL(act_pars.sz);
   ptGluon_fld Umu(&LL);
+  ptGluon_fld Fmu(&LL);  // the Jacobi form's second field (Quenched.cc:633-636)
   if (allocate(Umu)) return 1;
   int t1 = 0;
   while (t1 < nspt_pars.Sweep) {
     t1 += nspt_pars.Beat;
-    NsptEvolve(Umu);
+    if (two_field)
+      NsptEvolve(Umu, Fmu);
+    else
+      NsptEvolve(Umu);
     if (update_parameters(Umu)) return 0;
   }
   finalize(Umu, t1);

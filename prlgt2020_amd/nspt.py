@@ -64,7 +64,8 @@ EXPORTS = ["nspt_create", "nspt_destroy", "nspt_last_error", "nspt_obs_len", "ns
            "nspt_rng_uniform", "nspt_rng_noise", "nspt_device_noise", "nspt_profile", "nspt_profile_read",
            "nspt_batch_create", "nspt_batch_destroy", "nspt_batch_last_error", "nspt_batch_evolve",
            "nspt_batch_replica", "nspt_batch_tune", "nspt_adjoint_drift", "nspt_loops",
-           "nspt_unitarize", "nspt_set_action", "nspt_set_twist"]
+           "nspt_unitarize", "nspt_set_action", "nspt_set_twist", "nspt_gauge_wilson",
+           "nspt_zero_modes_subtraction", "nspt_stochastic_gauge_fixing", "nspt_rng_create_rand55"]
 
 PROF_NAMES = ["drift", "update", "update_log", "zero_mode", "fa_div", "dft", "fa_gexp", "transform", "noise",
               "reduce", "rect", "plaq_field"]
@@ -89,6 +90,10 @@ def lib():
         l.nspt_fa_gauge_fixing.argtypes = [ctypes.c_void_p, ctypes.c_double]
         l.nspt_drift.argtypes = [ctypes.c_void_p, ctypes.c_double, _dp]
         l.nspt_rng_create_ranlxd.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+        l.nspt_rng_create_rand55.argtypes = [ctypes.c_long, ctypes.POINTER(ctypes.c_void_p)]
+        l.nspt_gauge_wilson.argtypes = [ctypes.c_void_p, ctypes.POINTER(StepParams), _dp]
+        l.nspt_zero_modes_subtraction.argtypes = [ctypes.c_void_p]
+        l.nspt_stochastic_gauge_fixing.argtypes = [ctypes.c_void_p, ctypes.c_double]
         l.nspt_rng_destroy.argtypes = [ctypes.c_void_p]
         l.nspt_rng_uniform.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
         l.nspt_rng_noise.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]
@@ -136,6 +141,19 @@ class Ranlxd:
         if getattr(self, "h", None) is not None and _lib is not None:
             _lib.nspt_rng_destroy(self.h)
             self.h = None
+
+
+class Rand55(Ranlxd):
+    """Host noise source of the reference's shipped build: RAND55 (MyRand,
+    MyRand.cc:30-130) seeded as Rand[0].init(seed), su(N) noise of SU3rand /
+    SU2rand (MyMath.cc:256-292, 326-346)."""
+
+    def __init__(self, seed):
+        h = ctypes.c_void_p()
+        rc = lib().nspt_rng_create_rand55(seed, ctypes.byref(h))
+        if rc != 0:
+            raise NsptError("nspt_rng_create_rand55 failed (%d)" % rc)
+        self.h = h
 
 
 class Context:
@@ -241,6 +259,20 @@ class Context:
 
     def fa_gauge_fixing(self, alpha):
         self._check(lib().nspt_fa_gauge_fixing(self.h, alpha), "fa_gauge_fixing")
+
+    def gauge_wilson(self, params, want_obs=True):
+        """The Langevin update stage alone (no zero modes, no gauge fixing)."""
+        obs = np.zeros(self.obs_len) if want_obs else None
+        self._check(lib().nspt_gauge_wilson(self.h, ctypes.byref(params), _ptr(obs) if want_obs else None),
+                    "gauge_wilson")
+        return obs
+
+    def zero_modes_subtraction(self):
+        self._check(lib().nspt_zero_modes_subtraction(self.h), "zero_modes_subtraction")
+
+    def stochastic_gauge_fixing(self, alpha):
+        """The sequential gauge fixing in checkerboard site order."""
+        self._check(lib().nspt_stochastic_gauge_fixing(self.h, alpha), "stochastic_gauge_fixing")
 
     def drift(self, eps):
         D = np.zeros((4 * self.V, self.S), np.complex128)

@@ -75,27 +75,33 @@ def _read_cols(path, ncols):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("nc", [3, 2])
-def test_driver_run_matches_oracle(tmp_path, nc):
-    """20 Euler steps (Jacobi, zero modes, FA alpha 0.05) of a cold 4^4 lattice
-    at N = 6 with the reference's noise stream (srand(1); ranlxd(rand()))."""
+@pytest.mark.parametrize("nc,steps,opts", [(3, 100, ()), (2, 20, ()), (3, 20, ("-J", "1")),
+                                           (3, 20, ("-D", "2")), (2, 20, ("-J", "1", "-I", "1", "-D", "2"))])
+def test_driver_run_matches_oracle(tmp_path, nc, steps, opts):
+    """Euler steps (Jacobi, zero modes, FA alpha 0.05) of a cold 4^4 lattice at
+    N = 6 with the reference's noise stream (srand(1); ranlxd(rand())), or
+    (-D 2) the shipped build's RAND55 stream (Rand[0].init(1)).  SU(3) over
+    100 steps is the CPU-runnable workload (4^4, 6 orders, eps 0.05, seed 1)
+    in full.  -J 1: the Jacobi two-field NsptEvolve(Umu, Fmu), stage by
+    stage; -I 1: RK2."""
     from oracle import oracle as O
-    L, N, eps, alpha, steps, beat = 4, 6, 0.05, 0.05, 20, 10
+    L, N, eps, alpha, beat = 4, 6, 0.05, 0.05, 10
     write_inputs(tmp_path, L=L, sweep=steps, beat=beat, tau=eps, alpha=alpha, ptord=N, seed=1)
-    print("driver run: SU(%d) 4^4 N=6 Euler FA, %d steps" % (nc, steps), flush=True)
-    out = subprocess.run([driver(nc, 6)], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    print("driver run: SU(%d) 4^4 N=6 FA, %d steps, options %s" % (nc, steps, " ".join(opts)), flush=True)
+    out = subprocess.run([driver(nc, 6)] + list(opts), cwd=tmp_path, capture_output=True, text=True, timeout=300)
     assert out.returncode == 0, out.stdout + out.stderr
     n = names(L, alpha, eps, N, 6, nc)
+    integrator = 1 if "-I" in opts else 0
 
     # oracle with the same noise
     Lt = (L, L, L, L)
     V = L ** 4
     U = O.cold_field(Lt, nc, N)
-    rng = O.Ranlxd(O.glibc_first_rand(1))
+    rng = O.Rand55(1) if "-D" in opts else O.Ranlxd(O.glibc_first_rand(1))
     plaq, nor, tru, nxm = [], [], [], []
     for t in range(steps):
         eta = rng.noise(nc, 4 * V)
-        obs = O.step(Lt, nc, N, U, eps, alpha, integrator=0, update=0, gauge_fix=1, zero_modes=1, eta=eta)
+        obs = O.step(Lt, nc, N, U, eps, alpha, integrator=integrator, update=0, gauge_fix=1, zero_modes=1, eta=eta)
         p, nr, tr = O.split_obs(obs, N)
         plaq.append(np.stack([p.real, p.imag], 1))
         nor.append(nr)
@@ -127,7 +133,7 @@ def test_driver_run_matches_oracle(tmp_path, nc):
     # checkpoints: after each beat (save_config 1) and at the end, ptGluon[V] + plaquette
     S = 1 + nc * nc * 6
     size = 4 * V * S * 16 + 7 * 16
-    for f in (n["conf"] + "_1.dat", n["conf"] + "_2.dat", n["conf"]):
+    for f in [n["conf"] + "_%d.dat" % (i + 1) for i in range(steps // beat)] + [n["conf"]]:
         assert os.path.getsize(tmp_path / f) == size, f
     raw = np.fromfile(tmp_path / n["conf"], dtype=np.complex128)
     field = raw[:4 * V * S].reshape(4 * V, S)

@@ -90,3 +90,28 @@ def test_batch_replicas_refuse_action_and_twist():
     with pytest.raises(nspt.NsptError, match="periodic"):
         rep.set_twist(0b0110)
     b.close()
+
+
+def test_batch_replicas_refuse_their_own_steps():
+    """A replica steps only through the batch: its captured graphs track its
+    U/F buffer roles and observable rows.  The run goes on unharmed."""
+    from prlgt2020_amd import nspt
+    Ls, ps = _replicas(nspt)
+    b = nspt.Batch(Ls[:1], 3, 4, ps[:1])
+    b.evolve(2)
+    rep = b.replicas[0]
+    p = ps[0]
+    q = nspt.Context.params(p.eps, p.alpha, p.integrator, p.gauge_fix, p.zero_modes, p.noise, p.seed, 100)
+    for call in (lambda: rep.step(q), lambda: rep.evolve(q, 1), lambda: rep.gauge_wilson(q),
+                 lambda: rep.set_noise(np.zeros((4 * rep.V, 9), np.complex128))):
+        with pytest.raises(nspt.NsptError, match="replica of a batch"):
+            call()
+    obs = b.evolve(1)
+    ref = nspt.Context(Ls[0], 3, 4)
+    q0 = nspt.Context.params(p.eps, p.alpha, p.integrator, p.gauge_fix, p.zero_modes, p.noise, p.seed, p.step)
+    ref.evolve(q0, 2, want_obs=False)
+    o = ref.evolve(q0, 1)
+    assert np.array_equal(b.replicas[0].download(), ref.download())
+    assert np.array_equal(obs[0], o)
+    ref.close()
+    b.close()

@@ -132,48 +132,16 @@ def test_orders_evolved_field(nc, N, relerr):
     _run_pair(L, nc, N, 1, 1, 1, eps=0.02, relerr=relerr, U=U)
 
 
-def _conditioning_envelope(L, nc, N, U, eta, trials=3):
-    """Per-order spread of the oracle's own step when its input is perturbed
-    by 1e-16 relative (one rounding of every input coefficient)."""
-    rng = np.random.default_rng(99)
-    U1 = U.copy()
-    O.step(L, nc, N, U1, 0.02, 0.05, integrator=1, update=0, gauge_fix=1, zero_modes=1, eta=eta)
-    from tests.conftest import rel_err_per_order
-    env = np.zeros(N + 1)
-    for _ in range(trials):
-        Up = U * (1 + 1e-16 * (rng.normal(size=U.shape) + 1j * rng.normal(size=U.shape)))
-        O.step(L, nc, N, Up, 0.02, 0.05, integrator=1, update=0, gauge_fix=1, zero_modes=1, eta=eta)
-        env = np.maximum(env, rel_err_per_order(Up, U1, nc, N))
-    return U1, env
-
-
-@pytest.mark.parametrize("nc,N", [(3, 20), (3, 24), (3, 32), (2, 20), (2, 24)])
-def test_high_orders_within_fp64_conditioning(nc, N, relerr):
-    """Orders above ~16: one step is ill-conditioned in FP64 (a 1e-16
-    relative perturbation of the input moves the oracle's own order-20 output
-    by ~1e-11), so the bar per order is max(1e-12, 2 x that spread)."""
-    nspt = _nspt()
-    L = (2, 2, 2, 4)
-    U = evolved_field(L, nc, N)
-    eta = nspt.Ranlxd(4321).noise(nc, 4 * 32)
-    Uo, env = _conditioning_envelope(L, nc, N, U, eta)
-    ctx = nspt.Context(L, nc, N)
-    ctx.upload(U)
-    ctx.set_noise(eta)
-    ctx.step(nspt.Context.params(0.02, 0.05, nspt.RK2, nspt.GF_FA, 1, nspt.NOISE_HOST))
-    e = relerr(ctx.download(), Uo, nc, N)
-    bar = np.maximum(1e-12, 2 * env)
-    print("err", e, "\nbar", bar)
-    assert (e <= bar).all(), (e, bar)
-
-
 @pytest.mark.parametrize("L", [(4, 2, 4, 2), (6, 3, 2, 5), (3, 4, 5, 2), (2, 6, 2, 6), (7, 2, 3, 7), (2, 2, 32, 2),
-                               (2, 8, 2, 12), (2, 2, 2, 32), (5, 2, 3, 24)])
+                               (2, 8, 2, 12), (2, 2, 2, 32), (5, 2, 3, 24), (16, 2, 2, 16), (2, 16, 16, 2),
+                               (2, 16, 2, 10), (16, 2, 20, 2), (2, 2, 16, 2), (10, 20, 2, 2), (4, 4, 4, 16)])
 def test_fa_alone(relerr, L):
-    """FA gauge fixing: the unrolled line FFTs (extents 2..6, 8, 12, 24, 32;
-    the 32-site line runs in the wide kernel variant), the direct DFT of
-    other extents (7), and the batch of consecutive sites in the middle pass
-    (up to 32 of them: its tile rows hold L_1 * C + 1 sites)."""
+    """FA gauge fixing: the unrolled line FFTs (extents 2..6, 8, 10, 12, 16,
+    20, 24, 32; 16 is the only line length of a 16^4 lattice, and every
+    extent here appears on both tile axes of passes 0 and 1; the 32-site
+    line runs in the wide kernel variant), the direct DFT of other extents
+    (7), and the batch of consecutive sites in the middle pass (up to 32 of
+    them: its tile rows hold L_1 * C + 1 sites)."""
     nspt = _nspt()
     nc, N = 3, 6
     U = O.random_field(L, nc, N, seed=21)
@@ -183,6 +151,74 @@ def test_fa_alone(relerr, L):
     Uo = U.copy()
     O.fa_gauge_fixing(L, nc, N, Uo, 0.05)
     assert relerr(ctx.download(), Uo, nc, N).max() <= TOL
+
+
+@pytest.mark.parametrize("nc,integrator", [(3, 0), (3, 1), (2, 1)])
+def test_stage_entry_points_match_oracle(nc, integrator, relerr):
+    """gauge_wilson, zero_modes_subtraction and FAstochastic_gauge_fixing
+    called one by one (the reference's stage sequence, nspt.cc:963-1033) =
+    one oracle step, fields and observables, two steps."""
+    nspt = _nspt()
+    L, N = (2, 4, 2, 4), 6
+    V = 64
+    U = O.random_field(L, nc, N, seed=8, scale=0.4)
+    ctx = nspt.Context(L, nc, N)
+    ctx.upload(U)
+    rng = nspt.Ranlxd(99)
+    Uo = U.copy()
+    for s in range(2):
+        eta = rng.noise(nc, 4 * V)
+        ctx.set_noise(eta)
+        obs_g = ctx.gauge_wilson(nspt.Context.params(0.02, 0.05, integrator, nspt.GF_FA, 1, nspt.NOISE_HOST))
+        ctx.zero_modes_subtraction()
+        ctx.fa_gauge_fixing(0.05)
+        obs_o = O.step(L, nc, N, Uo, 0.02, 0.05, integrator=integrator, update=0, gauge_fix=1, zero_modes=1,
+                       eta=eta)
+        e = relerr(ctx.download(), Uo, nc, N)
+        print("step", s, "per-order rel err", e)
+        assert e.max() <= TOL, e
+        assert np.abs(obs_g - obs_o).max() <= TOL * max(1.0, np.abs(obs_o).max())
+    ctx.close()
+
+
+@pytest.mark.parametrize("nc", [3, 2])
+def test_zero_modes_subtraction_alone(nc, relerr):
+    """U <- exp(reH(log U - MM)), MM the per-direction mean of log U
+    (nspt.cc:605-654): against the oracle's zero_modes with MM = sum_x log U."""
+    nspt = _nspt()
+    L, N = (2, 2, 4, 6), 6
+    V = 96
+    U = O.random_field(L, nc, N, seed=12, scale=0.4)
+    MM = np.zeros((4, O.ser_len(nc, N)), np.complex128)
+    for l in range(4 * V):
+        MM[l % 4] += O.log(nc, N, U[l])
+    Uo = U.copy()
+    O.zero_modes(L, nc, N, Uo, MM)
+    ctx = nspt.Context(L, nc, N)
+    ctx.upload(U)
+    ctx.zero_modes_subtraction()
+    assert relerr(ctx.download(), Uo, nc, N).max() <= TOL
+    ctx.close()
+
+
+@pytest.mark.parametrize("nc", [3, 2])
+def test_stochastic_gauge_fixing_alone(nc, relerr):
+    """The sequential gauge fixing (nspt.cc:660-862) in checkerboard site
+    order, as the reference's alpha -0.05, against the oracle."""
+    nspt = _nspt()
+    L, N = (2, 4, 2, 4), 6
+    U = O.random_field(L, nc, N, seed=13, scale=0.4)
+    Uo = U.copy()
+    O.sgf_checkerboard(L, nc, N, Uo, -0.05)
+    ctx = nspt.Context(L, nc, N)
+    ctx.upload(U)
+    ctx.stochastic_gauge_fixing(-0.05)
+    assert relerr(ctx.download(), Uo, nc, N).max() <= TOL
+    ctx.close()
+    odd = nspt.Context((2, 3, 2, 2), nc, 2)
+    with pytest.raises(nspt.NsptError, match="even lattice extents"):
+        odd.stochastic_gauge_fixing(-0.05)
+    odd.close()
 
 
 @pytest.mark.parametrize("nc", [3, 2])

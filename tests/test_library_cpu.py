@@ -71,6 +71,30 @@ def test_host_noise_matches_reference_golden():
     assert np.allclose(got, want.reshape(got.shape), rtol=0, atol=0)
 
 
+def test_host_rand55_matches_reference_golden():
+    """RAND55 of the reference's shipped build (MyRand.cc:30-130), seeded as
+    Rand[0].init(1): uniforms, and the su(3) / su(2) noise of SU3rand /
+    SU2rand (MyMath.cc:256-292, 326-346), bit for bit against the vectors the
+    reference's own sources wrote."""
+    from prlgt2020_amd import nspt
+    g = np.load(os.path.join(ROOT, "tests", "golden", "rng.npz"))
+    assert np.array_equal(nspt.Rand55(1).uniform(3000), g["rand55_ranf_seed1"])
+    for nc, key in ((3, "su3rand_rand55_seed1"), (2, "su2rand_rand55_seed1")):
+        want = g[key]
+        got = nspt.Rand55(1).noise(nc, want.shape[0])
+        assert np.array_equal(got, want.reshape(got.shape)), nc
+
+
+def test_host_rand55_matches_oracle():
+    """Longer streams and a large seed (the reference's signed seeding sum
+    wraps) against the oracle's restatement."""
+    from oracle import oracle as O
+    from prlgt2020_amd import nspt
+    for seed in (7, 2682328493):
+        assert np.array_equal(nspt.Rand55(seed).noise(3, 900), O.Rand55(seed).noise(3, 900))
+        assert np.array_equal(nspt.Rand55(seed).noise(2, 900), O.Rand55(seed).noise(2, 900))
+
+
 def test_package_fails_loudly_without_library(tmp_path):
     code = ("import sys; sys.path.insert(0, %r)\n"
             "from prlgt2020_amd import nspt\n"
