@@ -257,6 +257,11 @@ struct TwistState {
 };
 TwistState g_tw;
 
+// Loops over links or sites run in parallel (OpenMP) on lattices large
+// enough to pay for it; every sum is still formed in the serial order, so
+// the results do not depend on it.
+bool g_par = false;
+
 // ---------------------------------------------------------------- lattice
 struct Latt {
   int L[4];
@@ -293,6 +298,7 @@ struct Latt {
     tw = g_tw.mask;
     for (int d = 0; d < 4; d++) L[d] = sz[d];
     V = L[0] * L[1] * L[2] * L[3];
+    g_par = V >= 4096;
     up.resize(4 * V);
     dn.resize(4 * V);
     for (int s = 0; s < V; s++) {
@@ -436,24 +442,53 @@ struct Obs {
     w2.assign(N + 1, cx{0, 0});
     improved = impr;
   }
-  void accumulate2(const Ser& W2) {
-    std::vector<cx> w(W2.N + 1);
-    ser_tr(W2, w.data());
-    for (int k = 0; k <= W2.N; k++) w2[k] += w[k];
+  // One link's terms (w: Tr W per order; nrm, tr: Re Tr(U_k U_k^dag) and
+  // Tr U_k; w2: Tr W2 of an improved action), recorded by a parallel loop
+  // and added in link order afterwards, so that every sum runs in the
+  // serial order.
+  struct Rec {
+    std::vector<cx> w, tr, w2;
+    std::vector<real> nrm;
+    int mu = 0;
+  };
+  static void record2(const Ser& W2, Rec& r) {
+    r.w2.assign(W2.N + 1, cx{0, 0});
+    ser_tr(W2, r.w2.data());
   }
-  void accumulate(const Ser& U, const Ser& W, int mu) {
+  static void record(const Ser& U, const Ser& W, int mu, Rec& r) {
     const int N = U.N, nc = U.nc;
-    std::vector<cx> w(N + 1);
-    ser_tr(W, w.data());
-    w1[0] += w[0];
+    r.mu = mu;
+    r.w.assign(N + 1, cx{0, 0});
+    ser_tr(W, r.w.data());
+    r.nrm.assign(N, 0);
+    r.tr.assign(N, cx{0, 0});
     for (int k = 1; k <= N; k++) {
-      w1[k] += w[k];
       cx d[9] = {}, p[9] = {};
       mat_dag(nc, U.ord(k), d);
       mat_mul(nc, U.ord(k), d, p);
-      norm[k - 1] += mat_tr(nc, p).re;
-      trU[mu * N + (k - 1)] += mat_tr(nc, U.ord(k));
+      r.nrm[k - 1] = mat_tr(nc, p).re;
+      r.tr[k - 1] = mat_tr(nc, U.ord(k));
     }
+  }
+  void add(const Rec& r) {
+    const int N = (int)norm.size();
+    for (int k = 0; k <= N; k++) w1[k] += r.w[k];
+    for (int k = 0; k < N; k++) {
+      norm[k] += r.nrm[k];
+      trU[r.mu * N + k] += r.tr[k];
+    }
+    if (!r.w2.empty())
+      for (int k = 0; k <= N; k++) w2[k] += r.w2[k];
+  }
+  void accumulate2(const Ser& W2) {
+    Rec r;
+    record2(W2, r);
+    for (int k = 0; k <= W2.N; k++) w2[k] += r.w2[k];
+  }
+  void accumulate(const Ser& U, const Ser& W, int mu) {
+    Rec r;
+    record(U, W, mu, r);
+    add(r);
   }
   // out: plaq[2(N+1)], norm[N], trU[2*4N] normalised as written to file
   void write(int nc, int V, double* out) const {
@@ -486,16 +521,18 @@ struct Obs {
 // (nspt.cc:383-417, Quenched.cc:584-589).  Optionally accumulates observables.
 // c1 != 0: improved action (nspt.cc:385-412, Quenched.cc:313-322),
 // W = tau_g reH(c0 U Sigma + c1 U Sigma_2x1) with c0 = 1 - 8 c1.
-Ser drift(const Field& F, int n, int mu, real tau_g, Obs* obs, double c1 = 0.0) {
+Ser drift(const Field& F, int n, int mu, real tau_g, Obs* obs, double c1 = 0.0, Obs::Rec* rec = nullptr) {
   Ser U = F.get(n, mu);
   Ser W = ser_mul(U, staple_sum(F, n, mu));
   if (obs) obs->accumulate(U, W, mu);
+  if (rec) Obs::record(U, W, mu, *rec);
   if (c1 != 0.0) {
     Ser R(F.nc, F.N);
     for (int nu = 0; nu < 4; nu++)
       if (nu != mu) R = ser_add(R, staple2x1(F, n, mu, nu));
     Ser W2 = ser_mul(U, R);
     if (obs) obs->accumulate2(W2);
+    if (rec) Obs::record2(W2, *rec);
     ser_scale(W, 1.0 - 8.0 * c1);
     ser_scale(W2, c1);
     W = ser_add(W, W2);
@@ -520,7 +557,6 @@ void sub_noise(Ser& W, real stau, const double* eta) {
 void dft_many(const int* L, int howmany, int stride, cx* data, int sign) {
   const int V = L[0] * L[1] * L[2] * L[3];
   const int sstride[4] = {L[1] * L[2] * L[3], L[2] * L[3], L[3], 1};
-  std::vector<cx> line, out;
   for (int d = 0; d < 4; d++) {
     const int n = L[d];
     std::vector<cx> tw(n);
@@ -528,11 +564,11 @@ void dft_many(const int* L, int howmany, int stride, cx* data, int sign) {
       const real ang = 2 * kPi * j / n;
       tw[j] = cx{std::cos(ang), sign * std::sin(ang)};
     }
-    line.resize(n);
-    out.resize(n);
+#pragma omp parallel for schedule(dynamic, 8) if (g_par)
     for (int s = 0; s < V; s++) {
       // visit each line once: sites with coordinate 0 along d
       if ((s / sstride[d]) % n != 0) continue;
+      std::vector<cx> line(n), out(n);
       for (int comp = 0; comp < howmany; comp++) {
         for (int j = 0; j < n; j++) line[j] = data[(size_t)(s + j * sstride[d]) * stride + comp];
         for (int k = 0; k < n; k++) {
@@ -918,18 +954,20 @@ struct Geom {
 void zero_modes(Field& F, std::vector<Ser>& MM) {
   const Latt& Z = *F.lat;
   for (int mu = 0; mu < 4; mu++) ser_scale(MM[mu], real(1) / Z.V);  // MM *= rVol
-  for (int s = 0; s < Z.V; s++)
-    for (int mu = 0; mu < 4; mu++) {
-      Ser W = ser_sub(ser_log(F.get(s, mu)), MM[mu]);
-      ser_reH(W);
-      F.set(s, mu, ser_exp(W));
-    }
+#pragma omp parallel for schedule(dynamic, 64) if (g_par)
+  for (long l = 0; l < 4L * Z.V; l++) {
+    const int s = (int)(l / 4), mu = (int)(l % 4);
+    Ser W = ser_sub(ser_log(F.get(s, mu)), MM[mu]);
+    ser_reH(W);
+    F.set(s, mu, ser_exp(W));
+  }
 }
 
 void fa_gauge_fixing(Field& F, double alpha) {
   const Latt& Z = *F.lat;
   const int S = F.S, nc = F.nc, N = F.N;
   std::vector<cx> Wg((size_t)Z.V * S, cx{0, 0});
+#pragma omp parallel for schedule(dynamic, 16) if (g_par)
   for (int x = 0; x < Z.V; x++) {
     Ser W(nc, N);
     for (int mu = 0; mu < 4; mu++) {
@@ -951,6 +989,7 @@ void fa_gauge_fixing(Field& F, double alpha) {
       p2[d][i] = h;  // lattice.h:1144-1168
     }
   }
+#pragma omp parallel for schedule(static) if (g_par)
   for (int x = 1; x < Z.V; x++) {
     int c[4];
     Z.coords(x, c);
@@ -960,6 +999,7 @@ void fa_gauge_fixing(Field& F, double alpha) {
   }
   dft_many(Z.L, S, S, Wg.data(), +1);
   std::vector<Ser> G(Z.V);
+#pragma omp parallel for schedule(dynamic, 16) if (g_par)
   for (int x = 0; x < Z.V; x++) {
     Ser W(nc, N);
     std::memcpy(W.v.data(), &Wg[(size_t)x * S], sizeof(cx) * S);
@@ -967,9 +1007,11 @@ void fa_gauge_fixing(Field& F, double alpha) {
     ser_reH(W);
     G[x] = ser_exp(W);
   }
-  for (int x = 0; x < Z.V; x++)
-    for (int mu = 0; mu < 4; mu++)
-      F.set(x, mu, ser_mul(ser_mul(G[x], F.get(x, mu)), ser_dag(G[Z.up[x * 4 + mu]])));
+#pragma omp parallel for schedule(dynamic, 64) if (g_par)
+  for (long l = 0; l < 4L * Z.V; l++) {
+    const int x = (int)(l / 4), mu = (int)(l % 4);
+    F.set(x, mu, ser_mul(ser_mul(G[x], F.get(x, mu)), ser_dag(G[Z.up[x * 4 + mu]])));
+  }
 }
 
 // Default (non-FA) stochastic gauge fixing: sequential and in place
@@ -1079,40 +1121,48 @@ void step(const Geom& g, real* Udata, const Params& p, const double* eta, double
         MM[mu] = ser_add(MM[mu], ser_log(Un));
       }
   } else {
+    // Jacobi: every link from the frozen field, links in parallel; the
+    // observable and zero-mode sums are then formed in link order
+    const long nl = 4L * Z.V;
     std::vector<real> Fbuf((size_t)8 * Z.V * S);
     Field Fn{g.nc, g.N, S, &Z, Fbuf.data()};
-    std::vector<Ser> D1;
+    std::vector<Ser> D1, lg(nl);
+    std::vector<Obs::Rec> rec(obs_out ? nl : 0);
     if (p.integrator == 1) {
       // predictor U1 = exp(D(U) - sqrt(eps) eta) U
-      D1.resize(4 * Z.V);
-      for (int s = 0; s < Z.V; s++)
-        for (int mu = 0; mu < 4; mu++) {
-          D1[4 * s + mu] = drift(U, s, mu, tau_g, &obs, p.c1);
-          Ser W = D1[4 * s + mu];
-          sub_noise(W, stau, eta + (size_t)(4 * s + mu) * ne2);
-          Fn.set(s, mu, ser_mul(ser_exp(W), U.get(s, mu)));
-        }
-    }
-    std::vector<real> Gbuf;
-    for (int s = 0; s < Z.V; s++)
-      for (int mu = 0; mu < 4; mu++) {
-        Ser W;
-        if (p.integrator == 1) {
-          // corrector: drift at U1, combined with D(U); applied to U
-          W = rk2_combine(D1[4 * s + mu], drift(Fn, s, mu, tau_g, nullptr, p.c1), g.nc, p.eps);
-        } else {
-          W = drift(U, s, mu, tau_g, &obs, p.c1);
-        }
-        sub_noise(W, stau, eta + (size_t)(4 * s + mu) * ne2);
-        Ser Un = ser_mul(ser_exp(W), U.get(s, mu));
-        if (p.integrator == 1) {
-          if (Gbuf.empty()) Gbuf.resize((size_t)8 * Z.V * S);
-          std::memcpy(Gbuf.data() + 2 * (size_t)(4 * s + mu) * S, Un.v.data(), sizeof(cx) * S);
-        } else {
-          Fn.set(s, mu, Un);
-        }
-        MM[mu] = ser_add(MM[mu], ser_log(Un));
+      D1.resize(nl);
+#pragma omp parallel for schedule(dynamic, 16) if (g_par)
+      for (long l = 0; l < nl; l++) {
+        const int s = (int)(l / 4), mu = (int)(l % 4);
+        D1[l] = drift(U, s, mu, tau_g, nullptr, p.c1, obs_out ? &rec[l] : nullptr);
+        Ser W = D1[l];
+        sub_noise(W, stau, eta + (size_t)l * ne2);
+        Fn.set(s, mu, ser_mul(ser_exp(W), U.get(s, mu)));
       }
+    }
+    std::vector<real> Gbuf(p.integrator == 1 ? (size_t)8 * Z.V * S : 0);
+#pragma omp parallel for schedule(dynamic, 16) if (g_par)
+    for (long l = 0; l < nl; l++) {
+      const int s = (int)(l / 4), mu = (int)(l % 4);
+      Ser W;
+      if (p.integrator == 1) {
+        // corrector: drift at U1, combined with D(U); applied to U
+        W = rk2_combine(D1[l], drift(Fn, s, mu, tau_g, nullptr, p.c1), g.nc, p.eps);
+      } else {
+        W = drift(U, s, mu, tau_g, nullptr, p.c1, obs_out ? &rec[l] : nullptr);
+      }
+      sub_noise(W, stau, eta + (size_t)l * ne2);
+      Ser Un = ser_mul(ser_exp(W), U.get(s, mu));
+      if (p.integrator == 1)
+        std::memcpy(Gbuf.data() + 2 * (size_t)l * S, Un.v.data(), sizeof(cx) * S);
+      else
+        Fn.set(s, mu, Un);
+      lg[l] = ser_log(Un);
+    }
+    for (long l = 0; l < nl; l++) {
+      if (obs_out) obs.add(rec[l]);
+      MM[l % 4] = ser_add(MM[l % 4], lg[l]);
+    }
     // Umu = Fmu (nspt.cc:992)
     if (p.integrator == 1)
       std::memcpy(Udata, Gbuf.data(), sizeof(real) * Gbuf.size());

@@ -93,3 +93,45 @@ def test_high_orders_accuracy_against_extended_precision(nc, N):
         e_gpu = err_vs_truth(gpu_step(L, nc, N, U, eta, form), hi, lo, nc, N)
         print("GPU %-10s vs long double:" % form, e_gpu)
         assert (e_gpu <= bar).all(), (form, e_gpu, bar)
+
+
+def test_headline_step_16x4_N20():
+    """One RK2 + zero modes + FA step of the headline workload's lattice and
+    orders (SU(3) 16^4, N = 20, eps 0.02, alpha 0.05) on a field evolved 20
+    steps from the cold start, with ranlxd host noise: the GPU step's error
+    per order against the long-double oracle step is at most twice the FP64
+    oracle's own (or 2e-15 of the order's scale), and the two FP64 steps
+    agree to 1e-12 relative at every order where the FP64 oracle itself is
+    that accurate.  The oracle runs on the host's cores (OpenMP)."""
+    import time
+    nspt = _nspt()
+    L, nc, N = (16, 16, 16, 16), 3, 20
+    V = 16 ** 4
+    ctx = nspt.Context(L, nc, N)
+    ctx.evolve(nspt.Context.params(0.02, 0.05, nspt.RK2, nspt.GF_FA, 1, nspt.NOISE_DEVICE, seed=3), 20,
+               want_obs=False)
+    U = ctx.download()
+    eta = nspt.Ranlxd(1804289383).noise(nc, 4 * V)
+    ctx.set_noise(eta)
+    obs_g = ctx.step(nspt.Context.params(0.02, 0.05, nspt.RK2, nspt.GF_FA, 1, nspt.NOISE_HOST))
+    Ug = ctx.download()
+    ctx.close()
+    t = time.time()
+    Uc = U.copy()
+    obs_c = O.step(L, nc, N, Uc, 0.02, 0.05, integrator=1, update=0, gauge_fix=1, zero_modes=1, eta=eta)
+    print("\nFP64 oracle step: %.1f s" % (time.time() - t), flush=True)
+    t = time.time()
+    hi, lo = U.copy(), np.zeros_like(U)
+    O.step_extended(L, nc, N, hi, lo, 0.02, 0.05, integrator=1, gauge_fix=1, zero_modes=1, eta=eta)
+    print("long-double oracle step: %.1f s" % (time.time() - t), flush=True)
+    e_cpu = err_vs_truth(Uc, hi, lo, nc, N)
+    e_gpu = err_vs_truth(Ug, hi, lo, nc, N)
+    from tests.conftest import rel_err_per_order
+    e_gc = rel_err_per_order(Ug, Uc, nc, N)[1:]
+    np.set_printoptions(precision=1, linewidth=200)
+    print("FP64 oracle vs long double:", e_cpu)
+    print("GPU vs long double:        ", e_gpu)
+    print("GPU vs FP64 oracle:        ", e_gc)
+    assert (e_gpu <= np.maximum(2 * e_cpu, 2e-15)).all()
+    assert (e_gc[e_cpu < 2.5e-13] <= 1e-12).all()
+    assert np.abs(obs_g - obs_c).max() <= 1e-12 * np.abs(obs_c).max()
