@@ -83,6 +83,36 @@ def step_flops(N, V, integrator, nc=3, improved=False):
     return per_link * 4 * V
 
 
+def host_cpu():
+    """CPU model and the cores this process may use."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count()
+    return {"model": model, "nproc": avail, "cpu_count": os.cpu_count()}
+
+
+def calibration():
+    """Oracle / reference throughput ratio measured where the reference's
+    sources are (oracle/calibrate.py), if recorded."""
+    path = os.path.join(ROOT, "oracle", "calibration.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        c = json.load(f)
+    return {"ratio_oracle_over_reference": c["ratio"], "measured_on": c["cpu"],
+            "how": "same lattice, orders and mode (Euler on the fly, sequential gauge fixing, zero modes) timed "
+                   "for the oracle and the reference's own build on one core; rows in oracle/calibration.json"}
+
+
 def cpu_baseline(N, integrator, budget_s=12.0):
     """Oracle port, one core, 4^4 lattice with the workload's orders and step."""
     from oracle import oracle as O
@@ -273,7 +303,9 @@ def main():
     lib = nspt.lib()
     lib.nspt_profile(ctx.h, 1)
     t0 = time.perf_counter()
-    ctx.evolve(p, args.steps, want_obs=False)  # returns after the stream is synchronised
+    # the per-step observables (plaquette, norm, Tr U per order) are formed
+    # and copied back, as the reference writes them every step
+    obs = ctx.evolve(p, args.steps, want_obs=True)  # returns after the stream is synchronised
     dt = time.perf_counter() - t0
     prof = ctx.profile_read()
     lib.nspt_profile(ctx.h, 0)
@@ -345,7 +377,7 @@ def main():
                                   ", twisted boundary conditions in %s" % "".join(
                                       "xyzt"[d] for d in range(4) if (args.twist >> d) & 1) if args.twist else ""),
                    "lattice": list(L), "orders": N, "integrator": args.integrator, "replicas": 1},
-        "roofline": {"bound": "mfma", "kernel": name, "profile_names": pnames, "achieved": achieved,
+        "roofline": {"bound": "fp64-valu", "kernel": name, "profile_names": pnames, "achieved": achieved,
                      "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
                      "avg_launch_ms": avg_s * 1e3, "flops_per_launch": fl,
@@ -355,6 +387,9 @@ def main():
                      "drift_evaluation": drift_eval},
         "kernels_ms_per_step": {k: v[0] / args.steps for k, v in prof.items() if v[1]},
         "plaquette_order2": float(plaq[2].real),
+        "plaquette_order2_last_step": float(obs[-1][4]),
+        "peak_note": "78.6 TFLOP/s is AMD's published FP64 peak of MI355X (vector and matrix alike); the path runs "
+                     "on the FP64 VALU, no MFMA",
     }
     ctx.close()
     if not args.no_host_noise and not args.twist and not improved:
@@ -367,6 +402,11 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(N, 1 if integ == nspt.RK2 else 0)
         out["cpu_baseline"]["farm"] = farm
+        out["cpu_baseline"]["host"] = host_cpu()
+        cal = calibration()
+        if cal:
+            cal["reference_equivalent_value"] = out["cpu_baseline"]["value"] / cal["ratio_oracle_over_reference"]
+            out["cpu_baseline"]["calibration"] = cal
     print(json.dumps(out))
 
 

@@ -255,8 +255,9 @@ AdjointSet adjoint_set() {
 
 // compiled orders (NG = ptord + 1)
 AdjointSet find_adjoint(int nc, int ng) {
-  const AdjointSet all[] = {adjoint_set<3, 5>(), adjoint_set<3, 7>(), adjoint_set<3, 13>(),
-                            adjoint_set<2, 5>(), adjoint_set<2, 7>(), adjoint_set<2, 13>()};
+  const AdjointSet all[] = {adjoint_set<3, 5>(),  adjoint_set<3, 7>(),  adjoint_set<3, 9>(),  adjoint_set<3, 13>(),
+                            adjoint_set<3, 17>(), adjoint_set<3, 21>(), adjoint_set<2, 5>(),  adjoint_set<2, 7>(),
+                            adjoint_set<2, 9>(),  adjoint_set<2, 13>(), adjoint_set<2, 17>(), adjoint_set<2, 21>()};
   for (const auto& s : all)
     if (s.nc == nc && s.ng == ng) return s;
   return AdjointSet{0, 0, nullptr, nullptr};
@@ -275,7 +276,7 @@ extern "C" int nspt_adjoint_drift(const nspt_geom* geom, const double* U, double
   if (!geom || !U) return -1;
   const int nc = geom->nc, N = geom->ptord, NG = N + 1, NE = nc * nc;
   const AdjointSet set = find_adjoint(nc, NG);
-  if (!set.plaq) return -2;  // orders 4, 6, 12 are compiled
+  if (!set.plaq) return -2;  // orders 4, 6, 8, 12, 16, 20 are compiled
   long V = 1;
   for (int d = 0; d < 4; d++) {
     if (geom->sz[d] < 2 || geom->sz[d] > 32) return -1;

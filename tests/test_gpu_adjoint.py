@@ -16,7 +16,8 @@ def reh(m):
 
 
 @pytest.mark.parametrize("nc,L,N", [(3, (8, 8, 8, 8), 12), (3, (4, 4, 2, 4), 6), (2, (4, 2, 4, 4), 6),
-                                    (3, (2, 3, 2, 2), 4)])
+                                    (3, (2, 3, 2, 2), 4), (3, (2, 4, 2, 3), 8), (2, (2, 2, 3, 2), 16),
+                                    (3, (2, 2, 2, 3), 16), (3, (2, 2, 2, 2), 20), (2, (3, 2, 2, 2), 20)])
 def test_adjoint_drift(nc, L, N):
     from oracle import oracle as O
     from prlgt2020_amd import nspt

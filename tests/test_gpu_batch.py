@@ -115,3 +115,27 @@ def test_batch_replicas_refuse_their_own_steps():
     assert np.array_equal(obs[0], o)
     ref.close()
     b.close()
+
+
+def test_eps_sweep_batch_full_size():
+    """The batched epsilon sweep at its real size: SU(3) 4 x 12^4 + 4 x 16^4,
+    16 orders, RK2 + zero modes + FA, eps {0.01, 0.015, 0.02, 0.03} per
+    volume, distinct seeds.  Three steps of the batch equal, bit for bit,
+    three steps of each replica alone (fields and observables)."""
+    from prlgt2020_amd import nspt
+    N = 16
+    Ls = [(12, 12, 12, 12)] * 4 + [(16, 16, 16, 16)] * 4
+    eps = [0.01, 0.015, 0.02, 0.03] * 2
+    ps = [nspt.Context.params(e, 0.05, nspt.RK2, nspt.GF_FA, 1, nspt.NOISE_DEVICE, seed=100 + r, step=0)
+          for r, e in enumerate(eps)]
+    print("batch of 8 SU(3) replicas 4 x 12^4 + 4 x 16^4, N=16", flush=True)
+    b = nspt.Batch(Ls, 3, N, ps)
+    obs_b = b.evolve(3)
+    for r, (L, p) in enumerate(zip(Ls, ps)):
+        c = nspt.Context(L, 3, N)
+        q = nspt.Context.params(p.eps, p.alpha, p.integrator, p.gauge_fix, p.zero_modes, p.noise, p.seed, p.step)
+        o = c.evolve(q, 3)
+        assert np.array_equal(b.replicas[r].download(), c.download()), r
+        assert np.array_equal(obs_b[r], o), r
+        c.close()
+    b.close()
