@@ -244,6 +244,19 @@ int nspt_batch_tune(nspt_batch* b, int steps, int* qt_chosen, double* ms_per_ste
  * ignored. */
 int nspt_adjoint_drift(const nspt_geom* geom, const double* U, double eps, double* grad, double* drift);
 
+/* nspt_adjoint_drift without any allocation (no counterpart in the
+ * reference): for callers that cannot allocate device memory -- code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.
+ * U, grad and drift are DEVICE pointers in the engine's layout: element e of
+ * order k of link (site, mu) at complex index (k*nc*nc + e)*4V + mu*V + site,
+ * k = 0..ptord (grad and drift may be null).  The six gradient slots per
+ * link live in a static device array of 192 MiB held by each GPU until the
+ * process ends: 6 * 4V * (ptord+1) * nc^2 * 16 bytes must fit (SU(3) 8^4 at
+ * ptord 12 does), else -4 and nothing runs.  The kernels run on the null
+ * stream and the call returns when they are done.  Every call on a GPU
+ * shares that GPU's array: calls that overlap on one GPU race. */
+int nspt_adjoint_drift_dev(const nspt_geom* geom, const double* U, double eps, double* grad, double* drift);
+
 #ifdef __cplusplus
 }
 #endif

@@ -27,6 +27,7 @@
 // 6 x 4V series (plane stride 6 x 4V), series index slot*4V + link.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <string>
 
 #include "../../include/nspt.h"
@@ -271,25 +272,93 @@ int grid_n(long n) {
 }  // namespace
 }  // namespace nspt
 
-extern "C" int nspt_adjoint_drift(const nspt_geom* geom, const double* U, double eps, double* grad, double* drift) {
-  using namespace nspt;
-  if (!geom || !U) return -1;
-  const int nc = geom->nc, N = geom->ptord, NG = N + 1, NE = nc * nc;
-  const AdjointSet set = find_adjoint(nc, NG);
-  if (!set.plaq) return -2;  // orders 4, 6, 8, 12, 16, 20 are compiled
+// Scratch of nspt_adjoint_drift_dev: the six gradient slots of every link
+// (6 x 4V series of ptord+1 matrices), in a static device array, which each
+// GPU the library runs on holds until the process ends.  Host code reaches
+// it only through the address hipGetSymbolAddress returns on that device.
+constexpr size_t ADJ_SCRATCH_BYTES = size_t(192) << 20;
+static __device__ __align__(256) double2 scratch[ADJ_SCRATCH_BYTES / sizeof(double2)];
+
+namespace nspt {
+namespace {
+
+// Geometry of an adjoint call: 0, or -1 (bad extents) / -2 (order not compiled).
+int adjoint_geometry(const nspt_geom* geom, AdjointSet* set, Geo* g) {
+  const int NE = geom->nc * geom->nc, NG = geom->ptord + 1;
+  *set = find_adjoint(geom->nc, NG);
+  if (!set->plaq) return -2;
   long V = 1;
   for (int d = 0; d < 4; d++) {
     if (geom->sz[d] < 2 || geom->sz[d] > 32) return -1;
     V *= geom->sz[d];
   }
-  Geo g{};
-  for (int d = 0; d < 4; d++) g.L[d] = geom->sz[d];
-  g.st[3] = 1;
-  g.st[2] = g.L[3];
-  g.st[1] = g.L[2] * g.L[3];
-  g.st[0] = g.L[1] * g.L[2] * g.L[3];
-  g.V = (int)V;
-  g.nrt = NG;
+  *g = Geo{};
+  for (int d = 0; d < 4; d++) g->L[d] = geom->sz[d];
+  g->st[3] = 1;
+  g->st[2] = g->L[3];
+  g->st[1] = g->L[2] * g->L[3];
+  g->st[0] = g->L[1] * g->L[2] * g->L[3];
+  g->V = (int)V;
+  g->nrt = NG;
+  (void)NE;
+  return 0;
+}
+
+// Device address of `scratch` on the current device, looked up once per
+// device (the array's host-side name is no device address).
+constexpr int MAX_DEVICES = 64;
+double2* g_scratch[MAX_DEVICES] = {};
+std::mutex g_scratch_mu;
+
+int scratch_on_current_device(double2** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEVICES) return -3;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return -3;
+    g_scratch[dev] = static_cast<double2*>(p);
+  }
+  *out = g_scratch[dev];
+  return 0;
+}
+
+}  // namespace
+}  // namespace nspt
+
+extern "C" int nspt_adjoint_drift_dev(const nspt_geom* geom, const double* U, double eps, double* grad,
+                                      double* drift) {
+  using namespace nspt;
+  if (!geom) return -1;
+  AdjointSet set;
+  Geo g;
+  const int rcg = adjoint_geometry(geom, &set, &g);
+  if (rcg) return rcg;
+  const int nc = geom->nc, NE = nc * nc, N = geom->ptord;
+  const long V = g.V, P = 4 * V, S = (long)(N + 1) * NE;
+  if ((size_t)(6 * S * P) * sizeof(double2) > ADJ_SCRATCH_BYTES) return -4;
+  if (!U) return -1;
+  double2* g6 = nullptr;
+  if (scratch_on_current_device(&g6)) return -3;
+  const int lpb = 64 / NE;
+  const long nb = (V + lpb - 1) / lpb;
+  // every slot of every link is written by its plaquette: nothing to clear
+  set.plaq((int)(6 * nb), g, reinterpret_cast<const double2*>(U), g6, N);
+  set.drift((int)(4 * nb), g, reinterpret_cast<const double2*>(U), g6, reinterpret_cast<double2*>(grad),
+            reinterpret_cast<double2*>(drift), -eps / (2.0 * nc));
+  if (hipGetLastError() != hipSuccess) return -3;
+  return hipStreamSynchronize(nullptr) == hipSuccess ? 0 : -3;
+}
+
+extern "C" int nspt_adjoint_drift(const nspt_geom* geom, const double* U, double eps, double* grad, double* drift) {
+  using namespace nspt;
+  if (!geom || !U) return -1;
+  AdjointSet set;
+  Geo g;
+  const int rcg = adjoint_geometry(geom, &set, &g);
+  if (rcg) return rcg;  // orders 4, 6, 8, 12, 16, 20 are compiled
+  const int nc = geom->nc, N = geom->ptord, NG = N + 1, NE = nc * nc;
+  const long V = g.V;
   const int lpb = 64 / NE;
   const long nb = (V + lpb - 1) / lpb, P = 4 * V, S = (long)NG * NE;
   double2 *u = nullptr, *aos = nullptr, *g6 = nullptr, *go = nullptr, *dout = nullptr;

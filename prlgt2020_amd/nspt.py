@@ -65,7 +65,8 @@ EXPORTS = ["nspt_create", "nspt_destroy", "nspt_last_error", "nspt_obs_len", "ns
            "nspt_batch_create", "nspt_batch_destroy", "nspt_batch_last_error", "nspt_batch_evolve",
            "nspt_batch_replica", "nspt_batch_tune", "nspt_adjoint_drift", "nspt_loops",
            "nspt_unitarize", "nspt_set_action", "nspt_set_twist", "nspt_gauge_wilson",
-           "nspt_zero_modes_subtraction", "nspt_stochastic_gauge_fixing", "nspt_rng_create_rand55"]
+           "nspt_zero_modes_subtraction", "nspt_stochastic_gauge_fixing", "nspt_rng_create_rand55",
+           "nspt_adjoint_drift_dev"]
 
 PROF_NAMES = ["drift", "update", "update_log", "zero_mode", "fa_div", "dft", "fa_gexp", "transform", "noise",
               "reduce", "rect", "plaq_field"]
@@ -109,6 +110,8 @@ def lib():
         l.nspt_batch_replica.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
         l.nspt_batch_tune.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int), _dp]
         l.nspt_adjoint_drift.argtypes = [ctypes.POINTER(_Geom), _dp, ctypes.c_double, _dp, _dp]
+        l.nspt_adjoint_drift_dev.argtypes = [ctypes.POINTER(_Geom), ctypes.c_void_p, ctypes.c_double,
+                                             ctypes.c_void_p, ctypes.c_void_p]
         _lib = l
     return _lib
 
