@@ -65,7 +65,11 @@ struct KernelSet {
 // (NC, N, tile edge)).  The default tile edge is 4; the SU(3) instances used
 // by the production configurations also come with edge 3 for the autotuner.
 #ifdef NSPT_SINGLE_INSTANCE
-#define NSPT_INSTANCES(X) X(3, 20, 4)
+#ifndef NSPT_SI_N
+#define NSPT_SI_N 20
+#endif
+#define NSPT_SI_APPLY(X, a, b, c) X(a, b, c)
+#define NSPT_INSTANCES(X) NSPT_SI_APPLY(X, 3, NSPT_SI_N, 4)
 #else
 #define NSPT_INSTANCES(X) \
   X(3, 4, 4) X(3, 6, 4) X(3, 8, 4) X(3, 12, 4) X(3, 16, 4) X(3, 16, 3) X(3, 20, 4) X(3, 20, 3) X(3, 24, 4) X(3, 32, 4) \

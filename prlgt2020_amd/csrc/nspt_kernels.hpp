@@ -12,6 +12,12 @@
 #include "nspt_device.hpp"
 #include "nspt_internal.hpp"
 
+#ifndef NSPT_DRIFT_P_PREFETCH_U
+#define NSPT_DRIFT_P_PREFETCH_U 1
+#endif
+#ifndef NSPT_QT_DRIFT_P
+#define NSPT_QT_DRIFT_P NSPT_QT_DRIFT
+#endif
 #ifndef NSPT_WAVES_PER_EU
 #define NSPT_WAVES_PER_EU 1
 #endif
@@ -166,7 +172,17 @@ __device__ __forceinline__ int link_site(const Geo& g, int k) {
 // Loads are unmasked: lanes of invalid links carry a clamped, valid index.
 // With dag, the dagger of the series is staged (element (r, c) loaded,
 // conj written at (c, r)).
-template <int NC, int N>
+// The loads go out in chunks of STAGE_CHUNK orders with no branch between
+// them (an order at or above nrt reads plane 0 and is written as zero), so
+// a chunk waits for one memory latency; a per-order "load if k < nrt, then
+// write" compiles to a branch per order and waits for every load in turn.
+// (CH = 1 keeps the per-order form, which measured faster where the
+// staging follows a long product with every register in use: the gauge
+// transform above 24 orders.)
+#ifndef NSPT_STAGE_CHUNK
+#define NSPT_STAGE_CHUNK 16
+#endif
+template <int NC, int N, int CH = NSPT_STAGE_CHUNK>
 __device__ __forceinline__ void stage_global(const Lane<NC, N>& ln, double2* slot, const SerField& f, long idx,
                                              int nrt, bool dag = false) {
   constexpr int NE = NC * NC;
@@ -175,20 +191,33 @@ __device__ __forceinline__ void stage_global(const Lane<NC, N>& ln, double2* slo
   const long plane = (long)NE * f.P;
   const int pos = dag ? (ln.c * NC + ln.r) : ln.e;
   const double sy = dag ? -1.0 : 1.0;
-#if NSPT_BIG_FIELDS
-#pragma unroll
-  for (int k = 0; k < N; k++) {
-    const double2 v = (k < nrt) ? ld_plane(f.data, k * plane, off) : zero2();
-    slot[k * NE + pos] = make_double2(v.x, sy * v.y);
-  }
-#else
+#if !NSPT_BIG_FIELDS
   const __amdgpu_buffer_rsrc_t r = rsrc(f.data);
-#pragma unroll
-  for (int k = 0; k < N; k++) {
-    const double2 v = (k < nrt) ? ld_r(r, off, (unsigned)(k * plane * 16)) : zero2();
-    slot[k * NE + pos] = make_double2(v.x, sy * v.y);
-  }
 #endif
+#pragma unroll
+  for (int k0 = 0; k0 < N; k0 += CH) {
+    double2 v[CH];
+#pragma unroll
+    for (int i = 0; i < CH; i++) {
+      const int k = k0 + i;
+      if (k < N) {
+        const int kk = uniform(k < nrt ? k : 0);
+#if NSPT_BIG_FIELDS
+        v[i] = ld_plane(f.data, kk * plane, off);
+#else
+        v[i] = ld_r(r, off, (unsigned)(kk * plane * 16));
+#endif
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < CH; i++) {
+      const int k = k0 + i;
+      if (k < N) {
+        const double2 w = (k < nrt) ? v[i] : zero2();
+        slot[k * NE + pos] = make_double2(w.x, sy * w.y);
+      }
+    }
+  }
 }
 
 // Loads and stores of a link reached with crossing code `code`: on a
@@ -553,7 +582,8 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_drift_p(Geo g, SerFie
   // the plaquettes at x: P_mu,nu(x), or P_nu,mu(x)^dag for nu < mu (the
   // first loaded straight into Sigma, the other two after the products,
   // into the prefetch registers; staging them through LDS before the
-  // products measured 24% slower)
+  // products measured 5% slower at 16^4 N=20, and loading the RK2 D1 at the
+  // start of the finish 2.5% slower: the register budget decides)
   auto at_x = [&](int p, bool& dg) {
     const int nu = (p < mu) ? p : p + 1;
     dg = nu < mu;
@@ -578,7 +608,11 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_drift_p(Geo g, SerFie
   // needed for Sigma and the product, and they are staged straight from HBM.
   constexpr bool PF = N <= (NC == 2 ? 24 : 20);
   constexpr int NP = PF ? N : 1;
-  double2 PX[NP], PU[NP];
+  // with PFU the link U_nu(x-nu) is prefetched too; without it, it is
+  // staged straight from HBM (80 fewer live VGPRs during the products)
+  constexpr bool PFU = PF && NSPT_DRIFT_P_PREFETCH_U;
+  constexpr int NPU = PFU ? N : 1;
+  double2 PX[NP], PU[NPU];
   int cm = TW0;
   long ix, iu;
   bool xdag;
@@ -593,10 +627,8 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_drift_p(Geo g, SerFie
     iu = (long)nu * V + xm;
   };
   locate(0);
-  if constexpr (PF) {
-    gload_tw<NC, N>(g, ln, P, ix, cm, nrt, ok, PX);
-    gload_tw<NC, N>(g, ln, U, iu, cm, nrt, ok, PU);
-  }
+  if constexpr (PF) gload_tw<NC, N>(g, ln, P, ix, cm, nrt, ok, PX);
+  if constexpr (PFU) gload_tw<NC, N>(g, ln, U, iu, cm, nrt, ok, PU);
   double2 xf = gflag(P, ix, ok), uf = gflag(U, iu, ok);
   bool xd = xdag;
   double2 tf = zero2(), ufc = zero2();
@@ -611,13 +643,14 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_drift_p(Geo g, SerFie
     const bool even = (i & 1) == 0;
     double2 a0 = tf;
     if (even) {
-      if constexpr (PF) {
+      if constexpr (PF)
         ln.stage(ln.slotA, PX, !xd);
-        ln.stage(ln.slotB, PU);
-      } else {
+      else
         stage_global_tw(g, ln, ln.slotA, P, ix, cm, nrt, !xd);
+      if constexpr (PFU)
+        ln.stage(ln.slotB, PU);
+      else
         stage_global_tw(g, ln, ln.slotB, U, iu, cm, nrt);
-      }
       ufc = uf;
       a0 = xd ? xf : c_conj(xf);
     }
@@ -625,19 +658,17 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_drift_p(Geo g, SerFie
     if (even) {
       if (i + 2 < 6) {
         locate(i / 2 + 1);
-        if constexpr (PF) {
-          gload_tw<NC, N>(g, ln, P, ix, cm, nrt, ok, PX);
-          gload_tw<NC, N>(g, ln, U, iu, cm, nrt, ok, PU);
-        }
+        if constexpr (PF) gload_tw<NC, N>(g, ln, P, ix, cm, nrt, ok, PX);
+        if constexpr (PFU) gload_tw<NC, N>(g, ln, U, iu, cm, nrt, ok, PU);
         xf = gflag(P, ix, ok);
         uf = gflag(U, iu, ok);
         xd = xdag;
       } else {
-        if constexpr (PF) gload<NC, N>(U, idx, ln.e, nrt, ok, PU);  // U_mu(x), for the observables
+        if constexpr (PFU) gload<NC, N>(U, idx, ln.e, nrt, ok, PU);  // U_mu(x), for the observables
         uf = gflag(U, idx, ok);
       }
     }
-    const double2 f = mul<NC, N, 1, false, NSPT_QT_DRIFT>(ln, a0, ufc, true, Sig, M, even, ln.slotA, even, 1.0);
+    const double2 f = mul<NC, N, 1, false, NSPT_QT_DRIFT_P>(ln, a0, ufc, true, Sig, M, even, ln.slotA, even, 1.0);
     if (even)
       tf = c_conj(f);
     else
@@ -645,7 +676,7 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_drift_p(Geo g, SerFie
     __syncthreads();
   }
   // U_mu(x) into slot A; the other two plaquettes at x into Sigma
-  if constexpr (PF) {
+  if constexpr (PFU) {
     ln.stage(ln.slotA, PU);
     bool d1, d2;
     const long i1 = at_x(1, d1), i2 = at_x(2, d2);
@@ -858,7 +889,7 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_gauge_transform(Geo g
     if constexpr (PF)
       ln.stage(ln.slotA, P);
     else
-      stage_global(ln, ln.slotA, G, site, nrt);
+      stage_global<NC, N, 1>(ln, ln.slotA, G, site, nrt);
     __syncthreads();
     const double2 nf = mul_lds<NC, N>(ln, gf, gpf, true, M, ln.slotB, false);
     __syncthreads();

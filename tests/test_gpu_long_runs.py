@@ -1,0 +1,111 @@
+"""Long GPU runs against long CPU runs of the oracle, per order.
+
+The CPU runs are the committed fixtures of tests/golden/make_long_runs.py
+(independent ranlxd streams); the GPU runs draw device noise (Philox) with
+their own seeds, so the two sides are independent samples of the same
+stationary distribution.  The runs are seeded, hence reproducible.
+
+- The CPU-runnable workload's shape with the Jacobi update: SU(3) 4^4, 6
+  orders, Euler eps = 0.05, zero modes + FA (alpha 0.05).  The plaquette
+  of orders 2, 4 and 6 (written every step by the reference, nspt.cc:570-597)
+  agrees within 3 sigma combined.
+- RK2 on 2^4, 4 orders, at five eps, SU(3) and SU(2): the GPU's own
+  order-4 plaquette has no term linear in eps (weak order 2, the pin of the
+  corrector's coefficient, tests/test_oracle_rk2_pin.py), and orders 2 and 4
+  agree with the CPU runs at every eps (chi^2 of the ten differences below
+  its 0.999 quantile).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from tests.test_oracle_rk2_pin import combine, controlled_order4, equipartition, eps_fit, run_means
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _nspt():
+    from prlgt2020_amd import nspt
+    return nspt
+
+
+def gpu_bins(Ls, nc, N, params, therm, steps, nbin, chunk=2000):
+    """Batch of replicas: therm steps, then steps in nbin bins; returns the
+    (replicas, nbin, N+1) bin means of the real plaquette per order."""
+    nspt = _nspt()
+    b = nspt.Batch(Ls, nc, N, params)
+    b.evolve(therm, want_obs=False)
+    per = steps // nbin
+    assert per * nbin == steps and chunk % per == 0
+    out = np.zeros((len(Ls), nbin, N + 1))
+    done = 0
+    while done < steps:
+        n = min(chunk, steps - done)
+        obs = b.evolve(n)
+        for r, o in enumerate(obs):
+            re = o[:, 0:2 * (N + 1):2]  # real plaquette per order
+            out[r, done // per:(done + n) // per] = re.reshape(n // per, per, N + 1).mean(axis=1)
+        done += n
+    b.close()
+    return out
+
+
+def test_cpu_runnable_workload_per_order_statistics():
+    nspt = _nspt()
+    d = np.load(os.path.join(GOLD, "long_runs_config1.npz"))
+    L, N, eps, alpha = tuple(int(x) for x in d["L"]), int(d["N"]), float(d["eps"]), float(d["alpha"])
+    R = 8
+    ps = [nspt.Context.params(eps, alpha, nspt.EULER, nspt.GF_FA, 1, nspt.NOISE_DEVICE, seed=2001 + r)
+          for r in range(R)]
+    print("GPU: %d replicas SU(3) 4^4 N=6 Euler, %d steps each" % (R, int(d["steps"])), flush=True)
+    g = gpu_bins([L] * R, 3, N, ps, int(d["therm"]), int(d["steps"]), d["bins"].shape[1])
+    # orders 2..N (order 0 is the constant 1, order 1 the trace of a traceless matrix)
+    mg, sg = combine(*run_means(g[:, :, 2:]))
+    mc, sc = combine(*run_means(d["bins"][:, :, 2:]))
+    for k in range(2, N + 1):
+        print("order %d: GPU %.5f +- %.5f   CPU %.5f +- %.5f" % (k, mg[k - 2], sg[k - 2], mc[k - 2], sc[k - 2]))
+    for k in (2, 4, 6):
+        z = (mg[k - 2] - mc[k - 2]) / np.hypot(sg[k - 2], sc[k - 2])
+        assert abs(z) < 3, (k, z)
+
+
+@pytest.mark.parametrize("inc", [0, 1])
+def test_rk2_eps_scaling_and_cpu_agreement(inc):
+    nspt = _nspt()
+    d = np.load(os.path.join(GOLD, "long_runs_rk2.npz"))
+    nc = int(d["nc"][inc])
+    L, N, alpha = tuple(int(x) for x in d["L"]), int(d["N"]), float(d["alpha"])
+    eps = [float(e) for e in d["eps"]]
+    reps = {0.01: 16, 0.02: 8, 0.04: 4, 0.06: 4, 0.08: 4}
+    Ls, ps, which = [], [], []
+    for ie, e in enumerate(eps):
+        for r in range(reps[e]):
+            Ls.append(L)
+            ps.append(nspt.Context.params(e, alpha, nspt.RK2, nspt.GF_FA, 1, nspt.NOISE_DEVICE,
+                                          seed=3000 + 100 * nc + 10 * ie + r))
+            which.append(ie)
+    steps, nbin = 20000, 40
+    print("GPU: %d replicas SU(%d) 2^4 N=4 RK2 over eps %s, %d steps each" % (len(Ls), nc, eps, steps), flush=True)
+    g = gpu_bins(Ls, nc, N, ps, 400, steps, nbin)
+    which = np.array(which)
+    y, s, chi2 = [], [], 0.0
+    ifac = list(d["fac"]).index(1.0)
+    V = int(np.prod(L))
+    for ie, e in enumerate(eps):
+        mg, sg = combine(*run_means(g[which == ie][:, :, 2:]))
+        mc, sc = combine(*run_means(d["bins"][inc, ifac, ie][:, :, 2:]))
+        for k in (2, 4):
+            z = (mg[k - 2] - mc[k - 2]) / np.hypot(sg[k - 2], sc[k - 2])
+            chi2 += z * z
+            print("eps %.3f order %d: GPU %.5f +- %.5f  CPU %.5f +- %.5f  z %.2f"
+                  % (e, k, mg[k - 2], sg[k - 2], mc[k - 2], sc[k - 2], z))
+        m4, s4 = controlled_order4(g[which == ie], equipartition(nc, V))
+        y.append(m4)
+        s.append(s4)
+    p, sp, _ = eps_fit(np.array(eps), np.array(y), np.array(s))
+    print("GPU order 4: a %.4f +- %.4f, b %.3f +- %.3f, c %.2f +- %.2f; chi2(10) %.1f" %
+          (p[0], sp[0], p[1], sp[1], p[2], sp[2], chi2))
+    assert abs(p[1]) < 3 * sp[1]
+    assert chi2 < 29.59  # chi^2 with 10 degrees of freedom, 0.999 quantile

@@ -101,3 +101,16 @@ def test_extended_oracle_is_the_same_step(nc, integrator):
     assert rel_err_per_order(Ud, hi + lo, nc, N).max() <= 2e-14
     assert np.abs(lo).max() > 0  # the extended result carries digits beyond FP64
     assert np.abs(od - oe).max() <= 1e-14 * np.abs(od).max()
+
+
+def test_long_run_fixture_of_the_cpu_runnable_workload_reproduces():
+    """The first steps of run 0 of tests/golden/long_runs_config1.npz (SU(3)
+    4^4, 6 orders, Euler eps 0.05, zero modes + FA), recomputed now: the
+    long CPU runs the GPU statistics are compared with are this oracle's."""
+    import os
+    d = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "long_runs_config1.npz"))
+    L, N = tuple(int(x) for x in d["L"]), int(d["N"])
+    U = O.cold_field(L, 3, N)
+    st = d["stamps"]
+    got = O.run_stats(L, 3, N, U, float(d["eps"]), float(d["alpha"]), 0, 1, 1, int(d["seeds"][0]), len(st), len(st))
+    assert np.abs(got - st).max() <= 1e-12
