@@ -201,22 +201,26 @@ def host_noise_path(nspt, L, nc, N, eps, p_dev, steps=3):
             "steps": steps, "host_threads": 1}
 
 
-def batched_sweep(nsteps, warmup, sizes=(12, 16)):
-    """Epsilon-replica sweep: SU(3) 4 x a^4 + 4 x b^4 for sizes (a, b) (the
-    workload is 12, 16), N = 16, RK2 + zero modes + FA, eps in {0.01, 0.015,
-    0.02, 0.03} per volume, distinct seeds.  Plain: each replica stepped
-    alone (nspt_evolve, its own launches).  Batched: one graph-replayed step
-    of all replicas, tiles autotuned."""
+def batched_sweep(nsteps, warmup, sizes=(12, 16), per_volume=4):
+    """Epsilon-replica sweep: SU(3) per_volume x a^4 + per_volume x b^4 for
+    sizes (a, b) (the workload is 4 x 12^4 + 4 x 16^4), N = 16, RK2 + zero
+    modes + FA, eps in {0.01, 0.015, 0.02, 0.03} per volume (repeated with
+    further seeds when per_volume > 4), distinct seeds.  Plain: each replica
+    a context of its own, stepped alone (nspt_evolve, its own launches).
+    Batched: one graph-replayed step of all replicas, the step form
+    (branches or replica table) and tile edge autotuned; the other form is
+    timed too."""
     from prlgt2020_amd import nspt
     N = 16
-    Ls = [(sizes[0],) * 4] * 4 + [(sizes[1],) * 4] * 4
-    eps = [0.01, 0.015, 0.02, 0.03] * 2
+    Ls = [(sizes[0],) * 4] * per_volume + [(sizes[1],) * 4] * per_volume
+    eps = [0.01, 0.015, 0.02, 0.03] * (per_volume // 2)
     ps = [nspt.Context.params(e, 0.05, nspt.RK2, nspt.GF_FA, 1, nspt.NOISE_DEVICE, seed=100 + r, step=0)
           for r, e in enumerate(eps)]
     work = sum(4 * L[0] ** 4 * N for L in Ls)  # link-orders per step of all replicas
     b = nspt.Batch(Ls, 3, N, ps)
     b.evolve(warmup, want_obs=False)
-    qt, tune_ms = b.tune(steps=3)
+    qt, tune_ms = b.tune(steps=3)  # step form (branches / table) x tile edge
+    chosen_table = b.table
     # plain: the same replicas as contexts of their own, one after the other,
     # each with its own launches
     plain = [nspt.Context(L, 3, N) for L in Ls]
@@ -233,12 +237,24 @@ def batched_sweep(nsteps, warmup, sizes=(12, 16)):
     t0 = time.perf_counter()
     b.evolve(nsteps, want_obs=True)
     t_batch = time.perf_counter() - t0
+    # the other step form, timed the same way
+    b.set_form(not chosen_table)
+    b.evolve(1, want_obs=True)
+    t0 = time.perf_counter()
+    b.evolve(nsteps, want_obs=True)
+    t_other = time.perf_counter() - t0
     b.close()
-    return {"workload": "SU(3) 4x%d^4 + 4x%d^4, 16 orders, RK2 + zero modes + FA, eps {0.01,0.015,0.02,0.03} "
-                        "per volume, distinct seeds, device noise" % sizes,
+    t_tab, t_br = (t_batch, t_other) if chosen_table else (t_other, t_batch)
+    return {"workload": "SU(3) %dx%d^4 + %dx%d^4, 16 orders, RK2 + zero modes + FA, eps {0.01,0.015,0.02,0.03} "
+                        "per volume%s, distinct seeds, device noise"
+                        % (per_volume, sizes[0], per_volume, sizes[1],
+                           " (x%d seeds)" % (per_volume // 4) if per_volume > 4 else ""),
             "steps": nsteps, "plain_value": work * nsteps / t_plain, "batched_value": work * nsteps / t_batch,
-            "unit": "link-order updates/s", "speedup": t_plain / t_batch, "tile_edge": qt,
-            "tune_ms_per_step": [round(x, 3) for x in tune_ms if x > 0]}
+            "unit": "link-order updates/s", "speedup": t_plain / t_batch,
+            "form": "table" if chosen_table else "branches", "tile_edge": qt,
+            "branches_value": work * nsteps / t_br, "table_value": work * nsteps / t_tab,
+            "tune_ms_per_step": {"branches": [round(x, 3) for x in tune_ms[:2]],
+                                 "table": [round(x, 3) for x in tune_ms[2:4]]}}
 
 
 def main():
@@ -399,6 +415,8 @@ def main():
         # replicas too small to fill the GPU alone, where the graph and the
         # concurrent branches pay
         out["batched_small"] = batched_sweep(max(20, 4 * args.steps), 4, sizes=(4, 6))
+        # the same sweep with four seeds per (volume, eps): 32 replicas
+        out["batched_many"] = batched_sweep(max(20, 4 * args.steps), 4, sizes=(4, 6), per_volume=16)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(N, 1 if integ == nspt.RK2 else 0)
         out["cpu_baseline"]["farm"] = farm

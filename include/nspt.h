@@ -213,9 +213,15 @@ int nspt_profile_read(nspt_ctx* ctx, double* ms_total, long long* count);
  * replica per process, Quenched.cc:262).  Replicas share nc and ptord; each
  * has its own geometry and step parameters and must draw device noise
  * (NSPT_NOISE_DEVICE) with its own seed.  One step of every replica is
- * captured once in a HIP graph, one branch per replica running
- * concurrently, and replayed; results equal nspt_evolve on each replica
- * alone. */
+ * captured once in a HIP graph and replayed, in one of two forms:
+ *  - branches (the default): one graph branch per replica, each with the
+ *    replica's own launches, running concurrently;
+ *  - table: one launch per stage of the step for all replicas, blockIdx.y
+ *    selecting the replica whose fields, geometry, eps and seed the kernels
+ *    read from a device table (replicas must share the integrator, the
+ *    gauge fixing -- none or FA -- and the zero-mode choice).
+ * Either way the results equal nspt_evolve on each replica alone, bit for
+ * bit. */
 typedef struct nspt_batch nspt_batch;
 int nspt_batch_create(const nspt_geom* geoms, const nspt_step_params* params, int nrep, nspt_batch** out);
 int nspt_batch_destroy(nspt_batch* b);
@@ -225,10 +231,16 @@ const char* nspt_batch_last_error(const nspt_batch* b);
 int nspt_batch_evolve(nspt_batch* b, int nsteps, double* obs);
 /* Context of replica r (download, plaquette, ...), owned by the batch. */
 int nspt_batch_replica(nspt_batch* b, int r, nspt_ctx** ctx);
-/* Autotuner: for every compiled order-tile edge of the replicas' kernel
- * instance, time `steps` replays of the batch step; keep the fastest.
- * Fields and counters are restored afterwards.  ms_per_step (may be null,
- * up to 4 entries) receives the measured times in candidate order. */
+/* Step form: 0 branches, 1 table (refused, -1, when the replicas do not
+ * allow it). */
+int nspt_batch_set_form(nspt_batch* b, int table);
+int nspt_batch_form(const nspt_batch* b);
+/* Autotuner: for each step form (branches, then table when possible) and
+ * each compiled order-tile edge of the replicas' kernel instance, time
+ * `steps` replays of the batch step; keep the fastest form and edge.
+ * Fields and counters are restored afterwards, on failure too (and then the
+ * form and edge too).  ms_per_step (may be null, up to 8 entries) receives
+ * the measured times in candidate order. */
 int nspt_batch_tune(nspt_batch* b, int steps, int* qt_chosen, double* ms_per_step);
 
 /* The Langevin drift as a reverse-mode gradient of the order-truncated

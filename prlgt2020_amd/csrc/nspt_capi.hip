@@ -16,9 +16,17 @@
 #include "../../include/nspt.h"
 #include "nspt_device.hpp"
 #include "nspt_internal.hpp"
+#include "nspt_table.hpp"
 
+using nspt::DriftArgs;
+using nspt::DriftPArgs;
+using nspt::GexpArgs;
 using nspt::Geo;
 using nspt::KernelSet;
+using nspt::PlaqArgs;
+using nspt::TransformArgs;
+using nspt::UpdateArgs;
+using nspt::ZeroModeArgs;
 using nspt::SerField;
 using nspt::TwistTab;
 
@@ -100,8 +108,8 @@ __device__ __forceinline__ void philox(uint32_t c[4], uint32_t k0, uint32_t k1) 
 // mapping of MyMath.cc:295-322.
 // step_dev (may be null): a device counter added to step, so that a step
 // replayed from a captured graph draws fresh noise.
-__global__ void k_noise_philox(int nc, int V, unsigned long long seed, unsigned long long step0,
-                               const unsigned long long* step_dev, double2* eta) {
+__device__ __forceinline__ void noise_philox_body(int nc, int V, unsigned long long seed, unsigned long long step0,
+                                                  const unsigned long long* step_dev, double2* eta) {
   const unsigned long long step = step0 + (step_dev ? *step_dev : 0ull);
   for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < 4L * V; idx += (long)gridDim.x * blockDim.x) {
   const long mu = idx / V, site = idx % V;
@@ -143,6 +151,24 @@ __global__ void k_noise_philox(int nc, int V, unsigned long long seed, unsigned 
   }
 }
 
+__global__ void k_noise_philox(int nc, int V, unsigned long long seed, unsigned long long step0,
+                               const unsigned long long* step_dev, double2* eta) {
+  noise_philox_body(nc, V, seed, step0, step_dev, eta);
+}
+
+// One replica per blockIdx.y (the grid-stride loop covers each replica's
+// links with the common x extent).
+struct NoiseArgs {
+  int nc, V;
+  unsigned long long seed, step0;
+  const unsigned long long* step_dev;
+  double2* eta;
+};
+__global__ void k_noise_philox_tab(const NoiseArgs* __restrict__ t) {
+  const NoiseArgs& a = t[blockIdx.y];
+  noise_philox_body(a.nc, a.V, a.seed, a.step0, a.step_dev, a.eta);
+}
+
 // Deterministic column sums, one level of a tree over the four directions
 // at once (blockIdx.z = mu): rows [mu*in_mu + chunk*per, ...) of the rows
 // `rows` of direction mu (row stride ld) are summed into row
@@ -181,6 +207,32 @@ __global__ __launch_bounds__(MEAN_THREADS) void k_field_mean(const double2* f, l
   if (threadIdx.x == 0) out[col] = nspt::c_scale(1.0 / (double)V, red[0]);
 }
 
+// k_reduce_rows for the replicas of a batch: blockIdx.z = 4 replica + mu;
+// a replica whose tree has fewer levels, or fewer chunks at this level, has
+// nch below blockIdx.y + 1 (0: inactive at this level).
+struct ReduceArgs {
+  const double* in;
+  long ld, in_mu, rows;
+  int ncols;
+  long per;
+  double* out;
+  long out_ld, out_mu;
+  double scale;
+  int nch;
+};
+__global__ void k_reduce_rows_tab(const ReduceArgs* __restrict__ t) {
+  const ReduceArgs& a = t[blockIdx.z >> 2];
+  if ((int)blockIdx.y >= a.nch) return;
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= a.ncols) return;
+  const long mu = blockIdx.z & 3;
+  const long r0 = mu * a.in_mu + blockIdx.y * a.per;
+  const long r1 = min(mu * a.in_mu + a.rows, r0 + a.per);
+  double s = 0.0;
+  for (long r = r0; r < r1; r++) s += a.in[r * a.ld + col];
+  a.out[(mu * a.out_mu + blockIdx.y) * a.out_ld + col] = s * a.scale;
+}
+
 // Per-step observables from the per-mu block sums (layout per row:
 // plaq (N+1) cplx, norm N, trU N cplx), normalised as nspt.cc:570-597.
 // Advances the noise step and observable row counters of a captured step.
@@ -190,13 +242,20 @@ __global__ void k_count(unsigned long long* ctr) {
     ctr[1] += 1;
   }
 }
+// the counters of nrep replicas
+__global__ void k_count_tab(unsigned long long* const* __restrict__ ctrs, int nrep) {
+  for (int r = threadIdx.x; r < nrep; r += blockDim.x) {
+    ctrs[r][0] += 1;
+    ctrs[r][1] += 1;
+  }
+}
 
 // slot_dev (may be null): a device counter selecting the output row, for
 // steps replayed from a captured graph.
 // obs2_mu (may be null): per-direction sums of the rectangle plaquette of an
 // improved action, appended after trU (nspt.cc:575-578).
-__global__ void k_obs_finalize(const double* obs_mu, const double* obs2_mu, int nrt, int nc, int V, double* out,
-                               const unsigned long long* slot_dev, int obs_len) {
+__device__ __forceinline__ void obs_finalize_body(const double* obs_mu, const double* obs2_mu, int nrt, int nc, int V,
+                                                  double* out, const unsigned long long* slot_dev, int obs_len) {
   if (slot_dev) out += (long)(*slot_dev) * obs_len;
   const int nob = 2 * (nrt + 1) + 3 * nrt;
   const double pn = (nc == 3 ? 72.0 : 48.0) * V, nn = (nc == 3 ? 12.0 : 8.0) * V, tn = (nc == 3 ? 3.0 : 2.0) * V;
@@ -215,6 +274,22 @@ __global__ void k_obs_finalize(const double* obs_mu, const double* obs2_mu, int 
       for (int mu = 0; mu < 4; mu++) s += obs2_mu[mu * 2 * (nrt + 1) + t];
       out[2 * (nrt + 1) + nrt + 8 * nrt + t] = s / pn;
     }
+}
+__global__ void k_obs_finalize(const double* obs_mu, const double* obs2_mu, int nrt, int nc, int V, double* out,
+                               const unsigned long long* slot_dev, int obs_len) {
+  obs_finalize_body(obs_mu, obs2_mu, nrt, nc, V, out, slot_dev, obs_len);
+}
+struct ObsFinArgs {
+  const double* obs_mu;
+  int nrt, nc, V;
+  double* out;
+  const unsigned long long* slot_dev;
+  int obs_len;
+};
+// one block per replica
+__global__ void k_obs_finalize_tab(const ObsFinArgs* __restrict__ t) {
+  const ObsFinArgs& a = t[blockIdx.x];
+  obs_finalize_body(a.obs_mu, nullptr, a.nrt, a.nc, a.V, a.out, a.slot_dev, a.obs_len);
 }
 
 // ------------------------------------------------------------ FA transform
@@ -477,13 +552,13 @@ __host__ inline size_t fft_shm(const FftTile& t, const Geo& g, bool direct) {
 }
 
 template <int MAXN>
-__global__ __launch_bounds__(FFT_THREADS) void k_fa_fft(Geo g, int pass, const double2* Xsrc, double2* fb,
-                                                       double alpha) {
+__device__ __forceinline__ void fa_fft_body(int blk, const Geo& g, int pass, const double2* Xsrc, double2* fb,
+                                            double alpha) {
   extern __shared__ double2 sm[];
   const FftTile t = fft_tile(g, pass);
   const long V = g.V;
-  const long comp = blockIdx.x / t.tiles;
-  const int tt = (int)(blockIdx.x % t.tiles);
+  const long comp = blk / t.tiles;
+  const int tt = (int)(blk % t.tiles);
   const int L0 = g.L[0], L1 = g.L[1], L2 = g.L[2], L3 = g.L[3];
   // tile origin: (x2, first x3) in pass 1, (x0, first x1) in passes 0 and 2
   const int nch = (pass == 1 ? L3 : L1) / t.C;
@@ -620,6 +695,24 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fa_fft(Geo g, int pass, const d
     }
     line[site] = sh.tile[off];
   }
+}
+template <int MAXN>
+__global__ __launch_bounds__(FFT_THREADS) void k_fa_fft(Geo g, int pass, const double2* Xsrc, double2* fb,
+                                                       double alpha) {
+  fa_fft_body<MAXN>(blockIdx.x, g, pass, Xsrc, fb, alpha);
+}
+struct FftArgs {
+  Geo g;
+  const double2* Xsrc;
+  double2* fb;
+  double alpha;
+  int nblk;  // tiles x components of this replica
+};
+template <int MAXN>
+__global__ __launch_bounds__(FFT_THREADS) void k_fa_fft_tab(int pass, const FftArgs* __restrict__ t) {
+  const FftArgs& a = t[blockIdx.y];
+  if ((int)blockIdx.x >= a.nblk) return;
+  fa_fft_body<MAXN>(blockIdx.x, a.g, pass, a.Xsrc, a.fb, a.alpha);
 }
 
 // Smallest compiled N >= n; among those the tile edge qt if compiled, else
@@ -1444,6 +1537,17 @@ int nspt_profile_read(nspt_ctx* c, double* ms_total, long long* count) {
 // Two graphs are kept because a step swaps the roles of the U and F
 // buffers: graph 0 is captured for the current buffer roles, graph 1 for the
 // swapped ones; after each replay the host-side roles are swapped to match.
+// One launch of a replica-table step: which kernel, its grid, and its
+// table (an offset into the batch's packed tables until uploaded).
+struct TabLaunch {
+  enum Kind { NOISE, DRIFT, PLAQ, DRIFTP, REDUCE, OBSFIN, UPDATE, ZEROMODE, FFT, GEXP, TRANSFORM, COUNT } kind;
+  dim3 grid;
+  size_t shm = 0;
+  size_t off = 0;
+  const void* tab = nullptr;
+  int pass = 0, maxn = 16;
+};
+
 struct nspt_batch {
   std::vector<nspt_ctx*> reps;
   std::vector<nspt_step_params> params;
@@ -1454,6 +1558,15 @@ struct nspt_batch {
   int parity = 0;
   bool with_obs = false;
   int cap = 0;  // observable rows per replica held on the device
+  // Step form: one graph branch per replica (each replica's own launches),
+  // or (table) one launch per stage for every replica, each replica's
+  // arguments in a device table.  The table form needs replicas that share
+  // the integrator, gauge fixing (none or FA), zero-mode choice and kernel
+  // build.
+  bool table = false;
+  bool table_ok = false;
+  void* tab_dev[2] = {nullptr, nullptr};
+  std::vector<TabLaunch> tab_seq[2];
   std::string err;
   int fail(const std::string& m) {
     err = m;
@@ -1470,9 +1583,340 @@ void batch_drop_graphs(nspt_batch* b) {
   }
 }
 
+// ---- replica-table step
+int round8(long n) { return (int)((n + 7) / 8 * 8); }
+
+struct TablePlan {
+  std::vector<char> host;
+  std::vector<TabLaunch> seq;
+  template <class T>
+  size_t add(const std::vector<T>& v) {
+    const size_t off = (host.size() + 255) / 256 * 256;
+    host.resize(off + sizeof(T) * v.size());
+    std::memcpy(host.data() + off, v.data(), sizeof(T) * v.size());
+    return off;
+  }
+  template <class T>
+  void launch(TabLaunch::Kind k, dim3 grid, const std::vector<T>& v, size_t shm = 0, int pass = 0, int maxn = 16) {
+    TabLaunch L;
+    L.kind = k;
+    L.grid = grid;
+    L.shm = shm;
+    L.off = add(v);
+    L.pass = pass;
+    L.maxn = maxn;
+    seq.push_back(L);
+  }
+};
+
+// Deterministic reduction of each replica's (4 rows x ncols) groups, as
+// reduce_groups forms it alone: the same chunks per level, levels until
+// one chunk; replicas with fewer levels sit out the later ones.
+struct ReduceRep {
+  const double* in;
+  long ld, rows;
+  double* out;
+  double scale;
+  double* tmp;
+};
+void plan_reduce(TablePlan& tp, const std::vector<ReduceRep>& rr, int ncols) {
+  const size_t R = rr.size();
+  std::vector<const double*> src(R);
+  std::vector<long> src_ld(R), rows(R);
+  std::vector<size_t> half(R);
+  std::vector<bool> done(R, false);
+  for (size_t r = 0; r < R; r++) {
+    src[r] = rr[r].in;
+    src_ld[r] = rr[r].ld;
+    rows[r] = rr[r].rows;
+    half[r] = reduce_tmp_doubles(rr[r].rows, 4, ncols) / 2;
+  }
+  for (int level = 0;; level++) {
+    std::vector<ReduceArgs> lv(R);
+    int maxnch = 0;
+    for (size_t r = 0; r < R; r++) {
+      ReduceArgs a{};
+      a.ncols = ncols;
+      a.per = REDUCE_PER;
+      if (done[r]) {
+        a.nch = 0;
+        lv[r] = a;
+        continue;
+      }
+      const long nch = (rows[r] + REDUCE_PER - 1) / REDUCE_PER;
+      a.in = src[r];
+      a.ld = src_ld[r];
+      a.in_mu = rows[r];
+      a.rows = rows[r];
+      a.out_ld = ncols;
+      if (nch == 1) {
+        a.out = rr[r].out;
+        a.out_mu = 1;
+        a.scale = rr[r].scale;
+        a.nch = 1;
+        done[r] = true;
+      } else {
+        double* dst = rr[r].tmp + (level % 2) * half[r];
+        a.out = dst;
+        a.out_mu = nch;
+        a.scale = 1.0;
+        a.nch = (int)nch;
+        src[r] = dst;
+        src_ld[r] = ncols;
+        rows[r] = nch;
+      }
+      maxnch = std::max(maxnch, a.nch);
+      lv[r] = a;
+    }
+    if (maxnch == 0) break;
+    tp.launch(TabLaunch::REDUCE, dim3((ncols + 63) / 64, maxnch, 4 * (unsigned)R), lv);
+  }
+}
+
+// The launches of one step of every replica, parity q: q = 0 steps the
+// fields the contexts call U (writing F), q = 1 the other way round (the
+// roles after one step).  Mirrors do_step for the Jacobi Euler / RK2 step
+// with zero modes optional and no or FA gauge fixing, Wilson action,
+// periodic lattice, device noise.
+void plan_table_step(nspt_batch* b, int q, bool with_obs, TablePlan& tp) {
+  const size_t R = b->reps.size();
+  nspt_ctx* c0 = b->reps[0];
+  const nspt_step_params& p0 = b->params[0];
+  const bool rk2 = p0.integrator == NSPT_RK2;
+  const bool fa = p0.gauge_fix == NSPT_GF_FA;
+  const bool zm = p0.zero_modes != 0;
+  const bool need_log = zm || fa;
+  const bool pf = c0->plaq_form;
+  const SerField none{nullptr, nullptr, 0};
+  auto cur = [&](nspt_ctx* c) { return q == 0 ? c->U : c->F; };
+  auto nxt = [&](nspt_ctx* c) { return q == 0 ? c->F : c->U; };
+  long maxl = 0, maxs = 0, maxn4 = 0;
+  for (auto* c : b->reps) {
+    maxl = std::max(maxl, (long)c->nblk_links);
+    maxs = std::max(maxs, (long)c->nblk_sites);
+    maxn4 = std::max(maxn4, (long)grid_for(4L * c->V, 256));
+  }
+  const unsigned gl = round8(maxl), gs = round8(maxs), gp = round8(6 * maxs), nr = (unsigned)R;
+  // noise
+  {
+    std::vector<NoiseArgs> v(R);
+    for (size_t r = 0; r < R; r++) {
+      nspt_ctx* c = b->reps[r];
+      v[r] = NoiseArgs{c->nc, c->V, b->params[r].seed, b->params[r].step, c->ctr_dev, c->eta};
+    }
+    tp.launch(TabLaunch::NOISE, dim3((unsigned)maxn4, nr), v);
+  }
+  auto tau = [&](size_t r) { return -b->params[r].eps / (2.0 * b->reps[r]->nc); };
+  auto drift = [&](bool second) {
+    if (pf) {
+      std::vector<PlaqArgs> pv(R);
+      std::vector<DriftPArgs> dv(R);
+      for (size_t r = 0; r < R; r++) {
+        nspt_ctx* c = b->reps[r];
+        const SerField Uin = second ? nxt(c) : cur(c);
+        pv[r] = PlaqArgs{c->g, Uin, c->Pf, 6 * c->g.nb_mu};
+        dv[r] = DriftPArgs{c->g, Uin, c->Pf, (rk2 && !second) ? c->D1 : c->D, second ? c->D1 : none, tau(r),
+                           second ? c->nc * b->params[r].eps / 6.0 : 0.0, second ? 1 : 0,
+                           (!second && with_obs) ? c->obs_part : nullptr, none, 0.0, c->nblk_links};
+      }
+      tp.launch(TabLaunch::PLAQ, dim3(gp, nr), pv);
+      tp.launch(TabLaunch::DRIFTP, dim3(gl, nr), dv);
+    } else {
+      std::vector<DriftArgs> dv(R);
+      for (size_t r = 0; r < R; r++) {
+        nspt_ctx* c = b->reps[r];
+        const SerField Uin = second ? nxt(c) : cur(c);
+        dv[r] = DriftArgs{c->g, Uin, (rk2 && !second) ? c->D1 : c->D, second ? c->D1 : none, tau(r),
+                          second ? c->nc * b->params[r].eps / 6.0 : 0.0, second ? 1 : 0,
+                          (!second && with_obs) ? c->obs_part : nullptr, none, 0.0, c->nblk_links};
+      }
+      tp.launch(TabLaunch::DRIFT, dim3(gl, nr), dv);
+    }
+  };
+  auto update = [&](bool final) {
+    std::vector<UpdateArgs> v(R);
+    for (size_t r = 0; r < R; r++) {
+      nspt_ctx* c = b->reps[r];
+      v[r] = UpdateArgs{c->g, (rk2 && !final) ? c->D1 : c->D, c->eta, std::sqrt(b->params[r].eps), cur(c), nxt(c),
+                        (final && need_log) ? c->L : none, (final && zm) ? c->mm_part : nullptr, c->nblk_links};
+    }
+    tp.launch(TabLaunch::UPDATE, dim3(gl, nr), v);
+  };
+  drift(false);
+  if (with_obs) {
+    std::vector<ReduceRep> rr(R);
+    std::vector<ObsFinArgs> fv(R);
+    for (size_t r = 0; r < R; r++) {
+      nspt_ctx* c = b->reps[r];
+      rr[r] = ReduceRep{c->obs_part, c->nob, c->g.nb_mu, c->obs_mu, 1.0, c->red_tmp};
+      fv[r] = ObsFinArgs{c->obs_mu, c->nrt, c->nc, c->V, c->obs_dev, c->ctr_dev + 1, c->obs_len};
+    }
+    plan_reduce(tp, rr, c0->nob);
+    tp.launch(TabLaunch::OBSFIN, dim3(nr), fv);
+  }
+  if (rk2) {
+    update(false);
+    drift(true);
+  }
+  update(true);
+  if (zm) {
+    const int ncol = 2 * c0->nrt * c0->NE;
+    std::vector<ReduceRep> rr(R);
+    for (size_t r = 0; r < R; r++) {
+      nspt_ctx* c = b->reps[r];
+      rr[r] = ReduceRep{reinterpret_cast<const double*>(c->mm_part), ncol, c->g.nb_mu,
+                        reinterpret_cast<double*>(c->MM), 1.0 / (double)c->V, c->red_tmp};
+    }
+    plan_reduce(tp, rr, ncol);
+    if (!fa) {
+      std::vector<ZeroModeArgs> v(R);
+      for (size_t r = 0; r < R; r++) {
+        nspt_ctx* c = b->reps[r];
+        v[r] = ZeroModeArgs{c->g, c->L, c->MM, nxt(c), c->X, c->nblk_links};
+      }
+      tp.launch(TabLaunch::ZEROMODE, dim3(gl, nr), v);
+    }
+  }
+  if (fa) {
+    const long ncomp = (long)c0->nrt * c0->NE;
+    for (int pass = 0; pass < 3; pass++) {
+      std::vector<FftArgs> v(R);
+      int maxn = 16;
+      for (auto* c : b->reps) {
+        const FftTile t = fft_tile(c->g, pass);
+        if (std::max(t.na, t.nb) > 16) maxn = FFT_MAXN;
+      }
+      long gx = 0;
+      size_t shm = 0;
+      for (size_t r = 0; r < R; r++) {
+        nspt_ctx* c = b->reps[r];
+        const FftTile t = fft_tile(c->g, pass);
+        const bool direct = !fft_unrolled(t.na, maxn) || !fft_unrolled(t.nb, maxn);
+        shm = std::max(shm, fft_shm(t, c->g, direct));
+        v[r] = FftArgs{c->g, c->L.data, c->fb, b->params[r].alpha, (int)((long)t.tiles * ncomp)};
+        gx = std::max(gx, (long)t.tiles * ncomp);
+      }
+      tp.launch(TabLaunch::FFT, dim3((unsigned)gx, nr), v, shm, pass, maxn);
+    }
+    std::vector<GexpArgs> gv(R);
+    std::vector<TransformArgs> tv(R);
+    for (size_t r = 0; r < R; r++) {
+      nspt_ctx* c = b->reps[r];
+      gv[r] = GexpArgs{c->g, c->fb, c->G, c->nblk_sites};
+      tv[r] = TransformArgs{c->g, c->G, nxt(c), zm ? c->L : none,
+                            zm ? reinterpret_cast<const double2*>(c->MM) : nullptr, c->nblk_links};
+    }
+    tp.launch(TabLaunch::GEXP, dim3(gs, nr), gv);
+    tp.launch(TabLaunch::TRANSFORM, dim3(gl, nr), tv);
+  }
+  std::vector<unsigned long long*> ctrs(R);
+  for (size_t r = 0; r < R; r++) ctrs[r] = b->reps[r]->ctr_dev;
+  tp.launch(TabLaunch::COUNT, dim3(1), ctrs);
+}
+
+void table_launch(nspt_batch* b, const TabLaunch& L) {
+  const KernelSet& ks = b->reps[0]->ks;
+  hipStream_t st = b->master;
+  const int gx = (int)L.grid.x, gy = (int)L.grid.y;
+  switch (L.kind) {
+    case TabLaunch::NOISE:
+      hipLaunchKernelGGL(k_noise_philox_tab, L.grid, dim3(256), 0, st, (const NoiseArgs*)L.tab);
+      break;
+    case TabLaunch::DRIFT: ks.drift_tab(gx, gy, st, (const nspt::DriftArgs*)L.tab); break;
+    case TabLaunch::PLAQ: ks.plaq_field_tab(gx, gy, st, (const nspt::PlaqArgs*)L.tab); break;
+    case TabLaunch::DRIFTP: ks.drift_p_tab(gx, gy, st, (const nspt::DriftPArgs*)L.tab); break;
+    case TabLaunch::REDUCE:
+      hipLaunchKernelGGL(k_reduce_rows_tab, L.grid, dim3(64), 0, st, (const ReduceArgs*)L.tab);
+      break;
+    case TabLaunch::OBSFIN:
+      hipLaunchKernelGGL(k_obs_finalize_tab, L.grid, dim3(256), 0, st, (const ObsFinArgs*)L.tab);
+      break;
+    case TabLaunch::UPDATE: ks.update_tab(gx, gy, st, (const nspt::UpdateArgs*)L.tab); break;
+    case TabLaunch::ZEROMODE: ks.zero_mode_tab(gx, gy, st, (const nspt::ZeroModeArgs*)L.tab); break;
+    case TabLaunch::FFT:
+      if (L.maxn == 16)
+        hipLaunchKernelGGL(k_fa_fft_tab<16>, L.grid, dim3(FFT_THREADS), L.shm, st, L.pass, (const FftArgs*)L.tab);
+      else
+        hipLaunchKernelGGL(k_fa_fft_tab<FFT_MAXN>, L.grid, dim3(FFT_THREADS), L.shm, st, L.pass,
+                           (const FftArgs*)L.tab);
+      break;
+    case TabLaunch::GEXP: ks.gexp_tab(gx, gy, st, (const nspt::GexpArgs*)L.tab); break;
+    case TabLaunch::TRANSFORM: ks.transform_tab(gx, gy, st, (const nspt::TransformArgs*)L.tab); break;
+    case TabLaunch::COUNT:
+      hipLaunchKernelGGL(k_count_tab, dim3(1), dim3(64), 0, st, (unsigned long long* const*)L.tab,
+                         (int)b->reps.size());
+      break;
+  }
+}
+
+// Whether the replicas can share one launch per stage.
+bool table_possible(const nspt_batch* b) {
+  const nspt_ctx* c0 = b->reps[0];
+  const nspt_step_params& p0 = b->params[0];
+  if (p0.gauge_fix != NSPT_GF_NONE && p0.gauge_fix != NSPT_GF_FA) return false;
+  for (size_t r = 0; r < b->reps.size(); r++) {
+    const nspt_ctx* c = b->reps[r];
+    const nspt_step_params& p = b->params[r];
+    if (p.integrator != p0.integrator || p.gauge_fix != p0.gauge_fix || (p.zero_modes != 0) != (p0.zero_modes != 0))
+      return false;
+    if (c->big != c0->big || c->plaq_form != c0->plaq_form || c->c1 != 0.0 || c->g.twist) return false;
+  }
+  return true;
+}
+
+void batch_free_tables(nspt_batch* b) {
+  for (int g = 0; g < 2; g++) {
+    if (b->tab_dev[g]) (void)hipFree(b->tab_dev[g]);
+    b->tab_dev[g] = nullptr;
+    b->tab_seq[g].clear();
+  }
+}
+
+// Plan and upload the tables of both parities.
+bool batch_plan_tables(nspt_batch* b, bool with_obs) {
+  batch_free_tables(b);
+  for (int q = 0; q < 2; q++) {
+    TablePlan tp;
+    plan_table_step(b, q, with_obs, tp);
+    if (hipMalloc(&b->tab_dev[q], tp.host.size()) != hipSuccess ||
+        hipMemcpy(b->tab_dev[q], tp.host.data(), tp.host.size(), hipMemcpyHostToDevice) != hipSuccess) {
+      b->err = "replica tables: allocation or upload failed";
+      return false;
+    }
+    for (auto& L : tp.seq) L.tab = static_cast<const char*>(b->tab_dev[q]) + L.off;
+    b->tab_seq[q] = tp.seq;
+  }
+  return true;
+}
+
 // Capture the two step graphs (with or without observables).
 bool batch_build(nspt_batch* b, bool with_obs) {
   batch_drop_graphs(b);
+  if (b->table) {
+    if (!batch_plan_tables(b, with_obs)) return false;
+    bool ok = true;
+    for (int g = 0; g < 2 && ok; g++) {
+      if (hipStreamBeginCapture(b->master, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        b->err = "hipStreamBeginCapture failed";
+        return false;
+      }
+      for (const auto& L : b->tab_seq[g]) table_launch(b, L);
+      hipGraph_t graph = nullptr;
+      if (hipStreamEndCapture(b->master, &graph) != hipSuccess || !graph) {
+        b->err = "hipStreamEndCapture failed";
+        ok = false;
+      }
+      if (ok && hipGraphInstantiate(&b->exec[g], graph, nullptr, nullptr, 0) != hipSuccess) {
+        b->err = "hipGraphInstantiate failed";
+        ok = false;
+      }
+      if (graph) (void)hipGraphDestroy(graph);
+    }
+    if (!ok) batch_drop_graphs(b);
+    b->parity = 0;
+    b->with_obs = with_obs;
+    return ok;
+  }
   for (auto* c : b->reps) {
     c->graph = true;
     c->prof = false;
@@ -1584,12 +2028,28 @@ int nspt_batch_create(const nspt_geom* geoms, const nspt_step_params* params, in
   b->join.resize(nrep);
   for (auto& e : b->join)
     if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return b->fail("event creation failed");
+  b->table_ok = table_possible(b);
   return batch_set_counters(b, 0, 0) ? 0 : -3;
 }
+
+int nspt_batch_set_form(nspt_batch* b, int table) {
+  if (!b) return -1;
+  if (table && !b->table_ok)
+    return b->fail("nspt_batch_set_form: the replicas differ in integrator, gauge fixing (none or FA only), "
+                   "zero modes or kernel build");
+  if ((table != 0) != b->table) {
+    b->table = table != 0;
+    batch_drop_graphs(b);
+  }
+  return 0;
+}
+
+int nspt_batch_form(const nspt_batch* b) { return b ? (b->table ? 1 : 0) : -1; }
 
 int nspt_batch_destroy(nspt_batch* b) {
   if (!b) return -1;
   batch_drop_graphs(b);
+  batch_free_tables(b);
   for (auto* c : b->reps) nspt_destroy(c);
   for (auto e : b->join) (void)hipEventDestroy(e);
   if (b->fork) (void)hipEventDestroy(b->fork);
@@ -1651,63 +2111,99 @@ int nspt_batch_evolve(nspt_batch* b, int nsteps, double* obs) {
 int nspt_batch_tune(nspt_batch* b, int steps, int* qt_chosen, double* ms_per_step) {
   if (!b || steps < 1) return -1;
   nspt_ctx* c0 = b->reps[0];
-  // candidate tile edges compiled for the replicas' instance
-  std::vector<int> cand;
+  // candidates: step form (graph branches, then the replica table when the
+  // replicas allow it) x compiled tile edge of the replicas' instance
+  std::vector<int> edges;
   for (int q : {4, 3, 5, 2}) {
     const KernelSet ks = find_set(c0->nc, c0->nrt, q, c0->big);
-    if (ks.qt == q && ks.nmax == c0->ks.nmax) cand.push_back(q);
+    if (ks.qt == q && ks.nmax == c0->ks.nmax) edges.push_back(q);
   }
+  std::vector<std::pair<bool, int>> cand;
+  for (bool tab : {false, true})
+    if (!tab || b->table_ok)
+      for (int q : edges) cand.push_back({tab, q});
+  const bool table0 = b->table;
+  const int qt0 = c0->ks.qt;
   // snapshot fields and counters: tuning steps are not part of the run
   std::vector<double2*> snap(b->reps.size(), nullptr);
   std::vector<unsigned long long> ctr(2 * b->reps.size());
-  for (size_t r = 0; r < b->reps.size(); r++) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  bool snapped = false;
+  double best = 1e30;
+  bool best_tab = table0;
+  int best_q = qt0;
+  int rc = 0;
+  auto set_form = [&](bool tab, int q) {
+    b->table = tab;
+    for (auto* c : b->reps) c->ks = find_set(c->nc, c->nrt, q, c->big);
+  };
+  for (size_t r = 0; r < b->reps.size() && rc == 0; r++) {
     nspt_ctx* c = b->reps[r];
     const size_t n = (size_t)c->nrt * c->NE * 4 * c->V + 4 * (size_t)c->V;
-    if (hipMalloc(&snap[r], n * sizeof(double2)) != hipSuccess) return b->fail("tune: snapshot allocation failed");
+    if (hipMalloc(&snap[r], n * sizeof(double2)) != hipSuccess) {
+      rc = b->fail("tune: snapshot allocation failed");
+      break;
+    }
     (void)hipMemcpyAsync(snap[r], c->U.data, (n - 4 * (size_t)c->V) * sizeof(double2), hipMemcpyDeviceToDevice,
                          b->master);
     (void)hipMemcpyAsync(snap[r] + (n - 4 * (size_t)c->V), c->U.flag, 4 * (size_t)c->V * sizeof(double2),
                          hipMemcpyDeviceToDevice, b->master);
     (void)hipMemcpyAsync(&ctr[2 * r], c->ctr_dev, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, b->master);
   }
-  if (!batch_sync(b)) return -3;
-  double best = 1e30;
-  int best_q = c0->ks.qt;
-  hipEvent_t e0, e1;
-  (void)hipEventCreate(&e0);
-  (void)hipEventCreate(&e1);
-  for (size_t i = 0; i < cand.size(); i++) {
-    for (auto* c : b->reps) c->ks = find_set(c->nc, c->nrt, cand[i], c->big);
-    if (!batch_build(b, false) || !batch_replay(b, 1)) return -3;
+  if (rc == 0 && !batch_sync(b)) rc = -3;
+  snapped = rc == 0;
+  if (rc == 0 && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) rc = b->fail("tune: events");
+  for (size_t i = 0; i < cand.size() && rc == 0; i++) {
+    set_form(cand[i].first, cand[i].second);
+    if (!batch_build(b, false) || !batch_replay(b, 1)) {
+      rc = -3;
+      break;
+    }
     (void)hipEventRecord(e0, b->master);
-    if (!batch_replay(b, steps)) return -3;
+    if (!batch_replay(b, steps)) {
+      rc = -3;
+      break;
+    }
     (void)hipEventRecord(e1, b->master);
-    if (!batch_sync(b)) return -3;
+    if (!batch_sync(b)) {
+      rc = -3;
+      break;
+    }
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, e0, e1);
     const double per = ms / steps;
-    if (ms_per_step && i < 4) ms_per_step[i] = per;
+    if (ms_per_step && i < 8) ms_per_step[i] = per;
     if (per < best) {
       best = per;
-      best_q = cand[i];
+      best_tab = cand[i].first;
+      best_q = cand[i].second;
     }
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  for (auto* c : b->reps) c->ks = find_set(c->nc, c->nrt, best_q, c->big);
+  // every exit restores the run: the form and tiles (the best on success,
+  // the original otherwise), the fields and the counters
+  if (rc == 0)
+    set_form(best_tab, best_q);
+  else
+    set_form(table0, qt0);
   batch_drop_graphs(b);
-  for (size_t r = 0; r < b->reps.size(); r++) {
-    nspt_ctx* c = b->reps[r];
-    const size_t n = (size_t)c->nrt * c->NE * 4 * c->V;
-    (void)hipMemcpyAsync(c->U.data, snap[r], n * sizeof(double2), hipMemcpyDeviceToDevice, b->master);
-    (void)hipMemcpyAsync(c->U.flag, snap[r] + n, 4 * (size_t)c->V * sizeof(double2), hipMemcpyDeviceToDevice,
-                         b->master);
-    (void)hipMemcpyAsync(c->ctr_dev, &ctr[2 * r], 2 * sizeof(unsigned long long), hipMemcpyHostToDevice, b->master);
+  if (snapped) {
+    for (size_t r = 0; r < b->reps.size(); r++) {
+      nspt_ctx* c = b->reps[r];
+      const size_t n = (size_t)c->nrt * c->NE * 4 * c->V;
+      (void)hipMemcpyAsync(c->U.data, snap[r], n * sizeof(double2), hipMemcpyDeviceToDevice, b->master);
+      (void)hipMemcpyAsync(c->U.flag, snap[r] + n, 4 * (size_t)c->V * sizeof(double2), hipMemcpyDeviceToDevice,
+                           b->master);
+      (void)hipMemcpyAsync(c->ctr_dev, &ctr[2 * r], 2 * sizeof(unsigned long long), hipMemcpyHostToDevice,
+                           b->master);
+    }
+    if (!batch_sync(b) && rc == 0) rc = -3;
   }
-  if (!batch_sync(b)) return -3;
-  for (auto* p : snap) (void)hipFree(p);
-  if (qt_chosen) *qt_chosen = best_q;
-  return 0;
+  for (auto* p : snap)
+    if (p) (void)hipFree(p);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (rc == 0 && qt_chosen) *qt_chosen = best_q;
+  return rc;
 }
 
 }  // extern "C"

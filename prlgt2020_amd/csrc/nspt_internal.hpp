@@ -6,6 +6,13 @@
 namespace nspt {
 
 struct SerField;
+struct DriftArgs;
+struct PlaqArgs;
+struct DriftPArgs;
+struct UpdateArgs;
+struct ZeroModeArgs;
+struct GexpArgs;
+struct TransformArgs;
 
 // Lattice geometry as the kernels see it.  st[d] is the site stride of
 // direction d (x3 fastest, lattice.h:37).
@@ -57,6 +64,16 @@ struct KernelSet {
   void (*gexp)(int nblk, hipStream_t, Geo, const double2* fb, SerField G) = nullptr;
   void (*transform)(int nblk, hipStream_t, Geo, SerField G, SerField U, SerField Xexp,
                     const double2* MM) = nullptr;
+  // the same kernels launched once for the replicas of a batch, each
+  // replica's arguments in a device table (grid: gx blocks, a multiple of 8
+  // covering the largest replica, times nrep)
+  void (*drift_tab)(int gx, int nrep, hipStream_t, const DriftArgs* tab) = nullptr;
+  void (*plaq_field_tab)(int gx, int nrep, hipStream_t, const PlaqArgs* tab) = nullptr;
+  void (*drift_p_tab)(int gx, int nrep, hipStream_t, const DriftPArgs* tab) = nullptr;
+  void (*update_tab)(int gx, int nrep, hipStream_t, const UpdateArgs* tab) = nullptr;
+  void (*zero_mode_tab)(int gx, int nrep, hipStream_t, const ZeroModeArgs* tab) = nullptr;
+  void (*gexp_tab)(int gx, int nrep, hipStream_t, const GexpArgs* tab) = nullptr;
+  void (*transform_tab)(int gx, int nrep, hipStream_t, const TransformArgs* tab) = nullptr;
   void (*link_pair)(int nblk, hipStream_t, Geo, SerField A, SerField out, int k) = nullptr;
   void (*loops)(int nblk, hipStream_t, Geo, SerField U, SerField L2, SerField L4, double* part) = nullptr;
 };

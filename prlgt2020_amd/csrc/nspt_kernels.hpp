@@ -11,6 +11,7 @@
 #pragma once
 #include "nspt_device.hpp"
 #include "nspt_internal.hpp"
+#include "nspt_table.hpp"
 
 #ifndef NSPT_DRIFT_P_PREFETCH_U
 #define NSPT_DRIFT_P_PREFETCH_U 1
@@ -118,15 +119,18 @@ __device__ __forceinline__ void twist_elem(const Geo& g, int code, int r, int c,
 #ifndef NSPT_BLOCK_ORDER
 #define NSPT_BLOCK_ORDER 2
 #endif
-__device__ __forceinline__ int block_order() {
+// Block p of n (a replica-table launch passes the replica's own count; its
+// grid's x extent is a multiple of 8, so x = p & 7 is still the XCD).
+__device__ __forceinline__ int block_order(int p, int n) {
 #if NSPT_BLOCK_ORDER >= 2
-  const int n = gridDim.x, p = blockIdx.x;
   const int q = n >> 3, r = n & 7, x = p & 7;
   return x * q + min(x, r) + (p >> 3);
 #else
-  return blockIdx.x;
+  (void)n;
+  return p;
 #endif
 }
+__device__ __forceinline__ int block_order() { return block_order(blockIdx.x, gridDim.x); }
 
 // Link block of this workgroup: direction mu, first index site0 in the list
 // of its sites (the lattice, or a checkerboard subset of count = V/2 sites),
@@ -364,7 +368,7 @@ __device__ __forceinline__ void drift_finish(const Lane<NC, N>& ln, double2* lds
 // mode 2: observables only.  obs_part (may be null): per block
 // [plaq (N+1) cplx][norm N][trU N cplx] of the block's mu, unnormalised.
 template <int NC, int N>
-__global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_drift(Geo g, SerField U, SerField D, SerField D1,
+__device__ __forceinline__ void k_drift_body(int blk_, int nblk_, const Geo& g, SerField U, SerField D, SerField D1,
                                                                  double tau_g, double rk2c, int mode,
                                                                  double* obs_part, SerField R, double c1) {
   constexpr int NE = NC * NC;
@@ -374,7 +378,7 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_drift(Geo g, SerField
   LN ln(lds);
   const int nrt = g.nrt;
   const int M = min(N, nrt) - 1;
-  const LinkBlock lb = link_block<LN::LPB>(g);
+  const LinkBlock lb = link_block_at<LN::LPB>(g, block_order(blk_, nblk_));
   const int mu = lb.mu;
   const int site0 = lb.site0;
   const int nvalid = min(LN::LPB, lb.count - site0);
@@ -488,6 +492,19 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_drift(Geo g, SerField
                       idx, ok, tau_g, mode, D, D1, rk2c);
 }
 
+template <int NC, int N>
+__global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_drift(Geo g, SerField U, SerField D, SerField D1,
+                                                                 double tau_g, double rk2c, int mode,
+                                                                 double* obs_part, SerField R, double c1) {
+  k_drift_body<NC, N>(blockIdx.x, gridDim.x, g, U, D, D1, tau_g, rk2c, mode, obs_part, R, c1);
+}
+template <int NC, int N>
+__global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_drift_tab(const DriftArgs* __restrict__ t) {
+  const DriftArgs& a = t[blockIdx.y];
+  if ((int)blockIdx.x >= a.nblk) return;
+  k_drift_body<NC, N>(blockIdx.x, a.nblk, a.g, a.U, a.D, a.D1, a.tau_g, a.rk2c, a.mode, a.obs_part, a.R, a.c1);
+}
+
 // ------------------------------------------------------------ drift from the plaquette field
 // The same drift without forming the staple sum: W = U_mu(x) Sigma_mu(x) is
 // a sum over nu != mu of the two plaquettes through the link, rotated to
@@ -505,13 +522,13 @@ __device__ __forceinline__ int plane_index(int a, int b) {
   return a == 0 ? b - 1 : (a == 1 ? b + 1 : 5);
 }
 template <int NC, int N>
-__global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_plaq_field(Geo g, SerField U, SerField P) {
+__device__ __forceinline__ void k_plaq_field_body(int blk_, int nblk_, const Geo& g, SerField U, SerField P) {
   using LN = Lane<NC, N>;
   __shared__ double2 lds[LN::LDS];
   LN ln(lds);
   const int nrt = g.nrt;
   const int M = min(N, nrt) - 1;
-  const int b = block_order();
+  const int b = block_order(blk_, nblk_);
   const int plane = b % 6, sb = b / 6;
   const int mu = plane < 3 ? 0 : (plane < 5 ? 1 : 2);
   const int nu = plane < 3 ? plane + 1 : (plane < 5 ? plane - 1 : 3);
@@ -554,12 +571,23 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_plaq_field(Geo g, Ser
   if (ok && ln.e == 0) P.flag[ip] = pf;
 }
 
+template <int NC, int N>
+__global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_plaq_field(Geo g, SerField U, SerField P) {
+  k_plaq_field_body<NC, N>(blockIdx.x, gridDim.x, g, U, P);
+}
+template <int NC, int N>
+__global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_plaq_field_tab(const PlaqArgs* __restrict__ t) {
+  const PlaqArgs& a = t[blockIdx.y];
+  if ((int)blockIdx.x >= a.nblk) return;
+  k_plaq_field_body<NC, N>(blockIdx.x, a.nblk, a.g, a.U, a.P);
+}
+
 // k_drift_p: the drift of k_drift (same modes, observables and improved
 // action) from the plaquette field of U.  Per nu: t = U_nu(x-nu)^dag X
 // (X = P_mu,nu(x-nu)^dag) in place in slot A, then Sigma += t U_nu(x-nu);
 // the plaquettes at x are added as loaded.
 template <int NC, int N>
-__global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_drift_p(Geo g, SerField U, SerField P, SerField D,
+__device__ __forceinline__ void k_drift_p_body(int blk_, int nblk_, const Geo& g, SerField U, SerField P, SerField D,
                                                                    SerField D1, double tau_g, double rk2c, int mode,
                                                                    double* obs_part, SerField R, double c1) {
   constexpr int NE = NC * NC;
@@ -569,7 +597,7 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_drift_p(Geo g, SerFie
   LN ln(lds);
   const int nrt = g.nrt;
   const int M = min(N, nrt) - 1;
-  const LinkBlock lb = link_block<LN::LPB>(g);
+  const LinkBlock lb = link_block_at<LN::LPB>(g, block_order(blk_, nblk_));
   const int mu = lb.mu;
   const int site0 = lb.site0;
   const int nvalid = min(LN::LPB, lb.count - site0);
@@ -709,6 +737,19 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_drift_p(Geo g, SerFie
                       idx, ok, tau_g, mode, D, D1, rk2c);
 }
 
+template <int NC, int N>
+__global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_drift_p(Geo g, SerField U, SerField P, SerField D,
+                                                                   SerField D1, double tau_g, double rk2c, int mode,
+                                                                   double* obs_part, SerField R, double c1) {
+  k_drift_p_body<NC, N>(blockIdx.x, gridDim.x, g, U, P, D, D1, tau_g, rk2c, mode, obs_part, R, c1);
+}
+template <int NC, int N>
+__global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_drift_p_tab(const DriftPArgs* __restrict__ t) {
+  const DriftPArgs& a = t[blockIdx.y];
+  if ((int)blockIdx.x >= a.nblk) return;
+  k_drift_p_body<NC, N>(blockIdx.x, a.nblk, a.g, a.U, a.P, a.D, a.D1, a.tau_g, a.rk2c, a.mode, a.obs_part, a.R, a.c1);
+}
+
 // ------------------------------------------------------------ update
 // Uout = exp(D - stau*eta at order 1) * Ubase  (nspt.cc:420-430, exp QCDpt.h:200-220);
 // optionally L = reH(log(Uout)) (QCDpt.h:176-198, 197-209) and per-block sums
@@ -716,7 +757,7 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_drift_p(Geo g, SerFie
 // generator reH(log U - mean) of nspt.cc:638 is L - mean(L).
 // eta: [idx*NE + e] (may be null).
 template <int NC, int N>
-__global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_update(Geo g, SerField D, const double2* eta,
+__device__ __forceinline__ void k_update_body(int blk_, int nblk_, const Geo& g, SerField D, const double2* eta,
                                                                   double stau, SerField Ubase, SerField Uout,
                                                                   SerField L, double2* mm_part) {
   constexpr int NE = NC * NC;
@@ -725,7 +766,7 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_update(Geo g, SerFiel
   LN ln(lds);
   const int nrt = g.nrt;
   const int n = min(N, nrt);
-  const LinkBlock lb = link_block<LN::LPB>(g);
+  const LinkBlock lb = link_block_at<LN::LPB>(g, block_order(blk_, nblk_));
   const int mu = lb.mu;
   const int site0 = lb.site0;
   const int nvalid = min(LN::LPB, lb.count - site0);
@@ -760,6 +801,19 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_update(Geo g, SerFiel
     mm_part[(long)lb.part * nrt * NE + o] = slot_sum<NC, N>(lds + LN::LPB * LN::SLOT, nvalid, o / NE, o % NE);
 }
 
+template <int NC, int N>
+__global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_update(Geo g, SerField D, const double2* eta,
+                                                                  double stau, SerField Ubase, SerField Uout,
+                                                                  SerField L, double2* mm_part) {
+  k_update_body<NC, N>(blockIdx.x, gridDim.x, g, D, eta, stau, Ubase, Uout, L, mm_part);
+}
+template <int NC, int N>
+__global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_update_tab(const UpdateArgs* __restrict__ t) {
+  const UpdateArgs& a = t[blockIdx.y];
+  if ((int)blockIdx.x >= a.nblk) return;
+  k_update_body<NC, N>(blockIdx.x, a.nblk, a.g, a.D, a.eta, a.stau, a.Ubase, a.Uout, a.L, a.mm_part);
+}
+
 // ------------------------------------------------------------ log field
 // L = log(U) for every link (FA gauge fixing without zero-mode removal).
 template <int NC, int N>
@@ -786,14 +840,14 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_log(Geo g, SerField U
 // X = reH(L - MM_mu), U = exp(X)  (nspt.cc:638-641).  MM: [(mu*nrt + k)*NE + e],
 // already multiplied by 1/V.  X is kept: it is log(U) for an FA step.
 template <int NC, int N>
-__global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_zero_mode(Geo g, SerField L, const double2* MM,
+__device__ __forceinline__ void k_zero_mode_body(int blk_, int nblk_, const Geo& g, SerField L, const double2* MM,
                                                                      SerField U, SerField X) {
   constexpr int NE = NC * NC;
   using LN = Lane<NC, N>;
   __shared__ double2 lds[LN::LDS];
   LN ln(lds);
   const int nrt = g.nrt;
-  const LinkBlock lb = link_block<LN::LPB>(g);
+  const LinkBlock lb = link_block_at<LN::LPB>(g, block_order(blk_, nblk_));
   const int mu = lb.mu;
   const int site0 = lb.site0;
   const int site = min(site0 + ln.j, g.V - 1);
@@ -816,17 +870,29 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_zero_mode(Geo g, SerF
   if (ok && ln.e == 0) U.flag[idx] = one2();
 }
 
+template <int NC, int N>
+__global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_zero_mode(Geo g, SerField L, const double2* MM,
+                                                                     SerField U, SerField X) {
+  k_zero_mode_body<NC, N>(blockIdx.x, gridDim.x, g, L, MM, U, X);
+}
+template <int NC, int N>
+__global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_zero_mode_tab(const ZeroModeArgs* __restrict__ t) {
+  const ZeroModeArgs& a = t[blockIdx.y];
+  if ((int)blockIdx.x >= a.nblk) return;
+  k_zero_mode_body<NC, N>(blockIdx.x, a.nblk, a.g, a.L, a.MM, a.U, a.X);
+}
+
 // ------------------------------------------------------------ FA: G = exp(reH(W))
 // W per site from the transformed buffer fb[(k*NE + e)*V + site] (already
 // multiplied by -alpha/(p^2 V)), G = exp(reH(W)) (nspt.cc:902-906).
 template <int NC, int N>
-__global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_fa_gexp(Geo g, const double2* fb, SerField G) {
+__device__ __forceinline__ void k_fa_gexp_body(int blk_, int nblk_, const Geo& g, const double2* fb, SerField G) {
   constexpr int NE = NC * NC;
   using LN = Lane<NC, N>;
   __shared__ double2 lds[LN::LDS];
   LN ln(lds);
   const int nrt = g.nrt;
-  const int site0 = block_order() * LN::LPB;
+  const int site0 = block_order(blk_, nblk_) * LN::LPB;
   const int site = min(site0 + ln.j, g.V - 1);
   const bool ok = ln.lds_ok && (site0 + ln.j) < g.V;
   double2 P[N];
@@ -843,20 +909,31 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_fa_gexp(Geo g, const 
   if (ok && ln.e == 0) G.flag[site] = one2();
 }
 
+template <int NC, int N>
+__global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_fa_gexp(Geo g, const double2* fb, SerField G) {
+  k_fa_gexp_body<NC, N>(blockIdx.x, gridDim.x, g, fb, G);
+}
+template <int NC, int N>
+__global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_fa_gexp_tab(const GexpArgs* __restrict__ t) {
+  const GexpArgs& a = t[blockIdx.y];
+  if ((int)blockIdx.x >= a.nblk) return;
+  k_fa_gexp_body<NC, N>(blockIdx.x, a.nblk, a.g, a.fb, a.G);
+}
+
 // ------------------------------------------------------------ gauge transform
 // U_mu(x) <- G(x) U_mu(x) G(x+mu)^dag, in place (nspt.cc:912-916).  With
 // Xexp.data set, U_mu(x) is not read but formed as exp(Xexp - MM_mu) (MM may
 // be null): the field after zero-mode removal (nspt.cc:638-641), from the
 // projected log L of the update and the mean MM (layout as in k_zero_mode).
 template <int NC, int N>
-__global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_gauge_transform(Geo g, SerField G, SerField U,
+__device__ __forceinline__ void k_gauge_transform_body(int blk_, int nblk_, const Geo& g, SerField G, SerField U,
                                                                            SerField Xexp, const double2* MM) {
   using LN = Lane<NC, N>;
   __shared__ double2 lds[LN::LDS];
   LN ln(lds);
   const int nrt = g.nrt;
   const int M = min(N, nrt) - 1;
-  const LinkBlock lb = link_block<LN::LPB>(g);
+  const LinkBlock lb = link_block_at<LN::LPB>(g, block_order(blk_, nblk_));
   const int mu = lb.mu;
   const int site0 = lb.site0;
   const int site = min(site0 + ln.j, g.V - 1);
@@ -915,6 +992,18 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_gauge_transform(Geo g
   unstage(ln, ln.slotB, P);
   gstore<NC, N>(U, idx, ln.e, nrt, ok, P);
   if (ok && ln.e == 0) U.flag[idx] = nf;
+}
+
+template <int NC, int N>
+__global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_gauge_transform(Geo g, SerField G, SerField U,
+                                                                           SerField Xexp, const double2* MM) {
+  k_gauge_transform_body<NC, N>(blockIdx.x, gridDim.x, g, G, U, Xexp, MM);
+}
+template <int NC, int N>
+__global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_gauge_transform_tab(const TransformArgs* __restrict__ t) {
+  const TransformArgs& a = t[blockIdx.y];
+  if ((int)blockIdx.x >= a.nblk) return;
+  k_gauge_transform_body<NC, N>(blockIdx.x, a.nblk, a.g, a.G, a.U, a.Xexp, a.MM);
 }
 
 // ------------------------------------------------------------ checkerboard gauge fixing
@@ -1387,6 +1476,28 @@ KernelSet make_kernel_set() {
   };
   s.transform = [](int nblk, hipStream_t st, Geo g, SerField G, SerField U, SerField Xexp, const double2* MM) {
     hipLaunchKernelGGL((k_gauge_transform<NC, N>), dim3(nblk), dim3(64), 0, st, g, G, U, Xexp, MM);
+  };
+  // replica-table launches: grid (gx, nrep), gx a multiple of 8
+  s.drift_tab = [](int gx, int nrep, hipStream_t st, const DriftArgs* t) {
+    hipLaunchKernelGGL((k_drift_tab<NC, N>), dim3(gx, nrep), dim3(64), 0, st, t);
+  };
+  s.plaq_field_tab = [](int gx, int nrep, hipStream_t st, const PlaqArgs* t) {
+    hipLaunchKernelGGL((k_plaq_field_tab<NC, N>), dim3(gx, nrep), dim3(64), 0, st, t);
+  };
+  s.drift_p_tab = [](int gx, int nrep, hipStream_t st, const DriftPArgs* t) {
+    hipLaunchKernelGGL((k_drift_p_tab<NC, N>), dim3(gx, nrep), dim3(64), 0, st, t);
+  };
+  s.update_tab = [](int gx, int nrep, hipStream_t st, const UpdateArgs* t) {
+    hipLaunchKernelGGL((k_update_tab<NC, N>), dim3(gx, nrep), dim3(64), 0, st, t);
+  };
+  s.zero_mode_tab = [](int gx, int nrep, hipStream_t st, const ZeroModeArgs* t) {
+    hipLaunchKernelGGL((k_zero_mode_tab<NC, N>), dim3(gx, nrep), dim3(64), 0, st, t);
+  };
+  s.gexp_tab = [](int gx, int nrep, hipStream_t st, const GexpArgs* t) {
+    hipLaunchKernelGGL((k_fa_gexp_tab<NC, N>), dim3(gx, nrep), dim3(64), 0, st, t);
+  };
+  s.transform_tab = [](int gx, int nrep, hipStream_t st, const TransformArgs* t) {
+    hipLaunchKernelGGL((k_gauge_transform_tab<NC, N>), dim3(gx, nrep), dim3(64), 0, st, t);
   };
   s.link_pair = [](int nblk, hipStream_t st, Geo g, SerField A, SerField out, int k) {
     constexpr int nb = link_pair_blocks<N>();

@@ -66,7 +66,7 @@ EXPORTS = ["nspt_create", "nspt_destroy", "nspt_last_error", "nspt_obs_len", "ns
            "nspt_batch_replica", "nspt_batch_tune", "nspt_adjoint_drift", "nspt_loops",
            "nspt_unitarize", "nspt_set_action", "nspt_set_twist", "nspt_gauge_wilson",
            "nspt_zero_modes_subtraction", "nspt_stochastic_gauge_fixing", "nspt_rng_create_rand55",
-           "nspt_adjoint_drift_dev"]
+           "nspt_adjoint_drift_dev", "nspt_batch_set_form", "nspt_batch_form"]
 
 PROF_NAMES = ["drift", "update", "update_log", "zero_mode", "fa_div", "dft", "fa_gexp", "transform", "noise",
               "reduce", "rect", "plaq_field"]
@@ -109,6 +109,8 @@ def lib():
         l.nspt_batch_evolve.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
         l.nspt_batch_replica.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
         l.nspt_batch_tune.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int), _dp]
+        l.nspt_batch_set_form.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        l.nspt_batch_form.argtypes = [ctypes.c_void_p]
         l.nspt_adjoint_drift.argtypes = [ctypes.POINTER(_Geom), _dp, ctypes.c_double, _dp, _dp]
         l.nspt_adjoint_drift_dev.argtypes = [ctypes.POINTER(_Geom), ctypes.c_void_p, ctypes.c_double,
                                              ctypes.c_void_p, ctypes.c_void_p]
@@ -351,10 +353,20 @@ class Batch:
             o += nsteps * n
         return out
 
+    def set_form(self, table):
+        """Step form: False one graph branch per replica, True one launch per
+        stage over a table of the replicas."""
+        self._check(lib().nspt_batch_set_form(self.h, 1 if table else 0), "batch_set_form")
+
+    @property
+    def table(self):
+        return lib().nspt_batch_form(self.h) == 1
+
     def tune(self, steps=3):
-        """Returns (chosen tile edge, ms per step of each candidate)."""
+        """Returns (chosen tile edge, ms per step of each candidate: the
+        branch form's tile edges, then the table form's)."""
         q = ctypes.c_int()
-        ms = np.zeros(4)
+        ms = np.zeros(8)
         self._check(lib().nspt_batch_tune(self.h, steps, ctypes.byref(q), _ptr(ms)), "batch_tune")
         return q.value, ms
 

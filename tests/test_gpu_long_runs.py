@@ -36,6 +36,7 @@ def gpu_bins(Ls, nc, N, params, therm, steps, nbin, chunk=2000):
     (replicas, nbin, N+1) bin means of the real plaquette per order."""
     nspt = _nspt()
     b = nspt.Batch(Ls, nc, N, params)
+    b.set_form(True)  # one launch per stage for all replicas
     b.evolve(therm, want_obs=False)
     per = steps // nbin
     assert per * nbin == steps and chunk % per == 0
