@@ -20,7 +20,8 @@ import sys
 import time
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-sys.path.insert(0, os.path.dirname(HERE))
+# the script's own folder would make "oracle" name oracle.py, not the package
+sys.path[:] = [os.path.dirname(HERE)] + [p for p in sys.path if os.path.abspath(p or ".") != HERE]
 from oracle import oracle as O  # noqa: E402
 
 # (lattice, orders, reference link-order updates/s on one core)
