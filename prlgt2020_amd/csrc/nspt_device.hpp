@@ -224,38 +224,22 @@ __device__ __forceinline__ void st_u(double2* base, unsigned off, double2 v) {
 //
 // ADD (with to_lds, dag false): order k is written as C[k] + scale*(A*B)_k,
 // C holding this lane's element of an addend series.
-template <int NC, int N, int OFF = 1, bool ADD = false, int Q = NSPT_QT>
-__device__ __forceinline__ double2 mul(const Lane<NC, N>& ln, double2 a0, double2 b0, bool has_b0,
-                                       double2 (&C)[N], int M, bool to_lds, double2* dst, bool dag,
-                                       double scale) {
+//
+// mul_bands is the band loop alone, for any OFF: every output index k in
+// [OFF, N) whose band runs is passed once to retire(k, v) with its Cauchy
+// sum v (exact up to M), after the last read of every operand index >= the
+// band's s; outputs below OFF are the caller's.  A caller whose operands
+// start at a higher order stores them from that order on (entry i of A
+// holding order i + pa, of B order i + pb) and takes OFF = pa + pb - 1, so
+// that no tile is spent on the zero orders.
+template <int NC, int N, int OFF, int Q, class Retire>
+__device__ __forceinline__ void mul_bands(const Lane<NC, N>& ln, int M, Retire&& retire) {
   constexpr int NE = NC * NC;
   static_assert(Q <= NSPT_QT_MAX, "LDS padding covers tiles up to NSPT_QT_MAX");
   constexpr int NO = 2 * Q - 1;
   M = uniform(M);
   const double2* pa = ln.slotA + ln.r * NC;
   const double2* pb = ln.slotB + ln.c;
-  const double2* own_a = ln.slotA + ln.e;
-  const double2* own_b = ln.slotB + ln.e;
-  const int pos = dag ? (ln.c * NC + ln.r) : ln.e;
-  const double sx = scale, sy = dag ? -scale : scale;
-  // Order k is final with value v (flag terms still to add).  The flag
-  // terms read order k of both operands before it is overwritten.
-  auto retire = [&](int k, double2 v) {
-    if (OFF == 1) {
-      v = c_add(v, c_mul(a0, own_b[k * NE]));
-      if (has_b0) v = c_add(v, c_mul(b0, own_a[k * NE]));
-    }
-    if (to_lds) {
-      if (ln.lds_ok) {
-        if (ADD)
-          dst[k * NE + pos] = make_double2(fma(sx, v.x, C[k].x), fma(sy, v.y, C[k].y));
-        else
-          dst[k * NE + pos] = make_double2(sx * v.x, sy * v.y);
-      }
-    } else {
-      C[k] = c_add(C[k], v);
-    }
-  };
   double2 acc[NO];
 #pragma unroll
   for (int o = 0; o < NO; o++) acc[o] = zero2();
@@ -324,11 +308,41 @@ __device__ __forceinline__ double2 mul(const Lane<NC, N>& ln, double2 a0, double
       for (int o = NO - 1; o >= 0; o--) acc[o] = (o >= Q) ? acc[o - Q] : zero2();
     }
   }
-  // outputs OFF .. OFF+QT-2 (carried past band 0, or flag terms only), and
-  // the flag-only index 0 of the flagged form
+  // outputs OFF .. OFF+QT-2 (carried past band 0, or flag terms only)
 #pragma unroll
   for (int d = 0; d + 1 < Q; d++)
     if (OFF + d < N) retire(OFF + d, acc[Q + d]);
+}
+
+template <int NC, int N, int OFF = 1, bool ADD = false, int Q = NSPT_QT>
+__device__ __forceinline__ double2 mul(const Lane<NC, N>& ln, double2 a0, double2 b0, bool has_b0,
+                                       double2 (&C)[N], int M, bool to_lds, double2* dst, bool dag,
+                                       double scale) {
+  constexpr int NE = NC * NC;
+  const double2* own_a = ln.slotA + ln.e;
+  const double2* own_b = ln.slotB + ln.e;
+  const int pos = dag ? (ln.c * NC + ln.r) : ln.e;
+  const double sx = scale, sy = dag ? -scale : scale;
+  // Order k is final with value v (flag terms still to add).  The flag
+  // terms read order k of both operands before it is overwritten.
+  auto retire = [&](int k, double2 v) {
+    if (OFF == 1) {
+      v = c_add(v, c_mul(a0, own_b[k * NE]));
+      if (has_b0) v = c_add(v, c_mul(b0, own_a[k * NE]));
+    }
+    if (to_lds) {
+      if (ln.lds_ok) {
+        if (ADD)
+          dst[k * NE + pos] = make_double2(fma(sx, v.x, C[k].x), fma(sy, v.y, C[k].y));
+        else
+          dst[k * NE + pos] = make_double2(sx * v.x, sy * v.y);
+      }
+    } else {
+      C[k] = c_add(C[k], v);
+    }
+  };
+  mul_bands<NC, N, OFF, Q>(ln, M, retire);
+  // the flag-only index 0 of the flagged form
   if (OFF == 1) retire(0, zero2());
   return OFF == 1 ? c_mul(a0, b0) : zero2();
 }
@@ -381,6 +395,163 @@ __device__ __forceinline__ void horner(const Lane<NC, N>& ln, int n) {
     }
   }
   __syncthreads();
+}
+
+// ------------------------------------------------------------ exp / log by Paterson-Stockmeyer
+// Instances of at least NSPT_PS_MIN_N orders evaluate exp and log as
+// p(X) = sum_{i <= n} c_i X^i in the Paterson-Stockmeyer form (Paterson and
+// Stockmeyer, SIAM J. Comput. 2 (1973) 60) with Y = X^3:
+//   X^2 = X X,  Y = X^2 X,
+//   R_J = q_J,  R_j = q_j + R_{j+1} Y  (j = J-1 .. 0, J = n/3),  p(X) = R_0,
+//   q_j = c_{3j} + c_{3j+1} X + c_{3j+2} X^2,
+// R_j truncated at order n - 3j (it is multiplied by Y j more times).  The
+// q_j cost no product (each lane adds its own elements), and every product
+// is stored from its operands' lowest order on (X^2 from order 2, Y from
+// order 3), so no tile is spent on their zero orders: at n = 20, 739 pairs
+// of orders against the 1330 of the Horner form.  The same terms are summed
+// in another association than the reference's Horner loop (QCDpt.h:176-220):
+// as accurate (against an extended-precision evaluation, exp to 2e-16 and
+// log within the Horner form's own error), but not bit-alike, so the
+// instances up to 16 orders, whose parity is held to the reference's
+// association, keep the Horner form.
+#ifndef NSPT_PS_MIN_N
+#define NSPT_PS_MIN_N 17
+#endif
+template <int N>
+constexpr bool use_ps() {
+  return N >= NSPT_PS_MIN_N;
+}
+// tile edge of its products
+#ifndef NSPT_QT_PS
+#define NSPT_QT_PS NSPT_QT
+#endif
+
+// 1/i! with two roundings: i! is exact in double up to 22!, and i!/22! is
+// an exact integer up to i = 32.
+__host__ __device__ constexpr double inv_factorial(int i) {
+  double f = 1.0, g = 1.0;
+  for (int k = 2; k <= i; k++) {
+    if (k <= 22)
+      f *= k;
+    else
+      g *= k;
+  }
+  return (1.0 / f) / g;
+}
+struct InvFactorials {
+  double v[33];
+  constexpr InvFactorials() : v() {
+    for (int i = 0; i < 33; i++) v[i] = inv_factorial(i);
+  }
+};
+__constant__ constexpr InvFactorials kInvFact{};
+
+// c_i of exp (1/i!) or of log(1 + X) ((-1)^{i+1}/i, c_0 = 0); 0 above n.
+template <bool IS_EXP>
+__device__ __forceinline__ double series_coef(int i, int n) {
+  if (i > n) return 0.0;
+  if (IS_EXP) return kInvFact.v[i];
+  return i == 0 ? 0.0 : ((i & 1) ? 1.0 : -1.0) / (double)i;
+}
+
+// p(X) for the generator X (its flag ignored, as in horner()), given as this
+// lane's element of every order; both slots are used.  On return slot A
+// holds orders 1..n of exp(X) (IS_EXP, flag 1) or log(1 + X) (flag 0).
+template <int NC, int N, bool IS_EXP>
+__device__ __forceinline__ void ps_series(const Lane<NC, N>& ln, int n, const double2 (&X)[N]) {
+  constexpr int NE = NC * NC;
+  constexpr int Q = NSPT_QT_PS;
+  n = uniform(n);
+  double2* const sa = ln.slotA;
+  double2* const sb = ln.slotB;
+  const int e = ln.e;
+  const bool own = ln.lds_ok;
+  // X^2 = X X into registers (order 1, index 0, is zero)
+  double2 X2[N];
+#pragma unroll
+  for (int k = 0; k < N; k++) X2[k] = zero2();
+  ln.stage(sa, X);
+  ln.stage(sb, X);
+  __syncthreads();
+  mul_bands<NC, N, 1, Q>(ln, n - 1, [&](int k, double2 v) { X2[k] = v; });
+  __syncthreads();
+  // X^2 from order 2 on into slot A; Y = X^2 X formed in place in slot B,
+  // from order 3 on
+  if (own) {
+#pragma unroll
+    for (int k = 1; k < N; k++) sa[(k - 1) * NE + e] = X2[k];
+  }
+  __syncthreads();
+  mul_bands<NC, N, 2, Q>(ln, n - 1, [&](int k, double2 v) {
+    if (own) sb[(k - 2) * NE + e] = v;
+  });
+  // Horner in Y, R_j formed in place in slot A (its flag r is c_{3j})
+  const int J = n / 3;
+  double r = series_coef<IS_EXP>(3 * J, n);
+  {
+    const double c1 = series_coef<IS_EXP>(3 * J + 1, n), c2 = series_coef<IS_EXP>(3 * J + 2, n);
+    __syncthreads();
+    if (own) {
+#pragma unroll
+      for (int k = 0; k < N; k++) sa[k * NE + e] = c_add(c_scale(c1, X[k]), c_scale(c2, X2[k]));
+    }
+  }
+  for (int j = J - 1; j >= 0; j--) {
+    __syncthreads();
+    const double c1 = series_coef<IS_EXP>(3 * j + 1, n), c2 = series_coef<IS_EXP>(3 * j + 2, n), rj = r;
+    // order k+1 of q_j + r_{j+1} Y, this lane's element
+    auto addend = [&](int k) {
+      double2 t = c_add(c_scale(c1, X[k]), c_scale(c2, X2[k]));
+      if (k >= 2) t = c_add(t, c_scale(rj, sb[(k - 2) * NE + e]));
+      return t;
+    };
+    mul_bands<NC, N, 3, Q>(ln, n - 3 * j - 1, [&](int k, double2 v) {
+      if (own) sa[k * NE + e] = c_add(v, addend(k));
+    });
+    // orders 1..3 lie below the product's lowest order
+    if (own) {
+#pragma unroll
+      for (int k = 0; k < 3 && k < N; k++) sa[k * NE + e] = addend(k);
+    }
+    r = series_coef<IS_EXP>(3 * j, n);
+  }
+  __syncthreads();
+}
+
+// exp(X)*B by ps_series and one product: X (flag 0) and B (flag bflag) as
+// this lane's elements of every order.  On return slot B holds exp(X)*B
+// (flag bflag), as after horner_exp_times().
+template <int NC, int N>
+__device__ __forceinline__ void ps_exp_times(const Lane<NC, N>& ln, int n, const double2 (&X)[N], double2 bflag,
+                                             const double2 (&bv)[N]) {
+  ps_series<NC, N, true>(ln, n, X);
+  ln.stage(ln.slotB, bv);
+  __syncthreads();
+  mul_lds<NC, N>(ln, one2(), bflag, true, n - 1, ln.slotB, false);
+  __syncthreads();
+}
+
+// The form each instance takes: exp(X)*B into slot B (X, B as lane
+// elements; the Horner form stages them itself) ...
+template <int NC, int N>
+__device__ __forceinline__ void series_exp_times(const Lane<NC, N>& ln, int n, double2 (&X)[N], double2 bflag,
+                                                 double2 (&bv)[N]) {
+  if constexpr (use_ps<N>()) {
+    ps_exp_times<NC, N>(ln, n, X, bflag, bv);
+  } else {
+    ln.stage(ln.slotA, X);
+    ln.stage(ln.slotB, bv);
+    horner_exp_times<NC, N>(ln, n, bflag, bv);
+  }
+}
+// ... and exp(X) or log(1 + X) into slot A, X staged in slot B by the caller
+// and also given as lane elements.
+template <int NC, int N, bool IS_EXP>
+__device__ __forceinline__ void series_fn(const Lane<NC, N>& ln, int n, const double2 (&X)[N]) {
+  if constexpr (use_ps<N>())
+    ps_series<NC, N, IS_EXP>(ln, n, X);
+  else
+    horner<NC, N, IS_EXP>(ln, n);
 }
 
 // reH of order k of the series in an LDS slot (QCDpt.cc:197-209):

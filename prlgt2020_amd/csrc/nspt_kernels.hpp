@@ -780,16 +780,14 @@ __device__ __forceinline__ void k_update_body(int blk_, int nblk_, const Geo& g,
   gload<NC, N>(Ubase, idx, ln.e, nrt, ok, S);
   const double2 ubf = gflag(Ubase, idx, ok);
   if (eta) W[0] = c_sub(W[0], c_scale(stau, eta[idx * NE + ln.e]));
-  ln.stage(ln.slotA, W);
-  ln.stage(ln.slotB, S);
-  // Unew = exp(W) * Ubase by Horner from the right, in place in slot B
-  horner_exp_times<NC, N>(ln, n, ubf, S);
+  // Unew = exp(W) * Ubase, in slot B
+  series_exp_times<NC, N>(ln, n, W, ubf, S);
   unstage(ln, ln.slotB, S);
   gstore<NC, N>(Uout, idx, ln.e, nrt, ok, S);
   if (ok && ln.e == 0) Uout.flag[idx] = ubf;
   if (!L.data) return;
   // L = reH(log(Unew)): generator Unew - 1 is already in slot B
-  horner<NC, N, false>(ln, n);
+  series_fn<NC, N, false>(ln, n, S);
 #pragma unroll
   for (int k = 0; k < N; k++) S[k] = reh_elem(ln, ln.slotA, k);
   gstore<NC, N>(L, idx, ln.e, nrt, ok, S);
@@ -829,8 +827,13 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_log(Geo g, SerField U
   const bool ok = ln.lds_ok && (site0 + ln.j) < g.V;
   const long idx = (long)mu * g.V + site;
   double2 S[N];
-  stage_global(ln, ln.slotB, U, idx, nrt);
-  horner<NC, N, false>(ln, min(N, nrt));
+  if constexpr (use_ps<N>()) {
+    gload<NC, N>(U, idx, ln.e, nrt, ok, S);
+    ps_series<NC, N, false>(ln, min(N, nrt), S);
+  } else {
+    stage_global(ln, ln.slotB, U, idx, nrt);
+    horner<NC, N, false>(ln, min(N, nrt));
+  }
   unstage(ln, ln.slotA, S);
   gstore<NC, N>(L, idx, ln.e, nrt, ok, S);
   if (ok && ln.e == 0 && L.flag) L.flag[idx] = zero2();
@@ -864,7 +867,7 @@ __device__ __forceinline__ void k_zero_mode_body(int blk_, int nblk_, const Geo&
   for (int k = 0; k < N; k++) P[k] = reh_elem(ln, ln.slotA, k);
   gstore<NC, N>(X, idx, ln.e, nrt, ok, P);
   ln.stage(ln.slotB, P);
-  horner<NC, N, true>(ln, min(N, nrt));
+  series_fn<NC, N, true>(ln, min(N, nrt), P);
   unstage(ln, ln.slotA, P);
   gstore<NC, N>(U, idx, ln.e, nrt, ok, P);
   if (ok && ln.e == 0) U.flag[idx] = one2();
@@ -903,7 +906,7 @@ __device__ __forceinline__ void k_fa_gexp_body(int blk_, int nblk_, const Geo& g
 #pragma unroll
   for (int k = 0; k < N; k++) P[k] = reh_elem(ln, ln.slotA, k);
   ln.stage(ln.slotB, P);
-  horner<NC, N, true>(ln, min(N, nrt));
+  series_fn<NC, N, true>(ln, min(N, nrt), P);
   unstage(ln, ln.slotA, P);
   gstore<NC, N>(G, site, ln.e, nrt, ok, P);
   if (ok && ln.e == 0) G.flag[site] = one2();
@@ -925,6 +928,12 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_fa_gexp_tab(const Gex
 // Xexp.data set, U_mu(x) is not read but formed as exp(Xexp - MM_mu) (MM may
 // be null): the field after zero-mode removal (nspt.cc:638-641), from the
 // projected log L of the update and the mean MM (layout as in k_zero_mode).
+#ifndef NSPT_PS_TRANSFORM_PREFETCH
+#define NSPT_PS_TRANSFORM_PREFETCH 0
+#endif
+#ifndef NSPT_PS_TRANSFORM_CHUNK
+#define NSPT_PS_TRANSFORM_CHUNK 1
+#endif
 template <int NC, int N>
 __device__ __forceinline__ void k_gauge_transform_body(int blk_, int nblk_, const Geo& g, SerField G, SerField U,
                                                                            SerField Xexp, const double2* MM) {
@@ -958,15 +967,25 @@ __device__ __forceinline__ void k_gauge_transform_body(int blk_, int nblk_, cons
     __syncthreads();
     unstage(ln, ln.slotB, Q);  // this lane's element of G(x+mu)^dag
     // G(x) is loaded into the free registers while the exponential runs
-    // (above 24 orders those registers are not free: it is loaded after)
-    constexpr bool PF = N <= 24;
-    if constexpr (PF) gload<NC, N>(G, site, ln.e, nrt, ok, P);
+    // (above 24 orders those registers are not free: it is loaded after;
+    // the Paterson-Stockmeyer form holds its generator in them)
+    constexpr bool PS = use_ps<N>();
+    constexpr bool PF = N <= 24 && (!PS || NSPT_PS_TRANSFORM_PREFETCH);
     const double2 gf = gflag(G, site, ok);
-    horner_exp_times<NC, N>(ln, M + 1, gpf, Q);
-    if constexpr (PF)
+    double2 Gx[PS && PF ? N : 1];
+    if constexpr (PS) {
+      if constexpr (PF) gload<NC, N>(G, site, ln.e, nrt, ok, Gx);
+      ps_exp_times<NC, N>(ln, M + 1, P, gpf, Q);
+    } else {
+      if constexpr (PF) gload<NC, N>(G, site, ln.e, nrt, ok, P);
+      horner_exp_times<NC, N>(ln, M + 1, gpf, Q);
+    }
+    if constexpr (PF && PS)
+      ln.stage(ln.slotA, Gx);
+    else if constexpr (PF)
       ln.stage(ln.slotA, P);
     else
-      stage_global<NC, N, 1>(ln, ln.slotA, G, site, nrt);
+      stage_global<NC, N, PS ? NSPT_PS_TRANSFORM_CHUNK : 1>(ln, ln.slotA, G, site, nrt);
     __syncthreads();
     const double2 nf = mul_lds<NC, N>(ln, gf, gpf, true, M, ln.slotB, false);
     __syncthreads();
@@ -1061,7 +1080,7 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_sgf(Geo g, SerField U
 #pragma unroll
   for (int k = 0; k < N; k++) W[k] = c_scale(alpha, W[k]);
   ln.stage(ln.slotB, W);
-  horner<NC, N, true>(ln, M + 1);
+  series_fn<NC, N, true>(ln, M + 1, W);
   unstage(ln, ln.slotA, W);  // W holds G from here on
   __syncthreads();
   for (int mu = 0; mu < 4; mu++) {
