@@ -45,23 +45,64 @@ def ex_flops(N, nc=3):
 
 ACTIONS = {"wil": 0.0, "tls": -1.0 / 12.0, "iwa": -0.331, "dbw": -1.4088}  # c1, Quenched.cc:313-322
 
+# instances of at least this many orders evaluate exp and log in the
+# Paterson-Stockmeyer form (NSPT_PS_MIN_N of the library's build)
+PS_MIN_N = 17
 
-def kernel_flops(name, N, V, nc=3, improved=False, staples=False):
-    """Algorithmic FP64 flops of one launch of a kernel (4V links or V sites).
+
+def pairs(T, p, q):
+    """Pairs of orders (i >= p, j >= q, i + j <= T) of a truncated product."""
+    m = T - p - q + 1
+    return m * (m + 1) // 2 if m > 0 else 0
+
+
+def ps_flops(N, nc=3):
+    """Executed flops of one exp or log in the Paterson-Stockmeyer form: X^2,
+    Y = X^3 and the N/3 steps in Y, each product over the pairs of its
+    operands' nonzero orders (a matrix product of two orders is 216 flops,
+    64 for SU(2)), plus each step's lane-local addend (12 flops per element
+    and order)."""
+    J = N // 3
+    pr = pairs(N, 1, 1) + pairs(N, 1, 2) + sum(pairs(N - 3 * j, 3, 1) for j in range(J))
+    return (216 if nc == 3 else 64) * pr + 12 * nc * nc * N * J
+
+
+def kernel_flops(name, N, V, nc=3, improved=False, cb_update=False, cb_gauge=False, fused=True, executed=False):
+    """FP64 flops of one launch of the kernel behind a profile name.
+
+    Canonical (executed=False): the reference's algorithm, a product SP, an
+    exp or log EX (Horner), exp(X) B = EX + SP.  Executed: what this
+    build's kernel forms, where exp(X) B by Horner from the right costs EX
+    alone and above 16 orders exp and log are ps_flops.
     The Jacobi drift is the plaquette field (3 products per plane and site,
-    4.5 per link) plus 6 products per link; the checkerboard sweep's drift
-    (staples=True) forms the staple sum: 13 products per link.  An improved
-    action adds U R to the drift and 72 products per link (six 5-link staples
-    in each of three planes, QCDenvNODEpt_test.h:591-607) in the rectangle
-    kernel, counted as the reference forms them."""
+    4.5 per link) plus 6 products per link; the checkerboard sweep
+    (cb_update) forms the staple sum, 13 products per link, one launch per
+    parity and direction (V/2 links); the checkerboard gauge fixing
+    (cb_gauge) is one launch per parity (an exp and 8 products per site).
+    With zero modes folded into the FA transform (fused), that kernel also
+    forms the zero-mode exponential.  An improved
+    action adds U R to the drift; the rectangle kernels count 72 products
+    per link as the reference forms them (six 5-link staples in each of three
+    planes, QCDenvNODEpt_test.h:591-607) and execute 38."""
     SP, EX = sp_flops(N, nc), ex_flops(N, nc)
-    drift = (13 if staples else 6) + (1 if improved else 0)
-    per_link = {"drift": drift * SP, "plaq_field": 4.5 * SP, "update": EX + SP, "update_log": 2 * EX + SP,
-                "zero_mode": EX, "transform": 2 * SP, "rect": 72 * SP}
+    if executed:
+        ps = N >= PS_MIN_N
+        E = ps_flops(N, nc) if ps else EX
+        ET = E + SP if ps else EX
+        rect = 38 * SP
+    else:
+        E, ET, rect = EX, EX + SP, 72 * SP
+    drift = (13 if cb_update else 6) + (1 if improved else 0)
+    if cb_update and name in ("drift", "update", "update_log"):
+        return {"drift": drift * SP, "update": ET, "update_log": ET + E}[name] * V / 2
+    if cb_gauge and name == "transform":
+        return (E + 8 * SP) * V / 2
+    per_link = {"drift": drift * SP, "plaq_field": 4.5 * SP, "update": ET, "update_log": ET + E,
+                "zero_mode": E, "transform": (ET + SP) if fused else 2 * SP, "rect": rect}
     if name in per_link:
         return per_link[name] * 4 * V
     if name == "fa_gexp":
-        return EX * V
+        return E * V
     return 0
 
 
@@ -333,23 +374,28 @@ def main():
     # the log), so launches, time and flops are summed per function first
     uses_pf = bool(prof.get("plaq_field", (0, 0))[1])
     groups = {}
+    step_exec = 0.0  # executed flops of the timed launches
     for pname, (ms, c) in prof.items():
         if not c:
             continue
         fname = kernel_function(pname, uses_pf, args.gauge != "fa")
-        fl1 = kernel_flops(pname, N, V, nc, improved, staples=integ == nspt.EULER_CHECKERBOARD)
-        if integ == nspt.EULER_CHECKERBOARD and pname in ("drift", "update", "update_log"):
-            fl1 /= 8  # one launch per (parity, direction): V/2 of the 4V links
-        g = groups.setdefault(fname, [0.0, 0, 0.0, []])
+        how = dict(cb_update=integ == nspt.EULER_CHECKERBOARD, cb_gauge=args.gauge != "fa", fused=args.gauge == "fa")
+        fl1 = kernel_flops(pname, N, V, nc, improved, **how)
+        fx1 = kernel_flops(pname, N, V, nc, improved, executed=True, **how)
+        step_exec += fx1 * c
+        g = groups.setdefault(fname, [0.0, 0, 0.0, [], 0.0])
         g[0] += ms
         g[1] += c
         g[2] += fl1 * c
         g[3].append(pname)
+        g[4] += fx1 * c
     name = max(groups, key=lambda k: groups[k][0])
-    tot_ms, cnt, fl_tot, pnames = groups[name]
+    tot_ms, cnt, fl_tot, pnames, fx_tot = groups[name]
     avg_s = tot_ms / cnt / 1e3
     fl = fl_tot / cnt
     achieved = fl / avg_s / 1e12 if fl else 0.0
+    fx = fx_tot / cnt
+    executed = {"flops_per_launch": fx, "achieved": fx / avg_s / 1e12, "frac": fx / avg_s / 1e12 / PEAK_FP64_TFLOPS}
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_su3_16x4_N20_rk2.md")
     if os.path.exists(tpath) and (args.L, N, args.integrator, args.action, args.twist) == (16, 20, "rk2", "wil", 0):
@@ -360,14 +406,17 @@ def main():
             if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == name:
                 traffic = float(f[2])
     step_tf = step_flops(N, V, integ, nc, improved) * args.steps / dt / 1e12
+    step_xtf = step_exec / dt / 1e12
     # a Jacobi drift evaluation is k_plaq_field + k_drift_p: at the canonical
     # 13 products per link (14 with an improved action) over both kernels
     drift_eval = None
     if prof.get("plaq_field", (0, 0))[1] and prof["drift"][1]:
         ev_ms = (prof["plaq_field"][0] + prof["drift"][0]) / prof["drift"][1]
-        ev_fl = kernel_flops("drift", N, V, nc, improved, staples=True)
+        ev_fl = (13 + (1 if improved else 0)) * sp_flops(N, nc) * 4 * V
+        ev_fx = kernel_flops("plaq_field", N, V, nc) + kernel_flops("drift", N, V, nc, improved)
         drift_eval = {"kernels": ["k_plaq_field", "k_drift_p"], "ms": ev_ms, "canonical_flops": ev_fl,
-                      "achieved": ev_fl / ev_ms / 1e9, "frac": ev_fl / ev_ms / 1e9 / PEAK_FP64_TFLOPS}
+                      "achieved": ev_fl / ev_ms / 1e9, "frac": ev_fl / ev_ms / 1e9 / PEAK_FP64_TFLOPS,
+                      "executed_flops": ev_fx, "executed_frac": ev_fx / ev_ms / 1e9 / PEAK_FP64_TFLOPS}
 
     out = {
         "metric": "link-order updates/s per Langevin step (FP64), plain single replica",
@@ -400,6 +449,12 @@ def main():
                      "traffic_gbs": traffic / avg_s / 1e9 if traffic else None,
                      "traffic_frac_of_hbm_peak": traffic / avg_s / 1e9 / PEAK_HBM_GBS if traffic else None,
                      "step_fp64_tflops": step_tf, "step_frac": step_tf / PEAK_FP64_TFLOPS,
+                     "executed": executed, "step_executed_tflops": step_xtf,
+                     "step_executed_frac": step_xtf / PEAK_FP64_TFLOPS,
+                     "flop_model": "achieved, frac and step_frac price every kernel at the reference's algorithm "
+                                   "(a product SP, an exp or log EX by Horner, 13 SP per drift); the executed "
+                                   "figures price what these kernels form (the plaquette-field drift, exp(X)B by "
+                                   "Horner from the right, and above 16 orders exp and log by Paterson-Stockmeyer)",
                      "drift_evaluation": drift_eval},
         "kernels_ms_per_step": {k: v[0] / args.steps for k, v in prof.items() if v[1]},
         "plaquette_order2": float(plaq[2].real),

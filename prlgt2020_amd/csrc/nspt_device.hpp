@@ -421,9 +421,10 @@ template <int N>
 constexpr bool use_ps() {
   return N >= NSPT_PS_MIN_N;
 }
-// tile edge of its products
+// tile edge of its products: they are longer than the Horner steps, and the
+// larger tile pays (as in the drift)
 #ifndef NSPT_QT_PS
-#define NSPT_QT_PS NSPT_QT
+#define NSPT_QT_PS NSPT_QT_DRIFT
 #endif
 
 // 1/i! with two roundings: i! is exact in double up to 22!, and i!/22! is
