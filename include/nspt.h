@@ -159,6 +159,18 @@ int nspt_stochastic_gauge_fixing(nspt_ctx* ctx, double alpha);
  * (nspt.cc:866-948). */
 int nspt_fa_gauge_fixing(nspt_ctx* ctx, double alpha);
 
+/* Momentum norms of the __K_MOM_ANALYSIS__ build (nspt.cc:919-944, after the
+ * gauge fixing of nspt.cc:833-857 and 919-944): the forward 4-D DFT of every
+ * component of the field (planGluon[0], Quenched.cc:792-815: FFTW_FORWARD,
+ * unnormalised), then per order k = 1..ptord and momentum p
+ *   knorm[(k-1)*V + p] = sum_mu Re Tr(Ut_mu,k(p) Ut_mu,k(p)^dag),
+ * p in the lattice's site order.  ptord * V doubles, the layout of the
+ * reference's per-step record in knorm_*.dat (Quenched.cc:549-560), whose
+ * fwrite reads knorm[0..allocORD-1] as one block.  The reference transforms
+ * the field back and multiplies it by 1/V (a change by rounding only); here
+ * the field is not touched.  Blocks until the values are on the host. */
+int nspt_momentum_norm(nspt_ctx* ctx, double* knorm);
+
 /* Langevin drift tau_g*reH(U*staples), tau_g = -eps/(2 nc), of the current
  * field for every link, in the host image layout with flag 0
  * (nspt.cc:383-417). */

@@ -20,6 +20,7 @@
 //                                              Quenched.cc:827-838 restated as a direct DFT)
 //   * plaquette_measure (OpenMP normalisation) nspt.cc:1316-1406
 //   * ComputeLoops (1x2, 2x2, 2x4, 4x4 loops)  nspt.cc:1156-1312
+//   * knorm (__K_MOM_ANALYSIS__)               nspt.cc:919-944, Quenched.cc:792-815
 //   * noise: SU3rand / SU2rand                 MyMath.cc:256-368
 //   * RAND55 + Box-Muller                      MyRand.h:81-108, MyRand.cc:73-130
 //   * ranlxd (portable path)                   ranlxd/ranlxd.cc:332-601
@@ -1197,6 +1198,39 @@ void plaquette_measure(const Geom& g, const double* Udata, double* out) {
   }
 }
 
+// __K_MOM_ANALYSIS__ momentum norms (nspt.cc:919-944): the forward DFT of
+// every component of the field, as planGluon[0] (Quenched.cc:792-815: the
+// 4 (1 + nc^2 N) complex of a site, FFTW_FORWARD, unnormalised), then per
+// order k = 1..N and momentum p (site order)
+//   out[(k-1) V + p] = sum_mu Re Tr(Ut_mu,k(p) Ut_mu,k(p)^dag),
+// the trace as the sum over rows of Re (A A^dag)_rr.  The field is read only
+// (the reference transforms back and multiplies by 1/V, nspt.cc:936-941).
+void knorm(const Geom& g, const double* Udata, double* out) {
+  Latt Z(g.L);
+  const int S = 1 + g.nc * g.nc * g.N, NE = g.nc * g.nc;
+  std::vector<cx> T((size_t)Z.V * 4 * S);
+  for (size_t i = 0; i < T.size(); i++) T[i] = cx{Udata[2 * i], Udata[2 * i + 1]};
+  dft_many(g.L, 4 * S, 4 * S, T.data(), -1);
+  for (int k = 0; k < g.N; k++)
+    for (int p = 0; p < Z.V; p++) {
+      double acc = 0.0;
+      for (int mu = 0; mu < 4; mu++) {
+        const cx* m = T.data() + (size_t)(4 * p + mu) * S + 1 + (size_t)k * NE;
+        double tr = 0.0;
+        for (int r = 0; r < g.nc; r++) {
+          double d = 0.0;
+          for (int c = 0; c < g.nc; c++) {
+            const cx v = m[r * g.nc + c];
+            d += v.re * v.re + v.im * v.im;  // Re (v conj(v))
+          }
+          tr += d;
+        }
+        acc += tr;
+      }
+      out[(size_t)k * Z.V + p] = acc;
+    }
+}
+
 // ComputeLoops (nspt.cc:1156-1312): per plane mu < nu and site x, the sums
 //   w21 += 2x1 + 1x2 rectangles, w22 += 2x2, w42 += 4x2 + 2x4, w44 += 4x4,
 // with the reference's neighbour chain (a .. t) and product order; out gets
@@ -1451,6 +1485,7 @@ void orc_set_twist(int nc, int mask, const double* om) {
 void orc_plaquette_measure(const int* gi, const double* U, double* out) {
   plaquette_measure(geom_from(gi), U, out);
 }
+void orc_knorm(const int* gi, const double* U, double* out) { knorm(geom_from(gi), U, out); }
 
 // data: V * stride complex; transforms components 0..howmany-1.
 void orc_dft_many(const int* L, int howmany, int stride, double* data, int sign) {

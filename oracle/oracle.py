@@ -296,6 +296,15 @@ def plaquette_measure(L, nc, N, U):
     return out
 
 
+def knorm(L, nc, N, U):
+    """Momentum norms of the __K_MOM_ANALYSIS__ build (nspt.cc:919-944):
+    array (N, V), row k-1 = sum_mu Re Tr(Ut Ut^dag) of order k per momentum."""
+    V = int(np.prod(L))
+    out = np.zeros((N, V), np.float64)
+    lib().orc_knorm(_geom(L, nc, N), _ptr(U), _ptr(out))
+    return out
+
+
 def compute_loops(L, nc, N, U):
     """Tr of the ComputeLoops sums w21, w22, w42, w44 (nspt.cc:1156-1312):
     array (4, N+1) complex, unnormalised."""

@@ -301,10 +301,15 @@ __global__ void k_obs_finalize_tab(const ObsFinArgs* __restrict__ t) {
 //           p^2 = sum_mu 4 sin^2(pi k_mu / L_mu) and the k = 0 mode zeroed,
 //           then backward FFT along 1 and 0;
 //   pass 2: backward FFT along 2 and 3.
+// The momentum norms of the whole field (nspt_momentum_norm) take two more
+// passes over the planes of one link direction:
+//   pass 3: the plane of the field itself (Xsrc + comp 4V), forward FFT
+//           along 2 and 3;
+//   pass 4: forward FFT along 0 and 1.
 // A block (one wave) holds one tile of L_da x L_db x C sites in LDS, C
-// consecutive values of a third direction: direction 3 in pass 1 (sites
-// (x0, x1) x C consecutive x3), direction 1 in passes 0 and 2 (C consecutive
-// (x2, x3) planes), so every load and store is a run of consecutive sites;
+// consecutive values of a third direction: direction 3 in passes 1 and 4
+// (sites (x0, x1) x C consecutive x3), direction 1 in passes 0, 2 and 3 (C
+// consecutive (x2, x3) planes), so every load and store is a run of consecutive sites;
 // C makes about 64 lines per axis, one per lane.  Each line is transformed by
 // one lane in registers: a mixed-radix Stockham FFT unrolled at compile time
 // for the extents in NSPT_FFT_SIZES, a direct DFT read from LDS for any
@@ -517,20 +522,22 @@ struct FftTile {
 
 // C: the largest divisor of the batch direction's extent that keeps the
 // tile within FFT_MAXE sites and the lines per axis within one per lane.
+__host__ __device__ inline bool fft_axes01(int pass) { return pass == 1 || pass == 4; }
 __host__ __device__ inline FftTile fft_tile(const Geo& g, int pass) {
   FftTile t;
-  t.da = pass == 1 ? 0 : 2;
-  t.db = pass == 1 ? 1 : 3;
+  const bool p01 = fft_axes01(pass);
+  t.da = p01 ? 0 : 2;
+  t.db = p01 ? 1 : 3;
   t.na = g.L[t.da];
   t.nb = g.L[t.db];
-  const int Lc = g.L[pass == 1 ? 3 : 1];
+  const int Lc = g.L[p01 ? 3 : 1];
   const int mn = t.na < t.nb ? t.na : t.nb;
   t.C = 1;
   for (int c = 2; c <= Lc; c++)
     if (Lc % c == 0 && t.na * t.nb * c <= FFT_MAXE && mn * c <= FFT_THREADS) t.C = c;
   t.E = t.na * t.nb * t.C;
   t.tiles = g.V / t.E;
-  if (pass == 1) {
+  if (p01) {
     t.ra = t.nb * t.C + 1;
     t.cs = 1;
     t.words = t.na * t.ra;  // the last offset is na*ra - 2
@@ -556,15 +563,16 @@ __device__ __forceinline__ void fa_fft_body(int blk, const Geo& g, int pass, con
                                             double alpha) {
   extern __shared__ double2 sm[];
   const FftTile t = fft_tile(g, pass);
+  const bool p01 = fft_axes01(pass);
   const long V = g.V;
   const long comp = blk / t.tiles;
   const int tt = (int)(blk % t.tiles);
   const int L0 = g.L[0], L1 = g.L[1], L2 = g.L[2], L3 = g.L[3];
-  // tile origin: (x2, first x3) in pass 1, (x0, first x1) in passes 0 and 2
-  const int nch = (pass == 1 ? L3 : L1) / t.C;
+  // tile origin: (x2, first x3) in passes 1 and 4, (x0, first x1) otherwise
+  const int nch = (p01 ? L3 : L1) / t.C;
   const int o0 = tt / nch;
   const int o1 = (tt % nch) * t.C;
-  const long base = pass == 1 ? (long)o0 * g.st[2] + o1 : (long)o0 * g.st[0] + (long)o1 * g.st[1];
+  const long base = p01 ? (long)o0 * g.st[2] + o1 : (long)o0 * g.st[0] + (long)o1 * g.st[1];
   FftShared sh;
   sh.twa = sm;
   sh.twb = sm + t.na;
@@ -623,7 +631,7 @@ __device__ __forceinline__ void fa_fft_body(int blk, const Geo& g, int pass, con
         // compiler keeps the loads together)
         const int l = min(l0 + u * FFT_THREADS, t.E - 1);
         int ia, ib, ic, site, off;
-        if constexpr (PASS == 1) {
+        if constexpr (PASS == 1 || PASS == 4) {
           ic = l % t.C;
           ib = (l / t.C) % t.nb;
           ia = l / (t.C * t.nb);
@@ -649,7 +657,8 @@ __device__ __forceinline__ void fa_fft_body(int blk, const Geo& g, int pass, con
           wx[u] = w.x;
           wy[u] = w.y;
         } else {
-          const double2 w = line[site];
+          // pass 3 reads the plane of the source field itself
+          const double2 w = PASS == 3 ? xc[site] : line[site];
           wx[u] = w.x;
           wy[u] = w.y;
         }
@@ -662,11 +671,15 @@ __device__ __forceinline__ void fa_fft_body(int blk, const Geo& g, int pass, con
     load_tile(std::integral_constant<int, 0>());
   else if (pass == 1)
     load_tile(std::integral_constant<int, 1>());
-  else
+  else if (pass == 2)
     load_tile(std::integral_constant<int, 2>());
+  else if (pass == 3)
+    load_tile(std::integral_constant<int, 3>());
+  else
+    load_tile(std::integral_constant<int, 4>());
   __syncthreads();
   FftAxis axa, axb;  // lines (ib, ic) along a, (ia, ic) along b
-  if (pass == 1) {
+  if (p01) {
     axa = FftAxis{t.na, t.nb, t.C, t.ra, t.C, 1, true};
     axb = FftAxis{t.nb, t.na, t.C, t.C, t.ra, 1, true};
   } else {
@@ -684,7 +697,7 @@ __device__ __forceinline__ void fa_fft_body(int blk, const Geo& g, int pass, con
   }
   for (int l = threadIdx.x; l < t.E; l += FFT_THREADS) {
     int site, off;
-    if (pass == 1) {
+    if (p01) {
       const int ic = l % t.C, ib = (l / t.C) % t.nb, ia = l / (t.C * t.nb);
       site = (int)(base + (long)ia * g.st[0] + (long)ib * g.st[1] + ic);
       off = ia * t.ra + ib * t.C + ic;
@@ -713,6 +726,22 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fa_fft_tab(int pass, const FftA
   const FftArgs& a = t[blockIdx.y];
   if ((int)blockIdx.x >= a.nblk) return;
   fa_fft_body<MAXN>(blockIdx.x, a.g, pass, a.Xsrc, a.fb, a.alpha);
+}
+
+// Momentum norms of one link direction (nspt.cc:925-934): out[k V + p] is
+// set (first) or increased by sum_e |fb[(k NE + e) V + p]|^2, fb holding the
+// 4-D transforms of the direction's planes; the directions are added in
+// order 0..3, the elements of a matrix in row-major order.
+__global__ void k_knorm_acc(const double2* __restrict__ fb, long V, int NE, int nrt, int first, double* out) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)nrt * V) return;
+  const long k = i / V, p = i - k * V;
+  double s = 0.0;
+  for (int e = 0; e < NE; e++) {
+    const double2 v = fb[(k * NE + e) * V + p];
+    s += v.x * v.x + v.y * v.y;
+  }
+  out[i] = first ? s : out[i] + s;
 }
 
 // Smallest compiled N >= n; among those the tile edge qt if compiled, else
@@ -786,6 +815,7 @@ struct nspt_ctx {
   TwistTab* twist_tab = nullptr;
   double *obs2_part = nullptr, *obs2_mu = nullptr;
   double2* fb = nullptr;
+  double* knorm_dev = nullptr;  // nspt_momentum_norm, allocated on first use
   double2* eta = nullptr;
   double2* eta_tmp = nullptr;
   double2* mm_part = nullptr;
@@ -949,23 +979,26 @@ bool obs_reduce(nspt_ctx* c, double* out_dev, bool with_w2 = false, long rows = 
 // FA gauge fixing of U with Xsrc = log U up to a constant per direction.
 // u_from_log: U is not in c->U but is exp(Xsrc - MM) (MM may be null),
 // formed by the transform.
-bool fa_apply(nspt_ctx* c, const SerField& Xsrc, double alpha, const double2* MM, bool u_from_log) {
+// One pass of k_fa_fft over the NE nrt planes of the FFT buffer (and of
+// Xsrc, read in passes 0 and 3 with plane stride 4V).
+bool fft_pass(nspt_ctx* c, int pass, const double2* Xsrc, double alpha) {
   const long ncomp = (long)c->nrt * c->NE;
-  for (int pass = 0; pass < 3; pass++) {
-    const FftTile t = fft_tile(c->g, pass);
-    const int maxn = std::max(t.na, t.nb) <= 16 ? 16 : FFT_MAXN;
-    const bool direct = !fft_unrolled(t.na, maxn) || !fft_unrolled(t.nb, maxn);
-    const dim3 grid((unsigned)((long)t.tiles * ncomp));
-    const size_t shm = fft_shm(t, c->g, direct);
-    Prof pr(c, pass == 0 ? NSPT_PROF_FA_DIV : NSPT_PROF_DFT);
-    if (maxn == 16)
-      hipLaunchKernelGGL(k_fa_fft<16>, grid, dim3(FFT_THREADS), shm, c->stream, c->g, pass,
-                         (const double2*)Xsrc.data, c->fb, alpha);
-    else
-      hipLaunchKernelGGL(k_fa_fft<FFT_MAXN>, grid, dim3(FFT_THREADS), shm, c->stream, c->g, pass,
-                         (const double2*)Xsrc.data, c->fb, alpha);
-    if (!launch_ok(c, pass == 0 ? "fa_div" : "fa_fft")) return false;
-  }
+  const FftTile t = fft_tile(c->g, pass);
+  const int maxn = std::max(t.na, t.nb) <= 16 ? 16 : FFT_MAXN;
+  const bool direct = !fft_unrolled(t.na, maxn) || !fft_unrolled(t.nb, maxn);
+  const dim3 grid((unsigned)((long)t.tiles * ncomp));
+  const size_t shm = fft_shm(t, c->g, direct);
+  Prof pr(c, pass == 0 ? NSPT_PROF_FA_DIV : NSPT_PROF_DFT);
+  if (maxn == 16)
+    hipLaunchKernelGGL(k_fa_fft<16>, grid, dim3(FFT_THREADS), shm, c->stream, c->g, pass, Xsrc, c->fb, alpha);
+  else
+    hipLaunchKernelGGL(k_fa_fft<FFT_MAXN>, grid, dim3(FFT_THREADS), shm, c->stream, c->g, pass, Xsrc, c->fb, alpha);
+  return launch_ok(c, pass == 0 ? "fa_div" : "fa_fft");
+}
+
+bool fa_apply(nspt_ctx* c, const SerField& Xsrc, double alpha, const double2* MM, bool u_from_log) {
+  for (int pass = 0; pass < 3; pass++)
+    if (!fft_pass(c, pass, (const double2*)Xsrc.data, alpha)) return false;
   {
     Prof pr(c, NSPT_PROF_FA_GEXP);
     c->ks.gexp(c->nblk_sites, c->stream, c->g, c->fb, c->G);
@@ -1478,6 +1511,26 @@ int nspt_fa_gauge_fixing(nspt_ctx* c, double alpha) {
   if (c->g.twist) return c->fail("the FA gauge fixing needs periodic boundary conditions") - 1;
   c->ks.logf(c->nblk_links, c->stream, c->g, c->U, c->L);
   if (!launch_ok(c, "log") || !fa_apply(c, c->L, alpha, nullptr, false)) return -3;
+  return c->hip_ok(hipStreamSynchronize(c->stream), "sync") ? 0 : -3;
+}
+
+int nspt_momentum_norm(nspt_ctx* c, double* knorm) {
+  if (!c || !knorm) return -1;
+  if (c->U.P != 4L * c->V) return c->fail("nspt_momentum_norm: unexpected field layout") - 1;
+  const long n = (long)c->nrt * c->V;
+  if (!c->knorm_dev && !c->alloc(&c->knorm_dev, (size_t)n)) return -2;
+  // per direction: its NE nrt planes through the FFT buffer (passes 3, 4),
+  // then their norms added
+  for (int mu = 0; mu < 4; mu++) {
+    if (!fft_pass(c, 3, (const double2*)c->U.data + (long)mu * c->V, 0.0) || !fft_pass(c, 4, (const double2*)c->U.data, 0.0))
+      return -3;
+    hipLaunchKernelGGL(k_knorm_acc, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, (const double2*)c->fb,
+                       (long)c->V, c->NE, c->nrt, mu == 0 ? 1 : 0, c->knorm_dev);
+    if (!launch_ok(c, "knorm")) return -3;
+  }
+  if (!c->hip_ok(hipMemcpyAsync(knorm, c->knorm_dev, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream),
+                 "knorm"))
+    return -3;
   return c->hip_ok(hipStreamSynchronize(c->stream), "sync") ? 0 : -3;
 }
 

@@ -513,7 +513,11 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_drift_tab(const Drift
 //   P_mu,nu(x) = U_mu(x) U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag,
 // and P_nu,mu = P_mu,nu^dag, so the six plaquettes of a site (3 products
 // each) serve all four of its links: 4.5 + 6 products per link instead of
-// the 13 of k_drift (no unitarity is assumed).
+// the 13 of k_drift.  The rotation of the lower plaquette equals the
+// reference's staple term U_mu(x) U_nu(x+mu-nu)^dag U_mu(x-nu)^dag U_nu(x-nu)
+// because U_nu(x-nu)^dag U_nu(x-nu) = 1, which the field satisfies order by
+// order: the cold start is 1, and every update and gauge transform
+// multiplies it by exponentials of antihermitian series (to rounding).
 //
 // k_plaq_field: P of plane p = (mu < nu) at every site, stored at series
 // index p V + site, planes (0,1) (0,2) (0,3) (1,2) (1,3) (2,3).
@@ -932,7 +936,7 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_fa_gexp_tab(const Gex
 #define NSPT_PS_TRANSFORM_PREFETCH 0
 #endif
 #ifndef NSPT_PS_TRANSFORM_CHUNK
-#define NSPT_PS_TRANSFORM_CHUNK 1
+#define NSPT_PS_TRANSFORM_CHUNK NSPT_STAGE_CHUNK
 #endif
 template <int NC, int N>
 __device__ __forceinline__ void k_gauge_transform_body(int blk_, int nblk_, const Geo& g, SerField G, SerField U,

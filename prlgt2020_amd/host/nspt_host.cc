@@ -15,6 +15,7 @@ act_params_t act_pars;
 mi355x_opts_t mi355x_opts = {0, 1, 1, 0, 1};
 Cplx* w = nullptr;
 std::ofstream plaqfile, normfile, trufile, loop_nxm, loop2x2;
+std::ofstream knormfile;
 
 double Cplx::mod() const { return std::sqrt(re * re + im * im); }
 
@@ -190,6 +191,14 @@ nspt_step_params step_params() {
           mi355x_opts.zero_modes, dev ? NSPT_NOISE_DEVICE : NSPT_NOISE_HOST, mi355x_opts.noise_seed, E.step};
 }
 
+// __K_MOM_ANALYSIS__ record of the current field (nspt.cc:919-944)
+void write_knorm(int V) {
+  if (!mi355x_opts.knorm || !knormfile.is_open()) return;
+  std::vector<double> rec((size_t)allocORD * V, 0.0);
+  if (nspt_momentum_norm(E.ctx, rec.data()) != 0) fail("nspt_momentum_norm");
+  knormfile.write(reinterpret_cast<const char*>(rec.data()), (std::streamsize)(sizeof(double) * rec.size()));
+}
+
 void host_noise(int V) {
   if (!E.rng) mi355x_seed_noise(mi355x_opts.noise_seed);
   E.eta.resize((size_t)4 * V * NC * NC * 2);
@@ -237,11 +246,13 @@ void zero_modes_subtraction(ptGluon_fld& Umu) {
 void stochastic_gauge_fixing(ptGluon_fld& Umu) {
   attach(Umu);
   if (nspt_stochastic_gauge_fixing(E.ctx, act_pars.alpha) != 0) fail("nspt_stochastic_gauge_fixing");
+  write_knorm(Umu.Z->Size);
 }
 
 void FAstochastic_gauge_fixing(ptGluon_fld& Umu) {
   attach(Umu);
   if (nspt_fa_gauge_fixing(E.ctx, act_pars.alpha) != 0) fail("nspt_fa_gauge_fixing");
+  write_knorm(Umu.Z->Size);
 }
 
 void NsptEvolve(ptGluon_fld& Umu, ptGluon_fld& Fmu) {
@@ -260,14 +271,20 @@ void NsptEvolve(ptGluon_fld& Umu) {
   nspt_step_params p = step_params();
   const int beat = nspt_pars.Beat;
   E.obs.resize((size_t)beat * ol);
+  // momentum norms after each step's gauge fixing: one step per call
+  const bool kn = mi355x_opts.knorm && mi355x_opts.fa != 0;
   if (p.noise == NSPT_NOISE_DEVICE) {
-    if (nspt_evolve(E.ctx, &p, beat, E.obs.data()) != 0) fail("nspt_evolve");
+    for (int t = 0; t < beat; t += kn ? 1 : beat) {
+      if (nspt_evolve(E.ctx, &p, kn ? 1 : beat, E.obs.data() + (size_t)t * ol) != 0) fail("nspt_evolve");
+      if (kn) write_knorm(Umu.Z->Size);
+    }
     E.step = p.step;
   } else {
     for (int t = 0; t < beat; t++) {
       host_noise(Umu.Z->Size);
       if (nspt_step(E.ctx, &p, E.obs.data() + (size_t)t * ol) != 0) fail("nspt_step");
       E.step++;
+      if (kn) write_knorm(Umu.Z->Size);
     }
   }
   for (int t = 0; t < beat; t++) write_step(E.obs.data() + (size_t)t * ol);

@@ -137,6 +137,7 @@ typedef struct {
   int action;        // GAUGE_ACTION (choices.h:101-113): 0 WIL, 1 TLSYM, 2 IWA, 3 DBW2
   int twist;         // TWISTED_BC (choices.h:43-55): bit d twists direction d; 0 periodic
   int wl2x2;         // __WILLOOP_2x2__ (choices.h:77): write the 2x2 loop file
+  int knorm;         // __K_MOM_ANALYSIS__: momentum norms after every gauge fixing
 } mi355x_opts_t;
 
 // The twist matrices of twistmatrices.h (not IDENTITY_TWIST), 4 x NC x NC
@@ -155,6 +156,13 @@ extern Cplx* w;
 // Output streams written once per Langevin step (nspt.cc:541-597), and the
 // Wilson-loop stream written by ComputeLoops.
 extern std::ofstream plaqfile, normfile, trufile, loop_nxm, loop2x2;
+// With mi355x_opts.knorm, every gauge fixing (nspt.cc:833-857, 919-944)
+// appends allocORD * V doubles to this binary stream: per order k =
+// 1..allocORD (zero above PTORD) and momentum p in site order, sum_mu Re
+// Tr(Ut Ut^dag) of the forward 4-D DFT of the field (nspt_momentum_norm).
+// The reference's fwrite of knorm[0] (nspt.cc:857, 944) reads its first
+// array as if the allocORD arrays were one block; this is that block.
+extern std::ofstream knormfile;
 
 // Seeds the host noise stream as the reference seeds Rand[0]: srand(seed);
 // ranlxd(rand()) (Quenched.cc:301-306), or, with device_noise 2, RAND55

@@ -108,7 +108,7 @@ int initialize(int argc, char** argv) {
   std::strcpy(nspt_pars.damon, "damocle.dag");
 
   int opt;
-  while ((opt = getopt(argc, argv, "l:s:b:d:r:g:a:t:p:c:n:I:F:Z:D:J:L:A:T:W:h")) != -1) {
+  while ((opt = getopt(argc, argv, "l:s:b:d:r:g:a:t:p:c:n:I:F:Z:D:J:L:A:T:W:K:h")) != -1) {
     switch (opt) {
       case 'l': act_pars.sz[0] = act_pars.sz[1] = act_pars.sz[2] = act_pars.sz[3] = std::atoi(optarg); break;
       case 's': nspt_pars.Sweep = std::atoi(optarg); break;
@@ -130,6 +130,7 @@ int initialize(int argc, char** argv) {
       case 'A': mi355x_opts.action = std::atoi(optarg); break;
       case 'T': mi355x_opts.twist = std::atoi(optarg); break;
       case 'W': mi355x_opts.wl2x2 = std::atoi(optarg); break;
+      case 'K': mi355x_opts.knorm = std::atoi(optarg); break;
       case 'h':
         std::printf("\n-l\t\tlattice size (L0 = L1 = L2 = L3)\n-s\t\tnumber of iterations (SWEEP)\n"
                     "-b\t\titerations between damocle reads (BEAT)\n-r\t\trandom seed\n-g\t\ttau gluons\n"
@@ -142,7 +143,8 @@ int initialize(int argc, char** argv) {
                     "-L\t\twith -t 1: start from the configuration saved with this lower allocORD\n"
                     "-A\t\tgauge action: 0 Wilson, 1 TLSYM, 2 IWA, 3 DBW2\n"
                     "-T\t\ttwisted boundary conditions: bit 0 x, 1 y, 2 z, 3 t (6: y and z)\n"
-                    "-W\t\t1: write the 2x2 Wilson loop file (__WILLOOP_2x2__)\n\n");
+                    "-W\t\t1: write the 2x2 Wilson loop file (__WILLOOP_2x2__)\n"
+                    "-K\t\t1: write the momentum norms after every gauge fixing (__K_MOM_ANALYSIS__)\n\n");
         std::exit(0);
     }
   }
@@ -196,6 +198,16 @@ int initialize(int argc, char** argv) {
     std::snprintf(nm, sizeof(nm), "%s%s_nf0_%dx%dx%dx%d_a%2.2f_tg%3.3f_o%d", conf_stem.c_str(), type_g(), z[0], z[1],
                   z[2], z[3], a, tg, low_order);
     low_confn = nm;
+  }
+  if (mi355x_opts.knorm && mi355x_opts.fa) {  // __K_MOM_ANALYSIS__ (Quenched.cc:549-560)
+    char kname[300];
+    std::snprintf(kname, sizeof(kname), "knorm_%s%s_nf0_%dx%dx%dx%d_a%2.2f_tg%3.3f_o%d.dat",
+                  mi355x_opts.fa == 1 ? "acc_" : "", type_g(), z[0], z[1], z[2], z[3], a, tg, PTORD);
+    knormfile.open(kname, std::ios::binary | std::ios::trunc);
+    if (!knormfile.is_open()) {
+      std::cout << "Error: cannot open " << kname << std::endl;
+      return 1;
+    }
   }
   std::snprintf(nspt_pars.logn, 256, "%s_nf0_%dx%dx%dx%d_a%2.2f_tg%3.3f_o%d.log", type_g(), z[0], z[1], z[2], z[3],
                 a, tg, PTORD);
@@ -342,6 +354,7 @@ int finalize(ptGluon_fld& Umu, int t) {
     WL2x2(Umu);
     loop2x2.close();
   }
+  if (knormfile.is_open()) knormfile.close();  // NsptFinalize (Quenched.cc:719-721)
   return 0;
 }
 

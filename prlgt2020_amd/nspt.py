@@ -60,7 +60,7 @@ _lib = None
 
 EXPORTS = ["nspt_create", "nspt_destroy", "nspt_last_error", "nspt_obs_len", "nspt_upload", "nspt_download",
            "nspt_cold_start", "nspt_set_noise", "nspt_step", "nspt_evolve", "nspt_plaquette",
-           "nspt_fa_gauge_fixing", "nspt_drift", "nspt_rng_create_ranlxd", "nspt_rng_destroy",
+           "nspt_fa_gauge_fixing", "nspt_momentum_norm", "nspt_drift", "nspt_rng_create_ranlxd", "nspt_rng_destroy",
            "nspt_rng_uniform", "nspt_rng_noise", "nspt_device_noise", "nspt_profile", "nspt_profile_read",
            "nspt_batch_create", "nspt_batch_destroy", "nspt_batch_last_error", "nspt_batch_evolve",
            "nspt_batch_replica", "nspt_batch_tune", "nspt_adjoint_drift", "nspt_loops",
@@ -89,6 +89,7 @@ def lib():
         l.nspt_step.argtypes = [ctypes.c_void_p, ctypes.POINTER(StepParams), _dp]
         l.nspt_evolve.argtypes = [ctypes.c_void_p, ctypes.POINTER(StepParams), ctypes.c_int, _dp]
         l.nspt_fa_gauge_fixing.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        l.nspt_momentum_norm.argtypes = [ctypes.c_void_p, _dp]
         l.nspt_drift.argtypes = [ctypes.c_void_p, ctypes.c_double, _dp]
         l.nspt_rng_create_ranlxd.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
         l.nspt_rng_create_rand55.argtypes = [ctypes.c_long, ctypes.POINTER(ctypes.c_void_p)]
@@ -264,6 +265,14 @@ class Context:
 
     def fa_gauge_fixing(self, alpha):
         self._check(lib().nspt_fa_gauge_fixing(self.h, alpha), "fa_gauge_fixing")
+
+    def momentum_norm(self):
+        """knorm of the __K_MOM_ANALYSIS__ build (nspt.cc:919-944): (ptord, V)
+        array, row k-1 = sum_mu Re Tr(Ut Ut^dag) of order k per momentum (site
+        order), Ut the unnormalised forward 4-D DFT of the field."""
+        out = np.zeros((self.N, self.V), np.float64)
+        self._check(lib().nspt_momentum_norm(self.h, out.ctypes.data_as(_dp)), "momentum_norm")
+        return out
 
     def gauge_wilson(self, params, want_obs=True):
         """The Langevin update stage alone (no zero modes, no gauge fixing)."""

@@ -299,3 +299,42 @@ def test_driver_wl2x2(tmp_path):
     got = _read_cols(tmp_path / ("plaq2x2_%s_o%d.txt" % (stem, N)), 3)
     assert got.shape == (3, 3)
     assert np.abs(got - np.array(want)).max() < 1e-12 * np.abs(want).max()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("opts", [(), ("-J", "1"), ("-I", "2", "-F", "2")])
+def test_driver_knorm_records(tmp_path, opts):
+    """-K 1 (__K_MOM_ANALYSIS__): after every gauge fixing one record of
+    allocORD * V doubles, the momentum norms of the field (nspt.cc:919-944),
+    in knorm_acc_* (FA) or knorm_* (sequential gauge fixing), names of
+    Quenched.cc:549-560.  FA runs: every record against the oracle's norms
+    along the oracle's trajectory under the same noise; the checkerboard run:
+    the last record against the oracle's norms of the saved configuration."""
+    from oracle import oracle as O
+    nc, L, N, eps, steps, beat = 3, 4, 6, 0.05, 20, 10
+    alpha = -0.05 if "-F" in opts else 0.05
+    write_inputs(tmp_path, L=L, sweep=steps, beat=beat, tau=eps, alpha=alpha, ptord=N, seed=1)
+    print("driver run: SU(3) 4^4 N=6 -K 1, options %s" % " ".join(opts), flush=True)
+    out = subprocess.run([driver(nc, 6), "-K", "1"] + list(opts), cwd=tmp_path, capture_output=True, text=True,
+                         timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    n = names(L, alpha, eps, N, 6, nc)
+    stem = "SU3_wil_nf0_4x4x4x4_a%2.2f_tg%3.3f_o%d.dat" % (abs(alpha), eps, N)
+    kfile = tmp_path / (("knorm_" if "-F" in opts else "knorm_acc_") + stem)
+    V = L ** 4
+    rec = np.fromfile(kfile, dtype=np.float64)
+    assert rec.size == steps * N * V
+    rec = rec.reshape(steps, N, V)
+    Lt = (L, L, L, L)
+    if "-F" in opts:
+        S = 1 + nc * nc * N
+        field = np.fromfile(tmp_path / n["conf"], dtype=np.complex128)[:4 * V * S].reshape(4 * V, S)
+        want = O.knorm(Lt, nc, N, field)
+        assert np.abs(rec[-1] - want).max() <= 1e-12 * np.abs(want).max()
+        return
+    U = O.cold_field(Lt, nc, N)
+    rng = O.Ranlxd(O.glibc_first_rand(1))
+    for t in range(steps):
+        O.step(Lt, nc, N, U, eps, alpha, integrator=0, update=0, gauge_fix=1, zero_modes=1, eta=rng.noise(nc, 4 * V))
+        want = O.knorm(Lt, nc, N, U)
+        assert np.abs(rec[t] - want).max() <= 1e-12 * np.abs(want).max(), t
