@@ -236,6 +236,7 @@ __global__ void k_g_soa_to_aos(const double2* soa, int S, int V, double2* aos) {
 
 struct AdjointSet {
   int nc, ng;
+  int lpb;  // links (plaquettes) per workgroup of these kernels: Lane<NC, NG>::LPB
   void (*plaq)(int nblk, Geo g, const double2* U, double2* G6, int K);
   void (*drift)(int nblk, Geo g, const double2* U, const double2* G6, double2* Gout, double2* Dout, double tau);
 };
@@ -245,6 +246,7 @@ AdjointSet adjoint_set() {
   AdjointSet s;
   s.nc = NC;
   s.ng = NG;
+  s.lpb = Lane<NC, NG>::LPB;
   s.plaq = [](int nblk, Geo g, const double2* U, double2* G6, int K) {
     hipLaunchKernelGGL((k_plaq_adjoint<NC, NG>), dim3(nblk), dim3(64), 0, 0, g, U, G6, K);
   };
@@ -261,7 +263,7 @@ AdjointSet find_adjoint(int nc, int ng) {
                             adjoint_set<2, 9>(),  adjoint_set<2, 13>(), adjoint_set<2, 17>(), adjoint_set<2, 21>()};
   for (const auto& s : all)
     if (s.nc == nc && s.ng == ng) return s;
-  return AdjointSet{0, 0, nullptr, nullptr};
+  return AdjointSet{0, 0, 0, nullptr, nullptr};
 }
 
 int grid_n(long n) {
@@ -340,8 +342,7 @@ extern "C" int nspt_adjoint_drift_dev(const nspt_geom* geom, const double* U, do
   if (!U) return -1;
   double2* g6 = nullptr;
   if (scratch_on_current_device(&g6)) return -3;
-  const int lpb = 64 / NE;
-  const long nb = (V + lpb - 1) / lpb;
+  const long nb = (V + set.lpb - 1) / set.lpb;
   // every slot of every link is written by its plaquette: nothing to clear
   set.plaq((int)(6 * nb), g, reinterpret_cast<const double2*>(U), g6, N);
   set.drift((int)(4 * nb), g, reinterpret_cast<const double2*>(U), g6, reinterpret_cast<double2*>(grad),
@@ -359,8 +360,7 @@ extern "C" int nspt_adjoint_drift(const nspt_geom* geom, const double* U, double
   if (rcg) return rcg;  // orders 4, 6, 8, 12, 16, 20 are compiled
   const int nc = geom->nc, N = geom->ptord, NG = N + 1, NE = nc * nc;
   const long V = g.V;
-  const int lpb = 64 / NE;
-  const long nb = (V + lpb - 1) / lpb, P = 4 * V, S = (long)NG * NE;
+  const long nb = (V + set.lpb - 1) / set.lpb, P = 4 * V, S = (long)NG * NE;
   double2 *u = nullptr, *aos = nullptr, *g6 = nullptr, *go = nullptr, *dout = nullptr;
   int rc = 0;
   auto ok = [&](hipError_t e) {

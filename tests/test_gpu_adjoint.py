@@ -17,7 +17,8 @@ def reh(m):
 
 @pytest.mark.parametrize("nc,L,N", [(3, (8, 8, 8, 8), 12), (3, (4, 4, 2, 4), 6), (2, (4, 2, 4, 4), 6),
                                     (3, (2, 3, 2, 2), 4), (3, (2, 4, 2, 3), 8), (2, (2, 2, 3, 2), 16),
-                                    (3, (2, 2, 2, 3), 16), (3, (2, 2, 2, 2), 20), (2, (3, 2, 2, 2), 20)])
+                                    (3, (2, 2, 2, 3), 16), (3, (2, 2, 2, 2), 20), (2, (3, 2, 2, 2), 20),
+                                    (3, (2, 2, 2, 4), 20), (2, (2, 2, 2, 4), 20)])
 def test_adjoint_drift(nc, L, N):
     from oracle import oracle as O
     from prlgt2020_amd import nspt
@@ -52,3 +53,30 @@ def test_adjoint_drift(nc, L, N):
         W = O.gmul(nc, N, U[l], S).reshape(N + 1, nc, nc)
         want = tau * reh(W)
         assert np.abs(D[l].reshape(N + 1, nc, nc) - want).max() < 1e-12 * max(1e-3, np.abs(want).max())
+
+
+@pytest.mark.parametrize("nc,L,N", [(3, (4, 4, 2, 4), 12), (3, (2, 2, 2, 4), 20), (2, (2, 4, 2, 2), 12)])
+def test_adjoint_drift_equals_the_production_drift_on_an_nspt_field(nc, L, N):
+    """On a field of the engine's own form (order 0 the flag times 1; here
+    U = exp(A), A antihermitian at orders 1..N) the adjoint drift equals the
+    production drift of nspt_drift (the staple sum of k_drift up to 16
+    orders, the plaquette-field form above) at every order 1..N, and its
+    order 0 (reH of the order-0 staple sum, a multiple of 1) is zero."""
+    from oracle import oracle as O
+    from prlgt2020_amd import nspt
+    eps = 0.02
+    ne = nc * nc
+    U = O.random_field(L, nc, N, seed=2, scale=0.3)
+    Ug = np.zeros((U.shape[0], (N + 1) * ne), np.complex128)
+    Ug[:, :ne] = U[:, :1] * np.eye(nc).ravel()
+    Ug[:, ne:] = U[:, 1:]
+    _, Da = nspt.adjoint_drift(L, nc, N, Ug, eps)
+    ctx = nspt.Context(L, nc, N)
+    ctx.upload(U)
+    Dp = ctx.drift(eps)
+    ctx.close()
+    scale = np.abs(Dp[:, 1:]).max()
+    err = np.abs(Da[:, ne:] - Dp[:, 1:]).max() / scale
+    print("SU(%d) %s N=%d: adjoint vs production drift %.2e" % (nc, "x".join(map(str, L)), N, err))
+    assert err <= 1e-12
+    assert np.abs(Da[:, :ne]).max() <= 1e-14 * scale
