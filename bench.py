@@ -47,7 +47,7 @@ ACTIONS = {"wil": 0.0, "tls": -1.0 / 12.0, "iwa": -0.331, "dbw": -1.4088}  # c1,
 
 # instances of at least this many orders evaluate exp and log in the
 # Paterson-Stockmeyer form (NSPT_PS_MIN_N of the library's build)
-PS_MIN_N = 17
+PS_MIN_N = 12
 
 
 def pairs(T, p, q):
@@ -67,17 +67,18 @@ def ps_flops(N, nc=3):
     return (216 if nc == 3 else 64) * pr + 12 * nc * nc * N * J
 
 
-def kernel_flops(name, N, V, nc=3, improved=False, cb_update=False, cb_gauge=False, fused=True, executed=False):
+def kernel_flops(name, N, V, nc=3, improved=False, cb_update=False, cb_gauge=False, fused=True, executed=False,
+                 plaq_form=True):
     """FP64 flops of one launch of the kernel behind a profile name.
 
     Canonical (executed=False): the reference's algorithm, a product SP, an
     exp or log EX (Horner), exp(X) B = EX + SP.  Executed: what this
     build's kernel forms, where exp(X) B by Horner from the right costs EX
-    alone and above 16 orders exp and log are ps_flops.
+    alone and from 12 orders on exp and log are ps_flops.
     The Jacobi drift is the plaquette field (3 products per plane and site,
-    4.5 per link) plus 6 products per link; the checkerboard sweep
-    (cb_update) forms the staple sum, 13 products per link, one launch per
-    parity and direction (V/2 links); the checkerboard gauge fixing
+    4.5 per link) plus 6 products per link (plaq_form), or the staple sum, 13
+    products per link, as up to 16 orders and in the checkerboard sweep
+    (cb_update: one launch per parity and direction, V/2 links); the checkerboard gauge fixing
     (cb_gauge) is one launch per parity (an exp and 8 products per site).
     With zero modes folded into the FA transform (fused), that kernel also
     forms the zero-mode exponential.  An improved
@@ -92,7 +93,7 @@ def kernel_flops(name, N, V, nc=3, improved=False, cb_update=False, cb_gauge=Fal
         rect = 38 * SP
     else:
         E, ET, rect = EX, EX + SP, 72 * SP
-    drift = (13 if cb_update else 6) + (1 if improved else 0)
+    drift = (13 if cb_update or not plaq_form else 6) + (1 if improved else 0)
     if cb_update and name in ("drift", "update", "update_log"):
         return {"drift": drift * SP, "update": ET, "update_log": ET + E}[name] * V / 2
     if cb_gauge and name == "transform":
@@ -242,6 +243,44 @@ def host_noise_path(nspt, L, nc, N, eps, p_dev, steps=3):
             "steps": steps, "host_threads": 1}
 
 
+def adjoint_bench(L=(8, 8, 8, 8), nc=3, N=12, calls=20):
+    """The adjoint drift at its configuration (SU(3) 8^4, 12 orders): the
+    entry point on device buffers (nspt_adjoint_drift_dev, which returns when
+    its kernels are done), timed over `calls` calls after two untimed ones.
+    It forms 13 full products of N+1 orders per link (8 per plaquette, 6
+    plaquettes per site; one per link for the drift), (N+1)(N+2)/2 pairs of
+    orders each."""
+    import ctypes
+    import torch
+    from prlgt2020_amd import nspt
+    V = int(np.prod(L))
+    ng, ne = N + 1, nc * nc
+    n = 4 * V * ng * ne  # complex
+    gen = torch.Generator(device="cuda").manual_seed(7)
+    U = 0.1 * torch.randn(2 * n, dtype=torch.float64, device="cuda", generator=gen)
+    G = torch.empty_like(U)
+    D = torch.empty_like(U)
+    geom = nspt._Geom((ctypes.c_int * 4)(*L), nc, N, N)
+    f = nspt.lib().nspt_adjoint_drift_dev
+    args = (ctypes.byref(geom), ctypes.c_void_p(U.data_ptr()), 0.02, ctypes.c_void_p(G.data_ptr()),
+            ctypes.c_void_p(D.data_ptr()))
+    torch.cuda.synchronize()
+    for _ in range(2):
+        if f(*args) != 0:
+            raise RuntimeError("nspt_adjoint_drift_dev failed")
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        f(*args)
+    dt = (time.perf_counter() - t0) / calls
+    flops = 13 * 4 * V * (216 if nc == 3 else 64) * ng * (ng + 1) // 2
+    return {"workload": "adjoint drift, SU(%d) %s, %d orders (order 0 a full matrix)"
+                        % (nc, "x".join(map(str, L)), N),
+            "ms_per_call": dt * 1e3, "link_orders_per_s": 4 * V * ng / dt, "flops_per_call": flops,
+            "tflops": flops / dt / 1e12, "frac": flops / dt / 1e12 / PEAK_FP64_TFLOPS, "calls": calls,
+            "note": "a call is two kernels (the plaquette adjoints, then the per-link sum and drift) on the null "
+                    "stream plus the host synchronisation; 16 384 links are few for 256 CUs"}
+
+
 def batched_sweep(nsteps, warmup, sizes=(12, 16), per_volume=4):
     """Epsilon-replica sweep: SU(3) per_volume x a^4 + per_volume x b^4 for
     sizes (a, b) (the workload is 4 x 12^4 + 4 x 16^4), N = 16, RK2 + zero
@@ -318,6 +357,7 @@ def main():
                          "matrices (6: y and z, its default); needs --gauge checkerboard")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched epsilon-replica sweep")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-drift line")
     ap.add_argument("--no-host-noise", action="store_true",
                     help="skip the host-noise path (ranlxd on the host, upload before each step)")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
@@ -333,6 +373,11 @@ def main():
     integ_cpu = {"rk2": 1, "euler": 0, "checkerboard": 0}[args.integrator]
     farm = None if args.no_cpu else cpu_farm(args.N, integ_cpu)
 
+    if args.copy_calib_mib or not args.no_adjoint:
+        # torch's device memory (the adjoint line's buffers, the calibration
+        # copy) needs torch to initialise the GPU before the library does
+        import torch
+        torch.empty(1, device="cuda")
     from prlgt2020_amd import nspt
     if args.copy_calib_mib:
         import torch
@@ -379,7 +424,8 @@ def main():
         if not c:
             continue
         fname = kernel_function(pname, uses_pf, args.gauge != "fa")
-        how = dict(cb_update=integ == nspt.EULER_CHECKERBOARD, cb_gauge=args.gauge != "fa", fused=args.gauge == "fa")
+        how = dict(cb_update=integ == nspt.EULER_CHECKERBOARD, cb_gauge=args.gauge != "fa", fused=args.gauge == "fa",
+                   plaq_form=uses_pf)
         fl1 = kernel_flops(pname, N, V, nc, improved, **how)
         fx1 = kernel_flops(pname, N, V, nc, improved, executed=True, **how)
         step_exec += fx1 * c
@@ -442,19 +488,20 @@ def main():
                                   ", twisted boundary conditions in %s" % "".join(
                                       "xyzt"[d] for d in range(4) if (args.twist >> d) & 1) if args.twist else ""),
                    "lattice": list(L), "orders": N, "integrator": args.integrator, "replicas": 1},
-        "roofline": {"bound": "fp64-valu", "kernel": name, "profile_names": pnames, "achieved": achieved,
-                     "peak": PEAK_FP64_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
-                     "avg_launch_ms": avg_s * 1e3, "flops_per_launch": fl,
+        "roofline": {"bound": "fp64-valu", "kernel": name, "profile_names": pnames,
+                     "achieved": executed["achieved"], "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": executed["frac"], "traffic": traffic,
+                     "avg_launch_ms": avg_s * 1e3, "flops_per_launch": executed["flops_per_launch"],
+                     "canonical": {"flops_per_launch": fl, "achieved": achieved, "frac": achieved / PEAK_FP64_TFLOPS},
                      "traffic_gbs": traffic / avg_s / 1e9 if traffic else None,
                      "traffic_frac_of_hbm_peak": traffic / avg_s / 1e9 / PEAK_HBM_GBS if traffic else None,
                      "step_fp64_tflops": step_tf, "step_frac": step_tf / PEAK_FP64_TFLOPS,
-                     "executed": executed, "step_executed_tflops": step_xtf,
-                     "step_executed_frac": step_xtf / PEAK_FP64_TFLOPS,
-                     "flop_model": "achieved, frac and step_frac price every kernel at the reference's algorithm "
-                                   "(a product SP, an exp or log EX by Horner, 13 SP per drift); the executed "
-                                   "figures price what these kernels form (the plaquette-field drift, exp(X)B by "
-                                   "Horner from the right, and above 16 orders exp and log by Paterson-Stockmeyer)",
+                     "step_executed_tflops": step_xtf, "step_executed_frac": step_xtf / PEAK_FP64_TFLOPS,
+                     "flop_model": "achieved and frac price the flops this kernel executes (the plaquette-field "
+                                   "drift, exp(X)B by Horner from the right, from 12 orders on exp and log by "
+                                   "Paterson-Stockmeyer); canonical, step_fp64_tflops and step_frac price the "
+                                   "reference's algorithm (a product SP, an exp or log EX by Horner, 13 SP per "
+                                   "drift evaluation); step_executed_* the flops the step's kernels execute",
                      "drift_evaluation": drift_eval},
         "kernels_ms_per_step": {k: v[0] / args.steps for k, v in prof.items() if v[1]},
         "plaquette_order2": float(plaq[2].real),
@@ -465,6 +512,8 @@ def main():
     ctx.close()
     if not args.no_host_noise and not args.twist and not improved:
         out["host_noise"] = host_noise_path(nspt, L, nc, N, args.eps, p)
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_bench()
     if not args.no_batched:
         out["batched"] = batched_sweep(max(4, args.steps), 2)
         # replicas too small to fill the GPU alone, where the graph and the

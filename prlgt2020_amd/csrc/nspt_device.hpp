@@ -408,14 +408,17 @@ __device__ __forceinline__ void horner(const Lane<NC, N>& ln, int n) {
 // q_j cost no product (each lane adds its own elements), and every product
 // is stored from its operands' lowest order on (X^2 from order 2, Y from
 // order 3), so no tile is spent on their zero orders: at n = 20, 739 pairs
-// of orders against the 1330 of the Horner form.  The same terms are summed
-// in another association than the reference's Horner loop (QCDpt.h:176-220):
-// as accurate (against an extended-precision evaluation, exp to 2e-16 and
-// log within the Horner form's own error), but not bit-alike, so the
-// instances up to 16 orders, whose parity is held to the reference's
-// association, keep the Horner form.
+// of orders against the 1330 of the Horner form (376 against 680 at 16,
+// 175 against 286 at 12).  The same terms are summed in another association
+// than the reference's Horner loop (QCDpt.h:176-220): as accurate (against
+// an extended-precision evaluation, exp to 2e-16 and log within the Horner
+// form's own error), and a step's agreement with the oracle is what it is
+// with the Horner form (on evolved fields at 12 and 16 orders the worst
+// per-order difference is the same to two digits: the step's conditioning
+// sets it).  Below 12 orders the products are too short for the fewer
+// pairs to pay for the extra product set-ups.
 #ifndef NSPT_PS_MIN_N
-#define NSPT_PS_MIN_N 17
+#define NSPT_PS_MIN_N 12
 #endif
 template <int N>
 constexpr bool use_ps() {

@@ -53,7 +53,7 @@ def test_executed_flop_model():
     at N - 3j), the pairs enumerated here one by one."""
     def brute(T, p, q):
         return sum(1 for i in range(p, T + 1) for j in range(q, T + 1) if i + j <= T)
-    for N in (17, 20, 24, 32):
+    for N in (12, 16, 20, 24, 32):
         J = N // 3
         pr = brute(N, 1, 1) + brute(N, 2, 1) + sum(brute(N - 3 * j, 1, 3) for j in range(J))
         assert bench.ps_flops(N) == 216 * pr + 12 * 9 * N * J
@@ -63,8 +63,9 @@ def test_executed_flop_model():
     # canonical: exp(X) U = EX + SP; the FA transform with the zero-mode exp folded in: EX + 2 SP
     assert bench.kernel_flops("update", 20, V) == (EX + SP) * 4 * V
     assert bench.kernel_flops("transform", 20, V) == (EX + 2 * SP) * 4 * V
-    # executed: Horner from the right costs the exp alone up to 16 orders
-    assert bench.kernel_flops("update", 16, V, executed=True) == bench.ex_flops(16) * 4 * V
+    # executed: Horner from the right costs the exp alone below 12 orders
+    assert bench.kernel_flops("update", 8, V, executed=True) == bench.ex_flops(8) * 4 * V
+    assert bench.kernel_flops("update", 16, V, executed=True) == (bench.ps_flops(16) + bench.sp_flops(16)) * 4 * V
     assert bench.kernel_flops("update", 20, V, executed=True) == (bench.ps_flops(20) + SP) * 4 * V
     # the rectangle kernels execute 38 of the 72 products the reference forms
     assert bench.kernel_flops("rect", 20, V, executed=True) * 72 == bench.kernel_flops("rect", 20, V) * 38
