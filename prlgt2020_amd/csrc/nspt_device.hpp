@@ -535,6 +535,27 @@ __device__ __forceinline__ void ps_exp_times(const Lane<NC, N>& ln, int n, const
   __syncthreads();
 }
 
+// exp(X)*B as ps_exp_times, with B staged into slot B by stage_b() once the
+// series is formed: above 20 orders of SU(3) the registers of a held B (4N
+// per lane) are what the series and its products need (holding it spilled
+// 14-30 VGPRs at 24 orders and 112-132 at 32), so B is read again from
+// memory instead.
+template <int NC, int N, class StageB>
+__device__ __forceinline__ void ps_exp_times_staged(const Lane<NC, N>& ln, int n, const double2 (&X)[N],
+                                                    double2 bflag, StageB&& stage_b) {
+  ps_series<NC, N, true>(ln, n, X);
+  stage_b();
+  __syncthreads();
+  mul_lds<NC, N>(ln, one2(), bflag, true, n - 1, ln.slotB, false);
+  __syncthreads();
+}
+// Whether exp(X)*B holds B in registers through the series (the Horner form
+// always stages it first).
+template <int NC, int N>
+constexpr bool exp_times_holds_b() {
+  return !use_ps<N>() || N <= (NC == 2 ? 24 : 20);
+}
+
 // The form each instance takes: exp(X)*B into slot B (X, B as lane
 // elements; the Horner form stages them itself) ...
 template <int NC, int N>

@@ -779,13 +779,18 @@ __device__ __forceinline__ void k_update_body(int blk_, int nblk_, const Geo& g,
   const long idx = (long)mu * g.V + site;
 
   // generator W = D - stau*eta (order 1) into slot A, Ubase into slot B
+  // (above 20 orders of SU(3) Ubase is staged from memory after the series)
+  constexpr bool HOLD = exp_times_holds_b<NC, N>();
   double2 S[N], W[N];
   gload<NC, N>(D, idx, ln.e, nrt, ok, W);
-  gload<NC, N>(Ubase, idx, ln.e, nrt, ok, S);
+  if constexpr (HOLD) gload<NC, N>(Ubase, idx, ln.e, nrt, ok, S);
   const double2 ubf = gflag(Ubase, idx, ok);
   if (eta) W[0] = c_sub(W[0], c_scale(stau, eta[idx * NE + ln.e]));
   // Unew = exp(W) * Ubase, in slot B
-  series_exp_times<NC, N>(ln, n, W, ubf, S);
+  if constexpr (HOLD)
+    series_exp_times<NC, N>(ln, n, W, ubf, S);
+  else
+    ps_exp_times_staged<NC, N>(ln, n, W, ubf, [&]() { stage_global<NC, N>(ln, ln.slotB, Ubase, idx, nrt); });
   unstage(ln, ln.slotB, S);
   gstore<NC, N>(Uout, idx, ln.e, nrt, ok, S);
   if (ok && ln.e == 0) Uout.flag[idx] = ubf;
@@ -957,19 +962,24 @@ __device__ __forceinline__ void k_gauge_transform_body(int blk_, int nblk_, cons
   if (Xexp.data) {
     // T = exp(X - MM) G(x+mu)^dag by Horner from the right (the exponential
     // is never formed alone), then U = G(x) T
-    double2 Q[N];
+    // (above 20 orders of SU(3) G(x+mu)^dag is staged from memory after the
+    // series instead of held in Q)
+    constexpr bool HOLD = exp_times_holds_b<NC, N>();
+    double2 Q[HOLD ? N : 1];
     gload<NC, N>(Xexp, idx, ln.e, nrt, ok, P);
-    gload<NC, N>(G, xp, ln.e, nrt, ok, Q);
+    if constexpr (HOLD) gload<NC, N>(G, xp, ln.e, nrt, ok, Q);
     const double2 gpf = c_conj(gflag(G, xp, ok));
     if (MM) {
 #pragma unroll
       for (int k = 0; k < N; k++)
         if (k < nrt) P[k] = c_sub(P[k], MM[((long)mu * nrt + k) * (NC * NC) + ln.e]);
     }
-    ln.stage(ln.slotA, P);
-    ln.stage(ln.slotB, Q, true);
-    __syncthreads();
-    unstage(ln, ln.slotB, Q);  // this lane's element of G(x+mu)^dag
+    if constexpr (HOLD) {
+      ln.stage(ln.slotA, P);
+      ln.stage(ln.slotB, Q, true);
+      __syncthreads();
+      unstage(ln, ln.slotB, Q);  // this lane's element of G(x+mu)^dag
+    }
     // G(x) is loaded into the free registers while the exponential runs
     // (above 24 orders those registers are not free: it is loaded after;
     // the Paterson-Stockmeyer form holds its generator in them)
@@ -979,7 +989,11 @@ __device__ __forceinline__ void k_gauge_transform_body(int blk_, int nblk_, cons
     double2 Gx[PS && PF ? N : 1];
     if constexpr (PS) {
       if constexpr (PF) gload<NC, N>(G, site, ln.e, nrt, ok, Gx);
-      ps_exp_times<NC, N>(ln, M + 1, P, gpf, Q);
+      if constexpr (HOLD)
+        ps_exp_times<NC, N>(ln, M + 1, P, gpf, Q);
+      else
+        ps_exp_times_staged<NC, N>(ln, M + 1, P, gpf,
+                                   [&]() { stage_global<NC, N>(ln, ln.slotB, G, xp, nrt, true); });
     } else {
       if constexpr (PF) gload<NC, N>(G, site, ln.e, nrt, ok, P);
       horner_exp_times<NC, N>(ln, M + 1, gpf, Q);
