@@ -264,8 +264,9 @@ int nspt_batch_tune(nspt_batch* b, int steps, int* qt_chosen, double* ms_per_ste
  * f = sum_{x, mu<nu} Re Tr P_{mu nu}(x)_{ptord}; one reverse pass seeded at
  * the top order yields every order of the staple sum, Sigma_m =
  * G_{ptord-m}^dag.  drift (may be null, same layout): -eps/(2 NC) reH(U Sigma)
- * per order 0..ptord.  Compiled for ptord 4, 6, 8, 12, 16, 20; alloc_ord is
- * ignored. */
+ * per order 0..ptord.  Any ptord from 1 to 20: the instance compiled for
+ * the next of 4, 6, 8, 12, 16, 20 runs with the orders above ptord zero
+ * (-2 above 20); alloc_ord is ignored. */
 int nspt_adjoint_drift(const nspt_geom* geom, const double* U, double eps, double* grad, double* drift);
 
 /* nspt_adjoint_drift without any allocation (no counterpart in the

@@ -48,15 +48,20 @@ __device__ __forceinline__ int shift_site(const Geo& g, int s, int d, int delta)
 // index of direction b among the three directions other than a
 __device__ __forceinline__ int other_index(int a, int b) { return b < a ? b : b - 1; }
 
+// A field of ng orders (0..ng-1, ng <= NG at run time) is read into the
+// compiled NG orders with zeros above: the products then form the same
+// orders 0..ng-1 as an instance compiled for ng (orders of a truncated
+// product only feed upward, and the cotangents start at order ng-1), and
+// only those are stored.
 template <int NC, int NG>
-__device__ __forceinline__ void load_ser(const Lane<NC, NG>& ln, const double2* f, long P, long idx,
+__device__ __forceinline__ void load_ser(const Lane<NC, NG>& ln, const double2* f, long P, long idx, int ng,
                                          double2 (&X)[NG]) {
   constexpr int NE = NC * NC;
 #pragma unroll
-  for (int k = 0; k < NG; k++) X[k] = f[((long)k * NE + ln.e) * P + idx];
+  for (int k = 0; k < NG; k++) X[k] = (k < ng) ? f[((long)k * NE + ln.e) * P + idx] : zero2();
 }
 template <int NC, int NG>
-__device__ __forceinline__ void store_ser(const Lane<NC, NG>& ln, double2* f, long P, long idx, bool ok,
+__device__ __forceinline__ void store_ser(const Lane<NC, NG>& ln, double2* f, long P, long idx, bool ok, int ng,
                                           const double2 (&X)[NG], bool dag) {
   constexpr int NE = NC * NC;
   // the dagger of a lane-owned series needs the transposed element: go
@@ -64,7 +69,8 @@ __device__ __forceinline__ void store_ser(const Lane<NC, NG>& ln, double2* f, lo
   if (!dag) {
     if (ok)
 #pragma unroll
-      for (int k = 0; k < NG; k++) f[((long)k * NE + ln.e) * P + idx] = X[k];
+      for (int k = 0; k < NG; k++)
+        if (k < ng) f[((long)k * NE + ln.e) * P + idx] = X[k];
     return;
   }
   __syncthreads();
@@ -72,7 +78,8 @@ __device__ __forceinline__ void store_ser(const Lane<NC, NG>& ln, double2* f, lo
   __syncthreads();
   if (ok)
 #pragma unroll
-    for (int k = 0; k < NG; k++) f[((long)k * NE + ln.e) * P + idx] = ln.slotA[k * NE + ln.e];
+    for (int k = 0; k < NG; k++)
+      if (k < ng) f[((long)k * NE + ln.e) * P + idx] = ln.slotA[k * NE + ln.e];
 }
 
 // Full Cauchy product of slot A and slot B into registers: C_n, n = 0..NG-1.
@@ -100,16 +107,17 @@ __global__ __launch_bounds__(64, 1) void k_plaq_adjoint(Geo g, const double2* U,
   const long V = g.V, P = 4 * V;
   const int xpmu = shift_site(g, x, mu, 1), xpnu = shift_site(g, x, nu, 1);
   const long l1 = mu * V + x, l2 = nu * V + xpmu, l3 = mu * V + xpnu, l4 = nu * V + x;
+  const int ng = g.nrt;  // orders of the field (<= NG)
 
   double2 X[NG], T1[NG], T2[NG], Tb[NG], R[NG];
   // forward
-  load_ser(ln, U, P, l1, X);
+  load_ser(ln, U, P, l1, ng, X);
   ln.stage(ln.slotA, X);
-  load_ser(ln, U, P, l2, X);
+  load_ser(ln, U, P, l2, ng, X);
   ln.stage(ln.slotB, X);
   conv(ln, T1);
   ln.stage(ln.slotA, T1);
-  load_ser(ln, U, P, l3, X);
+  load_ser(ln, U, P, l3, ng, X);
   ln.stage(ln.slotB, X, true);
   conv(ln, T2);
   // Pbar = delta_{kK} 1
@@ -117,7 +125,7 @@ __global__ __launch_bounds__(64, 1) void k_plaq_adjoint(Geo g, const double2* U,
   for (int k = 0; k < NG; k++) Tb[k] = (k == K && ln.r == ln.c) ? one2() : zero2();
   // T2bar = adjA(Pbar, U4^dag): reversed Pbar times (U4^dag)^dag = U4
   ln.stage(ln.slotA, Tb, false, true);
-  load_ser(ln, U, P, l4, X);
+  load_ser(ln, U, P, l4, ng, X);
   ln.stage(ln.slotB, X);
   conv(ln, R);
   // U4bar = adjB(T2, Pbar)^dag: T2^dag times reversed Pbar, reversed, daggered
@@ -130,12 +138,12 @@ __global__ __launch_bounds__(64, 1) void k_plaq_adjoint(Geo g, const double2* U,
     double2 Y[NG];
 #pragma unroll
     for (int k = 0; k < NG; k++) Y[k] = R[NG - 1 - k];
-    store_ser(ln, G6, 6 * P, (long)(2 * other_index(nu, mu) + 0) * P + l4, ok, Y, true);
+    store_ser(ln, G6, 6 * P, (long)(2 * other_index(nu, mu) + 0) * P + l4, ok, ng, Y, true);
   }
   // T1bar = adjA(T2bar, U3^dag): reversed T2bar times U3
   __syncthreads();
   ln.stage(ln.slotA, Tb, false, true);
-  load_ser(ln, U, P, l3, X);
+  load_ser(ln, U, P, l3, ng, X);
   ln.stage(ln.slotB, X);
   conv(ln, R);
   // U3bar = adjB(T1, T2bar)^dag: T1^dag times reversed T2bar
@@ -148,22 +156,22 @@ __global__ __launch_bounds__(64, 1) void k_plaq_adjoint(Geo g, const double2* U,
     double2 Y[NG];
 #pragma unroll
     for (int k = 0; k < NG; k++) Y[k] = R[NG - 1 - k];
-    store_ser(ln, G6, 6 * P, (long)(2 * other_index(mu, nu) + 1) * P + l3, ok, Y, true);
+    store_ser(ln, G6, 6 * P, (long)(2 * other_index(mu, nu) + 1) * P + l3, ok, ng, Y, true);
   }
   // U1bar = adjA(T1bar, U2): reversed T1bar times U2^dag
   __syncthreads();
   ln.stage(ln.slotA, Tb, false, true);
-  load_ser(ln, U, P, l2, X);
+  load_ser(ln, U, P, l2, ng, X);
   ln.stage(ln.slotB, X, true);
   conv(ln, R);
   {
     double2 Y[NG];
 #pragma unroll
     for (int k = 0; k < NG; k++) Y[k] = R[NG - 1 - k];
-    store_ser(ln, G6, 6 * P, (long)(2 * other_index(mu, nu) + 0) * P + l1, ok, Y, false);
+    store_ser(ln, G6, 6 * P, (long)(2 * other_index(mu, nu) + 0) * P + l1, ok, ng, Y, false);
   }
   // U2bar = adjB(U1, T1bar): U1^dag times reversed T1bar
-  load_ser(ln, U, P, l1, X);
+  load_ser(ln, U, P, l1, ng, X);
   __syncthreads();
   ln.stage(ln.slotA, X, true);
   ln.stage(ln.slotB, Tb, false, true);
@@ -172,12 +180,13 @@ __global__ __launch_bounds__(64, 1) void k_plaq_adjoint(Geo g, const double2* U,
     double2 Y[NG];
 #pragma unroll
     for (int k = 0; k < NG; k++) Y[k] = R[NG - 1 - k];
-    store_ser(ln, G6, 6 * P, (long)(2 * other_index(nu, mu) + 1) * P + l2, ok, Y, false);
+    store_ser(ln, G6, 6 * P, (long)(2 * other_index(nu, mu) + 1) * P + l2, ok, ng, Y, false);
   }
 }
 
-// Per link: G = sum of the six slots; Sigma_m = G_{K-m}^dag (K = NG-1);
-// D = tau reH(U Sigma) per order 0..NG-1.  Gout / Dout may be null.
+// Per link: G = sum of the six slots; Sigma_m = G_{K-m}^dag (K = ng-1, the
+// field's top order); D = tau reH(U Sigma) per order 0..ng-1.  Gout / Dout
+// may be null.
 template <int NC, int NG>
 __global__ __launch_bounds__(64, 1) void k_adjoint_drift(Geo g, const double2* U, const double2* G6,
                                                          double2* Gout, double2* Dout, double tau) {
@@ -190,21 +199,32 @@ __global__ __launch_bounds__(64, 1) void k_adjoint_drift(Geo g, const double2* U
   const int x = min(sb * LN::LPB + ln.j, g.V - 1);
   const bool ok = ln.lds_ok && (sb * LN::LPB + ln.j) < g.V;
   const long V = g.V, P = 4 * V, idx = mu * V + x;
+  const int ng = g.nrt;
   double2 Gs[NG], X[NG];
 #pragma unroll
   for (int k = 0; k < NG; k++) Gs[k] = zero2();
   for (int slot = 0; slot < 6; slot++) {
-    load_ser(ln, G6, 6 * P, (long)slot * P + idx, X);
+    load_ser(ln, G6, 6 * P, (long)slot * P + idx, ng, X);
 #pragma unroll
     for (int k = 0; k < NG; k++) Gs[k] = c_add(Gs[k], X[k]);
   }
   if (Gout && ok)
 #pragma unroll
-    for (int k = 0; k < NG; k++) Gout[((long)k * NE + ln.e) * P + idx] = Gs[k];
+    for (int k = 0; k < NG; k++)
+      if (k < ng) Gout[((long)k * NE + ln.e) * P + idx] = Gs[k];
   if (!Dout) return;
-  load_ser(ln, U, P, idx, X);
+  load_ser(ln, U, P, idx, ng, X);
   ln.stage(ln.slotA, X);
-  ln.stage(ln.slotB, Gs, true, true);  // Sigma
+  // Sigma into slot B: order m is G_{ng-1-m}^dag (this lane writes the
+  // conjugate of its element at the transposed position), zero above ng-1
+  if (ln.lds_ok) {
+    const int pos = ln.c * NC + ln.r;
+#pragma unroll
+    for (int m = 0; m < NG; m++) ln.slotB[m * NE + pos] = zero2();
+#pragma unroll
+    for (int k = 0; k < NG; k++)
+      if (k < ng) ln.slotB[(ng - 1 - k) * NE + pos] = c_conj(Gs[k]);
+  }
   double2 W[NG];
   conv(ln, W);
   ln.stage(ln.slotA, W);
@@ -213,7 +233,8 @@ __global__ __launch_bounds__(64, 1) void k_adjoint_drift(Geo g, const double2* U
   for (int k = 0; k < NG; k++) X[k] = c_scale(tau, reh_elem(ln, ln.slotA, k));
   if (ok)
 #pragma unroll
-    for (int k = 0; k < NG; k++) Dout[((long)k * NE + ln.e) * P + idx] = X[k];
+    for (int k = 0; k < NG; k++)
+      if (k < ng) Dout[((long)k * NE + ln.e) * P + idx] = X[k];
 }
 
 // host image (4V x NG*NE complex, link order 4*site+mu) <-> SoA
@@ -256,13 +277,14 @@ AdjointSet adjoint_set() {
   return s;
 }
 
-// compiled orders (NG = ptord + 1)
+// compiled orders (NG = ptord + 1, ascending); a field of ng orders runs on
+// the smallest instance with NG >= ng
 AdjointSet find_adjoint(int nc, int ng) {
   const AdjointSet all[] = {adjoint_set<3, 5>(),  adjoint_set<3, 7>(),  adjoint_set<3, 9>(),  adjoint_set<3, 13>(),
                             adjoint_set<3, 17>(), adjoint_set<3, 21>(), adjoint_set<2, 5>(),  adjoint_set<2, 7>(),
                             adjoint_set<2, 9>(),  adjoint_set<2, 13>(), adjoint_set<2, 17>(), adjoint_set<2, 21>()};
   for (const auto& s : all)
-    if (s.nc == nc && s.ng == ng) return s;
+    if (s.nc == nc && s.ng >= ng) return s;
   return AdjointSet{0, 0, 0, nullptr, nullptr};
 }
 
@@ -284,9 +306,11 @@ static __device__ __align__(256) double2 scratch[ADJ_SCRATCH_BYTES / sizeof(doub
 namespace nspt {
 namespace {
 
-// Geometry of an adjoint call: 0, or -1 (bad extents) / -2 (order not compiled).
+// Geometry of an adjoint call: 0, or -1 (bad extents or order) / -2 (order
+// above the largest compiled instance).
 int adjoint_geometry(const nspt_geom* geom, AdjointSet* set, Geo* g) {
   const int NE = geom->nc * geom->nc, NG = geom->ptord + 1;
+  if (geom->ptord < 1) return -1;
   *set = find_adjoint(geom->nc, NG);
   if (!set->plaq) return -2;
   long V = 1;
@@ -357,7 +381,7 @@ extern "C" int nspt_adjoint_drift(const nspt_geom* geom, const double* U, double
   AdjointSet set;
   Geo g;
   const int rcg = adjoint_geometry(geom, &set, &g);
-  if (rcg) return rcg;  // orders 4, 6, 8, 12, 16, 20 are compiled
+  if (rcg) return rcg;  // any order up to 20 (instances 4, 6, 8, 12, 16, 20)
   const int nc = geom->nc, N = geom->ptord, NG = N + 1, NE = nc * nc;
   const long V = g.V;
   const long nb = (V + set.lpb - 1) / set.lpb, P = 4 * V, S = (long)NG * NE;

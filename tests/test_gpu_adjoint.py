@@ -18,7 +18,10 @@ def reh(m):
 @pytest.mark.parametrize("nc,L,N", [(3, (8, 8, 8, 8), 12), (3, (4, 4, 2, 4), 6), (2, (4, 2, 4, 4), 6),
                                     (3, (2, 3, 2, 2), 4), (3, (2, 4, 2, 3), 8), (2, (2, 2, 3, 2), 16),
                                     (3, (2, 2, 2, 3), 16), (3, (2, 2, 2, 2), 20), (2, (3, 2, 2, 2), 20),
-                                    (3, (2, 2, 2, 4), 20), (2, (2, 2, 2, 4), 20)])
+                                    (3, (2, 2, 2, 4), 20), (2, (2, 2, 2, 4), 20),
+                                    # orders between the compiled instances (run on the next one up)
+                                    (3, (2, 2, 3, 2), 10), (2, (3, 2, 2, 2), 3), (3, (2, 2, 2, 2), 17),
+                                    (3, (2, 2, 2, 3), 1)])
 def test_adjoint_drift(nc, L, N):
     from oracle import oracle as O
     from prlgt2020_amd import nspt
