@@ -151,8 +151,10 @@ def calibration():
     with open(path) as f:
         c = json.load(f)
     return {"ratio_oracle_over_reference": c["ratio"], "measured_on": c["cpu"],
-            "how": "same lattice, orders and mode (Euler on the fly, sequential gauge fixing, zero modes) timed "
-                   "for the oracle and the reference's own build on one core; rows in oracle/calibration.json"}
+            "how": "the SU(3) series algebra of one RK2 + zero-mode + FA link-step (30 products, 3 exp, 1 log) "
+                   "at %d orders, timed on one core for the oracle and for the reference's own QCDpt.cc / "
+                   "MyMath.cc compiled from its sources (oracle/_ref); the reference's step itself needs FFTW3, "
+                   "absent from the image; rows in oracle/calibration.json" % c.get("orders", 6)}
 
 
 def cpu_baseline(N, integrator, budget_s=12.0):
@@ -308,12 +310,20 @@ def batched_sweep(nsteps, warmup, sizes=(12, 16), per_volume=4):
           for r, e in enumerate(eps)]
     for c, p in zip(plain, pp):
         c.evolve(p, warmup, want_obs=False)
-    t0 = time.perf_counter()
+    t_rep = []
     for c, p in zip(plain, pp):
+        t0 = time.perf_counter()
         c.evolve(p, nsteps, want_obs=True)
-    t_plain = time.perf_counter() - t0
+        t_rep.append(time.perf_counter() - t0)
+    t_plain = sum(t_rep)
     for c in plain:
         c.close()
+    # each volume's plain rate; the best of them bounds what any schedule of
+    # these replicas can reach once each replica fills the GPU by itself
+    per_volume_value = {}
+    for size in sorted(set(sizes)):
+        rs = [r for r, L in enumerate(Ls) if L[0] == size]
+        per_volume_value[str(size)] = sum(4 * size ** 4 * N for _ in rs) * nsteps / sum(t_rep[r] for r in rs)
     t0 = time.perf_counter()
     b.evolve(nsteps, want_obs=True)
     t_batch = time.perf_counter() - t0
@@ -330,6 +340,7 @@ def batched_sweep(nsteps, warmup, sizes=(12, 16), per_volume=4):
                         % (per_volume, sizes[0], per_volume, sizes[1],
                            " (x%d seeds)" % (per_volume // 4) if per_volume > 4 else ""),
             "steps": nsteps, "plain_value": work * nsteps / t_plain, "batched_value": work * nsteps / t_batch,
+            "plain_value_per_volume": per_volume_value, "ceiling_value": max(per_volume_value.values()),
             "unit": "link-order updates/s", "speedup": t_plain / t_batch,
             "form": "table" if chosen_table else "branches", "tile_edge": qt,
             "branches_value": work * nsteps / t_br, "table_value": work * nsteps / t_tab,

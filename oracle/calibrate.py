@@ -1,31 +1,34 @@
-"""Calibration of the oracle's CPU throughput against the reference's own
-build (test infrastructure; writes oracle/calibration.json).
+"""Calibration of the oracle's CPU speed against the reference's own code
+(test infrastructure; writes oracle/calibration.json).
 
     OMP_NUM_THREADS=1 python oracle/calibrate.py
 
-The reference cannot run where the GPU is (its sources stay here), so the
-CPU baseline of a GPU run times the oracle and states this ratio beside it.
-The reference's rates below were measured on this container's host (Intel
-Xeon @ 2.70 GHz, one core, g++ 11.4 -O3 with the flags of its Makefile:2,
-allocORD rebuilt to 20 for the 20-order case) in its default mode: update
-on the fly (nspt.cc:365-441), the sequential gauge fixing (nspt.cc:660-862),
-zero modes removed, Euler, RAND55 noise; per-step time differenced between
-two run lengths.  The oracle is timed here in the same mode (update=1,
-gauge_fix=2) on the same lattices; ratio = oracle rate / reference rate.
+The reference's step cannot be built in this image (nspt.cc and Quenched.cc
+include FFTW3, QCDenvNODEpt_test.h:33, which is absent), and its sources do
+not travel to the GPU box, so the CPU baseline of a GPU run times the oracle
+and states this ratio beside it.  What can be compared is the work that
+dominates both steps: the series algebra.  oracle/_ref is the reference's
+own MyMath.cc / QCDpt.cc compiled from its sources (oracle/Makefile, the
+flags of its Makefile:2); ref_su3_algebra_seconds and orc_algebra_seconds
+time the same rounds of SU(3) series products, exponentials and logarithms
+(the mix of one RK2 + zero-mode + FA link-step: 30 products, 3 exp, 1 log)
+at the reference's compiled order count (allocORD = 6, choices.h:30) on one
+core.  ratio = oracle rate / reference rate.
 """
+import ctypes
 import json
 import os
 import platform
 import sys
-import time
+
+import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 # the script's own folder would make "oracle" name oracle.py, not the package
 sys.path[:] = [os.path.dirname(HERE)] + [p for p in sys.path if os.path.abspath(p or ".") != HERE]
 from oracle import oracle as O  # noqa: E402
 
-# (lattice, orders, reference link-order updates/s on one core)
-REFERENCE = [((8, 8, 8, 8), 6, 3.16e5), ((4, 4, 4, 4), 20, 7.09e4)]
+MIX = {"products": (30, 0, 0), "exp": (0, 1, 0), "log": (0, 0, 1), "step mix": (30, 3, 1)}
 
 
 def cpu_model():
@@ -38,34 +41,44 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
-def oracle_rate(L, N, steps):
-    nc = 3
-    V = L[0] * L[1] * L[2] * L[3]
-    U = O.cold_field(L, nc, N)
-    rng = O.Rand55(1)
-    for s in range(2):  # warm up, leave the cold start
-        O.step(L, nc, N, U, 0.05, -0.05, integrator=0, update=1, gauge_fix=2, zero_modes=1, eta=rng.noise(nc, 4 * V))
-    t = 0.0
-    for s in range(steps):
-        eta = rng.noise(nc, 4 * V)
-        t0 = time.perf_counter()
-        O.step(L, nc, N, U, 0.05, -0.05, integrator=0, update=1, gauge_fix=2, zero_modes=1, eta=eta)
-        t += time.perf_counter() - t0
-    return 4 * V * N * steps / t
+def series_pair(N, seed=3):
+    """Two SU(3) link series of N orders (order 0 = 1) from the oracle's
+    random field, as flat double arrays of 1 + 9N complex."""
+    U = O.random_field((2, 2, 2, 2), 3, N, seed=seed, scale=0.3)
+    return (np.ascontiguousarray(U[0], np.complex128).view(np.float64),
+            np.ascontiguousarray(U[1], np.complex128).view(np.float64))
 
 
 def main():
+    O.build()
+    ref = O.Ref()
+    N = ref.A  # the reference's allocORD
+    a, b = series_pair(N)
+    dp = ctypes.POINTER(ctypes.c_double)
+    rl = ref.l
+    rl.ref_su3_algebra_seconds.restype = ctypes.c_double
+    rl.ref_su3_algebra_seconds.argtypes = [dp, dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp]
+    ol = ctypes.CDLL(O.LIB_PATH)
+    ol.orc_algebra_seconds.restype = ctypes.c_double
+    ol.orc_algebra_seconds.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_int, dp]
+    rl.ref_set_ptord(N)
+    pa, pb = a.ctypes.data_as(dp), b.ctypes.data_as(dp)
+    cs_r, cs_o = ctypes.c_double(), ctypes.c_double()
     rows = []
-    for L, N, ref in REFERENCE:
-        r = oracle_rate(L, N, 6)
-        rows.append({"lattice": list(L), "orders": N, "mode": "Euler, on the fly, sequential gauge fixing, zero modes",
-                     "oracle": r, "reference": ref, "ratio": r / ref})
+    for name, (m, e, lg) in MIX.items():
+        rounds = max(4000, 400000 // (m + 10 * (e + lg)))
+        best_r = min(rl.ref_su3_algebra_seconds(pa, pb, rounds, m, e, lg, ctypes.byref(cs_r)) for _ in range(3))
+        best_o = min(ol.orc_algebra_seconds(3, N, pa, pb, rounds, m, e, lg, ctypes.byref(cs_o)) for _ in range(3))
+        rows.append({"ops": name, "per_round": {"products": m, "exp": e, "log": lg}, "rounds": rounds,
+                     "reference_s": best_r, "oracle_s": best_o, "ratio": best_r / best_o,
+                     "checksums_agree": abs(cs_r.value - cs_o.value) <= 1e-9 * max(1.0, abs(cs_r.value))})
         print(rows[-1], flush=True)
-    out = {"cpu": cpu_model(), "cores": 1, "rows": rows,
-           "ratio": sum(r["ratio"] for r in rows) / len(rows)}
+    out = {"cpu": cpu_model(), "cores": 1, "orders": N, "how": __doc__.strip().split("\n\n")[1].replace("\n", " "),
+           "rows": rows, "ratio": [r["ratio"] for r in rows if r["ops"] == "step mix"][0]}
     with open(os.path.join(HERE, "calibration.json"), "w") as f:
         json.dump(out, f, indent=1)
-    print(json.dumps(out))
+    print(json.dumps({k: out[k] for k in ("cpu", "orders", "ratio")}))
 
 
 if __name__ == "__main__":

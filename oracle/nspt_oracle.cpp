@@ -42,6 +42,7 @@
 // layout of ptSU3 {Cplx flag; SU3 ptU[allocORD];} with allocORD = N.
 // Field layout: link (site, mu) at series index 4*site + mu (QCDenvNODEpt_test.h:521).
 
+#include <chrono>
 #include <climits>
 #include <cmath>
 #include <cstdint>
@@ -1356,6 +1357,23 @@ void orc_ser_mul(int nc, int N, const double* a, const double* b, double* c) {
   to_ptr(ser_mul(from_ptr(nc, N, a), from_ptr(nc, N, b)), c);
 }
 void orc_ser_exp(int nc, int N, const double* a, double* c) { to_ptr(ser_exp(from_ptr(nc, N, a)), c); }
+// Seconds for `rounds` rounds of n_mul products a*b, n_exp exp(a) and n_log
+// log(b) (the speed calibration against the reference's own algebra,
+// oracle/calibrate.py); the results feed a checksum.
+double orc_algebra_seconds(int nc, int N, const double* a, const double* b, int rounds, int n_mul, int n_exp,
+                           int n_log, double* checksum) {
+  const Ser A = from_ptr(nc, N, a), B = from_ptr(nc, N, b);
+  double sum = 0.0;
+  const auto t0 = std::chrono::steady_clock::now();
+  for (int r = 0; r < rounds; r++) {
+    for (int i = 0; i < n_mul; i++) sum += ser_mul(A, B).v[1].re;
+    for (int i = 0; i < n_exp; i++) sum += ser_exp(A).v[1].re;
+    for (int i = 0; i < n_log; i++) sum += ser_log(B).v[1].re;
+  }
+  const auto t1 = std::chrono::steady_clock::now();
+  *checksum = sum;
+  return std::chrono::duration<double>(t1 - t0).count();
+}
 void orc_ser_log(int nc, int N, const double* a, double* c) { to_ptr(ser_log(from_ptr(nc, N, a)), c); }
 void orc_ser_reh(int nc, int N, const double* a, double* c) {
   Ser s = from_ptr(nc, N, a);

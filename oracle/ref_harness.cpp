@@ -10,6 +10,7 @@
 // memory image of ptSU3 / ptSU2 (QCDpt.h:47-48, 657-658).  The runtime order
 // PTORD (QCDpt.cc:31) is set through ref_set_ptord.
 
+#include <chrono>
 #include <cstring>
 
 #include "QCDpt.h"
@@ -110,6 +111,36 @@ void ref_su2rand_rand55(long seed, int n, double* out) {
     SU2 m = SU2rand(r);
     std::memcpy(out + 8 * i, m.whr, sizeof(m.whr));
   }
+}
+
+// Seconds for `rounds` rounds of the reference's SU(3) series algebra at the
+// current PTORD: per round n_mul products c = a*b, n_exp exponentials and
+// n_log logarithms of the series at a and b (memory images as above).  For
+// the speed calibration of the oracle (oracle/calibrate.py); the results
+// feed a checksum so that no operation is optimised away.
+double ref_su3_algebra_seconds(const double* a, const double* b, int rounds, int n_mul, int n_exp, int n_log,
+                               double* checksum) {
+  const ptSU3 A = load<ptSU3>(a), B = load<ptSU3>(b);
+  ptSU3 c;
+  double sum = 0.0;
+  const auto t0 = std::chrono::steady_clock::now();
+  for (int r = 0; r < rounds; r++) {
+    for (int i = 0; i < n_mul; i++) {
+      c = A * B;
+      sum += c.ptU[0].whr[0].re;
+    }
+    for (int i = 0; i < n_exp; i++) {
+      c = exp(A);
+      sum += c.ptU[0].whr[0].re;
+    }
+    for (int i = 0; i < n_log; i++) {
+      c = log(B);
+      sum += c.ptU[0].whr[0].re;
+    }
+  }
+  const auto t1 = std::chrono::steady_clock::now();
+  *checksum = sum;
+  return std::chrono::duration<double>(t1 - t0).count();
 }
 
 }  // extern "C"
