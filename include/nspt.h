@@ -228,10 +228,14 @@ int nspt_profile_read(nspt_ctx* ctx, double* ms_total, long long* count);
  * captured once in a HIP graph and replayed, in one of two forms:
  *  - branches (the default): one graph branch per replica, each with the
  *    replica's own launches, running concurrently;
- *  - table: one launch per stage of the step for all replicas, blockIdx.y
- *    selecting the replica whose fields, geometry, eps and seed the kernels
- *    read from a device table (replicas must share the integrator, the
- *    gauge fixing -- none or FA -- and the zero-mode choice).
+ *  - table: one launch per stage of the step for all replicas of a group,
+ *    blockIdx.y selecting the replica whose fields, geometry, eps and seed
+ *    the kernels read from a device table (replicas must share the
+ *    integrator, the gauge fixing -- none or FA -- and the zero-mode
+ *    choice); the replicas are dealt into 1 to 8 groups of about equal
+ *    work, each group's launches a graph branch of its own, so that one
+ *    group's stage fills the GPU where another's leaves its last round
+ *    part-empty.
  * Either way the results equal nspt_evolve on each replica alone, bit for
  * bit. */
 typedef struct nspt_batch nspt_batch;
@@ -243,16 +247,18 @@ const char* nspt_batch_last_error(const nspt_batch* b);
 int nspt_batch_evolve(nspt_batch* b, int nsteps, double* obs);
 /* Context of replica r (download, plaquette, ...), owned by the batch. */
 int nspt_batch_replica(nspt_batch* b, int r, nspt_ctx** ctx);
-/* Step form: 0 branches, 1 table (refused, -1, when the replicas do not
- * allow it). */
+/* Step form: 0 branches, g = 1..8 the table form in g groups (at most one
+ * per replica; refused, -1, when the replicas do not allow the table form).
+ * nspt_batch_form returns the form so set. */
 int nspt_batch_set_form(nspt_batch* b, int table);
 int nspt_batch_form(const nspt_batch* b);
-/* Autotuner: for each step form (branches, then table when possible) and
- * each compiled order-tile edge of the replicas' kernel instance, time
- * `steps` replays of the batch step; keep the fastest form and edge.
- * Fields and counters are restored afterwards, on failure too (and then the
- * form and edge too).  ms_per_step (may be null, up to 8 entries) receives
- * the measured times in candidate order. */
+/* Autotuner: for each step form (branches, then when possible the table in
+ * 1, 2 and 4 groups, groups of at least two replicas) and each compiled
+ * order-tile edge of the replicas' kernel instance, time `steps` replays
+ * of the batch step; keep the fastest form and edge.  Fields and counters
+ * are restored afterwards, on failure too (and then the form and edge too).
+ * ms_per_step (may be null, up to 16 entries) receives the measured times
+ * in candidate order (form-major). */
 int nspt_batch_tune(nspt_batch* b, int steps, int* qt_chosen, double* ms_per_step);
 
 /* The Langevin drift as a reverse-mode gradient of the order-truncated

@@ -1599,6 +1599,7 @@ struct TabLaunch {
   size_t off = 0;
   const void* tab = nullptr;
   int pass = 0, maxn = 16;
+  int nrep = 0;  // replicas of the table (COUNT)
 };
 
 struct nspt_batch {
@@ -1612,14 +1613,16 @@ struct nspt_batch {
   bool with_obs = false;
   int cap = 0;  // observable rows per replica held on the device
   // Step form: one graph branch per replica (each replica's own launches),
-  // or (table) one launch per stage for every replica, each replica's
-  // arguments in a device table.  The table form needs replicas that share
-  // the integrator, gauge fixing (none or FA), zero-mode choice and kernel
+  // or (table) one launch per stage for every replica of a group, each
+  // replica's arguments in a device table, the groups (1 to 8) concurrent
+  // graph branches.  The table form needs replicas that share the
+  // integrator, gauge fixing (none or FA), zero-mode choice and kernel
   // build.
   bool table = false;
   bool table_ok = false;
-  void* tab_dev[2] = {nullptr, nullptr};
-  std::vector<TabLaunch> tab_seq[2];
+  int groups = 1;
+  std::vector<void*> tab_dev[2];                   // per group
+  std::vector<std::vector<TabLaunch>> tab_seq[2];  // per group
   std::string err;
   int fail(const std::string& m) {
     err = m;
@@ -1731,10 +1734,17 @@ void plan_reduce(TablePlan& tp, const std::vector<ReduceRep>& rr, int ncols) {
 // roles after one step).  Mirrors do_step for the Jacobi Euler / RK2 step
 // with zero modes optional and no or FA gauge fixing, Wilson action,
 // periodic lattice, device noise.
-void plan_table_step(nspt_batch* b, int q, bool with_obs, TablePlan& tp) {
-  const size_t R = b->reps.size();
-  nspt_ctx* c0 = b->reps[0];
-  const nspt_step_params& p0 = b->params[0];
+void plan_table_step(nspt_batch* b, const std::vector<int>& idx, int q, bool with_obs, TablePlan& tp) {
+  const size_t R = idx.size();
+  nspt_ctx* c0 = b->reps[idx[0]];
+  const nspt_step_params& p0 = b->params[idx[0]];
+  // the group's replicas and step parameters, in table order
+  std::vector<nspt_ctx*> reps(R);
+  std::vector<nspt_step_params> params(R);
+  for (size_t r = 0; r < R; r++) {
+    reps[r] = b->reps[idx[r]];
+    params[r] = b->params[idx[r]];
+  }
   const bool rk2 = p0.integrator == NSPT_RK2;
   const bool fa = p0.gauge_fix == NSPT_GF_FA;
   const bool zm = p0.zero_modes != 0;
@@ -1744,7 +1754,7 @@ void plan_table_step(nspt_batch* b, int q, bool with_obs, TablePlan& tp) {
   auto cur = [&](nspt_ctx* c) { return q == 0 ? c->U : c->F; };
   auto nxt = [&](nspt_ctx* c) { return q == 0 ? c->F : c->U; };
   long maxl = 0, maxs = 0, maxn4 = 0;
-  for (auto* c : b->reps) {
+  for (auto* c : reps) {
     maxl = std::max(maxl, (long)c->nblk_links);
     maxs = std::max(maxs, (long)c->nblk_sites);
     maxn4 = std::max(maxn4, (long)grid_for(4L * c->V, 256));
@@ -1754,22 +1764,22 @@ void plan_table_step(nspt_batch* b, int q, bool with_obs, TablePlan& tp) {
   {
     std::vector<NoiseArgs> v(R);
     for (size_t r = 0; r < R; r++) {
-      nspt_ctx* c = b->reps[r];
-      v[r] = NoiseArgs{c->nc, c->V, b->params[r].seed, b->params[r].step, c->ctr_dev, c->eta};
+      nspt_ctx* c = reps[r];
+      v[r] = NoiseArgs{c->nc, c->V, params[r].seed, params[r].step, c->ctr_dev, c->eta};
     }
     tp.launch(TabLaunch::NOISE, dim3((unsigned)maxn4, nr), v);
   }
-  auto tau = [&](size_t r) { return -b->params[r].eps / (2.0 * b->reps[r]->nc); };
+  auto tau = [&](size_t r) { return -params[r].eps / (2.0 * reps[r]->nc); };
   auto drift = [&](bool second) {
     if (pf) {
       std::vector<PlaqArgs> pv(R);
       std::vector<DriftPArgs> dv(R);
       for (size_t r = 0; r < R; r++) {
-        nspt_ctx* c = b->reps[r];
+        nspt_ctx* c = reps[r];
         const SerField Uin = second ? nxt(c) : cur(c);
         pv[r] = PlaqArgs{c->g, Uin, c->Pf, 6 * c->g.nb_mu};
         dv[r] = DriftPArgs{c->g, Uin, c->Pf, (rk2 && !second) ? c->D1 : c->D, second ? c->D1 : none, tau(r),
-                           second ? c->nc * b->params[r].eps / 6.0 : 0.0, second ? 1 : 0,
+                           second ? c->nc * params[r].eps / 6.0 : 0.0, second ? 1 : 0,
                            (!second && with_obs) ? c->obs_part : nullptr, none, 0.0, c->nblk_links};
       }
       tp.launch(TabLaunch::PLAQ, dim3(gp, nr), pv);
@@ -1777,10 +1787,10 @@ void plan_table_step(nspt_batch* b, int q, bool with_obs, TablePlan& tp) {
     } else {
       std::vector<DriftArgs> dv(R);
       for (size_t r = 0; r < R; r++) {
-        nspt_ctx* c = b->reps[r];
+        nspt_ctx* c = reps[r];
         const SerField Uin = second ? nxt(c) : cur(c);
         dv[r] = DriftArgs{c->g, Uin, (rk2 && !second) ? c->D1 : c->D, second ? c->D1 : none, tau(r),
-                          second ? c->nc * b->params[r].eps / 6.0 : 0.0, second ? 1 : 0,
+                          second ? c->nc * params[r].eps / 6.0 : 0.0, second ? 1 : 0,
                           (!second && with_obs) ? c->obs_part : nullptr, none, 0.0, c->nblk_links};
       }
       tp.launch(TabLaunch::DRIFT, dim3(gl, nr), dv);
@@ -1789,8 +1799,8 @@ void plan_table_step(nspt_batch* b, int q, bool with_obs, TablePlan& tp) {
   auto update = [&](bool final) {
     std::vector<UpdateArgs> v(R);
     for (size_t r = 0; r < R; r++) {
-      nspt_ctx* c = b->reps[r];
-      v[r] = UpdateArgs{c->g, (rk2 && !final) ? c->D1 : c->D, c->eta, std::sqrt(b->params[r].eps), cur(c), nxt(c),
+      nspt_ctx* c = reps[r];
+      v[r] = UpdateArgs{c->g, (rk2 && !final) ? c->D1 : c->D, c->eta, std::sqrt(params[r].eps), cur(c), nxt(c),
                         (final && need_log) ? c->L : none, (final && zm) ? c->mm_part : nullptr, c->nblk_links};
     }
     tp.launch(TabLaunch::UPDATE, dim3(gl, nr), v);
@@ -1800,7 +1810,7 @@ void plan_table_step(nspt_batch* b, int q, bool with_obs, TablePlan& tp) {
     std::vector<ReduceRep> rr(R);
     std::vector<ObsFinArgs> fv(R);
     for (size_t r = 0; r < R; r++) {
-      nspt_ctx* c = b->reps[r];
+      nspt_ctx* c = reps[r];
       rr[r] = ReduceRep{c->obs_part, c->nob, c->g.nb_mu, c->obs_mu, 1.0, c->red_tmp};
       fv[r] = ObsFinArgs{c->obs_mu, c->nrt, c->nc, c->V, c->obs_dev, c->ctr_dev + 1, c->obs_len};
     }
@@ -1816,7 +1826,7 @@ void plan_table_step(nspt_batch* b, int q, bool with_obs, TablePlan& tp) {
     const int ncol = 2 * c0->nrt * c0->NE;
     std::vector<ReduceRep> rr(R);
     for (size_t r = 0; r < R; r++) {
-      nspt_ctx* c = b->reps[r];
+      nspt_ctx* c = reps[r];
       rr[r] = ReduceRep{reinterpret_cast<const double*>(c->mm_part), ncol, c->g.nb_mu,
                         reinterpret_cast<double*>(c->MM), 1.0 / (double)c->V, c->red_tmp};
     }
@@ -1824,7 +1834,7 @@ void plan_table_step(nspt_batch* b, int q, bool with_obs, TablePlan& tp) {
     if (!fa) {
       std::vector<ZeroModeArgs> v(R);
       for (size_t r = 0; r < R; r++) {
-        nspt_ctx* c = b->reps[r];
+        nspt_ctx* c = reps[r];
         v[r] = ZeroModeArgs{c->g, c->L, c->MM, nxt(c), c->X, c->nblk_links};
       }
       tp.launch(TabLaunch::ZEROMODE, dim3(gl, nr), v);
@@ -1835,18 +1845,18 @@ void plan_table_step(nspt_batch* b, int q, bool with_obs, TablePlan& tp) {
     for (int pass = 0; pass < 3; pass++) {
       std::vector<FftArgs> v(R);
       int maxn = 16;
-      for (auto* c : b->reps) {
+      for (auto* c : reps) {
         const FftTile t = fft_tile(c->g, pass);
         if (std::max(t.na, t.nb) > 16) maxn = FFT_MAXN;
       }
       long gx = 0;
       size_t shm = 0;
       for (size_t r = 0; r < R; r++) {
-        nspt_ctx* c = b->reps[r];
+        nspt_ctx* c = reps[r];
         const FftTile t = fft_tile(c->g, pass);
         const bool direct = !fft_unrolled(t.na, maxn) || !fft_unrolled(t.nb, maxn);
         shm = std::max(shm, fft_shm(t, c->g, direct));
-        v[r] = FftArgs{c->g, c->L.data, c->fb, b->params[r].alpha, (int)((long)t.tiles * ncomp)};
+        v[r] = FftArgs{c->g, c->L.data, c->fb, params[r].alpha, (int)((long)t.tiles * ncomp)};
         gx = std::max(gx, (long)t.tiles * ncomp);
       }
       tp.launch(TabLaunch::FFT, dim3((unsigned)gx, nr), v, shm, pass, maxn);
@@ -1854,7 +1864,7 @@ void plan_table_step(nspt_batch* b, int q, bool with_obs, TablePlan& tp) {
     std::vector<GexpArgs> gv(R);
     std::vector<TransformArgs> tv(R);
     for (size_t r = 0; r < R; r++) {
-      nspt_ctx* c = b->reps[r];
+      nspt_ctx* c = reps[r];
       gv[r] = GexpArgs{c->g, c->fb, c->G, c->nblk_sites};
       tv[r] = TransformArgs{c->g, c->G, nxt(c), zm ? c->L : none,
                             zm ? reinterpret_cast<const double2*>(c->MM) : nullptr, c->nblk_links};
@@ -1863,13 +1873,13 @@ void plan_table_step(nspt_batch* b, int q, bool with_obs, TablePlan& tp) {
     tp.launch(TabLaunch::TRANSFORM, dim3(gl, nr), tv);
   }
   std::vector<unsigned long long*> ctrs(R);
-  for (size_t r = 0; r < R; r++) ctrs[r] = b->reps[r]->ctr_dev;
+  for (size_t r = 0; r < R; r++) ctrs[r] = reps[r]->ctr_dev;
   tp.launch(TabLaunch::COUNT, dim3(1), ctrs);
+  tp.seq.back().nrep = (int)R;
 }
 
-void table_launch(nspt_batch* b, const TabLaunch& L) {
+void table_launch(nspt_batch* b, const TabLaunch& L, hipStream_t st) {
   const KernelSet& ks = b->reps[0]->ks;
-  hipStream_t st = b->master;
   const int gx = (int)L.grid.x, gy = (int)L.grid.y;
   switch (L.kind) {
     case TabLaunch::NOISE:
@@ -1896,8 +1906,7 @@ void table_launch(nspt_batch* b, const TabLaunch& L) {
     case TabLaunch::GEXP: ks.gexp_tab(gx, gy, st, (const nspt::GexpArgs*)L.tab); break;
     case TabLaunch::TRANSFORM: ks.transform_tab(gx, gy, st, (const nspt::TransformArgs*)L.tab); break;
     case TabLaunch::COUNT:
-      hipLaunchKernelGGL(k_count_tab, dim3(1), dim3(64), 0, st, (unsigned long long* const*)L.tab,
-                         (int)b->reps.size());
+      hipLaunchKernelGGL(k_count_tab, dim3(1), dim3(64), 0, st, (unsigned long long* const*)L.tab, L.nrep);
       break;
   }
 }
@@ -1918,26 +1927,53 @@ bool table_possible(const nspt_batch* b) {
 }
 
 void batch_free_tables(nspt_batch* b) {
-  for (int g = 0; g < 2; g++) {
-    if (b->tab_dev[g]) (void)hipFree(b->tab_dev[g]);
-    b->tab_dev[g] = nullptr;
-    b->tab_seq[g].clear();
+  for (int q = 0; q < 2; q++) {
+    for (void* p : b->tab_dev[q])
+      if (p) (void)hipFree(p);
+    b->tab_dev[q].clear();
+    b->tab_seq[q].clear();
   }
 }
 
-// Plan and upload the tables of both parities.
+// The replicas of each table group: largest first, each to the group with
+// the fewest links so far (ties to the lower group), so that the groups'
+// launches carry about the same work; ascending replica order within a group.
+std::vector<std::vector<int>> table_groups(const nspt_batch* b) {
+  const int R = (int)b->reps.size();
+  const int G = std::max(1, std::min(b->groups, R));
+  std::vector<int> order(R);
+  for (int r = 0; r < R; r++) order[r] = r;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return b->reps[x]->V > b->reps[y]->V; });
+  std::vector<std::vector<int>> grp(G);
+  std::vector<long> load(G, 0);
+  for (int r : order) {
+    const int g = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    grp[g].push_back(r);
+    load[g] += b->reps[r]->V;
+  }
+  for (auto& v : grp) std::sort(v.begin(), v.end());
+  return grp;
+}
+
+// Plan and upload the tables of both parities, one per group.
 bool batch_plan_tables(nspt_batch* b, bool with_obs) {
   batch_free_tables(b);
+  const auto grp = table_groups(b);
   for (int q = 0; q < 2; q++) {
-    TablePlan tp;
-    plan_table_step(b, q, with_obs, tp);
-    if (hipMalloc(&b->tab_dev[q], tp.host.size()) != hipSuccess ||
-        hipMemcpy(b->tab_dev[q], tp.host.data(), tp.host.size(), hipMemcpyHostToDevice) != hipSuccess) {
-      b->err = "replica tables: allocation or upload failed";
-      return false;
+    for (const auto& idx : grp) {
+      TablePlan tp;
+      plan_table_step(b, idx, q, with_obs, tp);
+      void* dev = nullptr;
+      if (hipMalloc(&dev, tp.host.size()) != hipSuccess ||
+          hipMemcpy(dev, tp.host.data(), tp.host.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        if (dev) (void)hipFree(dev);
+        b->err = "replica tables: allocation or upload failed";
+        return false;
+      }
+      b->tab_dev[q].push_back(dev);
+      for (auto& L : tp.seq) L.tab = static_cast<const char*>(dev) + L.off;
+      b->tab_seq[q].push_back(tp.seq);
     }
-    for (auto& L : tp.seq) L.tab = static_cast<const char*>(b->tab_dev[q]) + L.off;
-    b->tab_seq[q] = tp.seq;
   }
   return true;
 }
@@ -1953,7 +1989,21 @@ bool batch_build(nspt_batch* b, bool with_obs) {
         b->err = "hipStreamBeginCapture failed";
         return false;
       }
-      for (const auto& L : b->tab_seq[g]) table_launch(b, L);
+      const size_t G = b->tab_seq[g].size();
+      if (G == 1) {
+        for (const auto& L : b->tab_seq[g][0]) table_launch(b, L, b->master);
+      } else {
+        // each group a branch forked from the master stream, on the stream
+        // of a replica of its own (group j: replica j's stream)
+        (void)hipEventRecord(b->fork, b->master);
+        for (size_t j = 0; j < G; j++) {
+          hipStream_t st = b->reps[j]->stream;
+          (void)hipStreamWaitEvent(st, b->fork, 0);
+          for (const auto& L : b->tab_seq[g][j]) table_launch(b, L, st);
+          (void)hipEventRecord(b->join[j], st);
+          (void)hipStreamWaitEvent(b->master, b->join[j], 0);
+        }
+      }
       hipGraph_t graph = nullptr;
       if (hipStreamEndCapture(b->master, &graph) != hipSuccess || !graph) {
         b->err = "hipStreamEndCapture failed";
@@ -2087,17 +2137,20 @@ int nspt_batch_create(const nspt_geom* geoms, const nspt_step_params* params, in
 
 int nspt_batch_set_form(nspt_batch* b, int table) {
   if (!b) return -1;
+  if (table < 0 || table > 8) return b->fail("nspt_batch_set_form: form is 0 (branches) or 1-8 table groups");
   if (table && !b->table_ok)
     return b->fail("nspt_batch_set_form: the replicas differ in integrator, gauge fixing (none or FA only), "
                    "zero modes or kernel build");
-  if ((table != 0) != b->table) {
+  const int groups = table ? std::min(table, (int)b->reps.size()) : b->groups;
+  if ((table != 0) != b->table || groups != b->groups) {
     b->table = table != 0;
+    b->groups = groups;
     batch_drop_graphs(b);
   }
   return 0;
 }
 
-int nspt_batch_form(const nspt_batch* b) { return b ? (b->table ? 1 : 0) : -1; }
+int nspt_batch_form(const nspt_batch* b) { return b ? (b->table ? b->groups : 0) : -1; }
 
 int nspt_batch_destroy(nspt_batch* b) {
   if (!b) return -1;
@@ -2164,18 +2217,22 @@ int nspt_batch_evolve(nspt_batch* b, int nsteps, double* obs) {
 int nspt_batch_tune(nspt_batch* b, int steps, int* qt_chosen, double* ms_per_step) {
   if (!b || steps < 1) return -1;
   nspt_ctx* c0 = b->reps[0];
-  // candidates: step form (graph branches, then the replica table when the
-  // replicas allow it) x compiled tile edge of the replicas' instance
+  // candidates: step form (graph branches, then when the replicas allow it
+  // the replica table in 1, 2 and 4 concurrent groups, as far as there are
+  // replicas) x compiled tile edge of the replicas' instance
   std::vector<int> edges;
   for (int q : {4, 3, 5, 2}) {
     const KernelSet ks = find_set(c0->nc, c0->nrt, q, c0->big);
     if (ks.qt == q && ks.nmax == c0->ks.nmax) edges.push_back(q);
   }
-  std::vector<std::pair<bool, int>> cand;
-  for (bool tab : {false, true})
-    if (!tab || b->table_ok)
-      for (int q : edges) cand.push_back({tab, q});
-  const bool table0 = b->table;
+  std::vector<int> forms = {0};
+  if (b->table_ok)
+    for (int g : {1, 2, 4})
+      if (g == 1 || g <= (int)b->reps.size() / 2) forms.push_back(g);
+  std::vector<std::pair<int, int>> cand;
+  for (int f : forms)
+    for (int q : edges) cand.push_back({f, q});
+  const int form0 = b->table ? b->groups : 0;
   const int qt0 = c0->ks.qt;
   // snapshot fields and counters: tuning steps are not part of the run
   std::vector<double2*> snap(b->reps.size(), nullptr);
@@ -2183,11 +2240,12 @@ int nspt_batch_tune(nspt_batch* b, int steps, int* qt_chosen, double* ms_per_ste
   hipEvent_t e0 = nullptr, e1 = nullptr;
   bool snapped = false;
   double best = 1e30;
-  bool best_tab = table0;
+  int best_form = form0;
   int best_q = qt0;
   int rc = 0;
-  auto set_form = [&](bool tab, int q) {
-    b->table = tab;
+  auto set_form = [&](int form, int q) {
+    b->table = form != 0;
+    if (form) b->groups = form;
     for (auto* c : b->reps) c->ks = find_set(c->nc, c->nrt, q, c->big);
   };
   for (size_t r = 0; r < b->reps.size() && rc == 0; r++) {
@@ -2225,19 +2283,19 @@ int nspt_batch_tune(nspt_batch* b, int steps, int* qt_chosen, double* ms_per_ste
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, e0, e1);
     const double per = ms / steps;
-    if (ms_per_step && i < 8) ms_per_step[i] = per;
+    if (ms_per_step && i < 16) ms_per_step[i] = per;
     if (per < best) {
       best = per;
-      best_tab = cand[i].first;
+      best_form = cand[i].first;
       best_q = cand[i].second;
     }
   }
   // every exit restores the run: the form and tiles (the best on success,
   // the original otherwise), the fields and the counters
   if (rc == 0)
-    set_form(best_tab, best_q);
+    set_form(best_form, best_q);
   else
-    set_form(table0, qt0);
+    set_form(form0, qt0);
   batch_drop_graphs(b);
   if (snapped) {
     for (size_t r = 0; r < b->reps.size(); r++) {

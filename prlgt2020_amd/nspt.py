@@ -363,19 +363,26 @@ class Batch:
         return out
 
     def set_form(self, table):
-        """Step form: False one graph branch per replica, True one launch per
-        stage over a table of the replicas."""
-        self._check(lib().nspt_batch_set_form(self.h, 1 if table else 0), "batch_set_form")
+        """Step form: False (0) one graph branch per replica; True (1) one
+        launch per stage over a table of the replicas; an int g > 1 the
+        table form in g concurrent groups of replicas."""
+        self._check(lib().nspt_batch_set_form(self.h, int(table)), "batch_set_form")
 
     @property
     def table(self):
-        return lib().nspt_batch_form(self.h) == 1
+        return lib().nspt_batch_form(self.h) >= 1
+
+    @property
+    def form(self):
+        """0 branches, g >= 1 the table form in g groups."""
+        return lib().nspt_batch_form(self.h)
 
     def tune(self, steps=3):
-        """Returns (chosen tile edge, ms per step of each candidate: the
-        branch form's tile edges, then the table form's)."""
+        """Returns (chosen tile edge, ms per step of each candidate: for each
+        form -- branches, then the table in 1, 2 and 4 groups as the
+        replicas allow -- the compiled tile edges)."""
         q = ctypes.c_int()
-        ms = np.zeros(8)
+        ms = np.zeros(16)
         self._check(lib().nspt_batch_tune(self.h, steps, ctypes.byref(q), _ptr(ms)), "batch_tune")
         return q.value, ms
 

@@ -175,6 +175,33 @@ def test_table_form_equals_single_replicas(nc, N, integrator, gf, zm):
     b.close()
 
 
+@pytest.mark.parametrize("nc,N,groups", [(3, 6, 2), (3, 6, 5), (2, 20, 3), (3, 16, 4)])
+def test_grouped_table_form_equals_single_replicas(nc, N, groups):
+    """The table form split into concurrent groups of replicas (each group's
+    launches a graph branch of its own): fields and observables equal, bit
+    for bit, each replica evolved alone."""
+    from prlgt2020_amd import nspt
+    Ls = [(4, 4, 4, 4), (4, 4, 6, 4), (6, 4, 4, 4), (2, 2, 2, 4), (3, 4, 5, 2)]
+    ps = _sweep(nspt, nspt.RK2, nspt.GF_FA, 1, len(Ls))
+    print("table batch of %d SU(%d) replicas in %d groups, N=%d" % (len(Ls), nc, groups, N), flush=True)
+    b = nspt.Batch(Ls, nc, N, ps)
+    b.set_form(groups)
+    assert b.table and b.form == groups
+    obs_b = b.evolve(4)
+    obs_b2 = b.evolve(3)
+    for r, (L, p) in enumerate(zip(Ls, ps)):
+        c = nspt.Context(L, nc, N)
+        q = nspt.Context.params(p.eps, p.alpha, p.integrator, p.gauge_fix, p.zero_modes, p.noise, p.seed, p.step)
+        o1 = c.evolve(q, 4)
+        o2 = c.evolve(q, 3)
+        assert np.array_equal(b.replicas[r].download(), c.download()), r
+        assert np.array_equal(obs_b[r], o1) and np.array_equal(obs_b2[r], o2), r
+        c.close()
+    with pytest.raises(nspt.NsptError, match="form"):
+        b.set_form(9)
+    b.close()
+
+
 def test_table_form_refused_for_mixed_replicas_and_forms_switch():
     """Replicas with different integrators or gauge fixings cannot share one
     launch per stage; switching the form of a run keeps it exact."""
