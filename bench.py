@@ -508,19 +508,20 @@ def main():
                                       "xyzt"[d] for d in range(4) if (args.twist >> d) & 1) if args.twist else ""),
                    "lattice": list(L), "orders": N, "integrator": args.integrator, "replicas": 1},
         "roofline": {"bound": "fp64-valu", "kernel": name, "profile_names": pnames,
-                     "achieved": executed["achieved"], "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": executed["frac"], "traffic": traffic,
-                     "avg_launch_ms": avg_s * 1e3, "flops_per_launch": executed["flops_per_launch"],
-                     "canonical": {"flops_per_launch": fl, "achieved": achieved, "frac": achieved / PEAK_FP64_TFLOPS},
+                     "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
+                     "avg_launch_ms": avg_s * 1e3, "flops_per_launch": fl,
+                     "executed": executed,
                      "traffic_gbs": traffic / avg_s / 1e9 if traffic else None,
                      "traffic_frac_of_hbm_peak": traffic / avg_s / 1e9 / PEAK_HBM_GBS if traffic else None,
                      "step_fp64_tflops": step_tf, "step_frac": step_tf / PEAK_FP64_TFLOPS,
                      "step_executed_tflops": step_xtf, "step_executed_frac": step_xtf / PEAK_FP64_TFLOPS,
-                     "flop_model": "achieved and frac price the flops this kernel executes (the plaquette-field "
-                                   "drift, exp(X)B by Horner from the right, from 12 orders on exp and log by "
-                                   "Paterson-Stockmeyer); canonical, step_fp64_tflops and step_frac price the "
-                                   "reference's algorithm (a product SP, an exp or log EX by Horner, 13 SP per "
-                                   "drift evaluation); step_executed_* the flops the step's kernels execute",
+                     "flop_model": "achieved, frac, step_fp64_tflops and step_frac price the algorithmic flops of "
+                                   "the reference's algorithm (a product SP, an exp or log EX by Horner, 13 SP per "
+                                   "drift evaluation, exp(X)B = EX + SP: the canonical count per link-step); "
+                                   "executed and step_executed_* price the flops this build's kernels execute "
+                                   "(the plaquette-field drift, exp(X)B by Horner from the right, from 12 orders on "
+                                   "exp and log by Paterson-Stockmeyer), fewer for the same result",
                      "drift_evaluation": drift_eval},
         "kernels_ms_per_step": {k: v[0] / args.steps for k, v in prof.items() if v[1]},
         "plaquette_order2": float(plaq[2].real),
