@@ -219,6 +219,13 @@ enum {
 int nspt_profile(nspt_ctx* ctx, int enable);
 int nspt_profile_read(nspt_ctx* ctx, double* ms_total, long long* count);
 
+/* Measured peaks of the current GPU (no counterpart in the reference; for
+ * the roofline of bench.py): FP64 FMA throughput of a kernel of independent
+ * FMA chains, in TFLOP/s, and the HBM rate of a 1 GiB device-to-device copy
+ * (bytes read + written), in GB/s; best of three timed runs each.
+ * Allocates 2 GiB for the duration of the call. */
+int nspt_measure_peaks(double* fp64_tflops, double* hbm_gbs);
+
 /* Independent replicas stepped together: the epsilon-sweep setting of
  * NSPT, where many runs of different volume and time step feed the
  * eps -> 0 extrapolation (no counterpart in the reference, which runs one

@@ -1583,6 +1583,74 @@ int nspt_profile_read(nspt_ctx* c, double* ms_total, long long* count) {
 
 }  // extern "C"
 
+namespace {
+// FP64 FMA throughput: eight independent chains per lane, so that the
+// FMA latency is covered by the chains and the waves of a SIMD.
+__global__ void k_fma_probe(double* out, int iters, double a, double b) {
+  const double t = 1e-9 * (double)(blockIdx.x * blockDim.x + threadIdx.x);
+  double x[8];
+#pragma unroll
+  for (int c = 0; c < 8; c++) x[c] = t + 0.125 * c;
+  for (int i = 0; i < iters; i++) {
+#pragma unroll
+    for (int c = 0; c < 8; c++) x[c] = fma(x[c], a, b);
+  }
+  double s = 0.0;
+#pragma unroll
+  for (int c = 0; c < 8; c++) s += x[c];
+  out[blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+// Device-to-device stream copy, 16 bytes per lane per iteration.
+__global__ void k_copy_probe(const double2* __restrict__ src, double2* __restrict__ dst, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) dst[i] = src[i];
+}
+}  // namespace
+
+extern "C" int nspt_measure_peaks(double* fp64_tflops, double* hbm_gbs) {
+  if (!fp64_tflops || !hbm_gbs) return -1;
+  int dev = 0, ncu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
+    return -3;
+  const int threads = 256, blocks = ncu * 16, iters = 4096;
+  const long n = 64L << 20;  // 64 Mi double2 = 1 GiB per buffer
+  double* out = nullptr;
+  double2 *src = nullptr, *dst = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = 0;
+  auto ok = [&](hipError_t e) {
+    if (e != hipSuccess && rc == 0) rc = -3;
+    return rc == 0;
+  };
+  if (ok(hipMalloc(&out, sizeof(double) * threads * blocks)) && ok(hipMalloc(&src, sizeof(double2) * n)) &&
+      ok(hipMalloc(&dst, sizeof(double2) * n)) && ok(hipEventCreate(&e0)) && ok(hipEventCreate(&e1)) &&
+      ok(hipMemset(src, 0, sizeof(double2) * n))) {
+    float best_f = 1e30f, best_c = 1e30f;
+    for (int rep = 0; rep < 4 && rc == 0; rep++) {  // the first of each is a warm-up
+      float ms = 0.f;
+      hipLaunchKernelGGL(k_fma_probe, dim3(blocks), dim3(threads), 0, nullptr, out, 64, 1.0000001, 1e-7);
+      (void)hipEventRecord(e0, nullptr);
+      hipLaunchKernelGGL(k_fma_probe, dim3(blocks), dim3(threads), 0, nullptr, out, iters, 1.0000001, 1e-7);
+      (void)hipEventRecord(e1, nullptr);
+      if (ok(hipEventSynchronize(e1)) && ok(hipEventElapsedTime(&ms, e0, e1)) && rep > 0) best_f = std::min(best_f, ms);
+      (void)hipEventRecord(e0, nullptr);
+      hipLaunchKernelGGL(k_copy_probe, dim3(ncu * 32), dim3(256), 0, nullptr, (const double2*)src, dst, n);
+      (void)hipEventRecord(e1, nullptr);
+      if (ok(hipEventSynchronize(e1)) && ok(hipEventElapsedTime(&ms, e0, e1)) && rep > 0) best_c = std::min(best_c, ms);
+    }
+    ok(hipGetLastError());
+    if (rc == 0) {
+      *fp64_tflops = 2.0 * 8.0 * iters * (double)threads * blocks / (best_f * 1e-3) / 1e12;
+      *hbm_gbs = 2.0 * sizeof(double2) * (double)n / (best_c * 1e-3) / 1e9;
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  for (void* p : {(void*)out, (void*)src, (void*)dst})
+    if (p) (void)hipFree(p);
+  return rc;
+}
+
 // ================================================================== batches
 // Independent replicas stepped together.  A step of every replica is
 // captured once into a HIP graph -- one branch per replica, on the

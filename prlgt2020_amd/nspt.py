@@ -66,7 +66,7 @@ EXPORTS = ["nspt_create", "nspt_destroy", "nspt_last_error", "nspt_obs_len", "ns
            "nspt_batch_replica", "nspt_batch_tune", "nspt_adjoint_drift", "nspt_loops",
            "nspt_unitarize", "nspt_set_action", "nspt_set_twist", "nspt_gauge_wilson",
            "nspt_zero_modes_subtraction", "nspt_stochastic_gauge_fixing", "nspt_rng_create_rand55",
-           "nspt_adjoint_drift_dev", "nspt_batch_set_form", "nspt_batch_form"]
+           "nspt_adjoint_drift_dev", "nspt_batch_set_form", "nspt_batch_form", "nspt_measure_peaks"]
 
 PROF_NAMES = ["drift", "update", "update_log", "zero_mode", "fa_div", "dft", "fa_gexp", "transform", "noise",
               "reduce", "rect", "plaq_field"]
@@ -407,6 +407,16 @@ def adjoint_drift(L, nc, N, U, eps):
     if rc != 0:
         raise NsptError("nspt_adjoint_drift failed (%d)" % rc)
     return G, D
+
+
+def measure_peaks():
+    """(FP64 TFLOP/s of an FMA-chain kernel, HBM GB/s of a 1 GiB device copy)
+    measured on the current GPU."""
+    f, h = ctypes.c_double(), ctypes.c_double()
+    rc = lib().nspt_measure_peaks(ctypes.byref(f), ctypes.byref(h))
+    if rc != 0:
+        raise NsptError("nspt_measure_peaks failed (%d)" % rc)
+    return f.value, h.value
 
 
 def split_obs(obs, N):
