@@ -135,3 +135,32 @@ def test_headline_workload_full_length():
     want = -(nc * nc - 1) * (1 - 1.0 / V) / 4
     print("order-2 plaquette over steps 400-499: %.5f (equipartition %.5f)" % (p2.mean(), want))
     assert abs(p2.mean() / want - 1) < 0.01
+
+
+def test_eps_sweep_extrapolates_to_equipartition():
+    """The epsilon-replica sweep at its real size (4 x 12^4 + 4 x 16^4, 16
+    orders, RK2 + zero modes + FA, eps 0.01 ... 0.03 per volume, distinct
+    seeds) stepped as one batch for 600 steps (Langevin time 6 at the
+    smallest eps): per volume, the order-2 plaquette over the last 100
+    steps, fitted as a + b eps^2 (RK2 is weak order 2), extrapolates to the
+    equipartition value -(NC^2-1)(1-1/V)/4 within 1% -- the use the sweep's
+    outputs are for."""
+    nspt = _nspt()
+    N = 16
+    eps = [0.01, 0.015, 0.02, 0.03]
+    Ls = [(12,) * 4] * 4 + [(16,) * 4] * 4
+    ps = [nspt.Context.params(e, 0.05, nspt.RK2, nspt.GF_FA, 1, nspt.NOISE_DEVICE, seed=500 + r)
+          for r, e in enumerate(eps * 2)]
+    b = nspt.Batch(Ls, 3, N, ps)
+    b.evolve(500, want_obs=False)
+    obs = b.evolve(100)
+    b.close()
+    for v, L in ((0, 12), (1, 16)):
+        V = L ** 4
+        y = np.array([obs[4 * v + i][:, 4].mean() for i in range(4)])
+        A = np.stack([np.ones(4), np.array(eps) ** 2], axis=1)
+        a, bcoef = np.linalg.lstsq(A, y, rcond=None)[0]
+        want = -8 * (1 - 1.0 / V) / 4
+        print("%d^4: order-2 plaquette %s -> eps 0: %.5f (equipartition %.5f)" % (L, np.round(y, 5), a, want))
+        assert np.isfinite(y).all()
+        assert abs(a / want - 1) < 0.01
