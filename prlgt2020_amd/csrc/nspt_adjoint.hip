@@ -82,6 +82,38 @@ __device__ __forceinline__ void store_ser(const Lane<NC, NG>& ln, double2* f, lo
       if (k < ng) f[((long)k * NE + ln.e) * P + idx] = ln.slotA[k * NE + ln.e];
 }
 
+// A series of ng orders staged straight from memory into an LDS slot (or
+// its dagger), zero above ng: loads issued eight orders at a time, so that
+// no register array is held for it across a product.
+template <int NC, int NG>
+__device__ __forceinline__ void stage_ser(const Lane<NC, NG>& ln, double2* slot, const double2* f, long P, long idx,
+                                          int ng, bool dag) {
+  constexpr int NE = NC * NC;
+  if (!ln.lds_ok) return;
+  const int pos = dag ? (ln.c * NC + ln.r) : ln.e;
+  const double sy = dag ? -1.0 : 1.0;
+  constexpr int CH = 8;
+#pragma unroll
+  for (int k0 = 0; k0 < NG; k0 += CH) {
+    double2 v[CH];
+#pragma unroll
+    for (int i = 0; i < CH; i++)
+      if (k0 + i < NG) v[i] = (k0 + i < ng) ? f[((long)(k0 + i) * NE + ln.e) * P + idx] : zero2();
+#pragma unroll
+    for (int i = 0; i < CH; i++)
+      if (k0 + i < NG) slot[(k0 + i) * NE + pos] = make_double2(v[i].x, sy * v[i].y);
+  }
+}
+// The cotangent of the plaquette's order-K trace, delta_{kK} 1, staged with
+// its orders reversed (index n holds order NG-1-n).
+template <int NC, int NG>
+__device__ __forceinline__ void stage_pbar_rev(const Lane<NC, NG>& ln, double2* slot, int K) {
+  constexpr int NE = NC * NC;
+  if (!ln.lds_ok) return;
+#pragma unroll
+  for (int n = 0; n < NG; n++) slot[n * NE + ln.e] = (NG - 1 - n == K && ln.r == ln.c) ? one2() : zero2();
+}
+
 // Full Cauchy product of slot A and slot B into registers: C_n, n = 0..NG-1.
 template <int NC, int NG>
 __device__ __forceinline__ void conv(const Lane<NC, NG>& ln, double2 (&C)[NG]) {
@@ -94,14 +126,20 @@ __device__ __forceinline__ void conv(const Lane<NC, NG>& ln, double2 (&C)[NG]) {
 
 template <int NC, int NG>
 __global__ __launch_bounds__(64, 1) void k_plaq_adjoint(Geo g, const double2* U, double2* G6, int K) {
-  constexpr int NE = NC * NC;
   using LN = Lane<NC, NG>;
-  __shared__ double2 lds[LN::LDS + NE];
+  __shared__ double2 lds[LN::LDS + NC * NC];
+  // a third slot per link holds T1 between its product and its last use,
+  // where it still leaves room for four workgroups per CU (up to 13 orders
+  // of SU(3)); above that T1 stays in registers
+  constexpr bool T1_LDS = (LN::LDS + NC * NC + LN::LPB * LN::SLOT) * 16 * 4 <= 163840;
+  __shared__ double2 lds_t1[T1_LDS ? LN::LPB * LN::SLOT : 1];
   LN ln(lds);
+  double2* const slotC = lds_t1 + (T1_LDS ? ln.j * LN::SLOT : 0);
   const int b = blockIdx.x;
   const int pl = b % 6, sb = b / 6;
-  const int pm[6] = {0, 0, 0, 1, 1, 2}, pn[6] = {1, 2, 3, 2, 3, 3};
-  const int mu = pm[pl], nu = pn[pl];
+  // planes (0,1) (0,2) (0,3) (1,2) (1,3) (2,3)
+  const int mu = pl < 3 ? 0 : (pl < 5 ? 1 : 2);
+  const int nu = pl < 3 ? pl + 1 : (pl < 5 ? pl - 1 : 3);
   const int x = min(sb * LN::LPB + ln.j, g.V - 1);
   const bool ok = ln.lds_ok && (sb * LN::LPB + ln.j) < g.V;
   const long V = g.V, P = 4 * V;
@@ -109,78 +147,97 @@ __global__ __launch_bounds__(64, 1) void k_plaq_adjoint(Geo g, const double2* U,
   const long l1 = mu * V + x, l2 = nu * V + xpmu, l3 = mu * V + xpnu, l4 = nu * V + x;
   const int ng = g.nrt;  // orders of the field (<= NG)
 
-  double2 X[NG], T1[NG], T2[NG], Tb[NG], R[NG];
-  // forward
-  load_ser(ln, U, P, l1, ng, X);
-  ln.stage(ln.slotA, X);
-  load_ser(ln, U, P, l2, ng, X);
-  ln.stage(ln.slotB, X);
-  conv(ln, T1);
-  ln.stage(ln.slotA, T1);
-  load_ser(ln, U, P, l3, ng, X);
-  ln.stage(ln.slotB, X, true);
-  conv(ln, T2);
-  // Pbar = delta_{kK} 1
+  // The eight products of a plaquette, one per iteration (the product body
+  // is emitted once), ordered so that only two series live in registers
+  // across a product -- the current cotangent Tb and the product R (and T1
+  // where it does not wait in LDS); links are staged straight from memory
+  // and Pbar = delta_{kK} 1 is generated (inlined eight times with the links, T1, T2 and Pbar held in
+  // registers, the kernel spilled 238 VGPRs at 13 orders):
+  //   0: R = U1 U2                                 T1 = R
+  //   1: R = T1 U3^dag                             (T2)
+  //   2: T2^dag rev(Pbar)  -> U4bar = rev(.)^dag
+  //   3: R = rev(Pbar) U4                          Tb = T2bar = rev(R)
+  //   4: T1^dag rev(T2bar) -> U3bar = rev(.)^dag
+  //   5: R = rev(T2bar) U3                         Tb = T1bar = rev(R)
+  //   6: rev(T1bar) U2^dag -> U1bar = rev(.)
+  //   7: U1^dag rev(T1bar) -> U2bar = rev(.)
+  constexpr int NE = NC * NC;
+  const int etr = ln.c * NC + ln.r;
+  double2 Tb[NG], R[NG], T1[T1_LDS ? 1 : NG];
 #pragma unroll
-  for (int k = 0; k < NG; k++) Tb[k] = (k == K && ln.r == ln.c) ? one2() : zero2();
-  // T2bar = adjA(Pbar, U4^dag): reversed Pbar times (U4^dag)^dag = U4
-  ln.stage(ln.slotA, Tb, false, true);
-  load_ser(ln, U, P, l4, ng, X);
-  ln.stage(ln.slotB, X);
-  conv(ln, R);
-  // U4bar = adjB(T2, Pbar)^dag: T2^dag times reversed Pbar, reversed, daggered
-  ln.stage(ln.slotA, T2, true);
-  ln.stage(ln.slotB, Tb, false, true);
+  for (int k = 0; k < NG; k++) Tb[k] = zero2();
+  for (int st = 0; st < 8; st++) {
+    asm volatile("" ::: "memory");
+    __syncthreads();
+    if (st == 0) {
+      stage_ser(ln, ln.slotA, U, P, l1, ng, false);
+      stage_ser(ln, ln.slotB, U, P, l2, ng, false);
+    } else if (st == 1) {
+      if constexpr (T1_LDS) {
+        if (ln.lds_ok) {
 #pragma unroll
-  for (int k = 0; k < NG; k++) Tb[k] = R[NG - 1 - k];  // Tb = T2bar
-  conv(ln, R);
-  {
-    double2 Y[NG];
+          for (int k = 0; k < NG; k++) ln.slotA[k * NE + ln.e] = slotC[k * NE + ln.e];
+        }
+      } else {
+        ln.stage(ln.slotA, T1);
+      }
+      stage_ser(ln, ln.slotB, U, P, l3, ng, true);
+    } else if (st == 2) {
+      ln.stage(ln.slotA, R, true);
+      stage_pbar_rev(ln, ln.slotB, K);
+    } else if (st == 3) {
+      stage_pbar_rev(ln, ln.slotA, K);
+      stage_ser(ln, ln.slotB, U, P, l4, ng, false);
+    } else if (st == 4) {
+      if constexpr (T1_LDS) {
+        if (ln.lds_ok) {
 #pragma unroll
-    for (int k = 0; k < NG; k++) Y[k] = R[NG - 1 - k];
-    store_ser(ln, G6, 6 * P, (long)(2 * other_index(nu, mu) + 0) * P + l4, ok, ng, Y, true);
-  }
-  // T1bar = adjA(T2bar, U3^dag): reversed T2bar times U3
-  __syncthreads();
-  ln.stage(ln.slotA, Tb, false, true);
-  load_ser(ln, U, P, l3, ng, X);
-  ln.stage(ln.slotB, X);
-  conv(ln, R);
-  // U3bar = adjB(T1, T2bar)^dag: T1^dag times reversed T2bar
-  ln.stage(ln.slotA, T1, true);
-  ln.stage(ln.slotB, Tb, false, true);
+          for (int k = 0; k < NG; k++) ln.slotA[k * NE + etr] = c_conj(slotC[k * NE + ln.e]);
+        }
+      } else {
+        ln.stage(ln.slotA, T1, true);
+      }
+      ln.stage(ln.slotB, Tb, false, true);
+    } else if (st == 5) {
+      ln.stage(ln.slotA, Tb, false, true);
+      stage_ser(ln, ln.slotB, U, P, l3, ng, false);
+    } else if (st == 6) {
+      ln.stage(ln.slotA, Tb, false, true);
+      stage_ser(ln, ln.slotB, U, P, l2, ng, true);
+    } else {
+      stage_ser(ln, ln.slotA, U, P, l1, ng, true);
+      ln.stage(ln.slotB, Tb, false, true);
+    }
+    conv(ln, R);
+    if (st == 0) {
+      if constexpr (T1_LDS) {
+        if (ln.lds_ok) {
 #pragma unroll
-  for (int k = 0; k < NG; k++) Tb[k] = R[NG - 1 - k];  // Tb = T1bar
-  conv(ln, R);
-  {
-    double2 Y[NG];
+          for (int k = 0; k < NG; k++) slotC[k * NE + ln.e] = R[k];
+        }
+      } else {
 #pragma unroll
-    for (int k = 0; k < NG; k++) Y[k] = R[NG - 1 - k];
-    store_ser(ln, G6, 6 * P, (long)(2 * other_index(mu, nu) + 1) * P + l3, ok, ng, Y, true);
-  }
-  // U1bar = adjA(T1bar, U2): reversed T1bar times U2^dag
-  __syncthreads();
-  ln.stage(ln.slotA, Tb, false, true);
-  load_ser(ln, U, P, l2, ng, X);
-  ln.stage(ln.slotB, X, true);
-  conv(ln, R);
-  {
-    double2 Y[NG];
+        for (int k = 0; k < NG; k++) T1[k] = R[k];
+      }
+    } else if (st == 3 || st == 5) {
 #pragma unroll
-    for (int k = 0; k < NG; k++) Y[k] = R[NG - 1 - k];
-    store_ser(ln, G6, 6 * P, (long)(2 * other_index(mu, nu) + 0) * P + l1, ok, ng, Y, false);
-  }
-  // U2bar = adjB(U1, T1bar): U1^dag times reversed T1bar
-  load_ser(ln, U, P, l1, ng, X);
-  __syncthreads();
-  ln.stage(ln.slotA, X, true);
-  ln.stage(ln.slotB, Tb, false, true);
-  conv(ln, R);
-  {
-    double2 Y[NG];
+      for (int k = 0; k < NG; k++) Tb[k] = R[NG - 1 - k];
+    } else if (st != 1) {
+      // gradient of one link, rev(R) or its dagger: slot 2 * (index of the
+      // other direction) + (0 for the link at the plaquette's corner, 1 for
+      // the far one); formed in slot A, then stored
+      const long slot = st == 2   ? (long)(2 * other_index(nu, mu) + 0) * P + l4
+                        : st == 4 ? (long)(2 * other_index(mu, nu) + 1) * P + l3
+                        : st == 6 ? (long)(2 * other_index(mu, nu) + 0) * P + l1
+                                  : (long)(2 * other_index(nu, mu) + 1) * P + l2;
+      ln.stage(ln.slotA, R, st == 2 || st == 4, true);
+      __syncthreads();
+      if (ok) {
 #pragma unroll
-    for (int k = 0; k < NG; k++) Y[k] = R[NG - 1 - k];
-    store_ser(ln, G6, 6 * P, (long)(2 * other_index(nu, mu) + 1) * P + l2, ok, ng, Y, false);
+        for (int k = 0; k < NG; k++)
+          if (k < ng) G6[((long)k * NE + ln.e) * 6 * P + slot] = ln.slotA[k * NE + ln.e];
+      }
+    }
   }
 }
 
