@@ -258,19 +258,25 @@ __global__ __launch_bounds__(64, 1) void k_adjoint_drift(Geo g, const double2* U
   const long V = g.V, P = 4 * V, idx = mu * V + x;
   const int ng = g.nrt;
   double2 Gs[NG], X[NG];
+  // the link's own series first, then the six slots order by order: every
+  // load is independent of the sums, so they are all in flight together
+  // (the slots are summed in ascending order, as the gradient is defined)
+  if (Dout) load_ser(ln, U, P, idx, ng, X);
 #pragma unroll
-  for (int k = 0; k < NG; k++) Gs[k] = zero2();
-  for (int slot = 0; slot < 6; slot++) {
-    load_ser(ln, G6, 6 * P, (long)slot * P + idx, ng, X);
+  for (int k = 0; k < NG; k++) {
+    double2 sum = zero2();
+    if (k < ng) {
+      const double2* gk = G6 + ((long)k * NE + ln.e) * 6 * P + idx;
 #pragma unroll
-    for (int k = 0; k < NG; k++) Gs[k] = c_add(Gs[k], X[k]);
+      for (int slot = 0; slot < 6; slot++) sum = c_add(sum, gk[(long)slot * P]);
+    }
+    Gs[k] = sum;
   }
   if (Gout && ok)
 #pragma unroll
     for (int k = 0; k < NG; k++)
       if (k < ng) Gout[((long)k * NE + ln.e) * P + idx] = Gs[k];
   if (!Dout) return;
-  load_ser(ln, U, P, idx, ng, X);
   ln.stage(ln.slotA, X);
   // Sigma into slot B: order m is G_{ng-1-m}^dag (this lane writes the
   // conjugate of its element at the transposed position), zero above ng-1
