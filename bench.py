@@ -523,7 +523,7 @@ def main():
                      "flop_model": "achieved, frac, step_fp64_tflops and step_frac price the algorithmic flops of "
                                    "the reference's algorithm (a product SP, an exp or log EX by Horner, 13 SP per "
                                    "drift evaluation, exp(X)B = EX + SP: the canonical count per link-step); "
-                                   "executed and step_executed_* price the flops this build's kernels execute "
+                                   "executed and step_executed_* price the flops these kernels execute "
                                    "(the plaquette-field drift, exp(X)B by Horner from the right, from 12 orders on "
                                    "exp and log by Paterson-Stockmeyer), fewer for the same result",
                      "drift_evaluation": drift_eval},
