@@ -348,11 +348,11 @@ __device__ __forceinline__ double2 mul(const Lane<NC, N>& ln, double2 a0, double
 }
 
 // Product into LDS (in place allowed) / added to registers.
-template <int NC, int N>
+template <int NC, int N, int Q = NSPT_QT>
 __device__ __forceinline__ double2 mul_lds(const Lane<NC, N>& ln, double2 a0, double2 b0, bool has_b0, int M,
                                            double2* dst, bool dag, double scale = 1.0) {
   double2 unused[N];
-  return mul<NC, N>(ln, a0, b0, has_b0, unused, M, true, dst, dag, scale);
+  return mul<NC, N, 1, false, Q>(ln, a0, b0, has_b0, unused, M, true, dst, dag, scale);
 }
 
 // exp(X)*B by Horner from the right, T <- B + (X*T)/k for k = n..1, which
@@ -531,7 +531,7 @@ __device__ __forceinline__ void ps_exp_times(const Lane<NC, N>& ln, int n, const
   ps_series<NC, N, true>(ln, n, X);
   ln.stage(ln.slotB, bv);
   __syncthreads();
-  mul_lds<NC, N>(ln, one2(), bflag, true, n - 1, ln.slotB, false);
+  mul_lds<NC, N, NSPT_QT_PS>(ln, one2(), bflag, true, n - 1, ln.slotB, false);
   __syncthreads();
 }
 
@@ -546,7 +546,7 @@ __device__ __forceinline__ void ps_exp_times_staged(const Lane<NC, N>& ln, int n
   ps_series<NC, N, true>(ln, n, X);
   stage_b();
   __syncthreads();
-  mul_lds<NC, N>(ln, one2(), bflag, true, n - 1, ln.slotB, false);
+  mul_lds<NC, N, NSPT_QT_PS>(ln, one2(), bflag, true, n - 1, ln.slotB, false);
   __syncthreads();
 }
 // Whether exp(X)*B holds B in registers through the series (the Horner form

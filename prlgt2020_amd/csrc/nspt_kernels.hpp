@@ -1005,7 +1005,9 @@ __device__ __forceinline__ void k_gauge_transform_body(int blk_, int nblk_, cons
     else
       stage_global<NC, N, PS ? NSPT_PS_TRANSFORM_CHUNK : 1>(ln, ln.slotA, G, site, nrt);
     __syncthreads();
-    const double2 nf = mul_lds<NC, N>(ln, gf, gpf, true, M, ln.slotB, false);
+    // (the full-length products of the Paterson-Stockmeyer instances take its
+    // larger order tiles)
+    const double2 nf = mul_lds<NC, N, PS ? NSPT_QT_PS : NSPT_QT>(ln, gf, gpf, true, M, ln.slotB, false);
     __syncthreads();
     unstage(ln, ln.slotB, P);
     gstore<NC, N>(U, idx, ln.e, nrt, ok, P);

@@ -13,12 +13,12 @@ def test_kernel_function_groups_profile_names():
     kf = bench.kernel_function
     # the two launches of the RK2 update are one kernel function
     assert kf("update", True) == kf("update_log", True) == "k_update"
-    assert kf("drift", True) == "k_drift_p"
+    # the plaquette field and the drift from it are one drift evaluation
+    assert kf("drift", True) == kf("plaq_field", True) == "k_plaq_field+k_drift_p"
     assert kf("drift", False) == "k_drift"
     assert kf("fa_div", True) == kf("dft", True) == "k_fa_fft"
     assert kf("transform", True) == "k_gauge_transform"
     assert kf("transform", False, checkerboard_gauge=True) == "k_sgf"
-    assert kf("plaq_field", True) == "k_plaq_field"
     assert kf("fa_gexp", True) == "k_fa_gexp"
 
 
@@ -40,11 +40,16 @@ def test_traffic_lines_for_the_roofline_kernels():
         f = line.split()
         if len(f) == 3 and f[0] == "traffic_bytes_per_launch":
             found[f[1]] = float(f[2])
-    assert "k_update" in found and "k_drift_p" in found
+    assert "k_update" in found and "k_drift_p" in found and "k_plaq_field+k_drift_p" in found
     # HBM bytes per launch within a factor of three of the algorithmic bytes
-    # (2.69 GB for k_update, 2.65-3.41 GB for k_drift_p)
+    # (2.69 GB for k_update, 2.65-3.41 GB for k_drift_p, 4.55-5.31 GB for a
+    # drift evaluation)
     assert 2.69e9 / 3 < found["k_update"] < 3 * 2.69e9
     assert 2.65e9 / 3 < found["k_drift_p"] < 3 * 3.41e9
+    # the evaluation's line is the sum of its two kernels' lines
+    pair = found["k_plaq_field"] + found["k_drift_p"]
+    assert abs(found["k_plaq_field+k_drift_p"] - pair) <= 1e-3 * pair
+    assert 4.55e9 / 3 < pair < 3 * 5.31e9
 
 
 def test_executed_flop_model():
@@ -69,3 +74,24 @@ def test_executed_flop_model():
     assert bench.kernel_flops("update", 20, V, executed=True) == (bench.ps_flops(20) + SP) * 4 * V
     # the rectangle kernels execute 38 of the 72 products the reference forms
     assert bench.kernel_flops("rect", 20, V, executed=True) * 72 == bench.kernel_flops("rect", 20, V) * 38
+
+
+def test_kernel_groups_count_a_drift_evaluation_once():
+    """An RK2 step's profile: two plaquette fields and two drifts are two
+    drift evaluations at 13 products per link, their time summed; the two
+    updates are one function of two launches."""
+    N, V = 20, 16 ** 4
+    SP = bench.sp_flops(N)
+    prof = {"plaq_field": (3.0, 2), "drift": (5.0, 2), "update": (2.0, 1), "update_log": (3.0, 1),
+            "noise": (0.1, 1), "reduce": (0.0, 0)}
+    groups, step_exec = bench.kernel_groups(prof, N, V, plaq_form=True)
+    ms, cnt, fl, names, fx = groups["k_plaq_field+k_drift_p"]
+    assert (ms, cnt, sorted(names)) == (8.0, 2, ["drift", "plaq_field"])
+    assert fl == 2 * 13 * SP * 4 * V
+    assert fx == 2 * (4.5 + 6) * SP * 4 * V
+    assert groups["k_update"][:2] == [5.0, 2]
+    assert "k_reduce_rows" not in groups
+    assert step_exec == sum(g[4] for g in groups.values())
+    # the staple form is its own kernel, at the same 13 products per link
+    groups, _ = bench.kernel_groups({"drift": (4.0, 2)}, N, V, plaq_form=False)
+    assert groups["k_drift"][1] == 2 and groups["k_drift"][2] == 2 * 13 * SP * 4 * V
