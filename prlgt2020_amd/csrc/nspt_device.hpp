@@ -655,6 +655,15 @@ __device__ __forceinline__ void gaccum(const SerField& f, long idx, int e, int n
 __device__ __forceinline__ double2 gflag(const SerField& f, long idx, bool ok) {
   return (ok && f.flag) ? f.flag[idx] : zero2();
 }
+// The flag of series idx in every lane, for callers whose idx is a valid
+// series index in every lane (lanes past the block's links read a clamped
+// site and never use their flag): an unconditional load.  gflag's per-lane
+// condition makes a divergent join that waits for the load at once, and,
+// vector loads completing in order, for every load issued before it -- the
+// next operands a kernel prefetches during a product included.
+__device__ __forceinline__ double2 gflag_all(const SerField& f, long idx) {
+  return f.flag ? f.flag[idx] : zero2();
+}
 
 }  // inline namespace NSPT_VARIANT
 }  // namespace nspt

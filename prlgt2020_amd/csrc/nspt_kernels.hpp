@@ -446,11 +446,11 @@ __device__ __forceinline__ void k_drift_body(int blk_, int nblk_, const Geo& g, 
     const int kind = kind_of(i);
     if (kind != 3) {
       if constexpr (PF) gload_tw<NC, N>(g, ln, U, ia, ca, nrt, ok, PA);
-      paf = gflag(U, ia, ok);
+      paf = gflag_all(U, ia);
     }
     if (kind == 0 || kind == 2 || kind == 3) {
       if constexpr (PF) gload_tw<NC, N>(g, ln, U, ib, cb, nrt, ok, PB);
-      pbf = gflag(U, ib, ok);
+      pbf = gflag_all(U, ib);
     }
   };
   fetch(0);
@@ -551,7 +551,7 @@ __device__ __forceinline__ void k_plaq_field_body(int blk_, int nblk_, const Geo
   double2 PA[N], PB[N];
   gload<NC, N>(U, nu * V + site, ln.e, nrt, ok, PA);
   gload_tw<NC, N>(g, ln, U, mu * V + xpnu, cpn, nrt, ok, PB);
-  double2 fa = gflag(U, nu * V + site, ok), fb = gflag(U, mu * V + xpnu, ok);
+  double2 fa = gflag_all(U, nu * V + site), fb = gflag_all(U, mu * V + xpnu);
   ln.stage(ln.slotB, PB);
 #pragma unroll
   for (int k = 0; k < N; k++) PB[k] = zero2();
@@ -563,7 +563,7 @@ __device__ __forceinline__ void k_plaq_field_body(int blk_, int nblk_, const Geo
     if (i < 2) {
       const long il = (i == 0) ? nu * V + xpmu : mu * V + site;
       gload_tw<NC, N>(g, ln, U, il, i == 0 ? cpm : TW0, nrt, ok, PA);
-      fa = gflag(U, il, ok);
+      fa = gflag_all(U, il);
     }
     const double2 f = mul<NC, N, 1, false, NSPT_QT_DRIFT>(ln, a0, fb, true, PB, M, i < 2, ln.slotB, i == 0, 1.0);
     fb = (i == 0) ? c_conj(f) : f;
@@ -661,7 +661,7 @@ __device__ __forceinline__ void k_drift_p_body(int blk_, int nblk_, const Geo& g
   locate(0);
   if constexpr (PF) gload_tw<NC, N>(g, ln, P, ix, cm, nrt, ok, PX);
   if constexpr (PFU) gload_tw<NC, N>(g, ln, U, iu, cm, nrt, ok, PU);
-  double2 xf = gflag(P, ix, ok), uf = gflag(U, iu, ok);
+  double2 xf = gflag_all(P, ix), uf = gflag_all(U, iu);
   bool xd = xdag;
   double2 tf = zero2(), ufc = zero2();
   // one product per iteration (the product body is emitted once):
@@ -692,12 +692,12 @@ __device__ __forceinline__ void k_drift_p_body(int blk_, int nblk_, const Geo& g
         locate(i / 2 + 1);
         if constexpr (PF) gload_tw<NC, N>(g, ln, P, ix, cm, nrt, ok, PX);
         if constexpr (PFU) gload_tw<NC, N>(g, ln, U, iu, cm, nrt, ok, PU);
-        xf = gflag(P, ix, ok);
-        uf = gflag(U, iu, ok);
+        xf = gflag_all(P, ix);
+        uf = gflag_all(U, iu);
         xd = xdag;
       } else {
         if constexpr (PFU) gload<NC, N>(U, idx, ln.e, nrt, ok, PU);  // U_mu(x), for the observables
-        uf = gflag(U, idx, ok);
+        uf = gflag_all(U, idx);
       }
     }
     const double2 f = mul<NC, N, 1, false, NSPT_QT_DRIFT_P>(ln, a0, ufc, true, Sig, M, even, ln.slotA, even, 1.0);
@@ -1356,8 +1356,8 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_link_pair(Geo g, SerF
     const bool okb = ln.lds_ok && (lb.site0 + ln.j) < g.V;
     gload<NC, N>(A, ix, ln.e, nrt, okb, PA);
     gload_tw<NC, N>(g, ln, A, ix2, c2, nrt, okb, PB);
-    fa = gflag(A, ix, okb);
-    fb = gflag(A, ix2, okb);
+    fa = gflag_all(A, ix);
+    fb = gflag_all(A, ix2);
   };
   load(b0);
   for (int i = 0; i < NB && b0 + i < nblk; i++) {
