@@ -461,8 +461,15 @@ __device__ __forceinline__ double series_coef(int i, int n) {
 // p(X) for the generator X (its flag ignored, as in horner()), given as this
 // lane's element of every order; both slots are used.  On return slot A
 // holds orders 1..n of exp(X) (IS_EXP, flag 1) or log(1 + X) (flag 0).
-template <int NC, int N, bool IS_EXP>
-__device__ __forceinline__ void ps_series(const Lane<NC, N>& ln, int n, const double2 (&X)[N]) {
+//
+// after_stage() runs once X is staged, while the first product runs: a
+// caller issues there the loads it needs only after the series.
+struct NoHook {
+  __device__ void operator()() const {}
+};
+template <int NC, int N, bool IS_EXP, class AfterStage = NoHook>
+__device__ __forceinline__ void ps_series(const Lane<NC, N>& ln, int n, const double2 (&X)[N],
+                                          AfterStage&& after_stage = NoHook{}) {
   constexpr int NE = NC * NC;
   constexpr int Q = NSPT_QT_PS;
   n = uniform(n);
@@ -477,6 +484,7 @@ __device__ __forceinline__ void ps_series(const Lane<NC, N>& ln, int n, const do
   ln.stage(sa, X);
   ln.stage(sb, X);
   __syncthreads();
+  after_stage();
   mul_bands<NC, N, 1, Q>(ln, n - 1, [&](int k, double2 v) { X2[k] = v; });
   __syncthreads();
   // X^2 from order 2 on into slot A; Y = X^2 X formed in place in slot B,
@@ -525,11 +533,13 @@ __device__ __forceinline__ void ps_series(const Lane<NC, N>& ln, int n, const do
 // exp(X)*B by ps_series and one product: X (flag 0) and B (flag bflag) as
 // this lane's elements of every order.  On return slot B holds exp(X)*B
 // (flag bflag), as after horner_exp_times().
-template <int NC, int N>
+// load_b() (if given) fills bv once X is staged; with dag_b the dagger of
+// bv is staged as B.
+template <int NC, int N, class LoadB = NoHook>
 __device__ __forceinline__ void ps_exp_times(const Lane<NC, N>& ln, int n, const double2 (&X)[N], double2 bflag,
-                                             const double2 (&bv)[N]) {
-  ps_series<NC, N, true>(ln, n, X);
-  ln.stage(ln.slotB, bv);
+                                             const double2 (&bv)[N], LoadB&& load_b = NoHook{}, bool dag_b = false) {
+  ps_series<NC, N, true>(ln, n, X, load_b);
+  ln.stage(ln.slotB, bv, dag_b);
   __syncthreads();
   mul_lds<NC, N, NSPT_QT_PS>(ln, one2(), bflag, true, n - 1, ln.slotB, false);
   __syncthreads();
@@ -558,12 +568,17 @@ constexpr bool exp_times_holds_b() {
 
 // The form each instance takes: exp(X)*B into slot B (X, B as lane
 // elements; the Horner form stages them itself) ...
-template <int NC, int N>
+//
+// load_b() fills bv: in the Paterson-Stockmeyer form it runs once X is
+// staged (B's loads then overlap the first product instead of adding to the
+// start's memory wait), in the Horner form first.
+template <int NC, int N, class LoadB = NoHook>
 __device__ __forceinline__ void series_exp_times(const Lane<NC, N>& ln, int n, double2 (&X)[N], double2 bflag,
-                                                 double2 (&bv)[N]) {
+                                                 double2 (&bv)[N], LoadB&& load_b = NoHook{}) {
   if constexpr (use_ps<N>()) {
-    ps_exp_times<NC, N>(ln, n, X, bflag, bv);
+    ps_exp_times<NC, N>(ln, n, X, bflag, bv, load_b);
   } else {
+    load_b();
     ln.stage(ln.slotA, X);
     ln.stage(ln.slotB, bv);
     horner_exp_times<NC, N>(ln, n, bflag, bv);

@@ -783,12 +783,11 @@ __device__ __forceinline__ void k_update_body(int blk_, int nblk_, const Geo& g,
   constexpr bool HOLD = exp_times_holds_b<NC, N>();
   double2 S[N], W[N];
   gload<NC, N>(D, idx, ln.e, nrt, ok, W);
-  if constexpr (HOLD) gload<NC, N>(Ubase, idx, ln.e, nrt, ok, S);
   const double2 ubf = gflag(Ubase, idx, ok);
   if (eta) W[0] = c_sub(W[0], c_scale(stau, eta[idx * NE + ln.e]));
-  // Unew = exp(W) * Ubase, in slot B
+  // Unew = exp(W) * Ubase, in slot B; Ubase is loaded once W is staged
   if constexpr (HOLD)
-    series_exp_times<NC, N>(ln, n, W, ubf, S);
+    series_exp_times<NC, N>(ln, n, W, ubf, S, [&]() { gload<NC, N>(Ubase, idx, ln.e, nrt, ok, S); });
   else
     ps_exp_times_staged<NC, N>(ln, n, W, ubf, [&]() { stage_global<NC, N>(ln, ln.slotB, Ubase, idx, nrt); });
   unstage(ln, ln.slotB, S);
@@ -965,16 +964,21 @@ __device__ __forceinline__ void k_gauge_transform_body(int blk_, int nblk_, cons
     // (above 20 orders of SU(3) G(x+mu)^dag is staged from memory after the
     // series instead of held in Q)
     constexpr bool HOLD = exp_times_holds_b<NC, N>();
+    constexpr bool PS = use_ps<N>();
     double2 Q[HOLD ? N : 1];
     gload<NC, N>(Xexp, idx, ln.e, nrt, ok, P);
-    if constexpr (HOLD) gload<NC, N>(G, xp, ln.e, nrt, ok, Q);
+    // the Horner form holds this lane's element of G(x+mu)^dag, staged
+    // first; the Paterson-Stockmeyer form holds G(x+mu) itself, loaded once
+    // the generator is staged (while the first product runs) and staged
+    // daggered after the series
+    if constexpr (HOLD && !PS) gload<NC, N>(G, xp, ln.e, nrt, ok, Q);
     const double2 gpf = c_conj(gflag(G, xp, ok));
     if (MM) {
 #pragma unroll
       for (int k = 0; k < N; k++)
         if (k < nrt) P[k] = c_sub(P[k], MM[((long)mu * nrt + k) * (NC * NC) + ln.e]);
     }
-    if constexpr (HOLD) {
+    if constexpr (HOLD && !PS) {
       ln.stage(ln.slotA, P);
       ln.stage(ln.slotB, Q, true);
       __syncthreads();
@@ -983,14 +987,16 @@ __device__ __forceinline__ void k_gauge_transform_body(int blk_, int nblk_, cons
     // G(x) is loaded into the free registers while the exponential runs
     // (above 24 orders those registers are not free: it is loaded after;
     // the Paterson-Stockmeyer form holds its generator in them)
-    constexpr bool PS = use_ps<N>();
     constexpr bool PF = N <= 24 && (!PS || NSPT_PS_TRANSFORM_PREFETCH);
     const double2 gf = gflag(G, site, ok);
     double2 Gx[PS && PF ? N : 1];
     if constexpr (PS) {
-      if constexpr (PF) gload<NC, N>(G, site, ln.e, nrt, ok, Gx);
+      if constexpr (PF && !HOLD) gload<NC, N>(G, site, ln.e, nrt, ok, Gx);
       if constexpr (HOLD)
-        ps_exp_times<NC, N>(ln, M + 1, P, gpf, Q);
+        ps_exp_times<NC, N>(ln, M + 1, P, gpf, Q, [&]() {
+          gload<NC, N>(G, xp, ln.e, nrt, ok, Q);
+          if constexpr (PF) gload<NC, N>(G, site, ln.e, nrt, ok, Gx);
+        }, true);
       else
         ps_exp_times_staged<NC, N>(ln, M + 1, P, gpf,
                                    [&]() { stage_global<NC, N>(ln, ln.slotB, G, xp, nrt, true); });
