@@ -285,7 +285,7 @@ int nspt_adjoint_drift(const nspt_geom* geom, const double* U, double eps, doubl
 /* nspt_adjoint_drift without any allocation (no counterpart in the
  * reference): for callers that cannot allocate device memory -- code under
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.
- * U, grad and drift are DEVICE pointers in the engine's layout: element e of
+ * U, grad and drift are DEVICE pointers in this layout (the adjoint's own): element e of
  * order k of link (site, mu) at complex index (k*nc*nc + e)*4V + mu*V + site,
  * k = 0..ptord (grad and drift may be null).  The six gradient slots per
  * link live in a static device array of 192 MiB held by each GPU until the

@@ -34,7 +34,8 @@ namespace {
 
 // ------------------------------------------------------------------ kernels
 
-// Host image (ptGluon[V], width A) -> device SoA field (orders < nrt).
+// Host image (ptGluon[V], width A) -> device field (orders < nrt, tiled
+// series as in nspt::ser_index).
 __global__ void k_aos_to_soa(const double2* aos, int A, int NE, int nrt, int V, SerField U) {
   const long P = 4L * V;
   const long n = (long)nrt * NE * P;
@@ -42,7 +43,7 @@ __global__ void k_aos_to_soa(const double2* aos, int A, int NE, int nrt, int V, 
     const long idx = t % P;
     const long ke = t / P;  // k*NE + e
     const long site = idx % V, mu = idx / V;
-    U.data[t] = aos[(4 * site + mu) * (1 + (long)NE * A) + 1 + ke];
+    U.data[nspt::ser_index(idx, (int)ke, nrt, NE)] = aos[(4 * site + mu) * (1 + (long)NE * A) + 1 + ke];
   }
   for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < P; idx += (long)gridDim.x * blockDim.x) {
     const long site = idx % V, mu = idx / V;
@@ -64,7 +65,7 @@ __global__ void k_soa_to_aos(SerField U, int A, int NE, int nrt, int V, double2*
     if (j == 0) {
       if (U.flag) v = U.flag[idx];
     } else if ((j - 1) / NE < nrt) {
-      v = U.data[(j - 1) * P + idx];
+      v = U.data[nspt::ser_index(idx, (int)(j - 1), nrt, NE)];
     }
     aos[t] = v;
   }
@@ -186,17 +187,19 @@ __global__ void k_reduce_rows(const double* in, long ld, long in_mu, long rows, 
 }
 
 // Mean over the sites of every (direction, order, element) column of a
-// link field: out[(mu*nrt + k)*NE + e] = (1/V) sum_x f[(k*NE + e)*P + mu*V + x]
-// (the layout of the zero-mode means MM).  One block per column; each
-// thread sums a fixed stride of sites, then a fixed tree: deterministic.
+// link field: out[(mu*nrt + k)*NE + e] = (1/V) sum_x (order k, element e
+// of series mu*V + x) (the layout of the zero-mode means MM).  One block per
+// column; each thread sums a fixed stride of sites, then a fixed tree:
+// deterministic.
 constexpr int MEAN_THREADS = 256;
-__global__ __launch_bounds__(MEAN_THREADS) void k_field_mean(const double2* f, long P, int V, int nrt, int NE,
+__global__ __launch_bounds__(MEAN_THREADS) void k_field_mean(const double2* f, int V, int nrt, int NE,
                                                              double2* out) {
   const int col = blockIdx.x;
   const int e = col % NE, k = (col / NE) % nrt, mu = col / (NE * nrt);
-  const double2* src = f + ((long)k * NE + e) * P + (long)mu * V;
+  const int ke = k * NE + e;
   double2 s = make_double2(0.0, 0.0);
-  for (int x = threadIdx.x; x < V; x += MEAN_THREADS) s = nspt::c_add(s, src[x]);
+  for (int x = threadIdx.x; x < V; x += MEAN_THREADS)
+    s = nspt::c_add(s, f[nspt::ser_index((long)mu * V + x, ke, nrt, NE)]);
   __shared__ double2 red[MEAN_THREADS];
   red[threadIdx.x] = s;
   __syncthreads();
@@ -559,8 +562,8 @@ __host__ inline size_t fft_shm(const FftTile& t, const Geo& g, bool direct) {
 }
 
 template <int MAXN>
-__device__ __forceinline__ void fa_fft_body(int blk, const Geo& g, int pass, const double2* Xsrc, double2* fb,
-                                            double alpha) {
+__device__ __forceinline__ void fa_fft_body(int blk, const Geo& g, int pass, const double2* Xsrc, int ne, int xmu,
+                                            double2* fb, double alpha) {
   extern __shared__ double2 sm[];
   const FftTile t = fft_tile(g, pass);
   const bool p01 = fft_axes01(pass);
@@ -613,7 +616,8 @@ __device__ __forceinline__ void fa_fft_body(int blk, const Geo& g, int pass, con
   }
 
   double2* line = fb + comp * V;
-  const double2* xc = Xsrc + comp * 4 * V;
+  // component comp (order and element) of link series idx of the source
+  auto xc = [&](long idx) { return Xsrc[nspt::ser_index(idx, (int)comp, g.nrt, ne)]; };
   // FFT_LOAD_BATCH sites per thread are loaded before any is stored to LDS,
   // so that their global-load latencies overlap (the pass is a constant
   // inside, so no branch separates the loads)
@@ -652,13 +656,13 @@ __device__ __forceinline__ void fa_fft_body(int blk, const Geo& g, int pass, con
 #pragma unroll
           for (int mu = 0; mu < 4; mu++) {
             const int xm = (x[mu] == 0) ? site + (g.L[mu] - 1) * g.st[mu] : site - g.st[mu];
-            w = nspt::c_add(w, nspt::c_sub(xc[mu * V + site], xc[mu * V + xm]));
+            w = nspt::c_add(w, nspt::c_sub(xc(mu * V + site), xc(mu * V + xm)));
           }
           wx[u] = w.x;
           wy[u] = w.y;
         } else {
-          // pass 3 reads the plane of the source field itself
-          const double2 w = PASS == 3 ? xc[site] : line[site];
+          // pass 3 reads direction xmu of the source field itself
+          const double2 w = PASS == 3 ? xc((long)xmu * V + site) : line[site];
           wx[u] = w.x;
           wy[u] = w.y;
         }
@@ -710,9 +714,9 @@ __device__ __forceinline__ void fa_fft_body(int blk, const Geo& g, int pass, con
   }
 }
 template <int MAXN>
-__global__ __launch_bounds__(FFT_THREADS) void k_fa_fft(Geo g, int pass, const double2* Xsrc, double2* fb,
-                                                       double alpha) {
-  fa_fft_body<MAXN>(blockIdx.x, g, pass, Xsrc, fb, alpha);
+__global__ __launch_bounds__(FFT_THREADS) void k_fa_fft(Geo g, int pass, const double2* Xsrc, int ne, int xmu,
+                                                       double2* fb, double alpha) {
+  fa_fft_body<MAXN>(blockIdx.x, g, pass, Xsrc, ne, xmu, fb, alpha);
 }
 struct FftArgs {
   Geo g;
@@ -720,12 +724,13 @@ struct FftArgs {
   double2* fb;
   double alpha;
   int nblk;  // tiles x components of this replica
+  int ne;    // elements per matrix of the source field
 };
 template <int MAXN>
 __global__ __launch_bounds__(FFT_THREADS) void k_fa_fft_tab(int pass, const FftArgs* __restrict__ t) {
   const FftArgs& a = t[blockIdx.y];
   if ((int)blockIdx.x >= a.nblk) return;
-  fa_fft_body<MAXN>(blockIdx.x, a.g, pass, a.Xsrc, a.fb, a.alpha);
+  fa_fft_body<MAXN>(blockIdx.x, a.g, pass, a.Xsrc, a.ne, 0, a.fb, a.alpha);
 }
 
 // Momentum norms of one link direction (nspt.cc:925-934): out[k V + p] is
@@ -876,7 +881,7 @@ struct nspt_ctx {
   bool alloc_field(SerField& f, long P, bool with_flag) {
     f.P = P;
     f.flag = nullptr;
-    if (!alloc(&f.data, (size_t)nrt * NE * P)) return false;
+    if (!alloc(&f.data, (size_t)nspt::ser_alloc(P, nrt, NE))) return false;
     if (with_flag && !alloc(&f.flag, (size_t)P)) return false;
     return true;
   }
@@ -980,8 +985,8 @@ bool obs_reduce(nspt_ctx* c, double* out_dev, bool with_w2 = false, long rows = 
 // u_from_log: U is not in c->U but is exp(Xsrc - MM) (MM may be null),
 // formed by the transform.
 // One pass of k_fa_fft over the NE nrt planes of the FFT buffer (and of
-// Xsrc, read in passes 0 and 3 with plane stride 4V).
-bool fft_pass(nspt_ctx* c, int pass, const double2* Xsrc, double alpha) {
+// the link field Xsrc, read in pass 0, and in pass 3 for direction xmu).
+bool fft_pass(nspt_ctx* c, int pass, const double2* Xsrc, double alpha, int xmu = 0) {
   const long ncomp = (long)c->nrt * c->NE;
   const FftTile t = fft_tile(c->g, pass);
   const int maxn = std::max(t.na, t.nb) <= 16 ? 16 : FFT_MAXN;
@@ -990,9 +995,11 @@ bool fft_pass(nspt_ctx* c, int pass, const double2* Xsrc, double alpha) {
   const size_t shm = fft_shm(t, c->g, direct);
   Prof pr(c, pass == 0 ? NSPT_PROF_FA_DIV : NSPT_PROF_DFT);
   if (maxn == 16)
-    hipLaunchKernelGGL(k_fa_fft<16>, grid, dim3(FFT_THREADS), shm, c->stream, c->g, pass, Xsrc, c->fb, alpha);
+    hipLaunchKernelGGL(k_fa_fft<16>, grid, dim3(FFT_THREADS), shm, c->stream, c->g, pass, Xsrc, c->NE, xmu, c->fb,
+                       alpha);
   else
-    hipLaunchKernelGGL(k_fa_fft<FFT_MAXN>, grid, dim3(FFT_THREADS), shm, c->stream, c->g, pass, Xsrc, c->fb, alpha);
+    hipLaunchKernelGGL(k_fa_fft<FFT_MAXN>, grid, dim3(FFT_THREADS), shm, c->stream, c->g, pass, Xsrc, c->NE, xmu,
+                       c->fb, alpha);
   return launch_ok(c, pass == 0 ? "fa_div" : "fa_fft");
 }
 
@@ -1181,9 +1188,9 @@ int nspt_create(const nspt_geom* geom, nspt_ctx** out) {
   c->V = (int)V;
   c->ks = find_set(c->nc, c->nrt);
   if (!c->ks.drift) return bad("nspt_create: no compiled kernel instance for this order (SU(3) up to 32, SU(2) up to 24)");
-  // fields whose order planes reach past 2 GiB (the plaquette field, 6V
-  // series, is the widest) take the instance built for them
-  c->big = (long)(c->ks.nmax - 1) * c->NE * 6L * V * 16 >= 0x80000000L;
+  // fields past 2 GiB (the plaquette field, 6V series, is the widest) take
+  // the instance built for them
+  c->big = nspt::ser_alloc(6L * V, c->ks.nmax, c->NE) * 16 >= 0x80000000L;
   if (c->big) c->ks = find_set(c->nc, c->nrt, 4, true);
   for (int d = 0; d < 4; d++) c->g.L[d] = geom->sz[d];
   c->g.st[3] = 1;
@@ -1246,7 +1253,8 @@ int nspt_obs_len(const nspt_ctx* c) { return c ? c->obs_len : -1; }
 int nspt_cold_start(nspt_ctx* c) {
   if (!c) return -1;
   const long P = 4L * c->V;
-  if (!c->hip_ok(hipMemsetAsync(c->U.data, 0, sizeof(double2) * (size_t)c->nrt * c->NE * P, c->stream), "memset"))
+  if (!c->hip_ok(hipMemsetAsync(c->U.data, 0, sizeof(double2) * (size_t)nspt::ser_alloc(P, c->nrt, c->NE), c->stream),
+                 "memset"))
     return -3;
   std::vector<double2> ones(P, make_double2(1.0, 0.0));
   if (!c->hip_ok(hipMemcpyAsync(c->U.flag, ones.data(), sizeof(double2) * P, hipMemcpyHostToDevice, c->stream),
@@ -1486,7 +1494,7 @@ int nspt_zero_modes_subtraction(nspt_ctx* c) {
   c->ks.logf(c->nblk_links, c->stream, c->g, c->U, c->L);
   if (!launch_ok(c, "log")) return -3;
   hipLaunchKernelGGL(k_field_mean, dim3(4 * c->nrt * c->NE), dim3(MEAN_THREADS), 0, c->stream,
-                     (const double2*)c->L.data, c->L.P, c->V, c->nrt, c->NE, reinterpret_cast<double2*>(c->MM));
+                     (const double2*)c->L.data, c->V, c->nrt, c->NE, reinterpret_cast<double2*>(c->MM));
   if (!launch_ok(c, "field mean")) return -3;
   c->ks.zero_mode(c->nblk_links, c->stream, c->g, c->L, c->MM, c->U, c->X);
   if (!launch_ok(c, "zero_mode")) return -3;
@@ -1522,7 +1530,7 @@ int nspt_momentum_norm(nspt_ctx* c, double* knorm) {
   // per direction: its NE nrt planes through the FFT buffer (passes 3, 4),
   // then their norms added
   for (int mu = 0; mu < 4; mu++) {
-    if (!fft_pass(c, 3, (const double2*)c->U.data + (long)mu * c->V, 0.0) || !fft_pass(c, 4, (const double2*)c->U.data, 0.0))
+    if (!fft_pass(c, 3, (const double2*)c->U.data, 0.0, mu) || !fft_pass(c, 4, (const double2*)c->U.data, 0.0))
       return -3;
     hipLaunchKernelGGL(k_knorm_acc, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, (const double2*)c->fb,
                        (long)c->V, c->NE, c->nrt, mu == 0 ? 1 : 0, c->knorm_dev);
@@ -1924,7 +1932,7 @@ void plan_table_step(nspt_batch* b, const std::vector<int>& idx, int q, bool wit
         const FftTile t = fft_tile(c->g, pass);
         const bool direct = !fft_unrolled(t.na, maxn) || !fft_unrolled(t.nb, maxn);
         shm = std::max(shm, fft_shm(t, c->g, direct));
-        v[r] = FftArgs{c->g, c->L.data, c->fb, params[r].alpha, (int)((long)t.tiles * ncomp)};
+        v[r] = FftArgs{c->g, c->L.data, c->fb, params[r].alpha, (int)((long)t.tiles * ncomp), c->NE};
         gx = std::max(gx, (long)t.tiles * ncomp);
       }
       tp.launch(TabLaunch::FFT, dim3((unsigned)gx, nr), v, shm, pass, maxn);
@@ -2318,7 +2326,7 @@ int nspt_batch_tune(nspt_batch* b, int steps, int* qt_chosen, double* ms_per_ste
   };
   for (size_t r = 0; r < b->reps.size() && rc == 0; r++) {
     nspt_ctx* c = b->reps[r];
-    const size_t n = (size_t)c->nrt * c->NE * 4 * c->V + 4 * (size_t)c->V;
+    const size_t n = (size_t)nspt::ser_alloc(4L * c->V, c->nrt, c->NE) + 4 * (size_t)c->V;
     if (hipMalloc(&snap[r], n * sizeof(double2)) != hipSuccess) {
       rc = b->fail("tune: snapshot allocation failed");
       break;
@@ -2368,7 +2376,7 @@ int nspt_batch_tune(nspt_batch* b, int steps, int* qt_chosen, double* ms_per_ste
   if (snapped) {
     for (size_t r = 0; r < b->reps.size(); r++) {
       nspt_ctx* c = b->reps[r];
-      const size_t n = (size_t)c->nrt * c->NE * 4 * c->V;
+      const size_t n = (size_t)nspt::ser_alloc(4L * c->V, c->nrt, c->NE);
       (void)hipMemcpyAsync(c->U.data, snap[r], n * sizeof(double2), hipMemcpyDeviceToDevice, b->master);
       (void)hipMemcpyAsync(c->U.flag, snap[r] + n, 4 * (size_t)c->V * sizeof(double2), hipMemcpyDeviceToDevice,
                            b->master);

@@ -27,19 +27,21 @@
 // the reference's (N-1)N(N+1)/6 matrix products per exp/log.  The generator
 // X stays in slot B and S in slot A for all steps.
 //
-// HBM layout of a series field with P series (structure of arrays):
-//   data[(k*NE + e)*P + idx]  double2, k = 0..N-1 is order k+1
-//   flag[idx]                 double2
+// HBM layout of a series field with P series (structure of arrays in
+// tiles of TILE = 64 consecutive series):
+//   data[((idx/TILE)*nrt*NE + k*NE + e)*TILE + idx%TILE]  double2, k = 0..nrt-1
+//                                                         is order k+1
+//   flag[idx]                                             double2
 #pragma once
 #include <hip/hip_runtime.h>
 
-// The order planes of a large field (32^4 at N = 32: 0.6 GB apart) lie
-// beyond the 32-bit byte offset of one buffer descriptor: every kernel
-// instance is also compiled with NSPT_BIG_FIELDS = 1, which reads each plane
-// at a scalar 64-bit address plus a 32-bit lane offset and which the host
-// picks for such fields (the per-plane addresses cost scalar registers the
-// common case should not pay).  The two builds live in different inline
-// namespaces, so that their kernels and helpers are distinct symbols.
+// A large field (32^4 at N = 32: 19 GB) lies beyond the 32-bit byte offset
+// of one buffer descriptor: every kernel instance is also compiled with
+// NSPT_BIG_FIELDS = 1, which addresses each lane's series with 64 bits and
+// which the host picks for fields past 2 GiB (the 64-bit lane addresses
+// cost registers and address arithmetic the common case should not pay).
+// The two builds live in different inline namespaces, so that their
+// kernels and helpers are distinct symbols.
 #ifndef NSPT_BIG_FIELDS
 #define NSPT_BIG_FIELDS 0
 #endif
@@ -54,8 +56,27 @@ namespace nspt {
 struct SerField {
   double2* data;
   double2* flag;  // may be null for fields whose flag is implied
-  long P;         // number of series (plane stride)
+  long P;         // number of series
 };
+
+// A series field holds its series in tiles of NSPT_TILE consecutive
+// indices: the tile of series idx holds every order and element of its
+// NSPT_TILE series, order-major, then element, then series, so that order k,
+// element e of series idx is complex entry ser_index(idx, k*NE + e).  One
+// link's series then spans one tile (184 KB at 20 orders of SU(3)) instead
+// of nrt*NE planes P*16 bytes apart: its loads touch one or two pages where
+// the plane layout touched a page per order and element (the GPU's L1
+// address-translation cache then missed 9-10% of the product kernels'
+// accesses), and the NSPT_TILE series of one (k, e) are still contiguous.
+// The allocation is rounded up to whole tiles.
+constexpr int NSPT_TILE = 64;
+constexpr int NSPT_TILE_SHIFT = 6;
+__host__ __device__ inline long ser_alloc(long P, int nrt, int NE) {
+  return (P + NSPT_TILE - 1) / NSPT_TILE * NSPT_TILE * nrt * NE;
+}
+__host__ __device__ inline long ser_index(long idx, int ke, int nrt, int NE) {
+  return (idx >> NSPT_TILE_SHIFT) * ((long)nrt * NE * NSPT_TILE) + (long)ke * NSPT_TILE + (idx & (NSPT_TILE - 1));
+}
 
 inline namespace NSPT_VARIANT {
 
@@ -164,7 +185,7 @@ struct Lane {
 // Load of element `off` (32-bit, per lane) of an array whose base is
 // wave-uniform: a raw buffer load with a scalar resource descriptor, so the
 // lane carries only a 32-bit offset.  Offsets past 2 GiB are not supported
-// (every per-plane offset here is far below that).
+// (fields past 2 GiB take the NSPT_BIG_FIELDS build).
 __device__ __forceinline__ const double2* uniform_ptr(const double2* p) {
   const unsigned long long a = reinterpret_cast<unsigned long long>(p);
   const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a);
@@ -180,14 +201,6 @@ __device__ __forceinline__ double2 ld_r(__amdgpu_buffer_rsrc_t r, unsigned off, 
   double2 d;
   __builtin_memcpy(&d, &v, sizeof(d));
   return d;
-}
-__device__ __forceinline__ double2 ld_u(const double2* base, unsigned off) { return ld_r(rsrc(base), off, 0u); }
-// 16-byte load at element `off` (per lane) of the plane starting `pofs`
-// elements (wave-uniform, any size) after `base`: a scalar 64-bit plane
-// address plus a 32-bit per-lane byte offset.
-__device__ __forceinline__ double2 ld_plane(const double2* base, long pofs, unsigned off) {
-  const char* p = reinterpret_cast<const char*>(uniform_ptr(base + pofs));
-  return *reinterpret_cast<const double2*>(p + (off * 16u));
 }
 __device__ __forceinline__ void st_u(double2* base, unsigned off, double2 v) {
   *reinterpret_cast<double2*>(reinterpret_cast<char*>(base) + (size_t)(off * 16u)) = v;
@@ -620,23 +633,25 @@ __device__ __forceinline__ void unstage(const Lane<NC, N>& ln, const double2* sl
   for (int k = 0; k < N; k++) X[k] = slot[k * NE + ln.e];
 }
 
-// Order plane k of a field starts k * plane elements after order 0.  While
-// the last plane starts within 2 GiB of the field, one buffer descriptor
-// and a scalar byte offset per order reach every plane; NSPT_BIG_FIELDS
-// builds address each plane on its own (see the top of this file).
+// Order k of a series lies k * NE * NSPT_TILE entries after its order 0.
+// While a field stays within 2 GiB, one buffer descriptor, a 32-bit lane
+// offset and a scalar byte offset per order reach every entry;
+// NSPT_BIG_FIELDS builds address each lane's series with 64 bits (see the
+// top of this file).
 // Global loads/stores of a lane-owned series.  Orders >= nrt read as zero and
 // are not stored.  Loads are unmasked (lanes of invalid links hold clamped,
 // valid indices); stores are masked by ok.
 template <int NC, int N>
 __device__ __forceinline__ void gload(const SerField& f, long idx, int e, int nrt, bool ok, double2 (&X)[N]) {
   constexpr int NE = NC * NC;
-  const unsigned off = (unsigned)(e * f.P + idx);
-  const long plane = (long)NE * f.P;
+  constexpr int plane = NE * NSPT_TILE;  // entries from order k to k+1 of a series
   (void)ok;
 #if NSPT_BIG_FIELDS
+  const double2* p = f.data + ser_index(idx, e, nrt, NE);
 #pragma unroll
-  for (int k = 0; k < N; k++) X[k] = (k < nrt) ? ld_plane(f.data, k * plane, off) : zero2();
+  for (int k = 0; k < N; k++) X[k] = (k < nrt) ? p[k * plane] : zero2();
 #else
+  const unsigned off = (unsigned)ser_index(idx, e, nrt, NE);
   const __amdgpu_buffer_rsrc_t r = rsrc(f.data);
 #pragma unroll
   for (int k = 0; k < N; k++) X[k] = (k < nrt) ? ld_r(r, off, (unsigned)(k * plane * 16)) : zero2();
@@ -645,27 +660,31 @@ __device__ __forceinline__ void gload(const SerField& f, long idx, int e, int nr
 template <int NC, int N>
 __device__ __forceinline__ void gstore(const SerField& f, long idx, int e, int nrt, bool ok, const double2 (&X)[N]) {
   constexpr int NE = NC * NC;
+  constexpr int plane = NE * NSPT_TILE;
   if (!ok) return;
-  const unsigned off = (unsigned)(e * f.P + idx);
-  const long plane = (long)NE * f.P;
+#if NSPT_BIG_FIELDS
+  double2* p = f.data + ser_index(idx, e, nrt, NE);
+#pragma unroll
+  for (int k = 0; k < N; k++)
+    if (k < nrt) p[k * plane] = X[k];
+#else
+  const unsigned off = (unsigned)ser_index(idx, e, nrt, NE);
 #pragma unroll
   for (int k = 0; k < N; k++)
     if (k < nrt) st_u(f.data + k * plane, off, X[k]);
+#endif
 }
 // f += X (own element), or f = X when first.
 template <int NC, int N>
 __device__ __forceinline__ void gaccum(const SerField& f, long idx, int e, int nrt, bool ok, const double2 (&X)[N],
                                        bool first) {
   constexpr int NE = NC * NC;
+  constexpr int plane = NE * NSPT_TILE;
   if (!ok) return;
-  const unsigned off = (unsigned)(e * f.P + idx);
-  const long plane = (long)NE * f.P;
+  double2* p = f.data + ser_index(idx, e, nrt, NE);
 #pragma unroll
   for (int k = 0; k < N; k++)
-    if (k < nrt) {
-      double2* p = f.data + k * plane;
-      st_u(p, off, first ? X[k] : c_add(ld_u(p, off), X[k]));
-    }
+    if (k < nrt) p[k * plane] = first ? X[k] : c_add(p[k * plane], X[k]);
 }
 __device__ __forceinline__ double2 gflag(const SerField& f, long idx, bool ok) {
   return (ok && f.flag) ? f.flag[idx] : zero2();

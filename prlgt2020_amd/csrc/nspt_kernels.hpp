@@ -190,12 +190,14 @@ template <int NC, int N, int CH = NSPT_STAGE_CHUNK>
 __device__ __forceinline__ void stage_global(const Lane<NC, N>& ln, double2* slot, const SerField& f, long idx,
                                              int nrt, bool dag = false) {
   constexpr int NE = NC * NC;
+  constexpr int plane = NE * NSPT_TILE;
   if (!ln.lds_ok) return;
-  const unsigned off = (unsigned)(ln.e * f.P + idx);
-  const long plane = (long)NE * f.P;
   const int pos = dag ? (ln.c * NC + ln.r) : ln.e;
   const double sy = dag ? -1.0 : 1.0;
-#if !NSPT_BIG_FIELDS
+#if NSPT_BIG_FIELDS
+  const double2* p = f.data + ser_index(idx, ln.e, nrt, NE);
+#else
+  const unsigned off = (unsigned)ser_index(idx, ln.e, nrt, NE);
   const __amdgpu_buffer_rsrc_t r = rsrc(f.data);
 #endif
 #pragma unroll
@@ -207,7 +209,7 @@ __device__ __forceinline__ void stage_global(const Lane<NC, N>& ln, double2* slo
       if (k < N) {
         const int kk = uniform(k < nrt ? k : 0);
 #if NSPT_BIG_FIELDS
-        v[i] = ld_plane(f.data, kk * plane, off);
+        v[i] = p[kk * plane];
 #else
         v[i] = ld_r(r, off, (unsigned)(kk * plane * 16));
 #endif
