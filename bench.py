@@ -330,8 +330,9 @@ def batched_sweep(nsteps, warmup, sizes=(12, 16), per_volume=4):
     further seeds when per_volume > 4), distinct seeds.  Plain: each replica
     a context of its own, stepped alone (nspt_evolve, its own launches).
     Batched: one graph-replayed step of all replicas, the step form
-    (branches, or the replica table in 1, 2 or 4 concurrent groups) and
-    tile edge autotuned; the other forms are timed too."""
+    (branches, or the replica table in 1, 2 or 4 concurrent groups), tile
+    edge and occupancy (links per wave, waves per SIMD) autotuned; the other
+    forms are timed too."""
     from prlgt2020_amd import nspt
     N = 16
     Ls = [(sizes[0],) * 4] * per_volume + [(sizes[1],) * 4] * per_volume
@@ -341,7 +342,11 @@ def batched_sweep(nsteps, warmup, sizes=(12, 16), per_volume=4):
     work = sum(4 * L[0] ** 4 * N for L in Ls)  # link-orders per step of all replicas
     b = nspt.Batch(Ls, 3, N, ps)
     b.evolve(warmup, want_obs=False)
-    qt, tune_ms = b.tune(steps=3)  # step form (branches / table in 1, 2, 4 groups) x tile edge
+    # stage 1: step form (branches / table in 1, 2, 4 groups) x tile edge;
+    # stage 2: the occupancy variants of the chosen edge in the chosen form
+    qt, tune_ms = b.tune(steps=3)
+    cands = b.tune_candidates()
+    shape = b.kernel_shape()
     chosen = b.form
     # plain: the same replicas as contexts of their own, one after the other,
     # each with its own launches
@@ -368,7 +373,7 @@ def batched_sweep(nsteps, warmup, sizes=(12, 16), per_volume=4):
     t0 = time.perf_counter()
     b.evolve(nsteps, want_obs=True)
     t_batch = time.perf_counter() - t0
-    # every other step form, timed the same way (the tile edge stays the tuned one)
+    # every other step form, timed the same way (the kernel build stays the tuned one)
     forms = [0, 1] + [g for g in (2, 4) if g <= len(Ls) // 2]
     name = lambda f: "branches" if f == 0 else ("table" if f == 1 else "table in %d groups" % f)
     form_values = {name(chosen): work * nsteps / t_batch}
@@ -381,10 +386,10 @@ def batched_sweep(nsteps, warmup, sizes=(12, 16), per_volume=4):
         b.evolve(nsteps, want_obs=True)
         form_values[name(f)] = work * nsteps / (time.perf_counter() - t0)
     b.close()
-    # the tuner's candidates, form-major over the tile edges compiled for
-    # SU(3) at 16 orders (4 and 3)
-    tune = {"%s, edge %d" % (name(f), q): round(float(tune_ms[i]), 3)
-            for i, (f, q) in enumerate((f, q) for f in forms for q in (4, 3))}
+    # the tuner's candidates: form x tile edge, then the occupancy variants
+    label = lambda c: "%s, edge %d%s" % (name(c[0]), c[1], "" if c[2] == 0 and c[3] == 1 else
+                                        ", %d links/wave, %d waves/SIMD" % (c[2], c[3]))
+    tune = {label(c): round(float(tune_ms[i]), 3) for i, c in enumerate(cands)}
     return {"workload": "SU(3) %dx%d^4 + %dx%d^4, 16 orders, RK2 + zero modes + FA, eps {0.01,0.015,0.02,0.03} "
                         "per volume%s, distinct seeds, device noise"
                         % (per_volume, sizes[0], per_volume, sizes[1],
@@ -392,7 +397,8 @@ def batched_sweep(nsteps, warmup, sizes=(12, 16), per_volume=4):
             "steps": nsteps, "plain_value": work * nsteps / t_plain, "batched_value": work * nsteps / t_batch,
             "plain_value_per_volume": per_volume_value, "best_volume_plain_value": max(per_volume_value.values()),
             "unit": "link-order updates/s", "speedup": t_plain / t_batch,
-            "form": name(chosen), "tile_edge": qt, "form_values": form_values, "tune_ms_per_step": tune}
+            "form": name(chosen), "tile_edge": qt, "links_per_wave": shape[1], "waves_per_simd": shape[2],
+            "form_values": form_values, "tune_ms_per_step": tune}
 
 
 def main():

@@ -66,11 +66,20 @@ typedef struct {
 } nspt_step_params;
 
 /* Context life cycle.  The field starts cold (flag 1, orders 0), which is
- * init_status 0 of Quenched.cc:596-604. */
+ * init_status 0 of Quenched.cc:596-604.  The context runs the default kernel
+ * build of its order (the smallest compiled order >= ptord, tile edge 4,
+ * the build's own links per wave, one wave per SIMD), or the build that the
+ * environment names at creation when it is compiled: NSPT_TILE_EDGE,
+ * NSPT_LINKS_PER_WAVE, NSPT_WAVES_PER_SIMD (A/B runs; results differ from
+ * the default build's by rounding only). */
 int nspt_create(const nspt_geom* geom, nspt_ctx** out);
 int nspt_destroy(nspt_ctx* ctx);
 const char* nspt_last_error(const nspt_ctx* ctx);
 int nspt_obs_len(const nspt_ctx* ctx);
+/* The kernel build a context runs: order-tile edge, links per wave (the
+ * links one wave of a link kernel holds), waves per SIMD its register budget
+ * allows.  No counterpart in the reference (a CPU code). */
+int nspt_kernel_shape(const nspt_ctx* ctx, int* tile_edge, int* links_per_wave, int* waves_per_simd);
 
 /* Field transfer in the ptGluon[V] image (Umu.load / Umu.save payload,
  * QCDenvNODEpt_test.h:342-374). */
@@ -259,14 +268,23 @@ int nspt_batch_replica(nspt_batch* b, int r, nspt_ctx** ctx);
  * nspt_batch_form returns the form so set. */
 int nspt_batch_set_form(nspt_batch* b, int table);
 int nspt_batch_form(const nspt_batch* b);
-/* Autotuner: for each step form (branches, then when possible the table in
- * 1, 2 and 4 groups, groups of at least two replicas) and each compiled
- * order-tile edge of the replicas' kernel instance, time `steps` replays
- * of the batch step; keep the fastest form and edge.  Fields and counters
- * are restored afterwards, on failure too (and then the form and edge too).
- * ms_per_step (may be null, up to 16 entries) receives the measured times
- * in candidate order (form-major). */
+/* Autotuner, in two stages.  Stage 1: for each step form (branches, then
+ * when possible the table in 1, 2 and 4 groups, groups of at least two
+ * replicas) and each compiled order-tile edge of the replicas' order, at the
+ * default occupancy, time `steps` replays of the batch step.  Stage 2: in
+ * the fastest form and edge so far, each compiled occupancy variant of that
+ * edge (links per wave x waves per SIMD).  The fastest candidate is kept.
+ * Fields and counters are restored afterwards, on failure too (and then the
+ * form and kernel build too).  ms_per_step (may be null, up to 16 entries)
+ * receives the measured times in candidate order, which
+ * nspt_batch_tune_candidates describes. */
 int nspt_batch_tune(nspt_batch* b, int steps, int* qt_chosen, double* ms_per_step);
+/* The last nspt_batch_tune's candidates, 4 ints each: form (0 branches, g
+ * the table in g groups), tile edge, links per wave (0: the build's
+ * default), waves per SIMD.  Writes up to max; returns their number. */
+int nspt_batch_tune_candidates(const nspt_batch* b, int* desc, int max);
+/* The kernel build the batch's replicas run (as nspt_kernel_shape). */
+int nspt_batch_kernel_shape(const nspt_batch* b, int* tile_edge, int* links_per_wave, int* waves_per_simd);
 
 /* The Langevin drift as a reverse-mode gradient of the order-truncated
  * Wilson action (no counterpart in the reference, whose drift is the staple

@@ -2,8 +2,9 @@
 
     python -m prlgt2020_amd.build [--force] [-j N]
 
-Each (NC, N) kernel instance is one translation unit (nspt_inst.hip compiled
-with -DINST_NC/-DINST_N, once more with -DINST_BIG=1 for fields past 2 GiB)
+Each kernel instance is one translation unit (nspt_inst.hip compiled with
+-DINST_NC/-DINST_N/-DINST_QT/-DINST_LPW/-DINST_WPE, once more with
+-DINST_BIG=1 for fields past 2 GiB)
 so they compile in parallel; the host side
 (nspt_capi.hip, nspt_rng.cpp) is linked with them into
 prlgt2020_amd/libnspt.so.  The instance list mirrors NSPT_INSTANCES of
@@ -41,7 +42,12 @@ def instances():
     # variant for quick experiments)
     block = text[text.rindex("#define NSPT_INSTANCES"):]
     block = block[:block.index("\n\n")]
-    return [(int(a), int(b), int(q)) for a, b, q in re.findall(r"X\((\d+),\s*(\d+),\s*(\d+)\)", block)]
+    return [tuple(int(v) for v in m)
+            for m in re.findall(r"X\((\d+),\s*(\d+),\s*(\d+),\s*(\d+),\s*(\d+)\)", block)]
+
+
+def _inst_obj(nc, n, q, lpw, wpe, big):
+    return os.path.join(OBJ, "inst_%d_%d_q%d_l%d_w%d%s.o" % (nc, n, q, lpw, wpe, "_big" if big else ""))
 
 
 def _newer(target, sources):
@@ -63,12 +69,13 @@ def build(force=False, jobs=None, verbose=False):
     hdrs = [os.path.join(CSRC, h) for h in HEADERS] + [os.path.join(HERE, "..", "include", "nspt.h")]
     jobs = jobs or min(8, os.cpu_count() or 4)
     tasks = []
-    for nc, n, q in instances():
+    for nc, n, q, lpw, wpe in instances():
         src = os.path.join(CSRC, "nspt_inst.hip")
         for big in (0, 1):
-            obj = os.path.join(OBJ, "inst_%d_%d_q%d%s.o" % (nc, n, q, "_big" if big else ""))
+            obj = _inst_obj(nc, n, q, lpw, wpe, big)
             if force or _newer(obj, [src] + hdrs):
                 tasks.append([HIPCC] + HIPFLAGS + ["-DINST_NC=%d" % nc, "-DINST_N=%d" % n, "-DINST_QT=%d" % q,
+                                                   "-DINST_LPW=%d" % lpw, "-DINST_WPE=%d" % wpe,
                                                    "-DINST_BIG=%d" % big, "-c", src, "-o", obj])
     capi_obj = os.path.join(OBJ, "nspt_capi.o")
     if force or _newer(capi_obj, [os.path.join(CSRC, "nspt_capi.hip")] + hdrs):
@@ -83,8 +90,7 @@ def build(force=False, jobs=None, verbose=False):
         for r in ex.map(_run, tasks):
             if verbose and r.stderr:
                 print(r.stderr, file=sys.stderr)
-    objs = [os.path.join(OBJ, "inst_%d_%d_q%d%s.o" % (nc, n, q, sfx)) for nc, n, q in instances()
-            for sfx in ("", "_big")] + [capi_obj, adj_obj, rng_obj]
+    objs = [_inst_obj(*inst, big) for inst in instances() for big in (0, 1)] + [capi_obj, adj_obj, rng_obj]
     relinked = False
     if force or tasks or _newer(LIB, objs):
         _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", LIB + ".tmp"] + objs)

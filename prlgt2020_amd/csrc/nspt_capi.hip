@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -749,37 +750,48 @@ __global__ void k_knorm_acc(const double2* __restrict__ fb, long V, int NE, int 
   out[i] = first ? s : out[i] + s;
 }
 
-// Smallest compiled N >= n; among those the tile edge qt if compiled, else
-// the default edge 4.
-KernelSet find_set(int nc, int n, int qt = 4, bool big = false) {
-  KernelSet best;
-  auto consider = [&](KernelSet s) {
-    if (s.nc != nc || s.nmax < n || s.big != big) return;
-    if (best.nmax == 0 || s.nmax < best.nmax) {
-      best = s;
-    } else if (s.nmax == best.nmax && s.qt == qt && best.qt != qt) {
-      best = s;
-    }
+// Smallest compiled N >= n; among those the shape (tile edge qt, links per
+// wave lpw, waves per SIMD wpe) if compiled, else the default shape (edge 4,
+// lpw 0, one wave per SIMD).
+KernelSet find_set(int nc, int n, int qt = 4, bool big = false, int lpw = 0, int wpe = 1) {
+  int nmax = 0;
+  auto order = [&](KernelSet s) {
+    if (s.nc == nc && s.nmax >= n && s.big == big && (nmax == 0 || s.nmax < nmax)) nmax = s.nmax;
   };
-#define NSPT_CONSIDER(a, b, q)                  \
-  consider(nspt::kernel_set_##a##_##b##_q##q()); \
-  consider(nspt::kernel_set_##a##_##b##_q##q##_big());
-  NSPT_INSTANCES(NSPT_CONSIDER)
-#undef NSPT_CONSIDER
-  if (best.nmax && best.qt != qt) {
-    KernelSet dflt = best;
-    best = KernelSet();
-    auto prefer4 = [&](KernelSet s) {
-      if (s.nc == nc && s.nmax == dflt.nmax && s.qt == 4 && s.big == big) best = s;
-    };
-#define NSPT_CONSIDER4(a, b, q)                 \
-  prefer4(nspt::kernel_set_##a##_##b##_q##q()); \
-  prefer4(nspt::kernel_set_##a##_##b##_q##q##_big());
-    NSPT_INSTANCES(NSPT_CONSIDER4)
-#undef NSPT_CONSIDER4
-    if (!best.nmax) best = dflt;
-  }
-  return best;
+  KernelSet want, dflt;
+  auto pick = [&](KernelSet s) {
+    if (s.nc != nc || s.nmax != nmax || s.big != big) return;
+    if (s.qt == qt && s.lpw == lpw && s.wpe == wpe) want = s;
+    if (s.qt == 4 && s.lpw == 0 && s.wpe == 1) dflt = s;
+  };
+#define NSPT_ORDER(a, b, q, l, w)                        \
+  order(nspt::kernel_set_##a##_##b##_q##q##_l##l##_w##w()); \
+  order(nspt::kernel_set_##a##_##b##_q##q##_l##l##_w##w##_big());
+  NSPT_INSTANCES(NSPT_ORDER)
+#undef NSPT_ORDER
+  if (!nmax) return KernelSet();
+#define NSPT_PICK(a, b, q, l, w)                        \
+  pick(nspt::kernel_set_##a##_##b##_q##q##_l##l##_w##w()); \
+  pick(nspt::kernel_set_##a##_##b##_q##q##_l##l##_w##w##_big());
+  NSPT_INSTANCES(NSPT_PICK)
+#undef NSPT_PICK
+  return want.nmax ? want : dflt;
+}
+
+// Fewest links per wave among the compiled builds of (nc, nmax): the
+// per-block partial-sum buffers are sized for it, so that the autotuner can
+// move a context to any of them.
+int min_links_per_wave(int nc, int nmax) {
+  int m = 64;
+  auto see = [&](KernelSet s) {
+    if (s.nc == nc && s.nmax == nmax) m = std::min(m, s.lpb);
+  };
+#define NSPT_SEE(a, b, q, l, w)                        \
+  see(nspt::kernel_set_##a##_##b##_q##q##_l##l##_w##w()); \
+  see(nspt::kernel_set_##a##_##b##_q##q##_l##l##_w##w##_big());
+  NSPT_INSTANCES(NSPT_SEE)
+#undef NSPT_SEE
+  return m;
 }
 
 int grid_for(long n, int block) {
@@ -1158,6 +1170,24 @@ bool ensure_obs(nspt_ctx* c, int n) {
   return true;
 }
 
+// Moves a context to kernel build ks: the link and site blocks follow its
+// links per wave (the partial-sum buffers are sized for the fewest, see
+// min_links_per_wave).
+void set_kernels(nspt_ctx* c, const KernelSet& ks) {
+  c->ks = ks;
+  c->g.nb_mu = (c->V + ks.lpb - 1) / ks.lpb;
+  c->g.nb_half = (c->V / 2 + ks.lpb - 1) / ks.lpb;
+  c->nblk_links = 4 * c->g.nb_mu;
+  c->nblk_sites = c->g.nb_mu;
+}
+
+// A kernel-build choice from the environment (NSPT_TILE_EDGE,
+// NSPT_LINKS_PER_WAVE, NSPT_WAVES_PER_SIMD; A/B runs and tests), else dflt.
+int env_choice(const char* name, int dflt) {
+  const char* v = std::getenv(name);
+  return (v && *v) ? std::atoi(v) : dflt;
+}
+
 }  // namespace
 
 // ================================================================== C ABI
@@ -1186,12 +1216,16 @@ int nspt_create(const nspt_geom* geom, nspt_ctx** out) {
     V *= geom->sz[d];
   }
   c->V = (int)V;
-  c->ks = find_set(c->nc, c->nrt);
+  // the default build of the order, or the shape the environment names if
+  // it is compiled
+  const int qt = env_choice("NSPT_TILE_EDGE", 4), lpw = env_choice("NSPT_LINKS_PER_WAVE", 0),
+            wpe = env_choice("NSPT_WAVES_PER_SIMD", 1);
+  c->ks = find_set(c->nc, c->nrt, qt, false, lpw, wpe);
   if (!c->ks.drift) return bad("nspt_create: no compiled kernel instance for this order (SU(3) up to 32, SU(2) up to 24)");
   // fields past 2 GiB (the plaquette field, 6V series, is the widest) take
   // the instance built for them
   c->big = nspt::ser_alloc(6L * V, c->ks.nmax, c->NE) * 16 >= 0x80000000L;
-  if (c->big) c->ks = find_set(c->nc, c->nrt, 4, true);
+  if (c->big) c->ks = find_set(c->nc, c->nrt, qt, true, lpw, wpe);
   for (int d = 0; d < 4; d++) c->g.L[d] = geom->sz[d];
   c->g.st[3] = 1;
   c->g.st[2] = c->g.L[3];
@@ -1199,11 +1233,8 @@ int nspt_create(const nspt_geom* geom, nspt_ctx** out) {
   c->g.st[0] = c->g.L[1] * c->g.L[2] * c->g.L[3];
   c->g.V = c->V;
   c->g.nrt = c->nrt;
-  c->g.nb_mu = (c->V + c->ks.lpb - 1) / c->ks.lpb;
   c->g.sub = 0;
-  c->g.nb_half = (c->V / 2 + c->ks.lpb - 1) / c->ks.lpb;
-  c->nblk_links = 4 * c->g.nb_mu;
-  c->nblk_sites = c->g.nb_mu;
+  set_kernels(c, c->ks);
   c->nob = 2 * (c->nrt + 1) + 3 * c->nrt;
   c->obs_len = 2 * (c->nrt + 1) + c->nrt + 8 * c->nrt;
   // The plaquette form of the drift does fewer products but sums them in
@@ -1219,8 +1250,10 @@ int nspt_create(const nspt_geom* geom, nspt_ctx** out) {
   *out = c;
   if (!c->hip_ok(hipStreamCreate(&c->stream), "hipStreamCreate")) return -3;
   const long P = 4L * V;
-  // per-block partial sums: 4 nb_mu rows, or 8 nb_half for a checkerboard sweep
-  const long part_rows = std::max(4L * c->g.nb_mu, 8L * c->g.nb_half);
+  // per-block partial sums: 4 nb_mu rows, or 8 nb_half for a checkerboard
+  // sweep, for the build of this order with the fewest links per wave
+  const int lpb_min = std::min(c->ks.lpb, min_links_per_wave(c->nc, c->ks.nmax));
+  const long part_rows = std::max(4L * ((c->V + lpb_min - 1) / lpb_min), 8L * ((c->V / 2 + lpb_min - 1) / lpb_min));
   bool ok = c->alloc_field(c->U, P, true) && c->alloc_field(c->F, P, true) && c->alloc_field(c->D, P, false) &&
             c->alloc_field(c->D1, P, false) && c->alloc_field(c->L, P, true) && c->alloc_field(c->X, P, false) &&
             c->alloc_field(c->G, V, true) && (!c->plaq_form || c->alloc_field(c->Pf, 6L * V, true)) &&
@@ -1249,6 +1282,14 @@ int nspt_destroy(nspt_ctx* c) {
 const char* nspt_last_error(const nspt_ctx* c) { return c ? c->err.c_str() : "null context"; }
 
 int nspt_obs_len(const nspt_ctx* c) { return c ? c->obs_len : -1; }
+
+int nspt_kernel_shape(const nspt_ctx* c, int* qt, int* links_per_wave, int* waves_per_simd) {
+  if (!c || !c->ks.drift) return -1;
+  if (qt) *qt = c->ks.qt;
+  if (links_per_wave) *links_per_wave = c->ks.lpb;
+  if (waves_per_simd) *waves_per_simd = c->ks.wpe;
+  return 0;
+}
 
 int nspt_cold_start(nspt_ctx* c) {
   if (!c) return -1;
@@ -1699,6 +1740,9 @@ struct nspt_batch {
   int groups = 1;
   std::vector<void*> tab_dev[2];                   // per group
   std::vector<std::vector<TabLaunch>> tab_seq[2];  // per group
+  // the last autotuner run's candidates: form, tile edge, links per wave,
+  // waves per SIMD
+  std::vector<std::array<int, 4>> tune_cands;
   std::string err;
   int fail(const std::string& m) {
     err = m;
@@ -2293,36 +2337,47 @@ int nspt_batch_evolve(nspt_batch* b, int nsteps, double* obs) {
 int nspt_batch_tune(nspt_batch* b, int steps, int* qt_chosen, double* ms_per_step) {
   if (!b || steps < 1) return -1;
   nspt_ctx* c0 = b->reps[0];
-  // candidates: step form (graph branches, then when the replicas allow it
-  // the replica table in 1, 2 and 4 concurrent groups, as far as there are
-  // replicas) x compiled tile edge of the replicas' instance
+  // the compiled shapes (tile edge, links per wave, waves per SIMD) of the
+  // replicas' order
+  std::vector<std::array<int, 3>> shapes;
+  auto see = [&](KernelSet s) {
+    if (s.nc == c0->nc && s.nmax == c0->ks.nmax && s.big == c0->big) shapes.push_back({s.qt, s.lpw, s.wpe});
+  };
+#define NSPT_SEE(a, bb, q, l, w)                         \
+  see(nspt::kernel_set_##a##_##bb##_q##q##_l##l##_w##w()); \
+  see(nspt::kernel_set_##a##_##bb##_q##q##_l##l##_w##w##_big());
+  NSPT_INSTANCES(NSPT_SEE)
+#undef NSPT_SEE
+  // stage 1: step form (graph branches, then when the replicas allow it the
+  // replica table in 1, 2 and 4 concurrent groups, as far as there are
+  // replicas) x compiled tile edge at the default occupancy; stage 2: the
+  // occupancy variants compiled for the chosen edge, in the chosen form
   std::vector<int> edges;
-  for (int q : {4, 3, 5, 2}) {
-    const KernelSet ks = find_set(c0->nc, c0->nrt, q, c0->big);
-    if (ks.qt == q && ks.nmax == c0->ks.nmax) edges.push_back(q);
-  }
+  for (int q : {4, 3, 5, 2})
+    for (const auto& sh : shapes)
+      if (sh[0] == q && sh[1] == 0 && sh[2] == 1) edges.push_back(q);
   std::vector<int> forms = {0};
   if (b->table_ok)
     for (int g : {1, 2, 4})
       if (g == 1 || g <= (int)b->reps.size() / 2) forms.push_back(g);
-  std::vector<std::pair<int, int>> cand;
+  b->tune_cands.clear();
   for (int f : forms)
-    for (int q : edges) cand.push_back({f, q});
+    for (int q : edges) b->tune_cands.push_back({f, q, 0, 1});
+  const size_t stage1 = b->tune_cands.size();
   const int form0 = b->table ? b->groups : 0;
-  const int qt0 = c0->ks.qt;
+  const int qt0 = c0->ks.qt, lpw0 = c0->ks.lpw, wpe0 = c0->ks.wpe;
   // snapshot fields and counters: tuning steps are not part of the run
   std::vector<double2*> snap(b->reps.size(), nullptr);
   std::vector<unsigned long long> ctr(2 * b->reps.size());
   hipEvent_t e0 = nullptr, e1 = nullptr;
   bool snapped = false;
   double best = 1e30;
-  int best_form = form0;
-  int best_q = qt0;
+  std::array<int, 4> best_c = {form0, qt0, lpw0, wpe0};
   int rc = 0;
-  auto set_form = [&](int form, int q) {
-    b->table = form != 0;
-    if (form) b->groups = form;
-    for (auto* c : b->reps) c->ks = find_set(c->nc, c->nrt, q, c->big);
+  auto set_form = [&](const std::array<int, 4>& k) {
+    b->table = k[0] != 0;
+    if (k[0]) b->groups = k[0];
+    for (auto* c : b->reps) set_kernels(c, find_set(c->nc, c->nrt, k[1], c->big, k[2], k[3]));
   };
   for (size_t r = 0; r < b->reps.size() && rc == 0; r++) {
     nspt_ctx* c = b->reps[r];
@@ -2340,8 +2395,8 @@ int nspt_batch_tune(nspt_batch* b, int steps, int* qt_chosen, double* ms_per_ste
   if (rc == 0 && !batch_sync(b)) rc = -3;
   snapped = rc == 0;
   if (rc == 0 && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) rc = b->fail("tune: events");
-  for (size_t i = 0; i < cand.size() && rc == 0; i++) {
-    set_form(cand[i].first, cand[i].second);
+  for (size_t i = 0; i < b->tune_cands.size() && rc == 0; i++) {
+    set_form(b->tune_cands[i]);
     if (!batch_build(b, false) || !batch_replay(b, 1)) {
       rc = -3;
       break;
@@ -2362,16 +2417,22 @@ int nspt_batch_tune(nspt_batch* b, int steps, int* qt_chosen, double* ms_per_ste
     if (ms_per_step && i < 16) ms_per_step[i] = per;
     if (per < best) {
       best = per;
-      best_form = cand[i].first;
-      best_q = cand[i].second;
+      best_c = b->tune_cands[i];
+    }
+    // stage 2 joins the list once stage 1 is timed, in the best form and
+    // edge so far
+    if (i + 1 == stage1) {
+      for (const auto& sh : shapes)
+        if (sh[0] == best_c[1] && !(sh[1] == 0 && sh[2] == 1) && b->tune_cands.size() < 16)
+          b->tune_cands.push_back({best_c[0], sh[0], sh[1], sh[2]});
     }
   }
-  // every exit restores the run: the form and tiles (the best on success,
-  // the original otherwise), the fields and the counters
+  // every exit restores the run: the form and kernel build (the best on
+  // success, the original otherwise), the fields and the counters
   if (rc == 0)
-    set_form(best_form, best_q);
+    set_form(best_c);
   else
-    set_form(form0, qt0);
+    set_form({form0, qt0, lpw0, wpe0});
   batch_drop_graphs(b);
   if (snapped) {
     for (size_t r = 0; r < b->reps.size(); r++) {
@@ -2389,8 +2450,21 @@ int nspt_batch_tune(nspt_batch* b, int steps, int* qt_chosen, double* ms_per_ste
     if (p) (void)hipFree(p);
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
-  if (rc == 0 && qt_chosen) *qt_chosen = best_q;
+  if (rc == 0 && qt_chosen) *qt_chosen = best_c[1];
   return rc;
+}
+
+int nspt_batch_tune_candidates(const nspt_batch* b, int* desc, int max) {
+  if (!b || (max > 0 && !desc)) return -1;
+  const int n = (int)b->tune_cands.size();
+  for (int i = 0; i < n && i < max; i++)
+    for (int k = 0; k < 4; k++) desc[4 * i + k] = b->tune_cands[i][k];
+  return n;
+}
+
+int nspt_batch_kernel_shape(const nspt_batch* b, int* qt, int* links_per_wave, int* waves_per_simd) {
+  if (!b || b->reps.empty()) return -1;
+  return nspt_kernel_shape(b->reps[0], qt, links_per_wave, waves_per_simd);
 }
 
 }  // extern "C"

@@ -40,15 +40,34 @@
 // NSPT_BIG_FIELDS = 1, which addresses each lane's series with 64 bits and
 // which the host picks for fields past 2 GiB (the 64-bit lane addresses
 // cost registers and address arithmetic the common case should not pay).
-// The two builds live in different inline namespaces, so that their
-// kernels and helpers are distinct symbols.
+// Tile edge of the product: pairs of operand orders are taken QT x QT at a
+// time.
+#ifndef NSPT_QT
+#define NSPT_QT 4
+#endif
+// Occupancy shape of a build: links per wave (0: the choice of
+// links_per_wave below) and the waves per SIMD the kernels are compiled for
+// (their register budget: 512 / waves registers per lane).
+#ifndef NSPT_LPW
+#define NSPT_LPW 0
+#endif
+#ifndef NSPT_WAVES_PER_EU
+#define NSPT_WAVES_PER_EU 1
+#endif
+// Every build (field size x tile edge x occupancy shape) lives in an inline
+// namespace of its own, so that its kernels, their host handles and its
+// helpers are distinct symbols: two builds of one template kernel under one
+// name would be merged by the linker into one handle, and every build would
+// launch the code registered first.
 #ifndef NSPT_BIG_FIELDS
 #define NSPT_BIG_FIELDS 0
 #endif
+#define NSPT_VARIANT_NAME2(f, q, l, w) f##_q##q##_l##l##_w##w
+#define NSPT_VARIANT_NAME(f, q, l, w) NSPT_VARIANT_NAME2(f, q, l, w)
 #if NSPT_BIG_FIELDS
-#define NSPT_VARIANT big_fields
+#define NSPT_VARIANT NSPT_VARIANT_NAME(big_fields, NSPT_QT, NSPT_LPW, NSPT_WAVES_PER_EU)
 #else
-#define NSPT_VARIANT std_fields
+#define NSPT_VARIANT NSPT_VARIANT_NAME(std_fields, NSPT_QT, NSPT_LPW, NSPT_WAVES_PER_EU)
 #endif
 
 namespace nspt {
@@ -98,11 +117,6 @@ __device__ __forceinline__ double2 one2() { return make_double2(1.0, 0.0); }
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// Tile edge of the product: pairs of operand orders are taken QT x QT at a
-// time.
-#ifndef NSPT_QT
-#define NSPT_QT 4
-#endif
 // The drift's products are full-length (truncated at the top order only), so
 // a larger tile pays there; the Horner steps of exp/log are mostly short
 // products, where the partial bands of a larger tile waste more than its
@@ -125,10 +139,13 @@ __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirs
 // fewer waves, fewer links per wave can pay: the choice maximises waves per
 // CU (LDS in 1 KiB granules, 160 KiB per CU, plus a flag per link) times
 // working lanes.  SU(3): 7 links up to N = 24, 4 at N = 32; SU(2): 16, but
-// 15 at N = 20 and 13 at N = 24.
+// 15 at N = 20 and 13 at N = 24.  A build with NSPT_LPW > 0 takes that many
+// (the occupancy variants the batch autotuner times).
 template <int NC, int N>
 constexpr int links_per_wave() {
   constexpr int NE = NC * NC;
+  static_assert(NSPT_LPW >= 0 && NSPT_LPW * NE <= 64, "links per wave fit one wave");
+  if (NSPT_LPW > 0) return NSPT_LPW;
   int best = 1, best_score = 0;
   for (int l = 64 / NE; l >= 1; l--) {
     const int bytes = (2 * l * (N * NE + NSPT_SLOT_PAD) + (NSPT_QT_MAX - 2) * NE + l) * 16;

@@ -44,6 +44,8 @@ struct TwistTab {
 // Launchers of one (NC, N) instance.  nblk is the grid size in waves.
 struct KernelSet {
   int nc = 0, nmax = 0, lpb = 0, qt = 0;  // qt: order-tile edge of the products
+  int lpw = 0;                            // links per wave asked of the build (0: the default, lpb)
+  int wpe = 1;                            // waves per SIMD the build's register budget allows
   bool big = false;                       // built for fields past 2 GiB (NSPT_BIG_FIELDS)
   void (*drift)(int nblk, hipStream_t, Geo, SerField U, SerField D, SerField D1, double tau_g, double rk2c, int mode,
                 double* obs, SerField R, double c1) = nullptr;
@@ -79,25 +81,31 @@ struct KernelSet {
 };
 
 // One function per compiled instance (nspt_inst.hip built once per
-// (NC, N, tile edge)).  The default tile edge is 4; the SU(3) instances used
-// by the production configurations also come with edge 3 for the autotuner.
+// X(NC, N, tile edge, links per wave, waves per SIMD); links per wave 0 is
+// the instance's own choice, links_per_wave()).  The default instance of an
+// order has tile edge 4, links per wave 0 and one wave per SIMD; the SU(3)
+// orders of the production configurations also come with tile edge 3, and
+// N = 16, 20 with occupancy variants (4 links per wave at one and at two
+// waves per SIMD), for the batch autotuner and for A/B runs.
 #ifdef NSPT_SINGLE_INSTANCE
 #ifndef NSPT_SI_N
 #define NSPT_SI_N 20
 #endif
-#define NSPT_SI_APPLY(X, a, b, c) X(a, b, c)
-#define NSPT_INSTANCES(X) NSPT_SI_APPLY(X, 3, NSPT_SI_N, 4)
+#define NSPT_SI_APPLY(X, a, b, c, l, w) X(a, b, c, l, w)
+#define NSPT_INSTANCES(X) NSPT_SI_APPLY(X, 3, NSPT_SI_N, 4, 0, 1)
 #else
-#define NSPT_INSTANCES(X) \
-  X(3, 4, 4) X(3, 6, 4) X(3, 8, 4) X(3, 12, 4) X(3, 16, 4) X(3, 16, 3) X(3, 20, 4) X(3, 20, 3) X(3, 24, 4) X(3, 32, 4) \
-  X(2, 4, 4) X(2, 6, 4) X(2, 8, 4) X(2, 12, 4) X(2, 16, 4) X(2, 20, 4) X(2, 24, 4)
+#define NSPT_INSTANCES(X)                                                                                           \
+  X(3, 4, 4, 0, 1) X(3, 6, 4, 0, 1) X(3, 8, 4, 0, 1) X(3, 12, 4, 0, 1) X(3, 16, 4, 0, 1) X(3, 16, 3, 0, 1)           \
+  X(3, 16, 4, 4, 1) X(3, 16, 4, 4, 2) X(3, 20, 4, 0, 1) X(3, 20, 3, 0, 1) X(3, 20, 4, 4, 2) X(3, 24, 4, 0, 1)        \
+  X(3, 32, 4, 0, 1) X(2, 4, 4, 0, 1) X(2, 6, 4, 0, 1) X(2, 8, 4, 0, 1) X(2, 12, 4, 0, 1) X(2, 16, 4, 0, 1)            \
+  X(2, 20, 4, 0, 1) X(2, 24, 4, 0, 1)
 #endif
 
 // Each instance comes in two builds: for fields whose order planes lie
 // within 2 GiB of the field's base, and (_big) for larger ones.
-#define NSPT_DECL_SET(nc, n, q) \
-  KernelSet kernel_set_##nc##_##n##_q##q();  \
-  KernelSet kernel_set_##nc##_##n##_q##q##_big();
+#define NSPT_DECL_SET(nc, n, q, l, w)                   \
+  KernelSet kernel_set_##nc##_##n##_q##q##_l##l##_w##w(); \
+  KernelSet kernel_set_##nc##_##n##_q##q##_l##l##_w##w##_big();
 NSPT_INSTANCES(NSPT_DECL_SET)
 #undef NSPT_DECL_SET
 

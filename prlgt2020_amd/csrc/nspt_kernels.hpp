@@ -19,9 +19,6 @@
 #ifndef NSPT_QT_DRIFT_P
 #define NSPT_QT_DRIFT_P NSPT_QT_DRIFT
 #endif
-#ifndef NSPT_WAVES_PER_EU
-#define NSPT_WAVES_PER_EU 1
-#endif
 
 namespace nspt {
 inline namespace NSPT_VARIANT {
@@ -1489,6 +1486,8 @@ KernelSet make_kernel_set() {
   s.nmax = N;
   s.big = NSPT_BIG_FIELDS != 0;
   s.lpb = Lane<NC, N>::LPB;
+  s.lpw = NSPT_LPW;
+  s.wpe = NSPT_WAVES_PER_EU;
   s.qt = NSPT_QT;
   s.drift = [](int nblk, hipStream_t st, Geo g, SerField U, SerField D, SerField D1, double tau_g, double rk2c,
                int mode, double* obs, SerField R, double c1) {

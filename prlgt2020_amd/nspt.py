@@ -66,7 +66,8 @@ EXPORTS = ["nspt_create", "nspt_destroy", "nspt_last_error", "nspt_obs_len", "ns
            "nspt_batch_replica", "nspt_batch_tune", "nspt_adjoint_drift", "nspt_loops",
            "nspt_unitarize", "nspt_set_action", "nspt_set_twist", "nspt_gauge_wilson",
            "nspt_zero_modes_subtraction", "nspt_stochastic_gauge_fixing", "nspt_rng_create_rand55",
-           "nspt_adjoint_drift_dev", "nspt_batch_set_form", "nspt_batch_form", "nspt_measure_peaks"]
+           "nspt_adjoint_drift_dev", "nspt_batch_set_form", "nspt_batch_form", "nspt_measure_peaks",
+           "nspt_kernel_shape", "nspt_batch_tune_candidates", "nspt_batch_kernel_shape"]
 
 PROF_NAMES = ["drift", "update", "update_log", "zero_mode", "fa_div", "dft", "fa_gexp", "transform", "noise",
               "reduce", "rect", "plaq_field"]
@@ -111,6 +112,10 @@ def lib():
         l.nspt_batch_replica.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
         l.nspt_batch_tune.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int), _dp]
         l.nspt_batch_set_form.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        _ip = ctypes.POINTER(ctypes.c_int)
+        l.nspt_kernel_shape.argtypes = [ctypes.c_void_p, _ip, _ip, _ip]
+        l.nspt_batch_kernel_shape.argtypes = [ctypes.c_void_p, _ip, _ip, _ip]
+        l.nspt_batch_tune_candidates.argtypes = [ctypes.c_void_p, _ip, ctypes.c_int]
         l.nspt_batch_form.argtypes = [ctypes.c_void_p]
         l.nspt_adjoint_drift.argtypes = [ctypes.POINTER(_Geom), _dp, ctypes.c_double, _dp, _dp]
         l.nspt_adjoint_drift_dev.argtypes = [ctypes.POINTER(_Geom), ctypes.c_void_p, ctypes.c_double,
@@ -201,6 +206,13 @@ class Context:
         U = np.ascontiguousarray(U, dtype=np.complex128)
         assert U.shape == (4 * self.V, self.S), U.shape
         self._check(lib().nspt_upload(self.h, _ptr(U)), "upload")
+
+    def kernel_shape(self):
+        """(tile edge, links per wave, waves per SIMD) of the kernel build
+        this context runs."""
+        v = [ctypes.c_int() for _ in range(3)]
+        self._check(lib().nspt_kernel_shape(self.h, *[ctypes.byref(x) for x in v]), "kernel_shape")
+        return tuple(x.value for x in v)
 
     def download(self):
         U = np.zeros((4 * self.V, self.S), np.complex128)
@@ -380,11 +392,27 @@ class Batch:
     def tune(self, steps=3):
         """Returns (chosen tile edge, ms per step of each candidate: for each
         form -- branches, then the table in 1, 2 and 4 groups as the
-        replicas allow -- the compiled tile edges)."""
+        replicas allow -- the compiled tile edges, then the occupancy
+        variants of the best edge in the best form; tune_candidates()
+        describes them)."""
         q = ctypes.c_int()
         ms = np.zeros(16)
         self._check(lib().nspt_batch_tune(self.h, steps, ctypes.byref(q), _ptr(ms)), "batch_tune")
         return q.value, ms
+
+    def tune_candidates(self):
+        """The last tune()'s candidates as (form, tile edge, links per wave
+        (0: the build's default), waves per SIMD) tuples, in its order."""
+        d = (ctypes.c_int * 64)()
+        n = lib().nspt_batch_tune_candidates(self.h, d, 16)
+        self._check(0 if n >= 0 else n, "batch_tune_candidates")
+        return [tuple(d[4 * i:4 * i + 4]) for i in range(n)]
+
+    def kernel_shape(self):
+        """(tile edge, links per wave, waves per SIMD) of the replicas' build."""
+        v = [ctypes.c_int() for _ in range(3)]
+        self._check(lib().nspt_batch_kernel_shape(self.h, *[ctypes.byref(x) for x in v]), "batch_kernel_shape")
+        return tuple(x.value for x in v)
 
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:

@@ -103,3 +103,26 @@ def test_package_fails_loudly_without_library(tmp_path):
     env = dict(os.environ, NSPT_LIB=str(tmp_path / "missing.so"))
     out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=120)
     assert "raised" in out.stdout and "not built" in out.stdout, out.stdout + out.stderr
+
+
+def test_every_kernel_build_has_its_own_launch_handles():
+    """Builds of one kernel template (tile edge, occupancy, field size) live
+    in namespaces of their own: each has its own host launch handle, so the
+    runtime launches each build's own code (one shared handle would make every
+    build run the code registered first)."""
+    from prlgt2020_amd import build
+    out = subprocess.run(["nm", "-C", "--defined-only", os.path.join(ROOT, "prlgt2020_amd", "libnspt.so")],
+                         capture_output=True, text=True, check=True).stdout
+    handles = {}
+    for line in out.splitlines():
+        m = re.match(r"([0-9a-f]+) V void nspt::(\w+)::k_update<(\d), (\d+)>\(", line)
+        if m:
+            handles[(m.group(2), int(m.group(3)), int(m.group(4)))] = m.group(1)
+    want = []
+    for nc, n, q, lpw, wpe in build.instances():
+        for f in ("std_fields", "big_fields"):
+            want.append(("%s_q%d_l%d_w%d" % (f, q, lpw, wpe), nc, n))
+    missing = [w for w in want if w not in handles]
+    assert not missing, missing
+    addrs = [handles[w] for w in want]
+    assert len(set(addrs)) == len(addrs)
