@@ -1126,8 +1126,11 @@ bool do_step(nspt_ctx* c, const nspt_step_params* p, double* obs_dev) {
     if (obs_dev && !obs_reduce(c, obs_dev, impr)) return false;
     {
       Prof pr(c, need_log ? NSPT_PROF_UPDATE_LOG : NSPT_PROF_UPDATE);
-      c->ks.update(c->nblk_links, c->stream, c->g, c->D, eta, stau, c->U, c->F, need_log ? c->L : none,
-                   p->zero_modes ? c->mm_part : nullptr);
+      // with zero modes the updated links are re-formed from exp(L - MM) by
+      // the zero-mode kernel or the FA gauge transform (series and flags):
+      // the update writes only L
+      c->ks.update(c->nblk_links, c->stream, c->g, c->D, eta, stau, c->U, p->zero_modes ? none : c->F,
+                   need_log ? c->L : none, p->zero_modes ? c->mm_part : nullptr);
     }
     if (!launch_ok(c, "update")) return false;
     std::swap(c->U, c->F);  // Umu = Fmu (nspt.cc:992)
@@ -1920,8 +1923,11 @@ void plan_table_step(nspt_batch* b, const std::vector<int>& idx, int q, bool wit
     std::vector<UpdateArgs> v(R);
     for (size_t r = 0; r < R; r++) {
       nspt_ctx* c = reps[r];
-      v[r] = UpdateArgs{c->g, (rk2 && !final) ? c->D1 : c->D, c->eta, std::sqrt(params[r].eps), cur(c), nxt(c),
-                        (final && need_log) ? c->L : none, (final && zm) ? c->mm_part : nullptr, c->nblk_links};
+      // the final update with zero modes writes only L (the links are
+      // re-formed from exp(L - MM) afterwards)
+      v[r] = UpdateArgs{c->g, (rk2 && !final) ? c->D1 : c->D, c->eta, std::sqrt(params[r].eps), cur(c),
+                        (final && zm) ? none : nxt(c), (final && need_log) ? c->L : none,
+                        (final && zm) ? c->mm_part : nullptr, c->nblk_links};
     }
     tp.launch(TabLaunch::UPDATE, dim3(gl, nr), v);
   };

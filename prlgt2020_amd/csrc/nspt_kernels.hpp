@@ -756,7 +756,8 @@ __global__ __launch_bounds__(64, NSPT_WAVES_PER_EU) void k_drift_p_tab(const Dri
 // ------------------------------------------------------------ update
 // Uout = exp(D - stau*eta at order 1) * Ubase  (nspt.cc:420-430, exp QCDpt.h:200-220);
 // optionally L = reH(log(Uout)) (QCDpt.h:176-198, 197-209) and per-block sums
-// of L for the zero mode (nspt.cc:432-439).  reH is linear, so the zero-mode
+// of L for the zero mode (nspt.cc:432-439).  Uout.data null: only L is
+// written (the caller re-forms the links from L).  reH is linear, so the zero-mode
 // generator reH(log U - mean) of nspt.cc:638 is L - mean(L).
 // eta: [idx*NE + e] (may be null).
 template <int NC, int N>
@@ -790,8 +791,10 @@ __device__ __forceinline__ void k_update_body(int blk_, int nblk_, const Geo& g,
   else
     ps_exp_times_staged<NC, N>(ln, n, W, ubf, [&]() { stage_global<NC, N>(ln, ln.slotB, Ubase, idx, nrt); });
   unstage(ln, ln.slotB, S);
-  gstore<NC, N>(Uout, idx, ln.e, nrt, ok, S);
-  if (ok && ln.e == 0) Uout.flag[idx] = ubf;
+  if (Uout.data) {
+    gstore<NC, N>(Uout, idx, ln.e, nrt, ok, S);
+    if (ok && ln.e == 0) Uout.flag[idx] = ubf;
+  }
   if (!L.data) return;
   // L = reH(log(Unew)): generator Unew - 1 is already in slot B
   series_fn<NC, N, false>(ln, n, S);
