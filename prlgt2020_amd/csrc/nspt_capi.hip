@@ -1423,9 +1423,14 @@ int nspt_loops(nspt_ctx* c, double* out) {
   const long rows = 6L * c->g.nb_mu;  // per rectangle type: 6 planes x site blocks
   if (!c->loop_part) {
     const long P = 4L * c->V;
+    // sized for the build of this order with the fewest links per wave (the
+    // batch autotuner may move a replica to it after this first call)
+    const int lpb_min = std::min(c->ks.lpb, min_links_per_wave(c->nc, c->ks.nmax));
+    const long rows_max = 6L * ((c->V + lpb_min - 1) / lpb_min);
     if (!((c->L2.data || c->alloc_field(c->L2, P, true)) && c->alloc_field(c->L4, P, true) &&
-          c->alloc(&c->loop_part, (size_t)6 * rows * ncols) &&
-          c->alloc(&c->loop_tmp, reduce_tmp_doubles(rows, 6, ncols)) && c->alloc(&c->loop_res, (size_t)6 * ncols))) {
+          c->alloc(&c->loop_part, (size_t)6 * rows_max * ncols) &&
+          c->alloc(&c->loop_tmp, reduce_tmp_doubles(rows_max, 6, ncols)) &&
+          c->alloc(&c->loop_res, (size_t)6 * ncols))) {
       c->loop_part = nullptr;
       return -2;
     }
